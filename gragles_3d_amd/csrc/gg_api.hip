@@ -1,0 +1,737 @@
+// Context management and the C ABI of libgg_levelset (include/gg_levelset.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gg_levelset.h"
+#include "gg_internal.h"
+
+using namespace gg;
+
+struct gg_ctx {
+  gg_params p{};
+  SimParams sp{};
+  bool fp32 = false;
+  size_t esz = 8;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int grid = 2048;
+
+  // per grain
+  int32_t* d_state = nullptr;
+  int32_t* d_newb = nullptr;
+  int32_t* d_exflag = nullptr;
+  double* d_volume = nullptr;
+  double* d_area = nullptr;
+  // distance roles
+  DescDev D[2]{};
+  int sel = 0;  // D[sel] is the input role
+  void* arena[2] = {nullptr, nullptr};
+  int64_t arena_cap = 0;
+  // IDLocal
+  IdlDev IL[2]{};
+  int il_cur = 0;  // IL[il_cur] holds the latest IDLocal content
+  // lists
+  ListDev LD{};
+  // plan
+  PlanDev P{};
+  int64_t* d_spec_off = nullptr;
+  void* d_spec = nullptr;
+  int64_t spec_cap = 0;
+  double* d_partials = nullptr;
+  // device scalars
+  int32_t* d_err = nullptr;
+  int32_t* d_step = nullptr;
+  int32_t* d_hist = nullptr;
+  int32_t hist_cap = 1 << 16;
+  int64_t* d_voxels = nullptr;
+  // tables
+  TablesDev TB{};
+  void* tb_mem[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  bool boxes_set = false;
+  bool lists_set = false;
+  // graph of two timesteps
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int graph_sel = -1, graph_il = -1;
+  // stage timing
+  bool timing = false;
+  hipEvent_t ev[12] = {};
+  double stage_ms[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<std::pair<int, int>> pending;  // (stage, event pair index)
+  std::vector<hipEvent_t> evpool;
+  int evused = 0;
+
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+int fail(gg_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(c, call)                                                                         \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      return fail((c), GG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+template <class T>
+int dalloc(gg_ctx* c, T** ptr, size_t count) {
+  void* p = nullptr;
+  size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) return fail(c, GG_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  c->allocs.push_back(p);
+  *ptr = (T*)p;
+  return GG_OK;
+}
+
+const double kPi = 3.14159265358979323846;
+
+Launch launch_of(gg_ctx* c) { return Launch{c->stream, c->grid}; }
+
+int check_device_error(gg_ctx* c) {
+  int32_t e = 0;
+  HIPCHK(c, hipMemcpy(&e, c->d_err, sizeof(e), hipMemcpyDeviceToHost));
+  if (e) {
+    std::string m = "device capacity error:";
+    if (e & kErrArena) m += " distance arena full;";
+    if (e & kErrList) m += " comparison list capacity exceeded;";
+    if (e & kErrBoxSize) m += " box side above 96 or box not cubic;";
+    if (e & kErrSpec) m += " spectral scratch full;";
+    if (e & kErrEmptyIO) m += " empty in/out intersection in redistancing;";
+    if (e & kErrHistory) m += " count history full;";
+    return fail(c, GG_ERR_CAPACITY, m);
+  }
+  return GG_OK;
+}
+
+// The C library's exp/cos/sin, called through volatile pointers so the
+// compiler cannot fold, fuse (sincos) or vectorise the calls: the FP64 tables
+// then carry exactly the bits of the host C library, as the CPU restatement's
+// do on the same host.
+double (*volatile libm_exp)(double) = ::exp;
+double (*volatile libm_cos)(double) = ::cos;
+double (*volatile libm_sin)(double) = ::sin;
+
+// host copies of the per-n tables
+int build_tables(gg_ctx* c) {
+  std::vector<int32_t> tw_off(kMaxN + 1, 0), g_off(kMaxN + 1, 0);
+  std::vector<double> twc, tws, gt;
+  std::vector<float> circ;
+  const int nn = c->p.conv_domain_size;
+  const double dt = c->p.dt;
+  for (int n = 1; n <= kMaxN; ++n) {
+    tw_off[n] = (int32_t)twc.size();
+    for (int t = 0; t < n; ++t) {
+      twc.push_back(libm_cos(2.0 * kPi * t / n));
+      tws.push_back(libm_sin(2.0 * kPi * t / n));
+    }
+    // spectral factor, the reference's expression evaluated left to right (src/box.cpp:760-761)
+    g_off[n] = (int32_t)gt.size();
+    const double nsq = nn * nn;
+    const double local_nsq = n * n;
+    const int h2 = n / 2;
+    for (int q = 0; q <= 3 * h2 * h2; ++q)
+      gt.push_back(libm_exp(-sqrt(2 / kPi) * q * 4.0 * dt * nsq / local_nsq * kPi * kPi) / (local_nsq * n));
+    // one-axis circulant row: c[m] = (1/n) sum_k f(min(k,n-k)) cos(2 pi k m / n)
+    std::vector<double> f(n);
+    for (int k = 0; k < n; ++k) {
+      int k2 = std::min(k, n - k);
+      f[k] = exp(-sqrt(2 / kPi) * (k2 * k2) * 4.0 * dt * nsq / local_nsq * kPi * kPi);
+    }
+    for (int m = 0; m < n; ++m) {
+      long double acc = 0;
+      for (int k = 0; k < n; ++k) acc += (long double)f[k] * cosl(2.0L * 3.141592653589793238462643383279L * ((k * m) % n) / n);
+      circ.push_back((float)(acc / n));
+    }
+  }
+  int32_t *d_two, *d_go;
+  double *d_twc, *d_tws, *d_gt;
+  float* d_circ;
+  int rc;
+  if ((rc = dalloc(c, &d_two, tw_off.size())) || (rc = dalloc(c, &d_go, g_off.size())) ||
+      (rc = dalloc(c, &d_twc, twc.size())) || (rc = dalloc(c, &d_tws, tws.size())) ||
+      (rc = dalloc(c, &d_gt, gt.size())) || (rc = dalloc(c, &d_circ, circ.size())))
+    return rc;
+  HIPCHK(c, hipMemcpy(d_two, tw_off.data(), tw_off.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(d_go, g_off.data(), g_off.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(d_twc, twc.data(), twc.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(d_tws, tws.data(), tws.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(d_gt, gt.data(), gt.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(d_circ, circ.data(), circ.size() * 4, hipMemcpyHostToDevice));
+  c->TB = TablesDev{d_two, d_twc, d_tws, d_go, d_gt, d_circ};
+  return GG_OK;
+}
+
+void cube_bounds_host(int32_t* b, int L) {
+  int grid_size = L - 1;
+  int w = b[3] - b[0], hgt = b[4] - b[1], d = b[5] - b[2];
+  if (w == hgt && w == d) return;
+  int ref = w > hgt ? w : hgt;
+  if (ref < d) ref = d;
+  if (ref > w) { int diff = ref - w; b[0] -= diff / 2; b[3] += diff / 2 + diff % 2; }
+  if (ref > hgt) { int diff = ref - hgt; b[1] -= diff / 2; b[4] += diff / 2 + diff % 2; }
+  if (ref > d) { int diff = ref - d; b[2] -= diff / 2; b[5] += diff / 2 + diff % 2; }
+  for (int a = 0; a < 3; ++a) {
+    if (b[a] < 0) { int delta = -b[a]; b[a + 3] += delta; b[a] += delta; }
+    else if (b[a + 3] > grid_size) { int delta = grid_size - b[a + 3]; b[a + 3] += delta; b[a] += delta; }
+  }
+}
+
+DescDev din(gg_ctx* c) { return c->D[c->sel]; }
+DescDev dout(gg_ctx* c) { return c->D[c->sel ^ 1]; }
+
+void plan_layout(gg_ctx* c, const DescDev& d) {
+  launch_plan(launch_of(c), c->sp, c->d_state, d.bounds, d.off, c->d_spec_off, c->P, c->arena_cap, c->spec_cap,
+              c->d_err, c->d_voxels);
+}
+
+// stage timing helpers
+hipEvent_t take_event(gg_ctx* c) {
+  if (c->evused >= (int)c->evpool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->evpool.push_back(e);
+  }
+  return c->evpool[c->evused++];
+}
+
+struct StageTimer {
+  gg_ctx* c;
+  int stage;
+  hipEvent_t a = nullptr;
+  StageTimer(gg_ctx* ctx, int s) : c(ctx), stage(s) {
+    if (c->timing) {
+      a = take_event(c);
+      if (a) hipEventRecord(a, c->stream);
+    }
+  }
+  ~StageTimer() {
+    if (c->timing && a) {
+      hipEvent_t b = take_event(c);
+      if (b) {
+        hipEventRecord(b, c->stream);
+        c->pending.push_back({stage, c->evused - 2});
+      }
+    }
+  }
+};
+
+void collect_timing(gg_ctx* c) {
+  for (auto& pr : c->pending) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c->evpool[pr.second], c->evpool[pr.second + 1]) == hipSuccess)
+      c->stage_ms[pr.first] += ms;
+  }
+  c->pending.clear();
+  c->evused = 0;
+}
+
+int require_boxes(gg_ctx* c) {
+  if (!c->boxes_set) return fail(c, GG_ERR_STATE, "gg_set_boxes has not been called");
+  return GG_OK;
+}
+
+// one run_sim iteration body, enqueued on the stream
+void enqueue_step(gg_ctx* c) {
+  Launch L = launch_of(c);
+  c->sel ^= 1;  // switchDistancebuffer (gridCoarsement's else branch)
+  {
+    StageTimer t(c, 0);
+    launch_conv(L, c->sp, c->P, din(c), dout(c), c->TB, c->d_spec_off, c->d_spec, c->fp32);
+    launch_idl_resize(L, c->sp, c->d_state, dout(c), c->IL[c->il_cur ^ 1]);
+  }
+  c->sel ^= 1;
+  {
+    StageTimer t(c, 1);
+    launch_lists(L, c->sp, c->d_state, din(c), dout(c), c->LD, c->d_newb, c->d_err);
+  }
+  {
+    StageTimer t(c, 2);
+    launch_comparison(L, c->sp, c->P, c->d_state, din(c), dout(c), c->LD, c->IL[c->il_cur], c->IL[c->il_cur ^ 1],
+                      c->d_newb, c->fp32);
+  }
+  c->il_cur ^= 1;
+  c->sel ^= 1;
+  {
+    StageTimer t(c, 3);
+    launch_hull(L, c->sp, c->P, din(c), c->d_exflag, c->d_partials, c->fp32);
+    launch_finalize(L, c->sp, c->P, c->d_state, c->LD, c->d_exflag, c->d_partials, c->d_newb, din(c), dout(c),
+                    c->d_volume, c->d_area);
+    plan_layout(c, dout(c));
+  }
+  {
+    StageTimer t(c, 4);
+    launch_redist(L, c->sp, c->P, c->d_state, din(c), dout(c), c->d_err, c->fp32);
+    launch_post_redist(L, c->sp, c->d_state, din(c), dout(c));
+  }
+  {
+    StageTimer t(c, 5);
+    launch_count(L, c->sp, c->d_state, c->d_step, c->d_hist, c->hist_cap, c->d_err);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int gg_create(const gg_params* params, gg_ctx** out) {
+  if (!params || !out) return GG_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return GG_ERR_DEVICE;
+  if (params->device < 0 || params->device >= ndev) return GG_ERR_ARG;
+  if (params->n_grains < 1 || params->ngridpoints < 2 || params->grid_blowup < 0 ||
+      (params->precision != GG_FP64_PLAIN && params->precision != GG_FP32_BATCHED))
+    return GG_ERR_ARG;
+  gg_ctx* c = new gg_ctx;
+  c->p = *params;
+  if (c->p.max_list <= 0) c->p.max_list = 512;
+  if (c->p.max_list > 1024 || c->p.ngridpoints > 32767) {
+    delete c;
+    return GG_ERR_ARG;
+  }
+  c->fp32 = params->precision == GG_FP32_BATCHED;
+  c->esz = c->fp32 ? 4 : 8;
+  c->sp = SimParams{params->n_grains, params->ngridpoints, params->grid_blowup, params->boundary_grain_tube,
+                    params->h, params->delta};
+  if (hipSetDevice(params->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return GG_ERR_HIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, params->device) == hipSuccess) c->grid = std::max(256, prop.multiProcessorCount * 8);
+  const int N = params->n_grains;
+  int rc;
+  int32_t* dummy;
+  (void)dummy;
+  if ((rc = dalloc(c, &c->d_state, N + 1)) || (rc = dalloc(c, &c->d_newb, 6 * (N + 1))) ||
+      (rc = dalloc(c, &c->d_exflag, N + 1)) || (rc = dalloc(c, &c->d_volume, N + 1)) ||
+      (rc = dalloc(c, &c->d_area, N + 1)) || (rc = dalloc(c, &c->d_err, 1)) || (rc = dalloc(c, &c->d_step, 1)) ||
+      (rc = dalloc(c, &c->d_hist, c->hist_cap)) || (rc = dalloc(c, &c->d_voxels, 1)) ||
+      (rc = dalloc(c, &c->d_spec_off, N + 1))) {
+    gg_destroy(c);
+    return rc;
+  }
+  for (int r = 0; r < 2; ++r) {
+    if ((rc = dalloc(c, &c->D[r].bounds, 6 * (N + 1))) || (rc = dalloc(c, &c->D[r].off, N + 1)) ||
+        (rc = dalloc(c, &c->IL[r].bounds, 6 * (N + 1))) || (rc = dalloc(c, &c->IL[r].off, N + 1)) ||
+        (rc = dalloc(c, &c->LD.buf[r], (size_t)(N + 1) * c->p.max_list)) || (rc = dalloc(c, &c->LD.len[r], N + 1))) {
+      gg_destroy(c);
+      return rc;
+    }
+  }
+  c->LD.max_list = c->p.max_list;
+  if ((rc = dalloc(c, &c->LD.sel, N + 1)) || (rc = dalloc(c, &c->P.cls_count, kClasses)) ||
+      (rc = dalloc(c, &c->P.cls_start, kClasses)) || (rc = dalloc(c, &c->P.grains, N + 1)) ||
+      (rc = dalloc(c, &c->P.grain_pos, N + 1)) || (rc = dalloc(c, &c->P.totals, kNumQ * kClasses)) ||
+      (rc = dalloc(c, &c->P.layer_base, kClasses)) || (rc = dalloc(c, &c->d_partials, (size_t)4 * (N + 1) * kMaxN))) {
+    gg_destroy(c);
+    return rc;
+  }
+  for (int q = 0; q < kNumQ; ++q)
+    if ((rc = dalloc(c, &c->P.pre[q], N + 1 + kClasses))) {
+      gg_destroy(c);
+      return rc;
+    }
+  // zero the small per-grain state
+  hipMemset(c->d_state, 0, sizeof(int32_t) * (N + 1));
+  hipMemset(c->d_exflag, 0, sizeof(int32_t) * (N + 1));
+  hipMemset(c->d_volume, 0, sizeof(double) * (N + 1));
+  hipMemset(c->d_area, 0, sizeof(double) * (N + 1));
+  hipMemset(c->d_err, 0, sizeof(int32_t));
+  hipMemset(c->d_step, 0, sizeof(int32_t));
+  hipMemset(c->LD.sel, 0, sizeof(int32_t) * (N + 1));
+  hipMemset(c->LD.len[0], 0, sizeof(int32_t) * (N + 1));
+  hipMemset(c->LD.len[1], 0, sizeof(int32_t) * (N + 1));
+  if ((rc = build_tables(c))) {
+    gg_destroy(c);
+    return rc;
+  }
+  upload_mc_tables();
+  if (hipDeviceSynchronize() != hipSuccess) {
+    gg_destroy(c);
+    return GG_ERR_HIP;
+  }
+  *out = c;
+  return GG_OK;
+}
+
+int gg_destroy(gg_ctx* c) {
+  if (!c) return GG_OK;
+  hipSetDevice(c->p.device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  for (auto e : c->evpool) hipEventDestroy(e);
+  for (void* p : c->allocs) hipFree(p);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return GG_OK;
+}
+
+const char* gg_last_error(const gg_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int gg_set_boxes(gg_ctx* c, const int32_t* bounds, const uint8_t* exists) {
+  if (!c || !bounds || !exists) return GG_ERR_ARG;
+  hipSetDevice(c->p.device);
+  const int N = c->p.n_grains, L = c->p.ngridpoints;
+  std::vector<int32_t> b(6 * (N + 1), 0), st(N + 1, kDeleted);
+  int64_t total = 0;
+  for (int g = 1; g <= N; ++g) {
+    if (!exists[g]) continue;
+    int32_t* r = &b[6 * g];
+    for (int a = 0; a < 6; ++a) r[a] = bounds[6 * g + a];
+    if (r[3] <= r[0] || r[4] <= r[1] || r[5] <= r[2]) return fail(c, GG_ERR_ARG, "empty box bounds");
+    cube_bounds_host(r, L);
+    int n = r[3] - r[0];
+    if (n > kMaxN) return fail(c, GG_ERR_CAPACITY, "box side above 96");
+    st[g] = kAlive;
+    total += (int64_t)n * n * n;
+  }
+  c->arena_cap = c->p.arena_voxels > 0 ? c->p.arena_voxels : 2 * total + 65536;
+  c->spec_cap = c->arena_cap;
+  for (int r = 0; r < 2; ++r) {
+    if (c->arena[r] == nullptr) {
+      int rc;
+      if (c->fp32) rc = dalloc(c, (float**)&c->arena[r], c->arena_cap);
+      else rc = dalloc(c, (double**)&c->arena[r], c->arena_cap);
+      if (rc) return rc;
+      if ((rc = dalloc(c, &c->IL[r].arena, c->arena_cap))) return rc;
+    }
+    c->D[r].arena = c->arena[r];
+  }
+  if (!c->fp32 && c->d_spec == nullptr) {
+    int rc = dalloc(c, (double2**)&c->d_spec, c->spec_cap);
+    if (rc) return rc;
+  }
+  HIPCHK(c, hipMemcpy(c->d_state, st.data(), 4 * (N + 1), hipMemcpyHostToDevice));
+  for (int r = 0; r < 2; ++r) {
+    HIPCHK(c, hipMemcpy(c->D[r].bounds, b.data(), 4 * b.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->IL[r].bounds, b.data(), 4 * b.size(), hipMemcpyHostToDevice));
+  }
+  c->sel = 0;
+  c->il_cur = 0;
+  plan_layout(c, c->D[0]);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int rc = check_device_error(c);
+  if (rc) return rc;
+  for (int r = 0; r < 2; ++r) {
+    if (r == 1) HIPCHK(c, hipMemcpy(c->D[1].off, c->D[0].off, 8 * (N + 1), hipMemcpyDeviceToDevice));
+    HIPCHK(c, hipMemcpy(c->IL[r].off, c->D[0].off, 8 * (N + 1), hipMemcpyDeviceToDevice));
+  }
+  // IDLocal starts zero (std::vector value-initialisation, src/box.cpp:221)
+  HIPCHK(c, hipMemset(c->IL[0].arena, 0, 4 * c->arena_cap));
+  HIPCHK(c, hipMemset(c->IL[1].arena, 0, 4 * c->arena_cap));
+  HIPCHK(c, hipDeviceSynchronize());
+  c->boxes_set = true;
+  return GG_OK;
+}
+
+int gg_init_distance(gg_ctx* c, const int32_t* idfield) {
+  if (!c || !idfield) return GG_ERR_ARG;
+  int rc = require_boxes(c);
+  if (rc) return rc;
+  hipSetDevice(c->p.device);
+  const int64_t L = c->p.ngridpoints;
+  int32_t* d_id = nullptr;
+  HIPCHK(c, hipMalloc(&d_id, sizeof(int32_t) * L * L * L));
+  hipError_t e = hipMemcpy(d_id, idfield, sizeof(int32_t) * L * L * L, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(d_id);
+    return fail(c, GG_ERR_HIP, std::string("hipMemcpy idfield: ") + hipGetErrorString(e));
+  }
+  Launch Lc = launch_of(c);
+  launch_init_phi(Lc, c->sp, c->P, din(c), d_id, c->fp32);
+  launch_redist(Lc, c->sp, c->P, c->d_state, din(c), dout(c), c->d_err, c->fp32);
+  launch_post_redist(Lc, c->sp, c->d_state, din(c), dout(c));
+  e = hipStreamSynchronize(c->stream);
+  hipFree(d_id);
+  if (e != hipSuccess) return fail(c, GG_ERR_HIP, std::string("init distance: ") + hipGetErrorString(e));
+  return check_device_error(c);
+}
+
+int gg_set_neighbour_lists(gg_ctx* c, const int32_t* offsets, const int32_t* ids) {
+  if (!c || !offsets) return GG_ERR_ARG;
+  hipSetDevice(c->p.device);
+  const int N = c->p.n_grains, M = c->p.max_list;
+  std::vector<int32_t> buf((size_t)(N + 1) * M, 0), len(N + 1, 0);
+  for (int g = 1; g <= N; ++g) {
+    int a = offsets[g], b = offsets[g + 1];
+    if (b - a > M) return fail(c, GG_ERR_CAPACITY, "initial neighbour list longer than max_list");
+    for (int i = a; i < b; ++i) {
+      if (ids[i] < 1 || ids[i] > N) return fail(c, GG_ERR_ARG, "neighbour id out of range");
+      buf[(size_t)g * M + (i - a)] = ids[i];
+    }
+    len[g] = b - a;
+  }
+  std::vector<int32_t> zero(N + 1, 0);
+  HIPCHK(c, hipMemcpy(c->LD.buf[0], buf.data(), 4 * buf.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->LD.len[0], len.data(), 4 * len.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->LD.sel, zero.data(), 4 * zero.size(), hipMemcpyHostToDevice));
+  c->lists_set = true;
+  return GG_OK;
+}
+
+int gg_switch_buffers(gg_ctx* c) {
+  if (!c) return GG_ERR_ARG;
+  c->sel ^= 1;
+  return GG_OK;
+}
+
+int gg_convolution(gg_ctx* c) {
+  if (!c) return GG_ERR_ARG;
+  int rc = require_boxes(c);
+  if (rc) return rc;
+  hipSetDevice(c->p.device);
+  StageTimer t(c, 0);
+  launch_conv(launch_of(c), c->sp, c->P, din(c), dout(c), c->TB, c->d_spec_off, c->d_spec, c->fp32);
+  launch_idl_resize(launch_of(c), c->sp, c->d_state, dout(c), c->IL[c->il_cur ^ 1]);
+  return GG_OK;
+}
+
+int gg_update_neighbour_lists(gg_ctx* c) {
+  if (!c) return GG_ERR_ARG;
+  int rc = require_boxes(c);
+  if (rc) return rc;
+  hipSetDevice(c->p.device);
+  StageTimer t(c, 1);
+  launch_lists(launch_of(c), c->sp, c->d_state, din(c), dout(c), c->LD, c->d_newb, c->d_err);
+  return GG_OK;
+}
+
+int gg_comparison(gg_ctx* c) {
+  if (!c) return GG_ERR_ARG;
+  int rc = require_boxes(c);
+  if (rc) return rc;
+  hipSetDevice(c->p.device);
+  StageTimer t(c, 2);
+  launch_comparison(launch_of(c), c->sp, c->P, c->d_state, din(c), dout(c), c->LD, c->IL[c->il_cur],
+                    c->IL[c->il_cur ^ 1], c->d_newb, c->fp32);
+  c->il_cur ^= 1;
+  return GG_OK;
+}
+
+int gg_level_set(gg_ctx* c) {
+  if (!c) return GG_ERR_ARG;
+  int rc = require_boxes(c);
+  if (rc) return rc;
+  hipSetDevice(c->p.device);
+  StageTimer t(c, 3);
+  Launch L = launch_of(c);
+  launch_hull(L, c->sp, c->P, din(c), c->d_exflag, c->d_partials, c->fp32);
+  launch_finalize(L, c->sp, c->P, c->d_state, c->LD, c->d_exflag, c->d_partials, c->d_newb, din(c), dout(c),
+                  c->d_volume, c->d_area);
+  plan_layout(c, dout(c));
+  return GG_OK;
+}
+
+int gg_redistancing(gg_ctx* c) {
+  if (!c) return GG_ERR_ARG;
+  int rc = require_boxes(c);
+  if (rc) return rc;
+  hipSetDevice(c->p.device);
+  StageTimer t(c, 4);
+  launch_redist(launch_of(c), c->sp, c->P, c->d_state, din(c), dout(c), c->d_err, c->fp32);
+  launch_post_redist(launch_of(c), c->sp, c->d_state, din(c), dout(c));
+  return GG_OK;
+}
+
+int gg_count_grains(gg_ctx* c, int32_t* count) {
+  if (!c || !count) return GG_ERR_ARG;
+  hipSetDevice(c->p.device);
+  {
+    StageTimer t(c, 5);
+    launch_count(launch_of(c), c->sp, c->d_state, c->d_step, c->d_hist, c->hist_cap, c->d_err);
+  }
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  int32_t st = 0;
+  HIPCHK(c, hipMemcpy(&st, c->d_step, 4, hipMemcpyDeviceToHost));
+  if (st < 1 || st > c->hist_cap) return fail(c, GG_ERR_CAPACITY, "count history full");
+  HIPCHK(c, hipMemcpy(count, c->d_hist + st - 1, 4, hipMemcpyDeviceToHost));
+  return GG_OK;
+}
+
+int gg_timestep(gg_ctx* c, int32_t nsteps, int32_t use_graph) {
+  if (!c || nsteps < 0) return GG_ERR_ARG;
+  int rc = require_boxes(c);
+  if (rc) return rc;
+  if (!c->lists_set) return fail(c, GG_ERR_STATE, "gg_set_neighbour_lists has not been called");
+  hipSetDevice(c->p.device);
+  if (!use_graph || c->timing) {
+    for (int s = 0; s < nsteps; ++s) enqueue_step(c);
+    return GG_OK;
+  }
+  int s = 0;
+  // the graph holds two timesteps (an even number of role swaps) and is valid
+  // for the role parity it was captured at
+  if (c->gexec && (c->graph_sel != c->sel || c->graph_il != c->il_cur)) {
+    if (nsteps > 0) {
+      enqueue_step(c);
+      ++s;
+    }
+  }
+  if (!c->gexec && nsteps - s >= 2) {
+    c->graph_sel = c->sel;
+    c->graph_il = c->il_cur;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(c);
+    enqueue_step(c);
+    hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
+    if (e != hipSuccess) return fail(c, GG_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return fail(c, GG_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    // capturing advanced the host-side role state by two steps; restore it
+    c->sel = c->graph_sel;
+    c->il_cur = c->graph_il;
+  }
+  while (nsteps - s >= 2 && c->gexec && c->graph_sel == c->sel && c->graph_il == c->il_cur) {
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    s += 2;
+  }
+  for (; s < nsteps; ++s) enqueue_step(c);
+  return GG_OK;
+}
+
+int gg_synchronize(gg_ctx* c) {
+  if (!c) return GG_ERR_ARG;
+  hipSetDevice(c->p.device);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->timing) collect_timing(c);
+  if (!c->boxes_set) return GG_OK;
+  return check_device_error(c);
+}
+
+int gg_get_count_history(gg_ctx* c, int32_t* counts, int32_t cap, int32_t* n) {
+  if (!c || !n) return GG_ERR_ARG;
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  int32_t st = 0;
+  HIPCHK(c, hipMemcpy(&st, c->d_step, 4, hipMemcpyDeviceToHost));
+  *n = st;
+  int m = std::min(std::min(st, cap), c->hist_cap);
+  if (counts && m > 0) HIPCHK(c, hipMemcpy(counts, c->d_hist, 4 * m, hipMemcpyDeviceToHost));
+  return GG_OK;
+}
+
+int gg_get_grain_stats(gg_ctx* c, int32_t* state, double* volume, double* area) {
+  if (!c) return GG_ERR_ARG;
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  const int N = c->p.n_grains;
+  if (state) HIPCHK(c, hipMemcpy(state, c->d_state, 4 * (N + 1), hipMemcpyDeviceToHost));
+  if (volume) HIPCHK(c, hipMemcpy(volume, c->d_volume, 8 * (N + 1), hipMemcpyDeviceToHost));
+  if (area) HIPCHK(c, hipMemcpy(area, c->d_area, 8 * (N + 1), hipMemcpyDeviceToHost));
+  return GG_OK;
+}
+
+int gg_get_bounds(gg_ctx* c, int32_t id, int32_t which, int32_t* b6) {
+  if (!c || !b6 || id < 0 || id > c->p.n_grains) return GG_ERR_ARG;
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  const DescDev& d = which ? dout(c) : din(c);
+  HIPCHK(c, hipMemcpy(b6, d.bounds + 6 * id, 24, hipMemcpyDeviceToHost));
+  return GG_OK;
+}
+
+int gg_download_distance(gg_ctx* c, int32_t id, int32_t which, double* dst, int64_t cap, int64_t* n) {
+  if (!c || !n || id < 1 || id > c->p.n_grains) return GG_ERR_ARG;
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  const DescDev& d = which ? dout(c) : din(c);
+  int32_t b[6];
+  int64_t off = 0;
+  HIPCHK(c, hipMemcpy(b, d.bounds + 6 * id, 24, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(&off, d.off + id, 8, hipMemcpyDeviceToHost));
+  const int64_t cnt = (int64_t)(b[3] - b[0]) * (b[4] - b[1]) * (b[5] - b[2]);
+  *n = cnt;
+  if (!dst) return GG_OK;
+  if (cnt > cap) return fail(c, GG_ERR_ARG, "destination too small");
+  if (c->fp32) {
+    std::vector<float> tmp(cnt);
+    HIPCHK(c, hipMemcpy(tmp.data(), (float*)d.arena + off, 4 * cnt, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < cnt; ++i) dst[i] = tmp[i];
+  } else {
+    HIPCHK(c, hipMemcpy(dst, (double*)d.arena + off, 8 * cnt, hipMemcpyDeviceToHost));
+  }
+  return GG_OK;
+}
+
+int gg_download_idlocal(gg_ctx* c, int32_t id, int32_t* b6, uint32_t* dst, int64_t cap, int64_t* n) {
+  if (!c || !n || !b6 || id < 1 || id > c->p.n_grains) return GG_ERR_ARG;
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  const IdlDev& il = c->IL[c->il_cur];
+  int64_t off = 0;
+  HIPCHK(c, hipMemcpy(b6, il.bounds + 6 * id, 24, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(&off, il.off + id, 8, hipMemcpyDeviceToHost));
+  const int64_t cnt = (int64_t)(b6[3] - b6[0]) * (b6[4] - b6[1]) * (b6[5] - b6[2]);
+  *n = cnt;
+  if (!dst) return GG_OK;
+  if (cnt > cap) return fail(c, GG_ERR_ARG, "destination too small");
+  HIPCHK(c, hipMemcpy(dst, il.arena + off, 4 * cnt, hipMemcpyDeviceToHost));
+  return GG_OK;
+}
+
+int gg_download_list(gg_ctx* c, int32_t id, int32_t which, int32_t* dst, int32_t cap, int32_t* n) {
+  if (!c || !n || id < 1 || id > c->p.n_grains) return GG_ERR_ARG;
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  int32_t sel = 0, len = 0;
+  HIPCHK(c, hipMemcpy(&sel, c->LD.sel + id, 4, hipMemcpyDeviceToHost));
+  const int b = which ? sel ^ 1 : sel;
+  HIPCHK(c, hipMemcpy(&len, c->LD.len[b] + id, 4, hipMemcpyDeviceToHost));
+  *n = len;
+  if (!dst) return GG_OK;
+  if (len > cap) return fail(c, GG_ERR_ARG, "destination too small");
+  if (len > 0)
+    HIPCHK(c, hipMemcpy(dst, c->LD.buf[b] + (size_t)id * c->p.max_list, 4 * len, hipMemcpyDeviceToHost));
+  return GG_OK;
+}
+
+int gg_get_new_bounds(gg_ctx* c, int32_t* bounds) {
+  if (!c || !bounds) return GG_ERR_ARG;
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpy(bounds, c->d_newb, 24 * (c->p.n_grains + 1), hipMemcpyDeviceToHost));
+  return GG_OK;
+}
+
+int gg_box_voxels(gg_ctx* c, int64_t* voxels) {
+  if (!c || !voxels) return GG_ERR_ARG;
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpy(voxels, c->d_voxels, 8, hipMemcpyDeviceToHost));
+  return GG_OK;
+}
+
+int gg_enable_stage_timing(gg_ctx* c, int32_t enable) {
+  if (!c) return GG_ERR_ARG;
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  c->timing = enable != 0;
+  return GG_OK;
+}
+
+int gg_get_stage_times(gg_ctx* c, double* ms6, int32_t reset) {
+  if (!c || !ms6) return GG_ERR_ARG;
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  for (int i = 0; i < 6; ++i) ms6[i] = c->stage_ms[i];
+  if (reset)
+    for (int i = 0; i < 6; ++i) c->stage_ms[i] = 0;
+  return GG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,356 @@
+// Heat-kernel convolution of every grain box (LSbox::executeConvolution,
+// src/box.cpp:414-539, Gaussian branch of convolutionGeneratorMKL :751-767).
+//
+// FP64 plain form: explicit 3-D DFT on the n^3 box, half spectrum along x,
+// spectral factor G, inverse DFT.  Every sum runs over its index from 0
+// upward starting at 0.0 with the products grouped as below, and the library
+// is built with -ffp-contract=off, so each value is the same IEEE double the
+// FP64 CPU restatement produces:
+//   forward x (real in):  re = re + v*c ; im = im - v*s
+//   forward y, z:         re = re + (a*c + b*s) ; im = im + (b*c - a*s)
+//   spectral factor:      re = re*G ; im = im*G
+//   inverse z, y:         re = re + (a*c - b*s) ; im = im + (a*s + b*c)
+//   inverse x (real out): acc = X0.re ; acc = acc + 2.0*(a*c - b*s), kx=1..(n-1)/2 ;
+//                         n even: acc = acc + (x odd ? -Xn/2.re : Xn/2.re)
+// with c = cos(2 pi t/n), s = sin(2 pi t/n) host tables and t = (index*k) mod n.
+// Three launches per size class: (x,y) forward per z-layer with the layer in
+// LDS, z forward + G + z inverse per block of 32 spectral columns, (y,x)
+// inverse per z-layer.
+//
+// FP32 batched form: the Gaussian factor is separable,
+// G(kx,ky,kz) = f(kx) f(ky) f(kz), so IDFT . diag(G) . DFT on the box equals
+// three 1-D circular convolutions with the real kernel
+// c[m] = (1/n) sum_k f(min(k,n-k)) cos(2 pi k m / n) (host FP64, stored FP32).
+// Two launches per size class: x then y per z-layer in LDS, then z per block of
+// 64 columns.
+#include "gg_internal.h"
+
+namespace gg {
+namespace {
+
+constexpr int kT = 256;
+
+template <int NMAX>
+__global__ __launch_bounds__(kT) void k_dft_xy_fwd(PlanDev P, int cls, DescDev din, TablesDev tb,
+                                                   const int64_t* __restrict__ spec_off, double2* spec) {
+  constexpr int R = (NMAX * (NMAX / 2 + 1) + kT - 1) / kT;
+  __shared__ double L[NMAX * (NMAX + 2)];
+  __shared__ double tc[NMAX], ts[NMAX];
+  const int total = P.totals[kQLayers * kClasses + cls];
+  const double* arena = (const double*)din.arena;
+  const int tid = threadIdx.x;
+  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    int g, z;
+    plan_tile(P, cls, kQLayers, tile, g, z);
+    const int n = box_n(din.bounds, g), n2 = n / 2 + 1, nc = n * n2;
+    const int to = tb.tw_off[n];
+    for (int i = tid; i < n; i += kT) { tc[i] = tb.twc[to + i]; ts[i] = tb.tws[to + i]; }
+    const double* src = arena + din.off[g] + (int64_t)z * n * n;
+    for (int i = tid; i < n * n; i += kT) L[i] = src[i];
+    __syncthreads();
+    double re[R], im[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int o = tid + r * kT;
+      double a = 0.0, b = 0.0;
+      if (o < nc) {
+        const int y = o / n2, kx = o - y * n2;
+        const double* v = L + y * n;
+        int t = 0;
+        for (int x = 0; x < n; ++x) {
+          a = a + v[x] * tc[t];
+          b = b - v[x] * ts[t];
+          t += kx;
+          if (t >= n) t -= n;
+        }
+      }
+      re[r] = a;
+      im[r] = b;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int o = tid + r * kT;
+      if (o < nc) { L[2 * o] = re[r]; L[2 * o + 1] = im[r]; }
+    }
+    __syncthreads();
+    double2* dst = spec + spec_off[g] + (int64_t)z * nc;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int o = tid + r * kT;
+      if (o < nc) {
+        const int ky = o / n2, kx = o - ky * n2;
+        double a = 0.0, b = 0.0;
+        int t = 0;
+        for (int y = 0; y < n; ++y) {
+          const double ar = L[2 * (y * n2 + kx)], ai = L[2 * (y * n2 + kx) + 1];
+          a = a + (ar * tc[t] + ai * ts[t]);
+          b = b + (ai * tc[t] - ar * ts[t]);
+          t += ky;
+          if (t >= n) t -= n;
+        }
+        dst[o] = make_double2(a, b);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int NMAX>
+__global__ __launch_bounds__(kT) void k_dft_z(PlanDev P, int cls, DescDev dref, TablesDev tb,
+                                              const int64_t* __restrict__ spec_off, double2* spec) {
+  constexpr int CB = kSpecBlock;
+  constexpr int R = (NMAX * CB + kT - 1) / kT;
+  __shared__ double2 B[NMAX * CB];
+  __shared__ double tc[NMAX], ts[NMAX];
+  const int total = P.totals[kQSpecCols * kClasses + cls];
+  const int tid = threadIdx.x;
+  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    int g, blk;
+    plan_tile(P, cls, kQSpecCols, tile, g, blk);
+    const int n = box_n(dref.bounds, g), n2 = n / 2 + 1, nc = n * n2;
+    const int c0 = blk * CB;
+    const int to = tb.tw_off[n];
+    const double* G = tb.gtab + tb.g_off[n];
+    for (int i = tid; i < n; i += kT) { tc[i] = tb.twc[to + i]; ts[i] = tb.tws[to + i]; }
+    double2* base = spec + spec_off[g];
+    for (int e = tid; e < n * CB; e += kT) {
+      const int z = e / CB, c = c0 + (e - z * CB);
+      B[e] = c < nc ? base[(int64_t)z * nc + c] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    double fr[R], fi[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int o = tid + r * kT;
+      const int kz = o / CB, cb = o - kz * CB, c = c0 + cb;
+      double a = 0.0, b = 0.0;
+      if (kz < n && c < nc) {
+        int t = 0;
+        for (int z = 0; z < n; ++z) {
+          const double2 v = B[z * CB + cb];
+          a = a + (v.x * tc[t] + v.y * ts[t]);
+          b = b + (v.y * tc[t] - v.x * ts[t]);
+          t += kz;
+          if (t >= n) t -= n;
+        }
+        const int ky = c / n2, kx = c - ky * n2;
+        const int kz2 = min(kz, n - kz), ky2 = min(ky, n - ky), kx2 = min(kx, n - kx);
+        const double gf = G[kz2 * kz2 + ky2 * ky2 + kx2 * kx2];
+        a = a * gf;
+        b = b * gf;
+      }
+      fr[r] = a;
+      fi[r] = b;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int o = tid + r * kT;
+      if (o < n * CB) B[o] = make_double2(fr[r], fi[r]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int o = tid + r * kT;
+      const int z = o / CB, cb = o - z * CB, c = c0 + cb;
+      if (z < n && c < nc) {
+        double a = 0.0, b = 0.0;
+        int t = 0;
+        for (int kz = 0; kz < n; ++kz) {
+          const double2 v = B[kz * CB + cb];
+          a = a + (v.x * tc[t] - v.y * ts[t]);
+          b = b + (v.x * ts[t] + v.y * tc[t]);
+          t += z;
+          if (t >= n) t -= n;
+        }
+        base[(int64_t)z * nc + c] = make_double2(a, b);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int NMAX>
+__global__ __launch_bounds__(kT) void k_dft_xy_inv(PlanDev P, int cls, DescDev dout, TablesDev tb,
+                                                   const int64_t* __restrict__ spec_off, const double2* spec) {
+  constexpr int R = (NMAX * (NMAX / 2 + 1) + kT - 1) / kT;
+  constexpr int R3 = (NMAX * NMAX + kT - 1) / kT;
+  __shared__ double L[NMAX * (NMAX + 2)];
+  __shared__ double tc[NMAX], ts[NMAX];
+  const int total = P.totals[kQLayers * kClasses + cls];
+  double* arena = (double*)dout.arena;
+  const int tid = threadIdx.x;
+  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    int g, z;
+    plan_tile(P, cls, kQLayers, tile, g, z);
+    const int n = box_n(dout.bounds, g), n2 = n / 2 + 1, nc = n * n2;
+    const int to = tb.tw_off[n];
+    for (int i = tid; i < n; i += kT) { tc[i] = tb.twc[to + i]; ts[i] = tb.tws[to + i]; }
+    const double2* src = spec + spec_off[g] + (int64_t)z * nc;
+    for (int i = tid; i < nc; i += kT) {
+      const double2 v = src[i];
+      L[2 * i] = v.x;
+      L[2 * i + 1] = v.y;
+    }
+    __syncthreads();
+    double re[R], im[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int o = tid + r * kT;
+      double a = 0.0, b = 0.0;
+      if (o < nc) {
+        const int y = o / n2, kx = o - y * n2;
+        int t = 0;
+        for (int ky = 0; ky < n; ++ky) {
+          const double ar = L[2 * (ky * n2 + kx)], ai = L[2 * (ky * n2 + kx) + 1];
+          a = a + (ar * tc[t] - ai * ts[t]);
+          b = b + (ar * ts[t] + ai * tc[t]);
+          t += y;
+          if (t >= n) t -= n;
+        }
+      }
+      re[r] = a;
+      im[r] = b;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int o = tid + r * kT;
+      if (o < nc) { L[2 * o] = re[r]; L[2 * o + 1] = im[r]; }
+    }
+    __syncthreads();
+    double* dst = arena + dout.off[g] + (int64_t)z * n * n;
+    const int half = (n - 1) / 2;
+#pragma unroll
+    for (int r = 0; r < R3; ++r) {
+      const int o = tid + r * kT;
+      if (o < n * n) {
+        const int y = o / n, x = o - y * n;
+        const double* row = L + 2 * (y * n2);
+        double acc = row[0];
+        int t = x;
+        for (int kx = 1; kx <= half; ++kx) {
+          acc = acc + 2.0 * (row[2 * kx] * tc[t] - row[2 * kx + 1] * ts[t]);
+          t += x;
+          if (t >= n) t -= n;
+        }
+        if ((n & 1) == 0) {
+          const double a = row[2 * (n / 2)];
+          acc = acc + ((x & 1) ? -a : a);
+        }
+        dst[o] = acc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---- FP32 separable circulant form ----------------------------------------
+template <int NMAX>
+__global__ __launch_bounds__(kT) void k_circ_xy(PlanDev P, int cls, DescDev din, DescDev dout, TablesDev tb) {
+  __shared__ float A[NMAX * NMAX];
+  __shared__ float B[NMAX * NMAX];
+  __shared__ float cr[NMAX];
+  const int total = P.totals[kQLayers * kClasses + cls];
+  const float* ain = (const float*)din.arena;
+  float* aout = (float*)dout.arena;
+  const int tid = threadIdx.x;
+  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    int g, z;
+    plan_tile(P, cls, kQLayers, tile, g, z);
+    const int n = box_n(din.bounds, g), nn = n * n;
+    for (int i = tid; i < n; i += kT) cr[i] = tb.circ[tb.tw_off[n] + i];
+    const float* src = ain + din.off[g] + (int64_t)z * nn;
+    for (int i = tid; i < nn; i += kT) A[i] = src[i];
+    __syncthreads();
+    for (int o = tid; o < nn; o += kT) {
+      const int y = o / n, x = o - y * n;
+      const float* row = A + y * n;
+      float acc = 0.f;
+      int idx = x;
+      for (int m = 0; m < n; ++m) {
+        acc = __builtin_fmaf(cr[idx], row[m], acc);
+        idx = idx == 0 ? n - 1 : idx - 1;
+      }
+      B[o] = acc;
+    }
+    __syncthreads();
+    float* dst = aout + dout.off[g] + (int64_t)z * nn;
+    for (int o = tid; o < nn; o += kT) {
+      const int y = o / n, x = o - y * n;
+      float acc = 0.f;
+      int idx = y;
+      for (int m = 0; m < n; ++m) {
+        acc = __builtin_fmaf(cr[idx], B[m * n + x], acc);
+        idx = idx == 0 ? n - 1 : idx - 1;
+      }
+      dst[o] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+template <int NMAX>
+__global__ __launch_bounds__(kT) void k_circ_z(PlanDev P, int cls, DescDev dout, TablesDev tb) {
+  constexpr int CB = kColBlock;
+  __shared__ float Lz[NMAX * CB];
+  __shared__ float cr[NMAX];
+  const int total = P.totals[kQCols * kClasses + cls];
+  float* aout = (float*)dout.arena;
+  const int tid = threadIdx.x;
+  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    int g, blk;
+    plan_tile(P, cls, kQCols, tile, g, blk);
+    const int n = box_n(dout.bounds, g), nn = n * n;
+    const int c0 = blk * CB;
+    for (int i = tid; i < n; i += kT) cr[i] = tb.circ[tb.tw_off[n] + i];
+    float* base = aout + dout.off[g];
+    for (int e = tid; e < n * CB; e += kT) {
+      const int z = e / CB, c = c0 + (e - z * CB);
+      Lz[e] = c < nn ? base[(int64_t)z * nn + c] : 0.f;
+    }
+    __syncthreads();
+    for (int o = tid; o < n * CB; o += kT) {
+      const int z = o / CB, cb = o - z * CB, c = c0 + cb;
+      if (c >= nn) continue;
+      float acc = 0.f;
+      int idx = z;
+      for (int m = 0; m < n; ++m) {
+        acc = __builtin_fmaf(cr[idx], Lz[m * CB + cb], acc);
+        idx = idx == 0 ? n - 1 : idx - 1;
+      }
+      base[(int64_t)z * nn + c] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+template <int NMAX>
+void conv_class(const Launch& L, const PlanDev& P, int cls, const DescDev& din, const DescDev& dout,
+                const TablesDev& tb, const int64_t* spec_off, void* spec, bool fp32) {
+  if (fp32) {
+    hipLaunchKernelGGL(k_circ_xy<NMAX>, dim3(L.grid), dim3(kT), 0, L.stream, P, cls, din, dout, tb);
+    hipLaunchKernelGGL(k_circ_z<NMAX>, dim3(L.grid), dim3(kT), 0, L.stream, P, cls, dout, tb);
+  } else {
+    hipLaunchKernelGGL(k_dft_xy_fwd<NMAX>, dim3(L.grid), dim3(kT), 0, L.stream, P, cls, din, tb, spec_off,
+                       (double2*)spec);
+    hipLaunchKernelGGL(k_dft_z<NMAX>, dim3(L.grid), dim3(kT), 0, L.stream, P, cls, dout, tb, spec_off,
+                       (double2*)spec);
+    hipLaunchKernelGGL(k_dft_xy_inv<NMAX>, dim3(L.grid), dim3(kT), 0, L.stream, P, cls, dout, tb, spec_off,
+                       (const double2*)spec);
+  }
+}
+
+}  // namespace
+
+void launch_conv(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, const DescDev& dout,
+                 const TablesDev& tb, const int64_t* spec_off, void* spec, bool fp32) {
+  (void)sp;
+  conv_class<32>(L, P, 0, din, dout, tb, spec_off, spec, fp32);
+  conv_class<48>(L, P, 1, din, dout, tb, spec_off, spec, fp32);
+  conv_class<64>(L, P, 2, din, dout, tb, spec_off, spec, fp32);
+  conv_class<96>(L, P, 3, din, dout, tb, spec_off, spec, fp32);
+}
+
+}  // namespace gg

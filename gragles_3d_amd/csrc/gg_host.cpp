@@ -1,0 +1,240 @@
+// Host-side set-up for the level-set library (include/gg_grainhdl.h): grid and
+// time variables, the seeded Voronoi microstructure and the initial neighbour
+// lists.  Plain C++ with std::thread workers; no GPU calls.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <atomic>
+#include <thread>
+#include <vector>
+
+#include "../../include/gg_grainhdl.h"
+
+namespace {
+
+const double kPi = 3.14159265358979323846;
+
+struct SplitMix64 {
+  uint64_t s;
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+void cube_bounds(int32_t* b, int L) {  // resizeToCube, src/dimensionalBuffer.h:139-219
+  int grid_size = L - 1;
+  int w = b[3] - b[0], hgt = b[4] - b[1], d = b[5] - b[2];
+  if (w == hgt && w == d) return;
+  int ref = w > hgt ? w : hgt;
+  if (ref < d) ref = d;
+  if (ref > w) { int diff = ref - w; b[0] -= diff / 2; b[3] += diff / 2 + diff % 2; }
+  if (ref > hgt) { int diff = ref - hgt; b[1] -= diff / 2; b[4] += diff / 2 + diff % 2; }
+  if (ref > d) { int diff = ref - d; b[2] -= diff / 2; b[5] += diff / 2 + diff % 2; }
+  for (int a = 0; a < 3; ++a) {
+    if (b[a] < 0) { int delta = -b[a]; b[a + 3] += delta; b[a] += delta; }
+    else if (b[a + 3] > grid_size) { int delta = grid_size - b[a + 3]; b[a + 3] += delta; b[a] += delta; }
+  }
+}
+
+// Run body(i) for i in [0, n) on nthreads host threads, items handed out one
+// at a time.
+template <class F>
+void parallel_for(int n, int nthreads, F body) {
+  if (nthreads < 1) {
+    nthreads = (int)std::thread::hardware_concurrency();
+    if (nthreads < 1) nthreads = 1;
+  }
+  nthreads = std::min(nthreads, std::max(1, n));
+  std::atomic<int> next{0};
+  auto worker = [&](int t) {
+    for (int i = next.fetch_add(1); i < n; i = next.fetch_add(1)) body(i, t);
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nthreads; ++t) pool.emplace_back(worker, t);
+  worker(0);
+  for (auto& th : pool) th.join();
+}
+
+int resolve_threads(int nthreads) {
+  if (nthreads > 0) return nthreads;
+  int n = (int)std::thread::hardware_concurrency();
+  return n > 0 ? n : 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gg_derive_params(int32_t n_grains, int32_t points_per_grain, int32_t border, double time_step_size,
+                     gg_params* out) {
+  if (!out || n_grains < 1 || points_per_grain < 1 || border < 0 || !(time_step_size > 0)) return GG_ERR_ARG;
+  std::memset(out, 0, sizeof(*out));
+  int rds = (int)(pow((double)n_grains, 1. / 3.0) * pow(kPi * 4 / 3, 1. / 3) / 2 * points_per_grain + 1);
+  out->n_grains = n_grains;
+  out->grid_blowup = border;
+  out->ngridpoints = rds + 2 * border;
+  out->h = 1.0 / rds;
+  out->delta = border / double(rds);
+  out->dt = time_step_size / double(rds * rds);
+  out->conv_domain_size = rds - 1;
+  out->boundary_grain_tube = 0;
+  out->precision = GG_FP64_PLAIN;
+  return GG_OK;
+}
+
+void gg_resize_to_cube(int32_t* bounds6, int32_t ngridpoints) { cube_bounds(bounds6, ngridpoints); }
+
+int gg_generate_voronoi(const gg_params* p, uint64_t seed, int32_t* idfield, int32_t* bounds, uint8_t* exists,
+                        int32_t nthreads) {
+  if (!p || !bounds || !exists) return GG_ERR_ARG;
+  const int N = p->n_grains, L = p->ngridpoints, gb = p->grid_blowup;
+  const double h = p->h;
+  std::vector<double> sx(N), sy(N), sz(N);
+  SplitMix64 rng{seed};
+  for (int i = 0; i < N; ++i) {
+    sx[i] = rng.uniform();
+    sy[i] = rng.uniform();
+    sz[i] = rng.uniform();
+  }
+  // bucket grid over [0,1)^3
+  int B = std::max(1, (int)std::cbrt((double)N));
+  const double cs = 1.0 / B;
+  std::vector<int> cstart((size_t)B * B * B + 1, 0), cell(N);
+  auto bidx = [&](double v) { int q = (int)(v * B); return q < 0 ? 0 : (q >= B ? B - 1 : q); };
+  for (int i = 0; i < N; ++i) {
+    cell[i] = (bidx(sz[i]) * B + bidx(sy[i])) * B + bidx(sx[i]);
+    cstart[cell[i] + 1]++;
+  }
+  for (size_t c = 0; c < (size_t)B * B * B; ++c) cstart[c + 1] += cstart[c];
+  std::vector<int> items(N), fillp(cstart.begin(), cstart.end() - 1);
+  for (int i = 0; i < N; ++i) items[fillp[cell[i]]++] = i;  // ascending id inside a bucket
+
+  const int T = resolve_threads(nthreads);
+  std::vector<int32_t> lo((size_t)T * (N + 1) * 3, INT32_MAX), hi((size_t)T * (N + 1) * 3, INT32_MIN);
+  parallel_for(L, T, [&](int k, int t) {
+    int32_t* mylo = &lo[(size_t)t * (N + 1) * 3];
+    int32_t* myhi = &hi[(size_t)t * (N + 1) * 3];
+    {
+      for (int i = 0; i < L; ++i) {
+        for (int j = 0; j < L; ++j) {
+          int32_t id = 0;
+          if (!(i < gb || j < gb || k < gb || i >= L - 1 - gb || j >= L - 1 - gb || k >= L - 1 - gb)) {
+            const double x = double((j - gb) * h), y = double((i - gb) * h), z = double((k - gb) * h);
+            const int bx = bidx(x), by = bidx(y), bz = bidx(z);
+            double best = 1e300;
+            int bid = -1;
+            for (int r = 0; r <= B; ++r) {
+              // buckets at Chebyshev ring r around (bx,by,bz)
+              for (int cz = bz - r; cz <= bz + r; ++cz) {
+                if (cz < 0 || cz >= B) continue;
+                for (int cy = by - r; cy <= by + r; ++cy) {
+                  if (cy < 0 || cy >= B) continue;
+                  for (int cx = bx - r; cx <= bx + r; ++cx) {
+                    if (cx < 0 || cx >= B) continue;
+                    if (std::max(std::abs(cz - bz), std::max(std::abs(cy - by), std::abs(cx - bx))) != r) continue;
+                    const int c = (cz * B + cy) * B + cx;
+                    for (int q = cstart[c]; q < cstart[c + 1]; ++q) {
+                      const int s = items[q];
+                      const double dx = x - sx[s], dy = y - sy[s], dz = z - sz[s];
+                      const double d2 = dx * dx + dy * dy + dz * dz;
+                      if (d2 < best || (d2 == best && s < bid)) {
+                        best = d2;
+                        bid = s;
+                      }
+                    }
+                  }
+                }
+              }
+              // every seed in ring r+1 or beyond lies at least r*cs away
+              const double rr = r * cs;
+              if (bid >= 0 && best < rr * rr) break;
+            }
+            id = bid + 1;
+            int32_t* l = mylo + 3 * id;
+            int32_t* u = myhi + 3 * id;
+            if (j < l[0]) l[0] = j;
+            if (j > u[0]) u[0] = j;
+            if (i < l[1]) l[1] = i;
+            if (i > u[1]) u[1] = i;
+            if (k < l[2]) l[2] = k;
+            if (k > u[2]) u[2] = k;
+          }
+          if (idfield) idfield[((size_t)k * L + i) * L + j] = id;
+        }
+      }
+    }
+  });
+  std::memset(bounds, 0, sizeof(int32_t) * 6 * (N + 1));
+  exists[0] = 0;
+  for (int g = 1; g <= N; ++g) {
+    int32_t l[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, u[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    for (int t = 0; t < T; ++t)
+      for (int a = 0; a < 3; ++a) {
+        l[a] = std::min(l[a], lo[((size_t)t * (N + 1) + g) * 3 + a]);
+        u[a] = std::max(u[a], hi[((size_t)t * (N + 1) + g) * 3 + a]);
+      }
+    if (l[0] == INT32_MAX) {
+      exists[g] = 0;
+      continue;
+    }
+    exists[g] = 1;
+    for (int a = 0; a < 3; ++a) {
+      bounds[6 * g + a] = std::max(0, l[a] - gb);
+      bounds[6 * g + 3 + a] = std::min(L, u[a] + 1 + gb);
+    }
+  }
+  return GG_OK;
+}
+
+int gg_initial_neighbour_lists(const gg_params* p, const int32_t* bounds, const uint8_t* exists, int32_t* offsets,
+                               int32_t* ids, int64_t cap, int64_t* n) {
+  if (!p || !bounds || !exists || !offsets || !n) return GG_ERR_ARG;
+  const int N = p->n_grains, L = p->ngridpoints;
+  std::vector<int32_t> cb(6 * (N + 1), 0);
+  std::vector<int> order;
+  int maxside = 1;
+  for (int g = 1; g <= N; ++g) {
+    if (!exists[g]) continue;
+    for (int a = 0; a < 6; ++a) cb[6 * g + a] = bounds[6 * g + a];
+    cube_bounds(&cb[6 * g], L);
+    maxside = std::max(maxside, cb[6 * g + 3] - cb[6 * g]);
+    order.push_back(g);
+  }
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return cb[6 * a] < cb[6 * b] || (cb[6 * a] == cb[6 * b] && a < b); });
+  std::vector<std::vector<int32_t>> nb(N + 1);
+  parallel_for((int)order.size(), 0, [&](int oi, int) {
+    const int g = order[oi];
+    const int32_t* a = &cb[6 * g];
+    // candidates h with h.x0 in [a.x0 - maxside, a.x1]
+    auto first = std::lower_bound(order.begin(), order.end(), a[0] - maxside,
+                                  [&](int q, int v) { return cb[6 * q] < v; });
+    std::vector<int32_t>& out = nb[g];
+    for (auto it = first; it != order.end() && cb[6 * *it] <= a[3]; ++it) {
+      const int hgr = *it;
+      if (hgr == g) continue;
+      const int32_t* b = &cb[6 * hgr];
+      // inclusive overlap, as the R-tree search and checkIntersection test it
+      if (a[0] > b[3] || a[3] < b[0] || a[1] > b[4] || a[4] < b[1] || a[2] > b[5] || a[5] < b[2]) continue;
+      out.push_back(hgr);
+    }
+    std::sort(out.begin(), out.end());
+  });
+  int64_t total = 0;
+  offsets[0] = 0;
+  offsets[1] = 0;
+  for (int g = 1; g <= N; ++g) {
+    offsets[g + 1] = (int32_t)(offsets[g] + nb[g].size());
+    total += (int64_t)nb[g].size();
+  }
+  *n = total;
+  if (total > cap || !ids) return total > cap ? GG_ERR_CAPACITY : GG_OK;
+  for (int g = 1; g <= N; ++g) std::copy(nb[g].begin(), nb[g].end(), ids + offsets[g]);
+  return GG_OK;
+}
+
+}  // extern "C"

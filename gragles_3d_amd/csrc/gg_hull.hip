@@ -1,0 +1,262 @@
+// Level-set stage: grain existence, volume and surface area from the marching-
+// cubes hull of the compared distance field, deletion of vanished grains and
+// the box resize (grainhdl::level_set, src/grainhdl.cpp:669-684 ->
+// LSbox::extractContour, src/box.cpp:1144-1161).
+//
+// k_hull walks the cells of one z-layer of a grain's input box exactly as
+// MarchingCubesAlgorithm::generateHull (src/marchingCubes.cpp:364-478): case
+// index from the eight corners (inside = phi >= 0), VertexInterp
+// (:484-499) on the cut edges, triangles from the case table, then the
+// signed-tetrahedron volume term (src/grainHull.cpp:40-56, including its
+// float(1/6) factor) and the triangle area (:120-131).  A grain exists iff
+// some cell is cut.  Each tile keeps Neumaier-compensated FP64 sums and writes
+// them to its own slot; k_finalize adds a grain's slots in layer order, so the
+// result does not depend on scheduling.
+#include "gg_internal.h"
+
+namespace gg {
+namespace {
+#include "mc_cases.inc"
+
+__constant__ int8_t c_tri[256][16];
+__constant__ uint16_t c_emask[256];
+
+// cube corner q -> (dx,dy,dz) (src/marchingCubes.cpp:378-386)
+__constant__ int8_t c_corner[8][3] = {{0, 0, 1}, {1, 0, 1}, {1, 0, 0}, {0, 0, 0},
+                                      {0, 1, 1}, {1, 1, 1}, {1, 1, 0}, {0, 1, 0}};
+// edge e -> corner pair (src/marchingCubes.cpp:393-464)
+__constant__ int8_t c_edge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                     {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+struct Acc {
+  double s, c;
+  __device__ void add(double x) {
+    double t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+  }
+  __device__ void merge(double s2, double c2) {
+    double t = s + s2;
+    double bp = t - s;
+    double e = (s - (t - bp)) + (s2 - bp);
+    s = t;
+    c = c + c2 + e;
+  }
+};
+
+constexpr int kHT = 256;
+
+template <class T>
+__global__ __launch_bounds__(kHT) void k_hull(SimParams sp, PlanDev P, DescDev din, int32_t* exflag,
+                                              double* partials) {
+  __shared__ double s_red[4][kHT / 64];
+  __shared__ int s_any[kHT / 64];
+  const int total = plan_total(P, kQLayers);
+  const T* arena = (const T*)din.arena;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    int c, g, kl;
+    plan_tile_any(P, kQLayers, tile, c, g, kl);
+    // slot of this tile: class base + tile index inside the class
+    int tin = tile;
+    for (int cc = 0; cc < c; ++cc) tin -= P.totals[kQLayers * kClasses + cc];
+    const int64_t slot = P.layer_base[c] + tin;
+    const int32_t* b = din.bounds + 6 * g;
+    const int x0 = b[0], y0 = b[1], z0 = b[2];
+    const int n = b[3] - b[0];
+    const T* box = arena + din.off[g];
+    Acc vol{0.0, 0.0}, area{0.0, 0.0};
+    int any = 0;
+    if (kl < n - 1) {
+      const int z = z0 + kl;
+      const int m1 = n - 1;
+      for (int v = tid; v < m1 * m1; v += kHT) {
+        const int yl = v / m1, xl = v - yl * m1;
+        double val[8];
+        unsigned ci = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          val[q] = (double)box[lin3(n, kl + c_corner[q][2], yl + c_corner[q][1], xl + c_corner[q][0])];
+          if (val[q] >= 0) ci |= 1u << q;
+        }
+        const int mask = c_emask[ci];
+        if (mask == 0) continue;
+        any = 1;
+        const int x = x0 + xl, y = y0 + yl;
+        double vx[12], vy[12], vz[12];
+#pragma unroll
+        for (int e = 0; e < 12; ++e) {
+          if (!(mask & (1 << e))) continue;
+          const int a = c_edge[e][0], bq = c_edge[e][1];
+          const double p1x = x + c_corner[a][0], p1y = y + c_corner[a][1], p1z = z + c_corner[a][2];
+          const double p2x = x + c_corner[bq][0], p2y = y + c_corner[bq][1], p2z = z + c_corner[bq][2];
+          const double v1 = val[a], v2 = val[bq];
+          if (fabs(v1) < 0.00001) {
+            vx[e] = p1x; vy[e] = p1y; vz[e] = p1z;
+          } else if (fabs(v2) < 0.00001) {
+            vx[e] = p2x; vy[e] = p2y; vz[e] = p2z;
+          } else if (fabs(v1 - v2) < 0.00001) {
+            vx[e] = p1x; vy[e] = p1y; vz[e] = p1z;
+          } else {
+            const double mu = v1 / (v1 - v2);
+            vx[e] = p1x + mu * (p2x - p1x);
+            vy[e] = p1y + mu * (p2y - p1y);
+            vz[e] = p1z + mu * (p2z - p1z);
+          }
+        }
+        for (int t = 0; t < 16 && c_tri[ci][t] >= 0; t += 3) {
+          const int e0 = c_tri[ci][t], e1 = c_tri[ci][t + 1], e2 = c_tri[ci][t + 2];
+          const double p0x = vx[e0], p0y = vy[e0], p0z = vz[e0];
+          const double p1x = vx[e1], p1y = vy[e1], p1z = vz[e1];
+          const double p2x = vx[e2], p2y = vy[e2], p2z = vz[e2];
+          const double v321 = p2x * p1y * p0z;
+          const double v231 = p1x * p2y * p0z;
+          const double v312 = p2x * p0y * p1z;
+          const double v132 = p0x * p2y * p1z;
+          const double v213 = p1x * p0y * p2z;
+          const double v123 = p0x * p1y * p2z;
+          vol.add((double)(1.0f / 6.0f) * (-v321 + v231 + v312 - v132 - v213 + v123));
+          const double abx = p0x - p1x, aby = p0y - p1y, abz = p0z - p1z;
+          const double acx = p0x - p2x, acy = p0y - p2y, acz = p0z - p2z;
+          const double cx = aby * acz - abz * acy;
+          const double cy = abz * acx - abx * acz;
+          const double cz = abx * acy - aby * acx;
+          area.add(sqrt(cx * cx + cy * cy + cz * cz) / 2.0);
+        }
+      }
+    }
+    // wavefront then workgroup reduction in a fixed order
+    double r[4] = {vol.s, vol.c, area.s, area.c};
+    for (int off = 32; off > 0; off >>= 1) {
+      double s2 = __shfl_down(r[0], off), c2 = __shfl_down(r[1], off);
+      double a2 = __shfl_down(r[2], off), d2 = __shfl_down(r[3], off);
+      Acc va{r[0], r[1]}, aa{r[2], r[3]};
+      va.merge(s2, c2);
+      aa.merge(a2, d2);
+      r[0] = va.s; r[1] = va.c; r[2] = aa.s; r[3] = aa.c;
+    }
+    const uint64_t anym = __ballot(any);
+    if (lane == 0) {
+      for (int q = 0; q < 4; ++q) s_red[q][wv] = r[q];
+      s_any[wv] = anym != 0ull;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      Acc va{s_red[0][0], s_red[1][0]}, aa{s_red[2][0], s_red[3][0]};
+      int an = s_any[0];
+      for (int w = 1; w < kHT / 64; ++w) {
+        va.merge(s_red[0][w], s_red[1][w]);
+        aa.merge(s_red[2][w], s_red[3][w]);
+        an |= s_any[w];
+      }
+      double* pp = partials + 4 * slot;
+      pp[0] = va.s; pp[1] = va.c; pp[2] = aa.s; pp[3] = aa.c;
+      if (an) atomicOr(&exflag[g], 1);
+    }
+    __syncthreads();
+  }
+}
+
+__device__ void cube_bounds(int* b, int maximumLength) {  // resizeToCube, src/dimensionalBuffer.h:139-219
+  const int grid_size = maximumLength - 1;
+  const int w = b[3] - b[0], hgt = b[4] - b[1], d = b[5] - b[2];
+  if (w == hgt && w == d) return;
+  int ref = w > hgt ? w : hgt;
+  if (ref < d) ref = d;
+  if (ref > w) { int diff = ref - w; b[0] -= diff / 2; b[3] += diff / 2 + diff % 2; }
+  if (ref > hgt) { int diff = ref - hgt; b[1] -= diff / 2; b[4] += diff / 2 + diff % 2; }
+  if (ref > d) { int diff = ref - d; b[2] -= diff / 2; b[5] += diff / 2 + diff % 2; }
+  for (int a = 0; a < 3; ++a) {
+    if (b[a] < 0) { int delta = -b[a]; b[a + 3] += delta; b[a] += delta; }
+    else if (b[a + 3] > grid_size) { int delta = grid_size - b[a + 3]; b[a + 3] += delta; b[a] += delta; }
+  }
+}
+
+// Per grain: vanished -> deleted; alive -> take the comparison list as the new
+// neighbour list, decide existence, finish volume/area, resize the output box
+// to the set-comparison bounds (+-grid_blowup, clamped, src/box.cpp:857-877)
+// made cubic.
+__global__ void k_finalize(SimParams sp, PlanDev P, DescDev din, int32_t* state, ListDev ld, int32_t* exflag,
+                           const double* __restrict__ partials, const int32_t* __restrict__ newb, DescDev dout,
+                           double* volume, double* area) {
+  for (int g = 1 + blockIdx.x * blockDim.x + threadIdx.x; g <= sp.N; g += gridDim.x * blockDim.x) {
+    const int s = state[g];
+    if (s == kVanished) {
+      state[g] = kDeleted;
+      continue;
+    }
+    if (s != kAlive) continue;
+    ld.sel[g] ^= 1;
+    const int ex = exflag[g];
+    exflag[g] = 0;
+    if (!ex) {
+      state[g] = kVanished;
+      continue;
+    }
+    const int pos = P.grain_pos[g];
+    int c = 0;
+    while (c + 1 < kClasses && pos >= P.cls_start[c + 1]) ++c;
+    const int idx = pos - P.cls_start[c];
+    const int first = P.pre[kQLayers][P.cls_start[c] + c + idx];
+    const int n = box_n(din.bounds, g);
+    const double* pp = partials + 4 * (P.layer_base[c] + first);
+    Acc va{pp[0], pp[1]}, aa{pp[2], pp[3]};
+    for (int l = 1; l < n; ++l) {
+      va.merge(pp[4 * l], pp[4 * l + 1]);
+      aa.merge(pp[4 * l + 2], pp[4 * l + 3]);
+    }
+    const double h = sp.h;
+    volume[g] = fabs(va.s + va.c) * h * h * h;
+    area[g] = (aa.s + aa.c) * h * h;
+    const int32_t* nb = newb + 6 * g;
+    int bb[6] = {nb[0] - sp.gb, nb[1] - sp.gb, nb[2] - sp.gb, nb[3] + sp.gb, nb[4] + sp.gb, nb[5] + sp.gb};
+    for (int a = 0; a < 3; ++a) {
+      if (bb[a] < 0) bb[a] = 0;
+      if (bb[a + 3] >= sp.ngp) bb[a + 3] = sp.ngp - 1;
+    }
+    cube_bounds(bb, sp.ngp);
+    for (int a = 0; a < 6; ++a) dout.bounds[6 * g + a] = bb[a];
+  }
+}
+
+}  // namespace
+
+void upload_mc_tables() {
+  int8_t tri[256][16];
+  uint16_t em[256];
+  static const int edge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                  {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+  for (int c = 0; c < 256; ++c) {
+    int m = 0;
+    for (int e = 0; e < 12; ++e)
+      if (((c >> edge[e][0]) & 1) != ((c >> edge[e][1]) & 1)) m |= 1 << e;
+    em[c] = (uint16_t)m;
+    for (int i = 0; i < 16; ++i) tri[c][i] = -1;
+    const char* s = kTriCases[c];
+    for (int i = 0; s[i]; ++i) {
+      char ch = s[i];
+      tri[c][i] = (int8_t)((ch >= '0' && ch <= '9') ? ch - '0' : ch - 'a' + 10);
+    }
+  }
+  (void)hipMemcpyToSymbol(HIP_SYMBOL(c_tri), tri, sizeof(tri));
+  (void)hipMemcpyToSymbol(HIP_SYMBOL(c_emask), em, sizeof(em));
+}
+
+void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, int32_t* exflag,
+                 double* partials, bool fp32) {
+  if (fp32)
+    hipLaunchKernelGGL(k_hull<float>, dim3(L.grid), dim3(kHT), 0, L.stream, sp, P, din, exflag, partials);
+  else
+    hipLaunchKernelGGL(k_hull<double>, dim3(L.grid), dim3(kHT), 0, L.stream, sp, P, din, exflag, partials);
+}
+
+void launch_finalize(const Launch& L, const SimParams& sp, const PlanDev& P, int32_t* state, const ListDev& ld,
+                     int32_t* exflag, const double* partials, const int32_t* newb, const DescDev& din,
+                     const DescDev& dout, double* volume, double* area) {
+  int blocks = (sp.N + 255) / 256;
+  hipLaunchKernelGGL(k_finalize, dim3(blocks), dim3(256), 0, L.stream, sp, P, din, state, ld, exflag, partials,
+                     newb, dout, volume, area);
+}
+
+}  // namespace gg

@@ -1,0 +1,175 @@
+// Internal declarations shared by the HIP translation units of libgg_levelset.
+//
+// Device data model (one context = every grain box of one simulation):
+//  * Two distance-buffer roles, "in" and "out" (the reference's
+//    m_inputDistance / m_outputDistance of every LSbox).  Each role is a
+//    DescDev: one arena holding all grains' boxes back to back plus per-grain
+//    bounds rows and arena offsets.  gg_switch_buffers swaps the roles on the
+//    host (no data moves).
+//  * Two IDLocal arenas with their own per-grain layout: the comparison reads
+//    the previous step's IDLocal (for voxels no neighbour covers) and writes
+//    the current one.
+//  * Two comparison-list buffers; per grain a selector says which one holds
+//    m_secondOrderNeighbours, the other receives m_comparisonList.
+//  * A work plan: alive grains grouped by box-size class (side <= 32, 48, 64,
+//    96) with per-class tile prefixes, so each stage runs one launch per class
+//    with LDS sized for that class, and every launch reads its tile count from
+//    device memory (no host synchronisation inside a timestep).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gg {
+
+constexpr int kClasses = 4;
+constexpr int kMaxN = 96;
+constexpr int kColBlock = 64;      // columns per z-sweep / z-convolution tile
+constexpr int kSpecBlock = 32;     // spectral columns per FP64 z-DFT tile
+
+__host__ __device__ inline int class_max(int c) { return c == 0 ? 32 : c == 1 ? 48 : c == 2 ? 64 : 96; }
+__host__ __device__ inline int size_class(int n) {
+  return n <= 32 ? 0 : n <= 48 ? 1 : n <= 64 ? 2 : n <= 96 ? 3 : -1;
+}
+
+enum { kDeleted = 0, kAlive = 1, kVanished = 2 };
+
+// device error bits
+enum {
+  kErrArena = 1,      // distance / IDLocal arena capacity exceeded
+  kErrList = 2,       // comparison list capacity exceeded
+  kErrBoxSize = 4,    // box side above kMaxN or box not cubic
+  kErrSpec = 8,       // spectral scratch capacity exceeded
+  kErrEmptyIO = 16,   // redistancing with an empty in/out intersection
+  kErrHistory = 32    // count history full
+};
+
+struct DescDev {
+  void* arena;       // double* (FP64) or float* (FP32)
+  int32_t* bounds;   // [N+1][6] {x0,y0,z0,x1,y1,z1}
+  int64_t* off;      // [N+1] offset of the box in the arena (elements)
+};
+
+struct IdlDev {
+  uint32_t* arena;
+  int32_t* bounds;
+  int64_t* off;
+};
+
+struct ListDev {
+  int32_t* buf[2];   // [(N+1)*max_list]
+  int32_t* len[2];   // [N+1]
+  int32_t* sel;      // [N+1] buffer index holding m_secondOrderNeighbours
+  int32_t max_list;
+};
+
+// Tile quantities carried per class by the plan.
+enum { kQLayers = 0, kQCols = 1, kQSpecCols = 2, kNumQ = 3 };
+
+struct PlanDev {
+  int32_t* cls_count;   // [kClasses]
+  int32_t* cls_start;   // [kClasses] first index into grains[]
+  int32_t* grains;      // [N] alive grain ids grouped by class, ascending id inside a class
+  int32_t* grain_pos;   // [N+1] index of a grain in grains[] (-1 if not alive)
+  int32_t* pre[kNumQ];  // [N + kClasses] per-class exclusive prefixes (segment c at cls_start[c]+c)
+  int32_t* totals;      // [kNumQ * kClasses]
+  int64_t* layer_base;  // [kClasses] first hull-partial slot of each class
+};
+
+struct SimParams {
+  int32_t N;
+  int32_t ngp;
+  int32_t gb;
+  int32_t tube;
+  double h;
+  double delta;
+};
+
+// Per-n tables (n = 1..kMaxN), host-computed in FP64.
+struct TablesDev {
+  const int32_t* tw_off;  // [kMaxN+1] offset into twc/tws and circ
+  const double* twc;      // cos(2 pi t / n)
+  const double* tws;      // sin(2 pi t / n)
+  const int32_t* g_off;   // [kMaxN+1] offset into gtab
+  const double* gtab;     // spectral factor G(s), s = kx2^2+ky2^2+kz2^2
+  const float* circ;      // FP32 circulant heat-kernel row (one axis) per n
+};
+
+// ---- helpers --------------------------------------------------------------
+__device__ inline int box_n(const int32_t* b, int g) { return b[6 * g + 3] - b[6 * g]; }
+
+// Locate tile t of class c for quantity q: grain id and local tile index.
+__device__ inline void plan_tile(const PlanDev& P, int c, int q, int t, int& g, int& local) {
+  const int start = P.cls_start[c];
+  const int cnt = P.cls_count[c];
+  const int32_t* pre = P.pre[q] + start + c;
+  int lo = 0, hi = cnt - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  g = P.grains[start + lo];
+  local = t - pre[lo];
+}
+
+// Flattened tile lookup over all classes for one quantity.
+__device__ inline bool plan_tile_any(const PlanDev& P, int q, int t, int& c, int& g, int& local) {
+  for (c = 0; c < kClasses; ++c) {
+    int tot = P.totals[q * kClasses + c];
+    if (t < tot) {
+      plan_tile(P, c, q, t, g, local);
+      return true;
+    }
+    t -= tot;
+  }
+  return false;
+}
+
+__device__ inline int plan_total(const PlanDev& P, int q) {
+  int s = 0;
+  for (int c = 0; c < kClasses; ++c) s += P.totals[q * kClasses + c];
+  return s;
+}
+
+template <class T>
+__device__ inline int sgnv(T v) { return (T(0) < v) - (v < T(0)); }
+
+__device__ inline int64_t lin3(int n, int z, int y, int x) { return ((int64_t)z * n + y) * n + x; }
+
+}  // namespace gg
+
+// ---- host-side launch wrappers (defined in the .hip files) -----------------
+struct gg_ctx;
+namespace gg {
+struct Launch {
+  hipStream_t stream;
+  int grid;   // workgroups per grid-stride launch
+};
+
+void launch_plan(const Launch& L, const SimParams& sp, const int32_t* state, const int32_t* bounds,
+                 int64_t* off_out, int64_t* spec_off, const PlanDev& P, int64_t arena_cap,
+                 int64_t spec_cap, int32_t* err, int64_t* voxels);
+void launch_init_phi(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din,
+                     const int32_t* idfield, bool fp32);
+void launch_conv(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din,
+                 const DescDev& dout, const TablesDev& tb, const int64_t* spec_off, void* spec, bool fp32);
+void launch_idl_resize(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& dout,
+                       const IdlDev& il);
+void launch_lists(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& din,
+                  const DescDev& dout, const ListDev& ld, int32_t* newb, int32_t* err);
+void launch_comparison(const Launch& L, const SimParams& sp, const PlanDev& P, const int32_t* state,
+                       const DescDev& din, const DescDev& dout, const ListDev& ld, const IdlDev& il_old,
+                       const IdlDev& il_new, int32_t* newb, bool fp32);
+void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, int32_t* exflag,
+                 double* partials, bool fp32);
+void launch_finalize(const Launch& L, const SimParams& sp, const PlanDev& P, int32_t* state,
+                     const ListDev& ld, int32_t* exflag, const double* partials, const int32_t* newb,
+                     const DescDev& din, const DescDev& dout, double* volume, double* area);
+void launch_redist(const Launch& L, const SimParams& sp, const PlanDev& P, const int32_t* state,
+                   const DescDev& din, const DescDev& dout, int32_t* err, bool fp32);
+void launch_post_redist(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& din,
+                        const DescDev& dout);
+void launch_count(const Launch& L, const SimParams& sp, const int32_t* state, int32_t* step,
+                  int32_t* history, int32_t history_cap, int32_t* err);
+void upload_mc_tables();
+}  // namespace gg

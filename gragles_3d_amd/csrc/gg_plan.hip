@@ -1,0 +1,221 @@
+// Work plan, arena layout and per-grain bookkeeping kernels.
+//
+// The plan groups the alive grains by box-size class and gives each class
+// exclusive prefixes of three tile counts (z-layers, 64-column blocks and
+// 32-column spectral blocks).  The same single-workgroup kernel lays out the
+// boxes of one distance role back to back in its arena (the device form of the
+// reference's per-LSbox DimensionalBuffer::resize, src/dimensionalBuffer.h:120-131)
+// and checks the arena capacities.
+#include "gg_internal.h"
+
+namespace gg {
+
+namespace {
+
+constexpr int kPlanThreads = 1024;
+
+__device__ int64_t block_scan_excl(int64_t v, int64_t* s, int64_t* total) {
+  const int t = threadIdx.x;
+  s[t] = v;
+  __syncthreads();
+  for (int d = 1; d < kPlanThreads; d <<= 1) {
+    int64_t add = t >= d ? s[t - d] : 0;
+    __syncthreads();
+    s[t] += add;
+    __syncthreads();
+  }
+  int64_t incl = s[t];
+  *total = s[kPlanThreads - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+__device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+__global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32_t* __restrict__ state,
+                                                       const int32_t* __restrict__ bounds, int64_t* off_out,
+                                                       int64_t* spec_off, PlanDev P, int64_t arena_cap,
+                                                       int64_t spec_cap, int32_t* err, int64_t* voxels) {
+  __shared__ int64_t s[kPlanThreads];
+  const int N = sp.N;
+  const int t = threadIdx.x;
+  const int chunk = (N + kPlanThreads - 1) / kPlanThreads;
+  const int gb = 1 + t * chunk;
+  const int ge = min(N + 1, gb + chunk);
+
+  // quantities per thread: [0..3] counts, [4..15] tile sums (q*4+c), [16] voxels, [17] spectrum
+  constexpr int NQ = 4 + kNumQ * kClasses + 2;
+  int64_t loc[NQ];
+  for (int q = 0; q < NQ; ++q) loc[q] = 0;
+  int bad = 0;
+  for (int g = gb; g < ge; ++g) {
+    P.grain_pos[g] = -1;
+    if (state[g] != kAlive) continue;
+    const int32_t* b = bounds + 6 * g;
+    int n = b[3] - b[0];
+    int c = size_class(n);
+    if (n < 1 || b[4] - b[1] != n || b[5] - b[2] != n || c < 0) { bad = 1; continue; }
+    loc[c] += 1;
+    loc[4 + kQLayers * kClasses + c] += n;
+    loc[4 + kQCols * kClasses + c] += ceil_div((int64_t)n * n, kColBlock);
+    loc[4 + kQSpecCols * kClasses + c] += ceil_div((int64_t)n * (n / 2 + 1), kSpecBlock);
+    loc[16] += (int64_t)n * n * n;
+    loc[17] += (int64_t)n * n * (n / 2 + 1);
+  }
+  if (bad) atomicOr(err, kErrBoxSize);
+  int64_t ex[NQ], tot[NQ];
+  for (int q = 0; q < NQ; ++q) ex[q] = block_scan_excl(loc[q], s, &tot[q]);
+
+  int64_t cls_start[kClasses];
+  {
+    int64_t acc = 0;
+    for (int c = 0; c < kClasses; ++c) { cls_start[c] = acc; acc += tot[c]; }
+  }
+  // A layout that does not fit its arena leaves an empty plan, so no kernel of
+  // this step touches memory beyond the arenas; the host reports the error bit.
+  const bool overflow = (off_out && tot[16] > arena_cap) || (spec_off && tot[17] > spec_cap);
+  if (t == 0) {
+    int64_t lb = 0;
+    for (int c = 0; c < kClasses; ++c) {
+      P.cls_count[c] = overflow ? 0 : (int32_t)tot[c];
+      P.cls_start[c] = (int32_t)cls_start[c];
+      for (int q = 0; q < kNumQ; ++q)
+        P.totals[q * kClasses + c] = overflow ? 0 : (int32_t)tot[4 + q * kClasses + c];
+      P.layer_base[c] = lb;
+      lb += tot[4 + kQLayers * kClasses + c];
+      for (int q = 0; q < kNumQ; ++q)
+        P.pre[q][cls_start[c] + c + tot[c]] = (int32_t)tot[4 + q * kClasses + c];
+    }
+    if (voxels) *voxels = tot[16];
+    if (off_out && tot[16] > arena_cap) atomicOr(err, kErrArena);
+    if (spec_off && tot[17] > spec_cap) atomicOr(err, kErrSpec);
+  }
+  for (int g = gb; g < ge; ++g) {
+    if (state[g] != kAlive) continue;
+    const int32_t* b = bounds + 6 * g;
+    int n = b[3] - b[0];
+    int c = size_class(n);
+    if (n < 1 || b[4] - b[1] != n || b[5] - b[2] != n || c < 0) continue;
+    int64_t pos = cls_start[c] + ex[c];
+    P.grains[pos] = g;
+    P.grain_pos[g] = (int32_t)pos;
+    int64_t seg = cls_start[c] + c + ex[c];
+    P.pre[kQLayers][seg] = (int32_t)ex[4 + kQLayers * kClasses + c];
+    P.pre[kQCols][seg] = (int32_t)ex[4 + kQCols * kClasses + c];
+    P.pre[kQSpecCols][seg] = (int32_t)ex[4 + kQSpecCols * kClasses + c];
+    ex[c] += 1;
+    ex[4 + kQLayers * kClasses + c] += n;
+    ex[4 + kQCols * kClasses + c] += ceil_div((int64_t)n * n, kColBlock);
+    ex[4 + kQSpecCols * kClasses + c] += ceil_div((int64_t)n * (n / 2 + 1), kSpecBlock);
+    if (off_out) off_out[g] = ex[16];
+    if (spec_off) spec_off[g] = ex[17];
+    ex[16] += (int64_t)n * n * n;
+    ex[17] += (int64_t)n * n * (n / 2 + 1);
+  }
+}
+
+// calculateDistanceFunction (src/box.cpp:374-397): +h inside the grain's
+// labelled voxels, -h elsewhere and on the border shell.
+template <class T>
+__global__ __launch_bounds__(256) void k_init_phi(SimParams sp, PlanDev P, DescDev din,
+                                                  const int32_t* __restrict__ idfield) {
+  const int total = plan_total(P, kQLayers);
+  T* arena = (T*)din.arena;
+  const int L = sp.ngp, lo = sp.gb, hi = sp.ngp - sp.gb - 1;
+  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    int c, g, kl;
+    plan_tile_any(P, kQLayers, tile, c, g, kl);
+    const int32_t* b = din.bounds + 6 * g;
+    const int n = b[3] - b[0];
+    const int k = b[2] + kl;
+    T* dst = arena + din.off[g] + (int64_t)kl * n * n;
+    for (int v = threadIdx.x; v < n * n; v += blockDim.x) {
+      int i = b[1] + v / n, j = b[0] + v % n;
+      T val;
+      if (i < lo || i > hi || j < lo || j > hi || k < lo || k > hi)
+        val = (T)(-sp.h);
+      else if (idfield[((int64_t)k * L + i) * L + j] == g)
+        val = (T)sp.h;
+      else
+        val = (T)(-sp.h);
+      dst[v] = val;
+    }
+  }
+}
+
+// resizeIDLocalToDistanceBuffer (src/box.cpp:559-567): the IDLocal buffer the
+// comparison writes takes the output buffer's bounds and layout.
+__global__ void k_idl_resize(SimParams sp, const int32_t* __restrict__ state, DescDev dout, IdlDev il) {
+  for (int g = 1 + blockIdx.x * blockDim.x + threadIdx.x; g <= sp.N; g += gridDim.x * blockDim.x) {
+    if (state[g] != kAlive) continue;
+    il.off[g] = dout.off[g];
+    for (int a = 0; a < 6; ++a) il.bounds[6 * g + a] = dout.bounds[6 * g + a];
+  }
+}
+
+// End of executeRedistancing (src/box.cpp:2582-2585): the input buffer takes
+// the output buffer's bounds (and, here, its arena layout).
+__global__ void k_post_redist(SimParams sp, const int32_t* __restrict__ state, DescDev din, DescDev dout) {
+  for (int g = 1 + blockIdx.x * blockDim.x + threadIdx.x; g <= sp.N; g += gridDim.x * blockDim.x) {
+    if (state[g] != kAlive) continue;
+    din.off[g] = dout.off[g];
+    for (int a = 0; a < 6; ++a) din.bounds[6 * g + a] = dout.bounds[6 * g + a];
+  }
+}
+
+// countGrains (src/grainhdl.cpp:936-943): grains not deleted; appended to the
+// on-device trajectory.
+__global__ __launch_bounds__(1024) void k_count(SimParams sp, const int32_t* __restrict__ state, int32_t* step,
+                                                int32_t* hist, int32_t cap, int32_t* err) {
+  __shared__ int s[1024];
+  int c = 0;
+  for (int g = 1 + threadIdx.x; g <= sp.N; g += 1024) c += state[g] != kDeleted;
+  s[threadIdx.x] = c;
+  __syncthreads();
+  for (int d = 512; d > 0; d >>= 1) {
+    if (threadIdx.x < d) s[threadIdx.x] += s[threadIdx.x + d];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int st = *step;
+    if (st < cap) hist[st] = s[0];
+    else atomicOr(err, kErrHistory);
+    *step = st + 1;
+  }
+}
+
+}  // namespace
+
+void launch_plan(const Launch& L, const SimParams& sp, const int32_t* state, const int32_t* bounds,
+                 int64_t* off_out, int64_t* spec_off, const PlanDev& P, int64_t arena_cap, int64_t spec_cap,
+                 int32_t* err, int64_t* voxels) {
+  hipLaunchKernelGGL(k_plan, dim3(1), dim3(kPlanThreads), 0, L.stream, sp, state, bounds, off_out, spec_off, P,
+                     arena_cap, spec_cap, err, voxels);
+}
+
+void launch_init_phi(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din,
+                     const int32_t* idfield, bool fp32) {
+  if (fp32)
+    hipLaunchKernelGGL(k_init_phi<float>, dim3(L.grid), dim3(256), 0, L.stream, sp, P, din, idfield);
+  else
+    hipLaunchKernelGGL(k_init_phi<double>, dim3(L.grid), dim3(256), 0, L.stream, sp, P, din, idfield);
+}
+
+void launch_idl_resize(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& dout,
+                       const IdlDev& il) {
+  int blocks = (sp.N + 255) / 256;
+  hipLaunchKernelGGL(k_idl_resize, dim3(blocks), dim3(256), 0, L.stream, sp, state, dout, il);
+}
+
+void launch_post_redist(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& din,
+                        const DescDev& dout) {
+  int blocks = (sp.N + 255) / 256;
+  hipLaunchKernelGGL(k_post_redist, dim3(blocks), dim3(256), 0, L.stream, sp, state, din, dout);
+}
+
+void launch_count(const Launch& L, const SimParams& sp, const int32_t* state, int32_t* step, int32_t* history,
+                  int32_t history_cap, int32_t* err) {
+  hipLaunchKernelGGL(k_count, dim3(1), dim3(1024), 0, L.stream, sp, state, step, history, history_cap, err);
+}
+
+}  // namespace gg

@@ -1,0 +1,243 @@
+// Redistancing of every grain box (LSbox::executeRedistancing,
+// src/box.cpp:2322-2586).
+//
+// The reference runs, per z-layer k in [izmin, izmax-1), an x-forward, an
+// x-backward, a y-forward and a y-backward sweep (each line independent), and
+// then a z-forward and a z-backward sweep over all layers (each (y,x) column
+// independent), and finally clamps to +-delta.  Layers are independent in the
+// first phase and columns in the second, so:
+//   k_redist_xy : one workgroup per (grain, output z-layer); the output layer
+//                 and the in/out-intersection of the input layer sit in LDS;
+//                 one lane per row for the x sweeps, one lane per column for
+//                 the y sweeps, each line swept sequentially in the reference's
+//                 order (so FP64 results are bit-identical).
+//   k_redist_z  : one lane per (y,x) column, 64 columns per tile; z-forward,
+//                 z-backward (with its interpolation step) and the clamp.
+// Loop ranges reproduce the reference's exactly, including the partial
+// coverage of rows/columns/layers outside the in/out intersection.
+#include "gg_internal.h"
+
+namespace gg {
+namespace {
+
+template <class T>
+__device__ inline T absv(T v) { return v < T(0) ? -v : v; }
+
+// One sweep step with interpolation on a sign change of the input values a
+// (at the current point) and b (at the next point): may update cur, returns
+// the candidate for the next point.
+template <class T>
+__device__ inline T step_interp(T a, T b, T& cur, T h) {
+  if (a * b <= T(0)) {
+    const T slope = (b - a) / h;
+    const T d = -a / slope;
+    if (absv(cur) > absv(d)) cur = -d * (T)sgnv(slope);
+  }
+  return cur + ((T)sgnv(b) * h);
+}
+
+template <class T, int NMAX>
+__global__ __launch_bounds__(128) void k_redist_xy(SimParams sp, PlanDev P, int cls, DescDev din, DescDev dout,
+                                                   int32_t* err) {
+  constexpr int S = NMAX + 1;
+  __shared__ T O[NMAX * S];
+  __shared__ T I[NMAX * S];
+  const int total = P.totals[kQLayers * kClasses + cls];
+  const T* ain = (const T*)din.arena;
+  T* aout = (T*)dout.arena;
+  const T h = (T)sp.h;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    int g, kl;
+    plan_tile(P, cls, kQLayers, tile, g, kl);
+    const int32_t* ob = dout.bounds + 6 * g;
+    const int32_t* ib = din.bounds + 6 * g;
+    const int ox0 = ob[0], oy0 = ob[1], oz0 = ob[2], ox1 = ob[3], oy1 = ob[4];
+    const int no = ox1 - ox0;
+    const int ni = ib[3] - ib[0];
+    const int k = oz0 + kl;
+    const int ixmin = max(ib[0], ox0), iymin = max(ib[1], oy0), izmin = max(ib[2], oz0);
+    const int ixmax = min(ib[3], ox1), iymax = min(ib[4], oy1), izmax = min(ib[5], ob[5]);
+    const bool empty = ixmin >= ixmax || iymin >= iymax || izmin >= izmax;
+    if (empty && tid == 0 && kl == 0) atomicOr(err, kErrEmptyIO);
+    for (int e = tid; e < no * no; e += nt) O[(e / no) * S + (e % no)] = T(-1);
+    const bool do_xy = !empty && k >= izmin && k < izmax - 1;
+    if (do_xy) {
+      const int wI = ixmax - ixmin, hI = iymax - iymin;
+      const T* src = ain + din.off[g] + (int64_t)(k - ib[2]) * ni * ni;
+      for (int e = tid; e < wI * hI; e += nt) {
+        const int i = iymin + e / wI, j = ixmin + e % wI;
+        I[(i - oy0) * S + (j - ox0)] = src[(int64_t)(i - ib[1]) * ni + (j - ib[0])];
+      }
+    }
+    __syncthreads();
+    if (do_xy) {
+      // x-forward then x-backward, one lane per row i in [iymin, oy1)
+      for (int r = tid; r < oy1 - iymin; r += nt) {
+        const int i = iymin + r;
+        T* orow = O + (i - oy0) * S;  // orow[j - ox0] = out(i, j, k)
+        const T* irow = I + (i - oy0) * S;
+        const bool inrow = i < iymax;
+        if (ixmin < ox1 - 1) {
+          T cur = orow[ixmin - ox0];
+          for (int j = ixmin; j < ox1 - 1; ++j) {
+            T nxt = orow[j + 1 - ox0];
+            T cand;
+            if (j < ixmax - 1 && inrow) cand = step_interp(irow[j - ox0], irow[j + 1 - ox0], cur, h);
+            else cand = cur + ((T)sgnv(nxt) * h);
+            if (absv(cand) < absv(nxt)) nxt = cand;
+            orow[j - ox0] = cur;
+            cur = nxt;
+          }
+          orow[ox1 - 1 - ox0] = cur;
+        }
+        if (ixmax - 1 > ox0) {
+          T cur = orow[ixmax - 1 - ox0];
+          for (int j = ixmax - 1; j > ox0; --j) {
+            T prv = orow[j - 1 - ox0];
+            T cand;
+            if (j > ixmin && inrow) cand = cur + ((T)sgnv(irow[j - 1 - ox0]) * h);
+            else cand = cur + (T)sgnv(prv) * h;
+            if (absv(cand) < absv(prv)) prv = cand;
+            cur = prv;
+            orow[j - 1 - ox0] = prv;
+          }
+        }
+      }
+      __syncthreads();
+      // y-forward then y-backward, one lane per column j in [ixmin, ox1)
+      for (int q = tid; q < ox1 - ixmin; q += nt) {
+        const int j = ixmin + q;
+        T* ocol = O + (j - ox0);  // ocol[(i - oy0) * S] = out(i, j, k)
+        const T* icol = I + (j - ox0);
+        const bool incol = j < ixmax;
+        if (iymin < oy1 - 1) {
+          T cur = ocol[(iymin - oy0) * S];
+          for (int i = iymin; i < oy1 - 1; ++i) {
+            T nxt = ocol[(i + 1 - oy0) * S];
+            T cand;
+            if (incol && i < iymax - 1) cand = step_interp(icol[(i - oy0) * S], icol[(i + 1 - oy0) * S], cur, h);
+            else cand = cur + ((T)sgnv(nxt) * h);
+            if (absv(cand) < absv(nxt)) nxt = cand;
+            ocol[(i - oy0) * S] = cur;
+            cur = nxt;
+          }
+          ocol[(oy1 - 1 - oy0) * S] = cur;
+        }
+        if (iymax - 1 > oy0) {
+          T cur = ocol[(iymax - 1 - oy0) * S];
+          for (int i = iymax - 1; i > oy0; --i) {
+            T prv = ocol[(i - 1 - oy0) * S];
+            T cand;
+            if (incol && i > iymin) cand = cur + ((T)sgnv(icol[(i - 1 - oy0) * S]) * h);
+            else cand = cur + ((T)sgnv(prv) * h);
+            if (absv(cand) < absv(prv)) prv = cand;
+            cur = prv;
+            ocol[(i - 1 - oy0) * S] = prv;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    T* dst = aout + dout.off[g] + (int64_t)kl * no * no;
+    for (int e = tid; e < no * no; e += nt) dst[e] = O[(e / no) * S + (e % no)];
+    __syncthreads();
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(64) void k_redist_z(SimParams sp, PlanDev P, DescDev din, DescDev dout) {
+  const int total = plan_total(P, kQCols);
+  const T* ain = (const T*)din.arena;
+  T* aout = (T*)dout.arena;
+  const T h = (T)sp.h;
+  const T dl = (T)sp.delta;
+  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    int c, g, blk;
+    plan_tile_any(P, kQCols, tile, c, g, blk);
+    const int32_t* ob = dout.bounds + 6 * g;
+    const int32_t* ib = din.bounds + 6 * g;
+    const int ox0 = ob[0], oy0 = ob[1], oz0 = ob[2], oz1 = ob[5];
+    const int no = ob[3] - ox0;
+    const int ni = ib[3] - ib[0];
+    const int col = blk * kColBlock + threadIdx.x;
+    if (col >= no * no) continue;
+    const int i = oy0 + col / no, j = ox0 + col % no;
+    const int ixmin = max(ib[0], ox0), iymin = max(ib[1], oy0), izmin = max(ib[2], oz0);
+    const int ixmax = min(ib[3], ob[3]), iymax = min(ib[4], ob[4]), izmax = min(ib[5], oz1);
+    const bool empty = ixmin >= ixmax || iymin >= iymax || izmin >= izmax;
+    const int64_t ls = (int64_t)no * no;
+    // out(i, j, k) = aout[ob_i + (k - oz0) * ls], in(i, j, k) = ain[ib_i + (k - iz0) * lsi]
+    const int64_t ob_i = dout.off[g] + col;
+    const int iz0 = ib[2];
+#define OUTZ(k) aout[ob_i + (int64_t)((k) - oz0) * ls]
+#define INZ(k) ain[ib_i + (int64_t)((k) - iz0) * lsi]
+    if (!empty && i >= iymin && j >= ixmin) {
+      const bool inside = i < iymax && j < ixmax;
+      const int64_t lsi = (int64_t)ni * ni;
+      const int64_t ib_i = din.off[g] + (int64_t)(i - ib[1]) * ni + (j - ib[0]);
+      if (izmin < oz1 - 1) {
+        T cur = OUTZ(izmin);
+        for (int k = izmin; k < oz1 - 1; ++k) {
+          T nxt = OUTZ(k + 1);
+          T cand;
+          if (k < izmax - 1 && inside) cand = step_interp(INZ(k), INZ(k + 1), cur, h);
+          else cand = cur + ((T)sgnv(nxt) * h);
+          if (absv(cand) < absv(nxt)) nxt = cand;
+          OUTZ(k) = cur;
+          cur = nxt;
+        }
+        OUTZ(oz1 - 1) = cur;
+      }
+      if (izmax - 1 > oz0) {
+        T cur = OUTZ(izmax - 1);
+        for (int k = izmax - 1; k > oz0; --k) {
+          T prv = OUTZ(k - 1);
+          T cand;
+          if (k > izmin && inside) cand = step_interp(INZ(k), INZ(k - 1), cur, h);
+          else cand = cur + ((T)sgnv(prv) * h);
+          if (absv(cand) < absv(prv)) prv = cand;
+          OUTZ(k) = cur;
+          cur = prv;
+        }
+        OUTZ(oz0) = cur;
+      }
+    }
+    // clampValues(-delta, delta) (src/dimensionalBufferReal.h:44-57)
+    for (int k = oz0; k < oz1; ++k) {
+      T v = OUTZ(k);
+      if (v < -dl) v = -dl;
+      else if (v > dl) v = dl;
+      OUTZ(k) = v;
+    }
+#undef OUTZ
+#undef INZ
+  }
+}
+
+template <class T, int NMAX>
+void xy_class(const Launch& L, const SimParams& sp, const PlanDev& P, int cls, const DescDev& din,
+              const DescDev& dout, int32_t* err) {
+  hipLaunchKernelGGL((k_redist_xy<T, NMAX>), dim3(L.grid), dim3(128), 0, L.stream, sp, P, cls, din, dout, err);
+}
+
+template <class T>
+void redist_all(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, const DescDev& dout,
+                int32_t* err) {
+  xy_class<T, 32>(L, sp, P, 0, din, dout, err);
+  xy_class<T, 48>(L, sp, P, 1, din, dout, err);
+  xy_class<T, 64>(L, sp, P, 2, din, dout, err);
+  xy_class<T, 96>(L, sp, P, 3, din, dout, err);
+  hipLaunchKernelGGL(k_redist_z<T>, dim3(L.grid), dim3(64), 0, L.stream, sp, P, din, dout);
+}
+
+}  // namespace
+
+void launch_redist(const Launch& L, const SimParams& sp, const PlanDev& P, const int32_t* state, const DescDev& din,
+                   const DescDev& dout, int32_t* err, bool fp32) {
+  (void)state;
+  if (fp32) redist_all<float>(L, sp, P, din, dout, err);
+  else redist_all<double>(L, sp, P, din, dout, err);
+}
+
+}  // namespace gg

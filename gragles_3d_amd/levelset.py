@@ -1,0 +1,326 @@
+"""ctypes binding of libgg_levelset.so (include/gg_levelset.h, include/gg_grainhdl.h).
+
+LevelSetContext mirrors the reference's per-grain operator surface as driven by
+grainhdl (src/grainhdl.cpp:836-919): convolution, switch, neighbour lists,
+comparison, level set, redistancing, grain count.  Every call goes to the HIP
+library; there is no other implementation behind it.  A missing or unloadable
+library raises, it never falls back.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgg_levelset.so")
+
+GG_OK, GG_ERR_ARG, GG_ERR_HIP, GG_ERR_CAPACITY, GG_ERR_STATE, GG_ERR_DEVICE = range(6)
+FP64_PLAIN, FP32_BATCHED = 0, 1
+DELETED, ALIVE, VANISHED = 0, 1, 2
+STAGES = ("convolution", "lists", "comparison", "level_set", "redistancing", "count")
+
+
+class GGParams(ctypes.Structure):
+    _fields_ = [
+        ("ngridpoints", ctypes.c_int32),
+        ("grid_blowup", ctypes.c_int32),
+        ("conv_domain_size", ctypes.c_int32),
+        ("boundary_grain_tube", ctypes.c_int32),
+        ("h", ctypes.c_double),
+        ("delta", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("precision", ctypes.c_int32),
+        ("n_grains", ctypes.c_int32),
+        ("max_list", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("arena_voxels", ctypes.c_int64),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class GGError(RuntimeError):
+    pass
+
+
+_lib = None
+
+_P = ctypes.POINTER
+_i32p, _u8p, _f64p, _u32p, _i64p = (_P(ctypes.c_int32), _P(ctypes.c_uint8), _P(ctypes.c_double),
+                                    _P(ctypes.c_uint32), _P(ctypes.c_int64))
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+
+# name -> (restype, argtypes): every exported symbol of include/*.h
+SIGNATURES = {
+    "gg_create": (ctypes.c_int, [_P(GGParams), _P(_vp)]),
+    "gg_destroy": (ctypes.c_int, [_vp]),
+    "gg_last_error": (ctypes.c_char_p, [_vp]),
+    "gg_set_boxes": (ctypes.c_int, [_vp, _i32p, _u8p]),
+    "gg_init_distance": (ctypes.c_int, [_vp, _i32p]),
+    "gg_set_neighbour_lists": (ctypes.c_int, [_vp, _i32p, _i32p]),
+    "gg_switch_buffers": (ctypes.c_int, [_vp]),
+    "gg_convolution": (ctypes.c_int, [_vp]),
+    "gg_update_neighbour_lists": (ctypes.c_int, [_vp]),
+    "gg_comparison": (ctypes.c_int, [_vp]),
+    "gg_level_set": (ctypes.c_int, [_vp]),
+    "gg_redistancing": (ctypes.c_int, [_vp]),
+    "gg_count_grains": (ctypes.c_int, [_vp, _i32p]),
+    "gg_timestep": (ctypes.c_int, [_vp, _i32, _i32]),
+    "gg_synchronize": (ctypes.c_int, [_vp]),
+    "gg_get_count_history": (ctypes.c_int, [_vp, _i32p, _i32, _i32p]),
+    "gg_get_grain_stats": (ctypes.c_int, [_vp, _i32p, _f64p, _f64p]),
+    "gg_get_bounds": (ctypes.c_int, [_vp, _i32, _i32, _i32p]),
+    "gg_download_distance": (ctypes.c_int, [_vp, _i32, _i32, _f64p, ctypes.c_int64, _i64p]),
+    "gg_download_idlocal": (ctypes.c_int, [_vp, _i32, _i32p, _u32p, ctypes.c_int64, _i64p]),
+    "gg_download_list": (ctypes.c_int, [_vp, _i32, _i32, _i32p, _i32, _i32p]),
+    "gg_get_new_bounds": (ctypes.c_int, [_vp, _i32p]),
+    "gg_box_voxels": (ctypes.c_int, [_vp, _i64p]),
+    "gg_enable_stage_timing": (ctypes.c_int, [_vp, _i32]),
+    "gg_get_stage_times": (ctypes.c_int, [_vp, _f64p, _i32]),
+    "gg_derive_params": (ctypes.c_int, [_i32, _i32, _i32, ctypes.c_double, _P(GGParams)]),
+    "gg_resize_to_cube": (None, [_i32p, _i32]),
+    "gg_generate_voronoi": (ctypes.c_int, [_P(GGParams), ctypes.c_uint64, _i32p, _i32p, _u8p, _i32]),
+    "gg_initial_neighbour_lists": (ctypes.c_int, [_P(GGParams), _i32p, _u8p, _i32p, _i32p, ctypes.c_int64,
+                                                  _i64p]),
+}
+
+
+def lib():
+    """Load the in-tree HIP library (raises if it is missing)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GGError("libgg_levelset.so is not built: run __graft_entry__.build() or "
+                          "python gragles_3d_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+# ---- host set-up (no GPU) --------------------------------------------------
+def derive_params(n_grains, points_per_grain, border, time_step_size=0.2):
+    p = GGParams()
+    rc = lib().gg_derive_params(int(n_grains), int(points_per_grain), int(border), float(time_step_size),
+                                ctypes.byref(p))
+    if rc != GG_OK:
+        raise GGError("gg_derive_params failed (%d)" % rc)
+    return p
+
+
+def resize_to_cube(bounds6, ngridpoints):
+    b = np.array(bounds6, dtype=np.int32)
+    lib().gg_resize_to_cube(_p(b, ctypes.c_int32), int(ngridpoints))
+    return b
+
+
+def generate_voronoi(params, seed, with_idfield=True, nthreads=0):
+    """Seeded Voronoi microstructure: (idfield[L,L,L] or None, bounds[N+1,6], exists[N+1])."""
+    N, L = params.n_grains, params.ngridpoints
+    idf = np.empty((L, L, L), dtype=np.int32) if with_idfield else None
+    bounds = np.zeros((N + 1, 6), dtype=np.int32)
+    exists = np.zeros(N + 1, dtype=np.uint8)
+    rc = lib().gg_generate_voronoi(ctypes.byref(params), ctypes.c_uint64(seed),
+                                   _p(idf, ctypes.c_int32) if idf is not None else None,
+                                   _p(bounds, ctypes.c_int32), _p(exists, ctypes.c_uint8), int(nthreads))
+    if rc != GG_OK:
+        raise GGError("gg_generate_voronoi failed (%d)" % rc)
+    return idf, bounds, exists
+
+
+def initial_neighbour_lists(params, bounds, exists):
+    N = params.n_grains
+    b = np.ascontiguousarray(bounds, dtype=np.int32)
+    e = np.ascontiguousarray(exists, dtype=np.uint8)
+    offs = np.zeros(N + 2, dtype=np.int32)
+    n = ctypes.c_int64(0)
+    lib().gg_initial_neighbour_lists(ctypes.byref(params), _p(b, ctypes.c_int32), _p(e, ctypes.c_uint8),
+                                     _p(offs, ctypes.c_int32), None, 0, ctypes.byref(n))
+    ids = np.zeros(max(int(n.value), 1), dtype=np.int32)
+    rc = lib().gg_initial_neighbour_lists(ctypes.byref(params), _p(b, ctypes.c_int32), _p(e, ctypes.c_uint8),
+                                          _p(offs, ctypes.c_int32), _p(ids, ctypes.c_int32), ids.size,
+                                          ctypes.byref(n))
+    if rc != GG_OK:
+        raise GGError("gg_initial_neighbour_lists failed (%d)" % rc)
+    return offs, ids[: int(n.value)]
+
+
+# ---- device context -----------------------------------------------------------
+class LevelSetContext:
+    """All grain boxes of one simulation on one GPU."""
+
+    def __init__(self, params, precision=FP64_PLAIN, max_list=512, device=0, arena_voxels=0):
+        p = GGParams()
+        ctypes.memmove(ctypes.byref(p), ctypes.byref(params), ctypes.sizeof(GGParams))
+        p.precision = int(precision)
+        p.max_list = int(max_list)
+        p.device = int(device)
+        p.arena_voxels = int(arena_voxels)
+        self.params = p
+        self.n_grains = p.n_grains
+        h = _vp()
+        rc = lib().gg_create(ctypes.byref(p), ctypes.byref(h))
+        if rc != GG_OK:
+            raise GGError("gg_create failed with code %d" % rc)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != GG_OK:
+            raise GGError("code %d: %s" % (rc, lib().gg_last_error(self.h).decode()))
+
+    # set-up
+    def set_boxes(self, bounds, exists):
+        b = np.ascontiguousarray(bounds, dtype=np.int32)
+        e = np.ascontiguousarray(exists, dtype=np.uint8)
+        self._chk(lib().gg_set_boxes(self.h, _p(b, ctypes.c_int32), _p(e, ctypes.c_uint8)))
+
+    def init_distance(self, idfield):
+        f = np.ascontiguousarray(idfield, dtype=np.int32)
+        self._chk(lib().gg_init_distance(self.h, _p(f, ctypes.c_int32)))
+
+    def set_neighbour_lists(self, offsets, ids):
+        o = np.ascontiguousarray(offsets, dtype=np.int32)
+        i = np.ascontiguousarray(ids, dtype=np.int32)
+        if i.size == 0:
+            i = np.zeros(1, dtype=np.int32)
+        self._chk(lib().gg_set_neighbour_lists(self.h, _p(o, ctypes.c_int32), _p(i, ctypes.c_int32)))
+
+    # stages
+    def switch(self):
+        self._chk(lib().gg_switch_buffers(self.h))
+
+    def convolution(self):
+        self._chk(lib().gg_convolution(self.h))
+
+    def update_lists(self):
+        self._chk(lib().gg_update_neighbour_lists(self.h))
+
+    def comparison(self):
+        self._chk(lib().gg_comparison(self.h))
+
+    def level_set(self):
+        self._chk(lib().gg_level_set(self.h))
+
+    def redistancing(self):
+        self._chk(lib().gg_redistancing(self.h))
+
+    def count(self):
+        c = ctypes.c_int32(0)
+        self._chk(lib().gg_count_grains(self.h, ctypes.byref(c)))
+        return c.value
+
+    def timestep(self, nsteps=1, use_graph=False):
+        self._chk(lib().gg_timestep(self.h, int(nsteps), 1 if use_graph else 0))
+
+    def synchronize(self):
+        self._chk(lib().gg_synchronize(self.h))
+
+    # inspection
+    def count_history(self):
+        n = ctypes.c_int32(0)
+        self._chk(lib().gg_get_count_history(self.h, None, 0, ctypes.byref(n)))
+        buf = np.zeros(max(n.value, 1), dtype=np.int32)
+        self._chk(lib().gg_get_count_history(self.h, _p(buf, ctypes.c_int32), buf.size, ctypes.byref(n)))
+        return buf[: n.value]
+
+    def stats(self):
+        N = self.n_grains
+        st = np.zeros(N + 1, dtype=np.int32)
+        v = np.zeros(N + 1)
+        a = np.zeros(N + 1)
+        self._chk(lib().gg_get_grain_stats(self.h, _p(st, ctypes.c_int32), _p(v, ctypes.c_double),
+                                           _p(a, ctypes.c_double)))
+        return st, v, a
+
+    def bounds(self, gid, which):
+        b = np.zeros(6, dtype=np.int32)
+        self._chk(lib().gg_get_bounds(self.h, int(gid), int(which), _p(b, ctypes.c_int32)))
+        return b
+
+    def phi(self, gid, which):
+        n = ctypes.c_int64(0)
+        self._chk(lib().gg_download_distance(self.h, int(gid), int(which), None, 0, ctypes.byref(n)))
+        buf = np.empty(max(n.value, 1), dtype=np.float64)
+        self._chk(lib().gg_download_distance(self.h, int(gid), int(which), _p(buf, ctypes.c_double), buf.size,
+                                             ctypes.byref(n)))
+        return self.bounds(gid, which), buf[: n.value]
+
+    def idlocal(self, gid):
+        b = np.zeros(6, dtype=np.int32)
+        n = ctypes.c_int64(0)
+        self._chk(lib().gg_download_idlocal(self.h, int(gid), _p(b, ctypes.c_int32), None, 0, ctypes.byref(n)))
+        buf = np.empty(max(n.value, 1), dtype=np.uint32)
+        self._chk(lib().gg_download_idlocal(self.h, int(gid), _p(b, ctypes.c_int32), _p(buf, ctypes.c_uint32),
+                                            buf.size, ctypes.byref(n)))
+        return b, buf[: n.value]
+
+    def neighbour_list(self, gid, which=0):
+        n = ctypes.c_int32(0)
+        self._chk(lib().gg_download_list(self.h, int(gid), int(which), None, 0, ctypes.byref(n)))
+        buf = np.zeros(max(n.value, 1), dtype=np.int32)
+        self._chk(lib().gg_download_list(self.h, int(gid), int(which), _p(buf, ctypes.c_int32), buf.size,
+                                         ctypes.byref(n)))
+        return buf[: n.value].copy()
+
+    def new_bounds(self):
+        b = np.zeros((self.n_grains + 1, 6), dtype=np.int32)
+        self._chk(lib().gg_get_new_bounds(self.h, _p(b, ctypes.c_int32)))
+        return b
+
+    def box_voxels(self):
+        v = ctypes.c_int64(0)
+        self._chk(lib().gg_box_voxels(self.h, ctypes.byref(v)))
+        return int(v.value)
+
+    def enable_stage_timing(self, on=True):
+        self._chk(lib().gg_enable_stage_timing(self.h, 1 if on else 0))
+
+    def stage_times(self, reset=True):
+        ms = np.zeros(6)
+        self._chk(lib().gg_get_stage_times(self.h, _p(ms, ctypes.c_double), 1 if reset else 0))
+        return dict(zip(STAGES, ms.tolist()))
+
+
+def setup_simulation(n_grains, points_per_grain, border, seed, precision=FP64_PLAIN, time_step_size=0.2,
+                     max_list=512, nthreads=0):
+    """Voronoi microstructure -> boxes -> initial distance -> neighbour lists, on the GPU.
+
+    Mirrors grainhdl::initializeSimulation + run_sim's prologue
+    (src/grainhdl.cpp:58-127, 832-834).  Returns (ctx, params, inputs) where
+    inputs holds the host arrays used, so an independent checker can start from
+    the same state.
+    """
+    p = derive_params(n_grains, points_per_grain, border, time_step_size)
+    idf, bounds, exists = generate_voronoi(p, seed, True, nthreads)
+    offs, ids = initial_neighbour_lists(p, bounds, exists)
+    ctx = LevelSetContext(p, precision=precision, max_list=max_list)
+    ctx.set_boxes(bounds, exists)
+    ctx.init_distance(idf)
+    ctx.set_neighbour_lists(offs, ids)
+    return ctx, p, dict(idfield=idf, bounds=bounds, exists=exists, offsets=offs, ids=ids)

@@ -1,0 +1,158 @@
+/*
+ * gg_levelset.h -- C ABI of the MI355X per-grain level-set timestep library.
+ *
+ * One context owns every grain box of one simulation on one GPU: the two
+ * distance-buffer arenas (the reference's m_inputDistance / m_outputDistance
+ * of every LSbox), the IDLocal arenas, the comparison lists, the spectral
+ * scratch and the per-size-class work plans.  Each stage entry point runs all
+ * grains in one batched launch per size class on the context's HIP stream.
+ *
+ * The entry points replace the LSbox operator surface that grainhdl's OpenMP
+ * loops call (paths relative to the GraGLeS_3D source tree):
+ *
+ *   gg_convolution             LSbox::executeConvolution        src/box.h:172, src/box.cpp:414-539
+ *   gg_switch_buffers          LSbox::switchInNOut              src/box.h:190, src/box.cpp:803-808
+ *                              (grainhdl::switchDistancebuffer  src/grainhdl.cpp:1166-1178)
+ *   gg_update_neighbour_lists  LSbox::computeSecondOrderNeighbours src/box.h:129, src/box.cpp:1090-1134
+ *   gg_comparison              LSbox::executeComparison + boundaryCondition + executeSetComparison
+ *                              src/box.h:123,192,128; src/box.cpp:891-966, 997-1088, 814-878
+ *                              (grainhdl::comparison_box        src/grainhdl.cpp:654-667)
+ *   gg_level_set               grainhdl::level_set -> LSbox::extractContour
+ *                              src/grainhdl.cpp:669-684, src/box.cpp:1144-1161 (exists, box
+ *                              resize + resizeToCube, grain volume and surface area)
+ *   gg_redistancing            LSbox::executeRedistancing       src/box.h:117, src/box.cpp:2322-2586
+ *   gg_count_grains            grainhdl::countGrains            src/grainhdl.cpp:936-943
+ *   gg_timestep                one run_sim iteration            src/grainhdl.cpp:836-919
+ *
+ * Conventions: grain ids are 1..n_grains, id 0 is the domain wall.  A bounds
+ * row is {xmin, ymin, zmin, xmax, ymax, zmax}, half-open, in global grid
+ * points; box storage is x fastest, then y, then z (src/dimensionalBuffer.h:75).
+ * Every call returns GG_OK (0) or an error code; gg_last_error() describes the
+ * last failure.  No call throws.  A context is used by one host thread.
+ */
+#ifndef GG_LEVELSET_H
+#define GG_LEVELSET_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gg_ctx gg_ctx;
+
+enum gg_status {
+  GG_OK = 0,
+  GG_ERR_ARG = 1,      /* invalid argument */
+  GG_ERR_HIP = 2,      /* a HIP runtime call failed */
+  GG_ERR_CAPACITY = 3, /* arena, list or box-size capacity exceeded on the device */
+  GG_ERR_STATE = 4,    /* call out of order (e.g. no boxes set) */
+  GG_ERR_DEVICE = 5    /* no usable GPU */
+};
+
+enum gg_precision {
+  GG_FP64_PLAIN = 0,  /* all FP64; bit-identical to the FP64 CPU restatement */
+  GG_FP32_BATCHED = 1 /* FP32 distance fields, FP64 hull reductions */
+};
+
+enum gg_grain_state { GG_DELETED = 0, GG_ALIVE = 1, GG_VANISHED = 2 };
+
+/* Grid and time variables of grainhdl (src/grainhdl.cpp:1419-1444, 237-239). */
+typedef struct gg_params {
+  int32_t ngridpoints;         /* grainhdl::ngridpoints = realDomainSize + 2*grid_blowup */
+  int32_t grid_blowup;         /* grainhdl::grid_blowup = DomainBorderSize */
+  int32_t conv_domain_size;    /* realDomainSize as read by the convolution (rds-1 in Voronoi mode) */
+  int32_t boundary_grain_tube; /* grainhdl::BoundaryGrainTube; never initialised by the reference, pass 0 */
+  double h;                    /* grainhdl::h = 1/realDomainSize */
+  double delta;                /* grainhdl::delta = DomainBorderSize/realDomainSize */
+  double dt;                   /* grainhdl::dt = TimeStepSize/realDomainSize^2 */
+  int32_t precision;           /* enum gg_precision */
+  int32_t n_grains;            /* Settings::NumberOfParticles */
+  int32_t max_list;            /* capacity of one comparison list (0 = 512, at most 1024) */
+  int32_t device;              /* HIP device ordinal */
+  int64_t arena_voxels;        /* voxels per distance arena (0 = 2x the initial sum of box volumes) */
+} gg_params;
+
+/* Create / destroy a context.  Replaces the per-grain buffer allocation of the
+ * LSbox constructors (src/box.cpp:111-119, 213-221) and grainhdl's per-thread
+ * FFT memory pools (src/grainhdl.cpp:598-609). */
+int gg_create(const gg_params* params, gg_ctx** out);
+int gg_destroy(gg_ctx* ctx);
+const char* gg_last_error(const gg_ctx* ctx);
+
+/* Box bounds of every grain as the LSbox constructor computes them before
+ * resizeToCube (src/box.cpp:185-221): rows 0..n_grains of 6 int32 (row 0
+ * ignored), exists[id] != 0 for grains that have a box.  The library applies
+ * resizeToCube(ngridpoints) to both buffers and sizes IDLocal to them. */
+int gg_set_boxes(gg_ctx* ctx, const int32_t* bounds, const uint8_t* exists);
+
+/* calculateDistanceFunction + executeRedistancing of every grain from the
+ * voxelised ID field (ngridpoints^3 int32, x fastest), src/box.cpp:374-397,
+ * driven by grainhdl::distanceInitialisation (src/grainhdl.cpp:580-597). */
+int gg_init_distance(gg_ctx* ctx, const int32_t* idfield);
+
+/* Initial m_secondOrderNeighbours of every grain (grainhdl::find_neighbors,
+ * src/grainhdl.cpp:1029-1054) as CSR: offsets[0..n_grains+1], ids. */
+int gg_set_neighbour_lists(gg_ctx* ctx, const int32_t* offsets, const int32_t* ids);
+
+/* Stage entry points (see the table above). */
+int gg_switch_buffers(gg_ctx* ctx);
+int gg_convolution(gg_ctx* ctx);
+int gg_update_neighbour_lists(gg_ctx* ctx);
+int gg_comparison(gg_ctx* ctx);
+int gg_level_set(gg_ctx* ctx);
+int gg_redistancing(gg_ctx* ctx);
+
+/* countGrains: synchronises and returns the number of grains not deleted. */
+int gg_count_grains(gg_ctx* ctx, int32_t* count);
+
+/* nsteps iterations of the run_sim body (switch, convolution, switch, lists,
+ * comparison, switch, level set, redistancing, count).  use_graph != 0 replays
+ * a captured HIP graph of two timesteps (captured on first use). */
+int gg_timestep(gg_ctx* ctx, int32_t nsteps, int32_t use_graph);
+
+/* Wait for the stream and report device-side capacity errors. */
+int gg_synchronize(gg_ctx* ctx);
+
+/* Grain-count trajectory recorded by each timestep's count stage: writes up to
+ * cap entries, *n receives the number of recorded steps. */
+int gg_get_count_history(gg_ctx* ctx, int32_t* counts, int32_t cap, int32_t* n);
+
+/* ---- inspection (copies device state to the host; synchronising) ---- */
+
+/* Per-grain state (enum gg_grain_state), grain volume and surface area
+ * (LSbox::m_volume / m_surface, src/grainHull.cpp:40-56, 120-131); arrays of
+ * n_grains+1.  Any pointer may be NULL. */
+int gg_get_grain_stats(gg_ctx* ctx, int32_t* state, double* volume, double* area);
+
+/* Bounds of grain id's input (which=0) or output (which=1) buffer. */
+int gg_get_bounds(gg_ctx* ctx, int32_t id, int32_t which, int32_t* bounds6);
+
+/* Copy grain id's distance buffer (which: 0 input, 1 output), converted to
+ * double, into dst (capacity cap elements); *n receives the element count. */
+int gg_download_distance(gg_ctx* ctx, int32_t id, int32_t which, double* dst, int64_t cap, int64_t* n);
+
+/* Copy grain id's IDLocal buffer (LSbox::m_IDLocal) and its bounds. */
+int gg_download_idlocal(gg_ctx* ctx, int32_t id, int32_t* bounds6, uint32_t* dst, int64_t cap, int64_t* n);
+
+/* Copy grain id's m_secondOrderNeighbours (which=0) or m_comparisonList (which=1). */
+int gg_download_list(gg_ctx* ctx, int32_t id, int32_t which, int32_t* dst, int32_t cap, int32_t* n);
+
+/* New bounds from executeSetComparison (m_new{X,Y,Z}{Min,Max}), rows of 6. */
+int gg_get_new_bounds(gg_ctx* ctx, int32_t* bounds);
+
+/* Sum of n^3 over the boxes of alive grains (the voxel-updates of one step). */
+int gg_box_voxels(gg_ctx* ctx, int64_t* voxels);
+
+/* Kernel timing of the dominant kernels: enable HIP-event timing of each
+ * stage on the context stream (adds an event pair around each stage). */
+int gg_enable_stage_timing(gg_ctx* ctx, int32_t enable);
+/* Accumulated milliseconds per stage since the last reset:
+ * [0] convolution [1] lists [2] comparison [3] level set [4] redistancing [5] count. */
+int gg_get_stage_times(gg_ctx* ctx, double* ms6, int32_t reset);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GG_LEVELSET_H */

@@ -1,0 +1,951 @@
+// CPU FP64 oracle of the GraGLeS_3D per-grain level-set timestep.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, bench.py's
+// cpu_baseline leg and __graft_entry__.smoke() load it, and only to compare the
+// HIP library against it or to time it as the CPU baseline.  Nothing in the
+// gragles_3d_amd package links, loads or calls it.
+//
+// It restates, grain by grain and loop by loop, the reference's operator
+// surface (paths relative to the GraGLeS_3D source tree):
+//   box storage            src/dimensionalBuffer.h:71-219, src/dimensionalBufferReal.h:44-57,
+//                          src/dimensionalBufferIDLocal.h:113-135
+//   distance init          src/box.cpp:374-397
+//   convolution            src/box.cpp:414-539, 685-795 (Gaussian mode, correction disabled)
+//   switch                 src/box.cpp:803-808, src/grainhdl.cpp:1166-1178
+//   neighbour lists        src/box.cpp:880-889, 1090-1134
+//   comparison             src/box.cpp:891-966, 968-990, 997-1088, 814-878
+//   level set / hull       src/grainhdl.cpp:669-684, src/box.cpp:1144-1161,
+//                          src/marchingCubes.cpp:364-499, src/grainHull.cpp:40-56, 120-155
+//   redistancing           src/box.cpp:2322-2586
+//   grain count            src/grainhdl.cpp:936-943
+//   stage order            src/grainhdl.cpp:836-919 (no coarsening, no output)
+//
+// The FFT libraries the reference calls (Intel MKL DFTI / FFTW3, versions not
+// pinned by the reference) are replaced by an explicit DFT whose every sum runs
+// in a fixed order; its definition is in convolve_box() below.  It is checked
+// against numpy's pocketfft (tests/golden) and against closed forms.
+//
+// Build: g++ -O3 -fopenmp -ffp-contract=off (oracle/Makefile).  No fast-math.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+const double kPi = 3.14159265358979323846;  // src/ggLS.h:25
+
+// ---------------------------------------------------------------------------
+// Box storage: a half-open [min,max) cube of the global grid, x fastest.
+// resize() sets the bounds and resizes the linear storage like std::vector:
+// old elements keep their linear positions, a grown tail is zero
+// (src/dimensionalBuffer.h:120-131).
+template <class T>
+struct Buf {
+  int x0 = 0, x1 = 1, y0 = 0, y1 = 1, z0 = 0, z1 = 1;
+  std::vector<T> v;
+
+  size_t lin(int y, int x, int z) const {
+    // same unsigned index arithmetic as src/dimensionalBuffer.h:75; an index
+    // outside the box wraps and is caught by at()
+    return (size_t)((unsigned)(z - z0) * (unsigned)(x1 - x0) * (unsigned)(y1 - y0) +
+                    (unsigned)(y - y0) * (unsigned)(x1 - x0) + (unsigned)(x - x0));
+  }
+  T& at(int y, int x, int z) { return v.at(lin(y, x, z)); }
+  const T& at(int y, int x, int z) const { return v.at(lin(y, x, z)); }
+  void resize(int ax0, int ay0, int az0, int ax1, int ay1, int az1) {
+    x0 = ax0; x1 = ax1; y0 = ay0; y1 = ay1; z0 = az0; z1 = az1;
+    v.resize((size_t)((unsigned)(x1 - x0) * (unsigned)(y1 - y0) * (unsigned)(z1 - z0)));
+  }
+  void fill(T val) { std::fill(v.begin(), v.end(), val); }
+};
+
+// resizeToCube (src/dimensionalBuffer.h:139-219) on a bounds array
+// {xmin,ymin,zmin,xmax,ymax,zmax}.
+void cube_bounds(int* b, int maximumLength) {
+  int grid_size = maximumLength - 1;
+  int w = b[3] - b[0], hgt = b[4] - b[1], d = b[5] - b[2];
+  if (w == hgt && w == d) return;
+  int ref = w > hgt ? w : hgt;
+  if (ref < d) ref = d;
+  if (ref > w) { int diff = ref - w; b[0] -= diff / 2; b[3] += diff / 2 + diff % 2; }
+  if (ref > hgt) { int diff = ref - hgt; b[1] -= diff / 2; b[4] += diff / 2 + diff % 2; }
+  if (ref > d) { int diff = ref - d; b[2] -= diff / 2; b[5] += diff / 2 + diff % 2; }
+  for (int a = 0; a < 3; ++a) {
+    if (b[a] < 0) { int delta = -b[a]; b[a + 3] += delta; b[a] += delta; }
+    else if (b[a + 3] > grid_size) { int delta = grid_size - b[a + 3]; b[a + 3] += delta; b[a] += delta; }
+  }
+}
+
+template <class T>
+void resize_to_cube(Buf<T>& buf, int L) {
+  int b[6] = {buf.x0, buf.y0, buf.z0, buf.x1, buf.y1, buf.z1};
+  int w = b[3] - b[0], hgt = b[4] - b[1], d = b[5] - b[2];
+  if (w == hgt && w == d) return;  // early return without resize (:146-147)
+  cube_bounds(b, L);
+  buf.resize(b[0], b[1], b[2], b[3], b[4], b[5]);
+}
+
+template <class T>
+int sgn(T val) { return (T(0) < val) - (val < T(0)); }  // src/utilities.h:27-30
+
+// ---------------------------------------------------------------------------
+// Marching-cubes triangle table (public-domain Lorensen/Bourke table; must be
+// identical to the reference's src/marchingCubes.cpp:106-362 so that the
+// triangulation, hence volume and area, agree).  Each string lists the cube
+// edges of the case's triangles, one hex digit per edge, three per triangle.
+#include "mc_cases.inc"
+
+int g_tri[256][16];
+int g_edge_mask[256];
+// cube corner c (bit c of the case index) -> offset (dx,dy,dz), src/marchingCubes.cpp:378-386
+const int kCorner[8][3] = {{0,0,1},{1,0,1},{1,0,0},{0,0,0},{0,1,1},{1,1,1},{1,1,0},{0,1,0}};
+// edge e -> (first corner, second corner), src/marchingCubes.cpp:393-464
+const int kEdge[12][2] = {{0,1},{1,2},{2,3},{3,0},{4,5},{5,6},{6,7},{7,4},{0,4},{1,5},{2,6},{3,7}};
+
+struct TablesInit {
+  TablesInit() {
+    for (int c = 0; c < 256; ++c) {
+      int m = 0;
+      for (int e = 0; e < 12; ++e)
+        if (((c >> kEdge[e][0]) & 1) != ((c >> kEdge[e][1]) & 1)) m |= 1 << e;
+      g_edge_mask[c] = m;
+      const char* s = kTriCases[c];
+      int n = (int)strlen(s);
+      for (int i = 0; i < 16; ++i) g_tri[c][i] = -1;
+      for (int i = 0; i < n; ++i) {
+        char ch = s[i];
+        g_tri[c][i] = (ch >= '0' && ch <= '9') ? ch - '0' : ch - 'a' + 10;
+      }
+    }
+  }
+} g_tables_init;
+
+struct V3 { double x, y, z; };
+
+// VertexInterp (src/marchingCubes.cpp:484-499) at iso level 0.
+V3 vertex_interp(V3 p1, V3 p2, double v1, double v2) {
+  if (std::abs(v1) < 0.00001) return p1;
+  if (std::abs(v2) < 0.00001) return p2;
+  if (std::abs(v1 - v2) < 0.00001) return p1;
+  double mu = v1 / (v1 - v2);
+  V3 p;
+  p.x = p1.x + mu * (p2.x - p1.x);
+  p.y = p1.y + mu * (p2.y - p1.y);
+  p.z = p1.z + mu * (p2.z - p1.z);
+  return p;
+}
+
+// Hull reductions over one distance buffer: exists (some cell has both
+// inside (phi>=0) and outside corners), the signed-tetrahedron volume sum and
+// the triangle-area sum, accumulated in cell order then triangle order exactly
+// as MarchingCubesAlgorithm::generateHull + GrainHull::computeGrainVolume /
+// computeSurfaceArea do.  Returns exists; vol/area are the raw sums (not
+// scaled by h).
+bool hull_reduce(const Buf<double>& B, double* vol_raw, double* area_raw) {
+  double volume = 0, surface = 0;
+  bool any = false;
+  V3 vert[12];
+  for (int z = B.z0; z < B.z1 - 1; ++z)
+    for (int y = B.y0; y < B.y1 - 1; ++y)
+      for (int x = B.x0; x < B.x1 - 1; ++x) {
+        double val[8];
+        V3 pos[8];
+        unsigned c = 0;
+        for (int q = 0; q < 8; ++q) {
+          int cx = x + kCorner[q][0], cy = y + kCorner[q][1], cz = z + kCorner[q][2];
+          val[q] = B.at(cy, cx, cz);
+          pos[q] = V3{(double)cx, (double)cy, (double)cz};
+          if (val[q] >= 0) c |= 1u << q;
+        }
+        int mask = g_edge_mask[c];
+        if (mask == 0) continue;
+        any = true;
+        for (int e = 0; e < 12; ++e)
+          if (mask & (1 << e)) {
+            int a = kEdge[e][0], b = kEdge[e][1];
+            vert[e] = vertex_interp(pos[a], pos[b], val[a], val[b]);
+          }
+        for (int t = 0; g_tri[c][t] != -1; t += 3) {
+          V3 p0 = vert[g_tri[c][t]], p1 = vert[g_tri[c][t + 1]], p2 = vert[g_tri[c][t + 2]];
+          double v321 = p2.x * p1.y * p0.z;
+          double v231 = p1.x * p2.y * p0.z;
+          double v312 = p2.x * p0.y * p1.z;
+          double v132 = p0.x * p2.y * p1.z;
+          double v213 = p1.x * p0.y * p2.z;
+          double v123 = p0.x * p1.y * p2.z;
+          volume += (double)(1.0f / 6.0f) * (-v321 + v231 + v312 - v132 - v213 + v123);
+          double abx = p0.x - p1.x, aby = p0.y - p1.y, abz = p0.z - p1.z;
+          double acx = p0.x - p2.x, acy = p0.y - p2.y, acz = p0.z - p2.z;
+          double cx = aby * acz - abz * acy;
+          double cy = abz * acx - abx * acz;
+          double cz = abx * acy - aby * acx;
+          surface += std::sqrt(cx * cx + cy * cy + cz * cz) / 2.0;
+        }
+      }
+  *vol_raw = volume;
+  *area_raw = surface;
+  return any;
+}
+
+// ---------------------------------------------------------------------------
+// Explicit DFT heat-kernel convolution of one n^3 box (stands in for
+// convolutionGeneratorMKL, src/box.cpp:685-795, Gaussian branch :751-767).
+//
+// Spectrum layout as the reference's: S[kx + n2*(ky + n*kz)], n2 = n/2+1.
+// Every sum runs over its index from 0 upward, starting at 0.0, with the
+// products grouped as written; tw_c[t] = cos(2*pi*t/n), tw_s[t] = sin(2*pi*t/n).
+//   forward x (real->half spectrum): re += v*c ; im -= v*s
+//   forward y, z (complex):          re += (a*c + b*s) ; im += (b*c - a*s)
+//   spectral factor:                  re *= G ; im *= G
+//   inverse z, y (complex):          re += (a*c - b*s) ; im += (a*s + b*c)
+//   inverse x (half spectrum->real): acc = X0.re ; acc += 2*(a*c - b*s) for
+//                                    kx=1..(n-1)/2 ; n even: acc += (-1)^x Xn/2.re
+// G(s), s = kx2^2+ky2^2+kz2^2 with k2 = min(k, n-k), is the reference's
+// expression evaluated left to right (src/box.cpp:760-761).
+struct ConvTables {
+  int n = 0;
+  std::vector<double> c, s, g;
+};
+
+// The tables are computed by the C library's exp/cos/sin called through
+// volatile pointers, so no compiler folds, fuses (sincos) or vectorises the
+// calls: every program that builds the tables this way on the same host gets
+// the same bits.
+double (*volatile libm_exp)(double) = ::exp;
+double (*volatile libm_cos)(double) = ::cos;
+double (*volatile libm_sin)(double) = ::sin;
+
+double gauss_factor(int sumsq, int n, int nn, double dt) {
+  double nsq = nn * nn;
+  double local_nsq = n * n;
+  return libm_exp(-sqrt(2 / kPi) * sumsq * 4.0 * dt * nsq / local_nsq * kPi * kPi) / (local_nsq * n);
+}
+
+ConvTables make_tables(int n, int nn, double dt) {
+  ConvTables T;
+  T.n = n;
+  T.c.resize(n);
+  T.s.resize(n);
+  for (int t = 0; t < n; ++t) {
+    T.c[t] = libm_cos(2.0 * kPi * t / n);
+    T.s[t] = libm_sin(2.0 * kPi * t / n);
+  }
+  int h2 = n / 2;
+  T.g.resize(3 * h2 * h2 + 1);
+  for (int q = 0; q <= 3 * h2 * h2; ++q) T.g[q] = gauss_factor(q, n, nn, dt);
+  return T;
+}
+
+void convolve_box(const ConvTables& T, const double* in, double* out) {
+  const int n = T.n, n2 = n / 2 + 1;
+  const double* C = T.c.data();
+  const double* S = T.s.data();
+  std::vector<double> ar((size_t)n * n * n2), ai((size_t)n * n * n2);
+  std::vector<double> br((size_t)n * n * n2), bi((size_t)n * n * n2);
+  auto sidx = [&](int kz, int ky, int kx) { return (size_t)kx + (size_t)n2 * (ky + (size_t)n * kz); };
+  // forward x: a[z][y][kx]
+  for (int z = 0; z < n; ++z)
+    for (int y = 0; y < n; ++y) {
+      const double* v = in + ((size_t)z * n + y) * n;
+      for (int kx = 0; kx < n2; ++kx) {
+        double re = 0.0, im = 0.0;
+        for (int x = 0; x < n; ++x) {
+          int t = (x * kx) % n;
+          re = re + v[x] * C[t];
+          im = im - v[x] * S[t];
+        }
+        ar[sidx(z, y, kx)] = re;
+        ai[sidx(z, y, kx)] = im;
+      }
+    }
+  // forward y: b[z][ky][kx]
+  for (int z = 0; z < n; ++z)
+    for (int kx = 0; kx < n2; ++kx)
+      for (int ky = 0; ky < n; ++ky) {
+        double re = 0.0, im = 0.0;
+        for (int y = 0; y < n; ++y) {
+          int t = (y * ky) % n;
+          double a = ar[sidx(z, y, kx)], b = ai[sidx(z, y, kx)];
+          re = re + (a * C[t] + b * S[t]);
+          im = im + (b * C[t] - a * S[t]);
+        }
+        br[sidx(z, ky, kx)] = re;
+        bi[sidx(z, ky, kx)] = im;
+      }
+  // forward z, spectral factor, inverse z: a[z][ky][kx]
+  std::vector<double> fr(n), fi(n);
+  for (int ky = 0; ky < n; ++ky)
+    for (int kx = 0; kx < n2; ++kx) {
+      int ky2 = std::min(ky, n - ky), kx2 = std::min(kx, n - kx);
+      for (int kz = 0; kz < n; ++kz) {
+        double re = 0.0, im = 0.0;
+        for (int z = 0; z < n; ++z) {
+          int t = (z * kz) % n;
+          double a = br[sidx(z, ky, kx)], b = bi[sidx(z, ky, kx)];
+          re = re + (a * C[t] + b * S[t]);
+          im = im + (b * C[t] - a * S[t]);
+        }
+        int kz2 = std::min(kz, n - kz);
+        double G = T.g[kz2 * kz2 + ky2 * ky2 + kx2 * kx2];
+        fr[kz] = re * G;
+        fi[kz] = im * G;
+      }
+      for (int z = 0; z < n; ++z) {
+        double re = 0.0, im = 0.0;
+        for (int kz = 0; kz < n; ++kz) {
+          int t = (z * kz) % n;
+          double a = fr[kz], b = fi[kz];
+          re = re + (a * C[t] - b * S[t]);
+          im = im + (a * S[t] + b * C[t]);
+        }
+        ar[sidx(z, ky, kx)] = re;
+        ai[sidx(z, ky, kx)] = im;
+      }
+    }
+  // inverse y: b[z][y][kx]
+  for (int z = 0; z < n; ++z)
+    for (int kx = 0; kx < n2; ++kx)
+      for (int y = 0; y < n; ++y) {
+        double re = 0.0, im = 0.0;
+        for (int ky = 0; ky < n; ++ky) {
+          int t = (y * ky) % n;
+          double a = ar[sidx(z, ky, kx)], b = ai[sidx(z, ky, kx)];
+          re = re + (a * C[t] - b * S[t]);
+          im = im + (a * S[t] + b * C[t]);
+        }
+        br[sidx(z, y, kx)] = re;
+        bi[sidx(z, y, kx)] = im;
+      }
+  // inverse x (half spectrum -> real)
+  const int half = (n - 1) / 2;
+  for (int z = 0; z < n; ++z)
+    for (int y = 0; y < n; ++y)
+      for (int x = 0; x < n; ++x) {
+        double acc = br[sidx(z, y, 0)];
+        for (int kx = 1; kx <= half; ++kx) {
+          int t = (x * kx) % n;
+          double a = br[sidx(z, y, kx)], b = bi[sidx(z, y, kx)];
+          acc = acc + 2.0 * (a * C[t] - b * S[t]);
+        }
+        if ((n & 1) == 0) {
+          double a = br[sidx(z, y, n / 2)];
+          acc = acc + ((x & 1) ? -a : a);
+        }
+        out[((size_t)z * n + y) * n + x] = acc;
+      }
+}
+
+// ---------------------------------------------------------------------------
+// Redistancing of one grain (src/box.cpp:2322-2586): out is overwritten with
+// -1, swept, clamped to +-delta.  The caller resizes `in` afterwards.
+void redistance(const Buf<double>& in, Buf<double>& out, double h, double delta) {
+  out.fill(-1.0);
+  const int ixmin = in.x0 < out.x0 ? out.x0 : in.x0;
+  const int iymin = in.y0 < out.y0 ? out.y0 : in.y0;
+  const int izmin = in.z0 < out.z0 ? out.z0 : in.z0;
+  const int ixmax = in.x1 < out.x1 ? in.x1 : out.x1;
+  const int iymax = in.y1 < out.y1 ? in.y1 : out.y1;
+  const int izmax = in.z1 < out.z1 ? in.z1 : out.z1;
+  double cand, slope, d;
+  for (int k = izmin; k < izmax - 1; ++k) {
+    // x forward
+    for (int i = iymin; i < out.y1; ++i)
+      for (int j = ixmin; j < out.x1 - 1; ++j) {
+        if (j < ixmax - 1 && i < iymax) {
+          double a = in.at(i, j, k), b = in.at(i, j + 1, k);
+          if (a * b <= 0.0) {
+            slope = (b - a) / h;
+            d = -a / slope;
+            if (std::abs(out.at(i, j, k)) > std::abs(d)) out.at(i, j, k) = -d * sgn(slope);
+          }
+          cand = out.at(i, j, k) + (sgn(b) * h);
+        } else {
+          cand = out.at(i, j, k) + (sgn(out.at(i, j + 1, k)) * h);
+        }
+        if (std::abs(cand) < std::abs(out.at(i, j + 1, k))) out.at(i, j + 1, k) = cand;
+      }
+    // x backward
+    for (int i = iymin; i < out.y1; ++i)
+      for (int j = ixmax - 1; j > out.x0; --j) {
+        if (j > ixmin && i < iymax)
+          cand = out.at(i, j, k) + (sgn(in.at(i, j - 1, k)) * h);
+        else
+          cand = out.at(i, j, k) + sgn(out.at(i, j - 1, k)) * h;
+        if (std::abs(cand) < std::abs(out.at(i, j - 1, k))) out.at(i, j - 1, k) = cand;
+      }
+    // y forward
+    for (int j = ixmin; j < out.x1; ++j)
+      for (int i = iymin; i < out.y1 - 1; ++i) {
+        if (j < ixmax && i < iymax - 1) {
+          double a = in.at(i, j, k), b = in.at(i + 1, j, k);
+          if (a * b <= 0.0) {
+            slope = (b - a) / h;
+            d = -a / slope;
+            if (std::abs(out.at(i, j, k)) > std::abs(d)) out.at(i, j, k) = -d * sgn(slope);
+          }
+          cand = out.at(i, j, k) + (sgn(b) * h);
+        } else {
+          cand = out.at(i, j, k) + (sgn(out.at(i + 1, j, k)) * h);
+        }
+        if (std::abs(cand) < std::abs(out.at(i + 1, j, k))) out.at(i + 1, j, k) = cand;
+      }
+    // y backward
+    for (int j = ixmin; j < out.x1; ++j)
+      for (int i = iymax - 1; i > out.y0; --i) {
+        if (j < ixmax && i > iymin)
+          cand = out.at(i, j, k) + (sgn(in.at(i - 1, j, k)) * h);
+        else
+          cand = out.at(i, j, k) + (sgn(out.at(i - 1, j, k)) * h);
+        if (std::abs(cand) < std::abs(out.at(i - 1, j, k))) out.at(i - 1, j, k) = cand;
+      }
+  }
+  // z forward
+  for (int k = izmin; k < out.z1 - 1; ++k)
+    for (int i = iymin; i < out.y1; ++i)
+      for (int j = ixmin; j < out.x1; ++j) {
+        if (k < izmax - 1 && i < iymax && j < ixmax) {
+          double a = in.at(i, j, k), b = in.at(i, j, k + 1);
+          if (a * b <= 0.0) {
+            slope = (b - a) / h;
+            d = -a / slope;
+            if (std::abs(out.at(i, j, k)) > std::abs(d)) out.at(i, j, k) = -d * sgn(slope);
+          }
+          cand = out.at(i, j, k) + (sgn(b) * h);
+        } else {
+          cand = out.at(i, j, k) + (sgn(out.at(i, j, k + 1)) * h);
+        }
+        if (std::abs(cand) < std::abs(out.at(i, j, k + 1))) out.at(i, j, k + 1) = cand;
+      }
+  // z backward (with interpolation, unlike x/y backward)
+  for (int k = izmax - 1; k > out.z0; --k)
+    for (int i = iymin; i < out.y1; ++i)
+      for (int j = ixmin; j < out.x1; ++j) {
+        if (k > izmin && i < iymax && j < ixmax) {
+          double a = in.at(i, j, k), b = in.at(i, j, k - 1);
+          if (a * b <= 0.0) {
+            slope = (b - a) / h;
+            d = -a / slope;
+            if (std::abs(out.at(i, j, k)) > std::abs(d)) out.at(i, j, k) = -d * sgn(slope);
+          }
+          cand = out.at(i, j, k) + (sgn(b) * h);
+        } else {
+          cand = out.at(i, j, k) + (sgn(out.at(i, j, k - 1)) * h);
+        }
+        if (std::abs(cand) < std::abs(out.at(i, j, k - 1))) out.at(i, j, k - 1) = cand;
+      }
+  // clampValues (src/dimensionalBufferReal.h:44-57)
+  for (double& v : out.v) {
+    if (v < -delta) { v = -delta; continue; }
+    if (v > delta) { v = delta; continue; }
+  }
+}
+
+// ---------------------------------------------------------------------------
+enum { kDeleted = 0, kAlive = 1, kVanished = 2 };
+
+struct Grain {
+  unsigned id = 0;
+  int state = kDeleted;
+  std::unique_ptr<Buf<double>> in, out;
+  Buf<uint32_t> idl;
+  std::vector<unsigned> second;   // m_secondOrderNeighbours
+  std::vector<unsigned> cmp;      // m_comparisonList
+  int nb[6] = {0, 0, 0, 0, 0, 0}; // m_new{X,Y,Z}{Min,Max}: xmin,ymin,zmin,xmax,ymax,zmax
+  bool touches_wall = false;
+  double volume = 0, area = 0;
+};
+
+}  // namespace
+
+struct orc_params {
+  int32_t ngridpoints;
+  int32_t grid_blowup;
+  int32_t conv_domain_size;     // realDomainSize seen by the convolution (rds-1 in Voronoi mode)
+  int32_t boundary_grain_tube;  // grainhdl::BoundaryGrainTube (indeterminate in the reference; 0)
+  double h;
+  double delta;
+  double dt;
+  int32_t n_grains;
+};
+
+struct orc_sim {
+  orc_params p;
+  std::vector<Grain> g;  // index = grain id; 0 is the domain wall pseudo grain
+  std::map<int, ConvTables> tables;
+  std::string err;
+
+  const ConvTables& tab(int n) {
+    auto it = tables.find(n);
+    if (it == tables.end()) it = tables.emplace(n, make_tables(n, p.conv_domain_size, p.dt)).first;
+    return it->second;
+  }
+  bool exists(unsigned id) const {
+    // getGrainByID(id)->grainExists(); a deleted grain (NULL) must never be asked
+    if (id == 0 || id >= g.size()) throw std::runtime_error("grain id out of range in list");
+    if (g[id].state == kDeleted) throw std::runtime_error("deleted grain referenced from a list");
+    return g[id].state == kAlive;
+  }
+};
+
+namespace {
+
+bool intersects(const Buf<double>& a, const Buf<double>& b) {  // checkIntersection, src/box.cpp:880-889
+  if (a.x0 > b.x1 || a.x1 < b.x0 || a.y0 > b.y1 || a.y1 < b.y0 || a.z0 > b.z1 || a.z1 < b.z0)
+    return false;
+  return true;
+}
+
+void second_order_neighbours(orc_sim& S, Grain& G) {  // src/box.cpp:1090-1134
+  if (G.state != kAlive) return;
+  std::vector<unsigned> cand;
+  G.cmp.clear();
+  for (unsigned nbid : G.second) {
+    if (S.exists(nbid)) G.cmp.push_back(nbid);
+    const Grain& nb = S.g[nbid];
+    for (unsigned hid : nb.second) {
+      if (S.exists(hid) && intersects(*G.in, *S.g[hid].in)) cand.push_back(hid);
+    }
+  }
+  for (unsigned c : cand) {
+    if (c == G.id) continue;
+    if (c == 0) continue;
+    bool in = false;
+    for (unsigned e : G.cmp)
+      if (e == c) { in = true; break; }
+    if (!in) G.cmp.push_back(c);
+  }
+}
+
+bool boundary_intersection(const orc_sim& S, const Grain& G) {  // src/box.cpp:968-990
+  int lo = S.p.grid_blowup + S.p.boundary_grain_tube;
+  int hi = S.p.ngridpoints - S.p.grid_blowup - S.p.boundary_grain_tube;
+  const Buf<double>& o = *G.out;
+  if (o.x0 > lo && o.x1 < hi && o.y0 > lo && o.y1 < hi && o.z0 > lo && o.z1 < hi) return false;
+  return true;
+}
+
+void boundary_condition(const orc_sim& S, Grain& G) {  // src/box.cpp:997-1088
+  const int gb = S.p.grid_blowup;
+  const double h = S.p.h;
+  const int m = S.p.ngridpoints;
+  const Buf<double>& in = *G.in;
+  Buf<double>& out = *G.out;
+  double dist = 0;
+  for (int k = in.z0; k < in.z1; ++k)
+    for (int i = in.y0; i < in.y1; ++i)
+      for (int j = in.x0; j < in.x1; ++j) {
+        double dXMin = -(j - gb + 1), dYMin = -(i - gb + 1), dZMin = -(k - gb + 1);
+        double dXMax = (j - (m - 1 - gb)), dYMax = (i - (m - 1 - gb)), dZMax = (k - (m - 1 - gb));
+        double dX = std::abs(dXMin) < std::abs(dXMax) ? dXMin : dXMax;
+        double dY = std::abs(dYMin) < std::abs(dYMax) ? dYMin : dYMax;
+        double dZ = std::abs(dZMin) < std::abs(dZMax) ? dZMin : dZMax;
+        if (dX > 0 && dY > 0 && dZ > 0)
+          dist = std::sqrt((double)dX * dX + dY * dY + dZ * dZ);
+        else if (dX < 0 && dY < 0 && dZ < 0) {
+          dist = std::max(dX, dY);
+          dist = std::max(dist, dZ);
+        } else if (dX >= 0) {
+          if (dY < 0 && dZ < 0) dist = dX;
+          else if (dY < 0 && dZ >= 0) dist = std::sqrt((double)dX * dX + dZ * dZ);
+          else if (dY >= 0 && dZ < 0) dist = std::sqrt((double)dX * dX + dY * dY);
+          // dY >= 0 && dZ >= 0: dist keeps its previous value
+        } else if (dY >= 0) {
+          if (dZ < 0 && dX < 0) dist = dY;
+          else if (dX < 0 && dZ >= 0) dist = std::sqrt((double)dY * dY + dZ * dZ);
+        } else if (dZ >= 0) {
+          if (dY < 0 && dX < 0) dist = dZ;
+        }
+        if (dist > -gb)
+          if (dist * h > out.at(i, j, k)) {
+            out.at(i, j, k) = dist * h;
+            G.idl.at(i, j, k) = 0;
+          }
+      }
+}
+
+void comparison(orc_sim& S, Grain& G) {  // executeComparison, src/box.cpp:891-966
+  if (G.state != kAlive) return;
+  Buf<double>& out = *G.out;
+  const Buf<double>& in = *G.in;
+  out.fill(-1.0);
+  G.second = G.cmp;
+  for (unsigned nbid : G.second) {
+    const Grain& nb = S.g[nbid];
+    const Buf<double>& nin = *nb.in;
+    int xa = in.x0 < nin.x0 ? nin.x0 : in.x0;
+    int xb = in.x1 > nin.x1 ? nin.x1 : in.x1;
+    int ya = in.y0 < nin.y0 ? nin.y0 : in.y0;
+    int yb = in.y1 > nin.y1 ? nin.y1 : in.y1;
+    int za = in.z0 < nin.z0 ? nin.z0 : in.z0;
+    int zb = in.z1 > nin.z1 ? nin.z1 : in.z1;
+    for (int k = za; k < zb; ++k)
+      for (int i = ya; i < yb; ++i)
+        for (int j = xa; j < xb; ++j) {
+          double d = nin.at(i, j, k);
+          if (d > out.at(i, j, k)) {
+            out.at(i, j, k) = d;
+            G.idl.at(i, j, k) = nb.id;
+          }
+        }
+  }
+  if (boundary_intersection(S, G)) {
+    G.touches_wall = true;
+    boundary_condition(S, G);
+  } else {
+    G.touches_wall = false;
+  }
+}
+
+void set_comparison(orc_sim& S, Grain& G) {  // executeSetComparison, src/box.cpp:814-878
+  Buf<double>& out = *G.out;
+  const Buf<double>& in = *G.in;
+  int nxmin = out.x1, nxmax = out.x0, nymin = out.y1, nymax = out.y0, nzmin = out.z1, nzmax = out.z0;
+  const double lim = 0.7 * S.p.delta;
+  for (int k = out.z0; k < out.z1; ++k)
+    for (int i = out.y0; i < out.y1; ++i)
+      for (int j = out.x0; j < out.x1; ++j) {
+        double a = in.at(i, j, k);
+        if (std::abs(a) < lim)
+          out.at(i, j, k) = 0.5 * (a - out.at(i, j, k));
+        else
+          out.at(i, j, k) = a;
+        if (out.at(i, j, k) >= 0) {
+          if (i < nymin) nymin = i;
+          if (i > nymax) nymax = i;
+          if (j < nxmin) nxmin = j;
+          if (j > nxmax) nxmax = j;
+          if (k < nzmin) nzmin = k;
+          if (k > nzmax) nzmax = k;
+        }
+      }
+  const int gb = S.p.grid_blowup, ngp = S.p.ngridpoints;
+  nxmin -= gb; nxmax += gb; nymin -= gb; nymax += gb; nzmin -= gb; nzmax += gb;
+  if (nxmin < 0) nxmin = 0;
+  if (nymin < 0) nymin = 0;
+  if (nzmin < 0) nzmin = 0;
+  if (nxmax >= ngp) nxmax = ngp - 1;
+  if (nymax >= ngp) nymax = ngp - 1;
+  if (nzmax >= ngp) nzmax = ngp - 1;
+  G.nb[0] = nxmin; G.nb[1] = nymin; G.nb[2] = nzmin;
+  G.nb[3] = nxmax; G.nb[4] = nymax; G.nb[5] = nzmax;
+}
+
+template <class F>
+void for_grains(orc_sim& S, F f) {
+  // static round-robin grain lists, src/IterativeGrainScheduler.cpp:13-26
+  const int N = (int)S.g.size() - 1;
+  std::string err;
+#pragma omp parallel
+  {
+    int nt = omp_get_num_threads(), t = omp_get_thread_num();
+    for (int id = 1 + t; id <= N; id += nt) {
+      try {
+        f(S.g[id]);
+      } catch (std::exception& e) {
+#pragma omp critical
+        err += std::string("grain ") + std::to_string(id) + ": " + e.what() + "\n";
+      }
+    }
+  }
+  if (!err.empty()) throw std::runtime_error(err);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C API (ctypes).  Every entry returns 0 on success, nonzero on error; the
+// message is available from orc_last_error().
+extern "C" {
+
+static std::string g_static_err;
+
+const char* orc_last_error(orc_sim* S) { return S ? S->err.c_str() : g_static_err.c_str(); }
+
+orc_sim* orc_create(const orc_params* p) {
+  orc_sim* S = new orc_sim;
+  S->p = *p;
+  S->g.resize((size_t)p->n_grains + 1);
+  for (int i = 0; i <= p->n_grains; ++i) S->g[i].id = (unsigned)i;
+  return S;
+}
+
+void orc_destroy(orc_sim* S) { delete S; }
+
+void orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+
+#define ORC_TRY(S, ...)                                       \
+  try {                                                       \
+    __VA_ARGS__;                                              \
+  } catch (std::exception & e) {                              \
+    (S)->err = e.what();                                      \
+    return 1;                                                 \
+  }                                                           \
+  return 0;
+
+// LSbox(id, vertices, IDField, owner) bounds handling (src/box.cpp:213-221):
+// bounds rows are {xmin,ymin,zmin,xmax,ymax,zmax} before resizeToCube.
+int orc_set_boxes(orc_sim* S, const int32_t* bounds, const uint8_t* exists) {
+  ORC_TRY(S, {
+    const int L = S->p.ngridpoints;
+    for (int id = 1; id <= S->p.n_grains; ++id) {
+      Grain& G = S->g[id];
+      const int32_t* b = bounds + 6 * id;
+      G.state = exists[id] ? kAlive : kDeleted;
+      G.in.reset(new Buf<double>);
+      G.out.reset(new Buf<double>);
+      G.in->resize(b[0], b[1], b[2], b[3], b[4], b[5]);
+      G.out->resize(b[0], b[1], b[2], b[3], b[4], b[5]);
+      resize_to_cube(*G.in, L);
+      resize_to_cube(*G.out, L);
+      G.idl.resize(G.out->x0, G.out->y0, G.out->z0, G.out->x1, G.out->y1, G.out->z1);
+    }
+  })
+}
+
+// distanceInitialisation (src/grainhdl.cpp:580-597) -> calculateDistanceFunction
+// (src/box.cpp:374-397).  idfield is ngridpoints^3, x fastest.
+int orc_init_distance(orc_sim* S, const int32_t* idfield) {
+  ORC_TRY(S, {
+    const int L = S->p.ngridpoints, gb = S->p.grid_blowup;
+    const int lo = gb, hi = L - gb - 1;
+    const double h = S->p.h;
+    for_grains(*S, [&](Grain& G) {
+      if (G.state != kAlive) return;
+      Buf<double>& in = *G.in;
+      for (int k = in.z0; k < in.z1; ++k)
+        for (int i = in.y0; i < in.y1; ++i)
+          for (int j = in.x0; j < in.x1; ++j) {
+            double v;
+            if (i < lo || i > hi || j < lo || j > hi || k < lo || k > hi)
+              v = -h;
+            else if ((int32_t)G.id == idfield[((size_t)k * L + i) * L + j])
+              v = h;
+            else
+              v = -h;
+            in.at(i, j, k) = v;
+          }
+      redistance(*G.in, *G.out, S->p.h, S->p.delta);
+      G.in->resize(G.out->x0, G.out->y0, G.out->z0, G.out->x1, G.out->y1, G.out->z1);
+    });
+  })
+}
+
+// find_neighbors result (src/grainhdl.cpp:1029-1054): CSR over grain ids 0..N.
+int orc_set_lists(orc_sim* S, const int32_t* offsets, const int32_t* ids) {
+  ORC_TRY(S, {
+    for (int id = 1; id <= S->p.n_grains; ++id) {
+      Grain& G = S->g[id];
+      G.second.assign(ids + offsets[id], ids + offsets[id + 1]);
+    }
+  })
+}
+
+int orc_switch(orc_sim* S) {
+  ORC_TRY(S, {
+    for (int id = 1; id <= S->p.n_grains; ++id) {
+      Grain& G = S->g[id];
+      if (G.state == kDeleted) continue;
+      std::swap(G.in, G.out);
+    }
+  })
+}
+
+int orc_convolution(orc_sim* S) {  // executeConvolution, src/box.cpp:414-539
+  ORC_TRY(S, {
+    // build tables serially so the parallel region only reads the map
+    for (int id = 1; id <= S->p.n_grains; ++id)
+      if (S->g[id].state == kAlive) S->tab(S->g[id].out->x1 - S->g[id].out->x0);
+    for_grains(*S, [&](Grain& G) {
+      if (G.state != kAlive) return;
+      int n = G.out->x1 - G.out->x0;
+      if (G.out->y1 - G.out->y0 != n || G.out->z1 - G.out->z0 != n)
+        throw std::runtime_error("convolution on a non-cubic box");
+      if (G.in->v.size() < (size_t)n * n * n) throw std::runtime_error("input buffer too small");
+      convolve_box(S->tables.at(n), G.in->v.data(), G.out->v.data());
+      G.idl.resize(G.out->x0, G.out->y0, G.out->z0, G.out->x1, G.out->y1, G.out->z1);
+    });
+  })
+}
+
+int orc_update_lists(orc_sim* S) {  // updateSecondOrderNeighbors, src/grainhdl.cpp:1015-1027
+  ORC_TRY(S, { for_grains(*S, [&](Grain& G) { second_order_neighbours(*S, G); }); })
+}
+
+int orc_comparison(orc_sim* S) {  // comparison_box, src/grainhdl.cpp:654-667
+  ORC_TRY(S, {
+    for_grains(*S, [&](Grain& G) {
+      if (G.state == kDeleted) return;
+      comparison(*S, G);
+      // executeSetComparison also runs for a vanished grain; its result is
+      // discarded with the grain at the next level_set, so it is skipped here.
+      if (G.state == kAlive) set_comparison(*S, G);
+    });
+  })
+}
+
+int orc_level_set(orc_sim* S) {  // level_set, src/grainhdl.cpp:669-684 + extractContour
+  ORC_TRY(S, {
+    const int L = S->p.ngridpoints;
+    const double h = S->p.h;
+    for_grains(*S, [&](Grain& G) {
+      if (G.state == kDeleted) return;
+      if (G.state == kVanished) {
+        G.state = kDeleted;
+        G.in.reset();
+        G.out.reset();
+        return;
+      }
+      double vr, ar;
+      bool ex = hull_reduce(*G.in, &vr, &ar);
+      if (!ex) {
+        G.state = kVanished;
+        return;
+      }
+      G.out->resize(G.nb[0], G.nb[1], G.nb[2], G.nb[3], G.nb[4], G.nb[5]);
+      resize_to_cube(*G.out, L);
+      G.volume = std::abs(vr) * h * h * h;
+      G.area = ar * h * h;
+    });
+  })
+}
+
+int orc_redistancing(orc_sim* S) {  // redistancing, src/grainhdl.cpp:720-748
+  ORC_TRY(S, {
+    for_grains(*S, [&](Grain& G) {
+      if (G.state != kAlive) return;
+      redistance(*G.in, *G.out, S->p.h, S->p.delta);
+      G.in->resize(G.out->x0, G.out->y0, G.out->z0, G.out->x1, G.out->y1, G.out->z1);
+    });
+  })
+}
+
+int orc_count(orc_sim* S) {  // countGrains, src/grainhdl.cpp:936-943
+  int c = 0;
+  for (int id = 1; id <= S->p.n_grains; ++id)
+    if (S->g[id].state != kDeleted) ++c;
+  return c;
+}
+
+// One run_sim iteration without coarsening or output (src/grainhdl.cpp:836-919).
+int orc_timestep(orc_sim* S, int nsteps, int32_t* counts) {
+  for (int s = 0; s < nsteps; ++s) {
+    if (orc_switch(S) || orc_convolution(S) || orc_switch(S) || orc_update_lists(S) ||
+        orc_comparison(S) || orc_switch(S) || orc_level_set(S) || orc_redistancing(S))
+      return 1;
+    if (counts) counts[s] = orc_count(S);
+  }
+  return 0;
+}
+
+int orc_get_state(orc_sim* S, int id) { return S->g.at(id).state; }
+
+// which: 0 = input buffer, 1 = output buffer.  bounds6 receives
+// {xmin,ymin,zmin,xmax,ymax,zmax}; returns the stored element count.
+long orc_get_bounds(orc_sim* S, int id, int which, int32_t* b) {
+  const Grain& G = S->g.at(id);
+  const Buf<double>* B = which ? G.out.get() : G.in.get();
+  if (!B) return -1;
+  b[0] = B->x0; b[1] = B->y0; b[2] = B->z0; b[3] = B->x1; b[4] = B->y1; b[5] = B->z1;
+  return (long)B->v.size();
+}
+
+long orc_get_phi(orc_sim* S, int id, int which, double* dst, long cap) {
+  const Grain& G = S->g.at(id);
+  const Buf<double>* B = which ? G.out.get() : G.in.get();
+  if (!B) return -1;
+  long n = (long)B->v.size();
+  if (n > cap) return -2;
+  std::memcpy(dst, B->v.data(), sizeof(double) * n);
+  return n;
+}
+
+long orc_get_idlocal(orc_sim* S, int id, int32_t* b, uint32_t* dst, long cap) {
+  const Grain& G = S->g.at(id);
+  b[0] = G.idl.x0; b[1] = G.idl.y0; b[2] = G.idl.z0; b[3] = G.idl.x1; b[4] = G.idl.y1; b[5] = G.idl.z1;
+  long n = (long)G.idl.v.size();
+  if (!dst) return n;
+  if (n > cap) return -2;
+  std::memcpy(dst, G.idl.v.data(), sizeof(uint32_t) * n);
+  return n;
+}
+
+// which: 0 = m_secondOrderNeighbours, 1 = m_comparisonList
+long orc_get_list(orc_sim* S, int id, int which, int32_t* dst, long cap) {
+  const Grain& G = S->g.at(id);
+  const std::vector<unsigned>& L = which ? G.cmp : G.second;
+  long n = (long)L.size();
+  if (n > cap) return -2;
+  for (long i = 0; i < n; ++i) dst[i] = (int32_t)L[i];
+  return n;
+}
+
+void orc_get_stats(orc_sim* S, double* volume, double* area, int32_t* newbounds) {
+  for (int id = 0; id <= S->p.n_grains; ++id) {
+    volume[id] = S->g[id].volume;
+    area[id] = S->g[id].area;
+    if (newbounds)
+      for (int a = 0; a < 6; ++a) newbounds[6 * id + a] = S->g[id].nb[a];
+  }
+}
+
+// ---- standalone helpers for unit tests -----------------------------------
+void orc_resize_to_cube(int32_t* b, int L) {
+  int w = b[3] - b[0], hgt = b[4] - b[1], d = b[5] - b[2];
+  if (w == hgt && w == d) return;
+  int t[6] = {b[0], b[1], b[2], b[3], b[4], b[5]};
+  cube_bounds(t, L);
+  for (int a = 0; a < 6; ++a) b[a] = t[a];
+}
+
+// Convolve one n^3 box with the heat kernel of (dt, nn).
+void orc_convolve_box(int n, int nn, double dt, const double* in, double* out) {
+  ConvTables T = make_tables(n, nn, dt);
+  convolve_box(T, in, out);
+}
+
+void orc_gauss_table(int n, int nn, double dt, double* g) {
+  int h2 = n / 2;
+  for (int q = 0; q <= 3 * h2 * h2; ++q) g[q] = gauss_factor(q, n, nn, dt);
+}
+
+// Hull reductions of one cube box at origin (x0,y0,z0): returns exists.
+int orc_hull_box(int n, int x0, int y0, int z0, const double* phi, double h, double* volume,
+                 double* area) {
+  Buf<double> B;
+  B.resize(x0, y0, z0, x0 + n, y0 + n, z0 + n);
+  std::memcpy(B.v.data(), phi, sizeof(double) * B.v.size());
+  double vr, ar;
+  bool ex = hull_reduce(B, &vr, &ar);
+  *volume = std::abs(vr) * h * h * h;
+  *area = ar * h * h;
+  return ex ? 1 : 0;
+}
+
+// Redistance one grain: in has bounds ib (6), out has bounds ob (6).
+void orc_redistance_box(const int32_t* ib, const double* in, const int32_t* ob, double* out,
+                        double h, double delta) {
+  Buf<double> I, O;
+  I.resize(ib[0], ib[1], ib[2], ib[3], ib[4], ib[5]);
+  O.resize(ob[0], ob[1], ob[2], ob[3], ob[4], ob[5]);
+  std::memcpy(I.v.data(), in, sizeof(double) * I.v.size());
+  redistance(I, O, h, delta);
+  std::memcpy(out, O.v.data(), sizeof(double) * O.v.size());
+}
+
+// Triangle table row access for the table-consistency test.
+int orc_tri_row(int c, int32_t* row16) {
+  for (int i = 0; i < 16; ++i) row16[i] = g_tri[c][i];
+  return g_edge_mask[c];
+}
+
+}  // extern "C"

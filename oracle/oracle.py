@@ -1,0 +1,248 @@
+"""ctypes wrapper of the CPU FP64 oracle (oracle/gg_oracle.cpp).
+
+Test infrastructure only: tests/, bench.py's cpu_baseline leg and
+__graft_entry__.smoke() use it as the checker.  The gragles_3d_amd package never
+imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libggoracle.so")
+
+DELETED, ALIVE, VANISHED = 0, 1, 2
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("ngridpoints", ctypes.c_int32),
+        ("grid_blowup", ctypes.c_int32),
+        ("conv_domain_size", ctypes.c_int32),
+        ("boundary_grain_tube", ctypes.c_int32),
+        ("h", ctypes.c_double),
+        ("delta", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("n_grains", ctypes.c_int32),
+    ]
+
+
+def build():
+    """Compile the oracle with its Makefile (g++ -O3 -fopenmp -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        i32p, u8p, f64p, u32p = P(ctypes.c_int32), P(ctypes.c_uint8), P(ctypes.c_double), P(ctypes.c_uint32)
+        vp = ctypes.c_void_p
+        sig = {
+            "orc_last_error": (ctypes.c_char_p, [vp]),
+            "orc_create": (vp, [P(OrcParams)]),
+            "orc_destroy": (None, [vp]),
+            "orc_set_threads": (None, [ctypes.c_int]),
+            "orc_set_boxes": (ctypes.c_int, [vp, i32p, u8p]),
+            "orc_init_distance": (ctypes.c_int, [vp, i32p]),
+            "orc_set_lists": (ctypes.c_int, [vp, i32p, i32p]),
+            "orc_switch": (ctypes.c_int, [vp]),
+            "orc_convolution": (ctypes.c_int, [vp]),
+            "orc_update_lists": (ctypes.c_int, [vp]),
+            "orc_comparison": (ctypes.c_int, [vp]),
+            "orc_level_set": (ctypes.c_int, [vp]),
+            "orc_redistancing": (ctypes.c_int, [vp]),
+            "orc_count": (ctypes.c_int, [vp]),
+            "orc_timestep": (ctypes.c_int, [vp, ctypes.c_int, i32p]),
+            "orc_get_state": (ctypes.c_int, [vp, ctypes.c_int]),
+            "orc_get_bounds": (ctypes.c_long, [vp, ctypes.c_int, ctypes.c_int, i32p]),
+            "orc_get_phi": (ctypes.c_long, [vp, ctypes.c_int, ctypes.c_int, f64p, ctypes.c_long]),
+            "orc_get_idlocal": (ctypes.c_long, [vp, ctypes.c_int, i32p, u32p, ctypes.c_long]),
+            "orc_get_list": (ctypes.c_long, [vp, ctypes.c_int, ctypes.c_int, i32p, ctypes.c_long]),
+            "orc_get_stats": (None, [vp, f64p, f64p, i32p]),
+            "orc_resize_to_cube": (None, [i32p, ctypes.c_int]),
+            "orc_convolve_box": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_double, f64p, f64p]),
+            "orc_gauss_table": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_double, f64p]),
+            "orc_hull_box": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, f64p,
+                                            ctypes.c_double, f64p, f64p]),
+            "orc_redistance_box": (None, [i32p, f64p, i32p, f64p, ctypes.c_double, ctypes.c_double]),
+            "orc_tri_row": (ctypes.c_int, [ctypes.c_int, i32p]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+# ---- standalone helpers ---------------------------------------------------
+def resize_to_cube(bounds, L):
+    b = np.ascontiguousarray(bounds, dtype=np.int32).copy()
+    lib().orc_resize_to_cube(_p(b, ctypes.c_int32), int(L))
+    return b
+
+
+def convolve_box(phi, nn, dt):
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    n = phi.shape[0]
+    assert phi.shape == (n, n, n)
+    out = np.empty_like(phi)
+    lib().orc_convolve_box(n, int(nn), float(dt), _p(phi, ctypes.c_double), _p(out, ctypes.c_double))
+    return out
+
+
+def gauss_table(n, nn, dt):
+    g = np.empty(3 * (n // 2) ** 2 + 1, dtype=np.float64)
+    lib().orc_gauss_table(n, int(nn), float(dt), _p(g, ctypes.c_double))
+    return g
+
+
+def hull_box(phi, origin, h):
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    n = phi.shape[0]
+    v = np.zeros(1)
+    a = np.zeros(1)
+    ex = lib().orc_hull_box(n, int(origin[0]), int(origin[1]), int(origin[2]), _p(phi, ctypes.c_double),
+                            float(h), _p(v, ctypes.c_double), _p(a, ctypes.c_double))
+    return bool(ex), float(v[0]), float(a[0])
+
+
+def redistance_box(in_bounds, phi_in, out_bounds, h, delta):
+    ib = np.ascontiguousarray(in_bounds, dtype=np.int32)
+    ob = np.ascontiguousarray(out_bounds, dtype=np.int32)
+    phi_in = np.ascontiguousarray(phi_in, dtype=np.float64)
+    out = np.empty((ob[5] - ob[2], ob[4] - ob[1], ob[3] - ob[0]), dtype=np.float64)
+    lib().orc_redistance_box(_p(ib, ctypes.c_int32), _p(phi_in, ctypes.c_double), _p(ob, ctypes.c_int32),
+                             _p(out, ctypes.c_double), float(h), float(delta))
+    return out
+
+
+def tri_row(c):
+    r = np.empty(16, dtype=np.int32)
+    m = lib().orc_tri_row(int(c), _p(r, ctypes.c_int32))
+    return m, r
+
+
+# ---- whole simulation -----------------------------------------------------
+class OracleSim:
+    """All grain boxes of one simulation, stage by stage, on the CPU in FP64."""
+
+    def __init__(self, ngridpoints, grid_blowup, conv_domain_size, h, delta, dt, n_grains,
+                 boundary_grain_tube=0):
+        p = OrcParams(ngridpoints, grid_blowup, conv_domain_size, boundary_grain_tube, h, delta, dt, n_grains)
+        self.n_grains = n_grains
+        self.h = lib().orc_create(ctypes.byref(p))
+
+    def close(self):
+        if self.h:
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise RuntimeError(lib().orc_last_error(self.h).decode())
+
+    def set_boxes(self, bounds, exists):
+        b = np.ascontiguousarray(bounds, dtype=np.int32)
+        e = np.ascontiguousarray(exists, dtype=np.uint8)
+        self._chk(lib().orc_set_boxes(self.h, _p(b, ctypes.c_int32), _p(e, ctypes.c_uint8)))
+
+    def init_distance(self, idfield):
+        f = np.ascontiguousarray(idfield, dtype=np.int32)
+        self._chk(lib().orc_init_distance(self.h, _p(f, ctypes.c_int32)))
+
+    def set_lists(self, offsets, ids):
+        o = np.ascontiguousarray(offsets, dtype=np.int32)
+        i = np.ascontiguousarray(ids, dtype=np.int32)
+        if i.size == 0:
+            i = np.zeros(1, dtype=np.int32)
+        self._chk(lib().orc_set_lists(self.h, _p(o, ctypes.c_int32), _p(i, ctypes.c_int32)))
+
+    def switch(self):
+        self._chk(lib().orc_switch(self.h))
+
+    def convolution(self):
+        self._chk(lib().orc_convolution(self.h))
+
+    def update_lists(self):
+        self._chk(lib().orc_update_lists(self.h))
+
+    def comparison(self):
+        self._chk(lib().orc_comparison(self.h))
+
+    def level_set(self):
+        self._chk(lib().orc_level_set(self.h))
+
+    def redistancing(self):
+        self._chk(lib().orc_redistancing(self.h))
+
+    def count(self):
+        return lib().orc_count(self.h)
+
+    def timestep(self, nsteps=1):
+        c = np.zeros(max(nsteps, 1), dtype=np.int32)
+        self._chk(lib().orc_timestep(self.h, int(nsteps), _p(c, ctypes.c_int32)))
+        return c[:nsteps]
+
+    def state(self, gid):
+        return lib().orc_get_state(self.h, int(gid))
+
+    def bounds(self, gid, which):
+        b = np.zeros(6, dtype=np.int32)
+        n = lib().orc_get_bounds(self.h, int(gid), int(which), _p(b, ctypes.c_int32))
+        return (b, n) if n >= 0 else (None, -1)
+
+    def phi(self, gid, which):
+        """Distance buffer of grain gid (which: 0 = input, 1 = output) as its linear storage."""
+        b, n = self.bounds(gid, which)
+        if b is None:
+            return None, None
+        buf = np.empty(max(n, 1), dtype=np.float64)
+        m = lib().orc_get_phi(self.h, int(gid), int(which), _p(buf, ctypes.c_double), buf.size)
+        return b, buf[:m]
+
+    def idlocal(self, gid):
+        b = np.zeros(6, dtype=np.int32)
+        n = lib().orc_get_idlocal(self.h, int(gid), _p(b, ctypes.c_int32), None, 0)
+        buf = np.empty(max(n, 1), dtype=np.uint32)
+        lib().orc_get_idlocal(self.h, int(gid), _p(b, ctypes.c_int32), _p(buf, ctypes.c_uint32), buf.size)
+        return b, buf[:n]
+
+    def neighbour_list(self, gid, which=0):
+        buf = np.empty(4096, dtype=np.int32)
+        n = lib().orc_get_list(self.h, int(gid), int(which), _p(buf, ctypes.c_int32), buf.size)
+        assert n >= 0
+        return buf[:n].copy()
+
+    def stats(self):
+        v = np.zeros(self.n_grains + 1)
+        a = np.zeros(self.n_grains + 1)
+        nb = np.zeros((self.n_grains + 1) * 6, dtype=np.int32)
+        lib().orc_get_stats(self.h, _p(v, ctypes.c_double), _p(a, ctypes.c_double), _p(nb, ctypes.c_int32))
+        return v, a, nb.reshape(-1, 6)
+
+    def states(self):
+        return np.array([self.state(i) for i in range(self.n_grains + 1)], dtype=np.int32)

@@ -1,0 +1,71 @@
+"""FP64 plain GPU path vs the FP64 CPU oracle: bit-exact, stage by stage.
+
+Each case prints its name, seed and shape before touching the GPU.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import gragles_3d_amd as G
+from parity import compare_all, make_pair, step_both
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("n_grains,ppg,border,seed", [(27, 12, 3, 11), (64, 10, 7, 5)])
+def test_stagewise_bitexact(n_grains, ppg, border, seed):
+    _say("stagewise n=%d ppg=%d border=%d seed=%d" % (n_grains, ppg, border, seed))
+    ctx, orc, p = make_pair(n_grains, ppg, border, seed)
+    compare_all(ctx, orc, "after init", check_phi=(1,), check_idlocal=True, check_lists=True)
+    for s in range(3):
+        step_both(ctx, orc, compare_every_stage=True, tag="step %d" % s)
+    ctx.close()
+
+
+def test_many_steps_with_vanishing():
+    # small grains, many steps: grains vanish and get deleted, lists shrink
+    _say("many steps n=64 ppg=6 border=3 seed=3")
+    ctx, orc, p = make_pair(64, 6, 3, 3)
+    counts_o = []
+    for s in range(40):
+        step_both(ctx, orc)
+        counts_o.append(orc.count())
+        if s % 10 == 9:
+            compare_all(ctx, orc, "step %d" % s)
+    compare_all(ctx, orc, "final")
+    st, _, _ = ctx.stats()
+    assert (st != G.DELETED).sum() == counts_o[-1]
+    assert counts_o[-1] < 64, "no grain vanished; the case does not exercise deletion"
+    ctx.close()
+
+
+def test_timestep_entry_matches_oracle():
+    _say("timestep entry n=125 ppg=8 border=3 seed=7")
+    ctx, orc, p = make_pair(125, 8, 3, 7)
+    ctx.timestep(6, use_graph=False)
+    counts_o = orc.timestep(6)
+    compare_all(ctx, orc, "after 6 timesteps")
+    np.testing.assert_array_equal(ctx.count_history(), counts_o)
+    ctx.close()
+
+
+def test_graph_replay_matches_eager():
+    _say("graph replay n=125 ppg=8 border=3 seed=9")
+    ctx_a, orc, p = make_pair(125, 8, 3, 9)
+    ctx_b, _, _ = make_pair(125, 8, 3, 9)
+    ctx_a.timestep(7, use_graph=False)
+    for _ in range(3):  # several replays with eager work and syncs between them
+        ctx_b.timestep(2, use_graph=True)
+        ctx_b.synchronize()
+    ctx_b.timestep(1, use_graph=True)
+    orc.timestep(7)
+    compare_all(ctx_a, orc, "eager after 7 steps")
+    compare_all(ctx_b, orc, "graph after 7 steps")
+    ctx_a.close()
+    ctx_b.close()
