@@ -1,0 +1,215 @@
+"""Benchmark of the per-grain level-set timestep on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 8000 grains from
+a seeded Voronoi tessellation (seed 2), 20 points per grain, domain border 3
+(323^3 interior, 329^3 grid), Gaussian convolution with TimeStepSize 0.2,
+isotropic, no coarsening.  A step is one pass of the hot path over every grain
+box: convolution, neighbour lists, comparison, level set (existence, volume,
+area, resize) and redistancing.  The timed steps of the FP32 batched path run
+as replays of a captured HIP graph; the all-FP64 path is timed the same way and
+reported beside it.
+
+value = voxel updates per second = sum over steps of sum_boxes n_b^3 / wall
+time of the timed steps, with every input resident in HBM.
+
+Output: one JSON line on stdout.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import gragles_3d_amd as G  # noqa: E402
+
+METRIC = "grain-box voxel updates/sec per timestep, FP32-batched vs FP64-plain; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kbar_and_voxels(ctx, n_grains):
+    """Exact mean number of comparison-list boxes covering a voxel, K = sum_i sum_{j in list(i)}
+    |B_i cap B_j| / sum_i |B_i|, over the grains alive now, from their box bounds."""
+    st, _, _ = ctx.stats()
+    alive = np.nonzero(st == G.ALIVE)[0]
+    B = {int(g): ctx.bounds(int(g), 1) for g in alive}
+    num = 0
+    den = 0
+    for g in alive:
+        b = B[int(g)]
+        den += int(b[3] - b[0]) ** 3
+        for j in ctx.neighbour_list(int(g), 0):
+            c = B.get(int(j))
+            if c is None:
+                continue
+            w = [max(0, min(b[a + 3], c[a + 3]) - max(b[a], c[a])) for a in range(3)]
+            num += w[0] * w[1] * w[2]
+    return num / max(den, 1), den
+
+
+def run_path(precision, inputs, p, steps, warmup, use_graph=True):
+    ctx = G.LevelSetContext(p, precision=precision)
+    ctx.set_boxes(inputs["bounds"], inputs["exists"])
+    ctx.init_distance(inputs["idfield"])
+    ctx.set_neighbour_lists(inputs["offsets"], inputs["ids"])
+    ctx.synchronize()
+    ctx.timestep(warmup, use_graph=use_graph)
+    ctx.synchronize()
+    vox_before = ctx.box_voxels()
+    t0 = time.perf_counter()
+    ctx.timestep(steps, use_graph=use_graph)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    vox_after = ctx.box_voxels()
+    return ctx, t1 - t0, vox_before, vox_after
+
+
+def kernel_pass(ctx, steps):
+    """Eager steps with every kernel segment bracketed by HIP events on the library's stream."""
+    ctx.enable_stage_timing(True)
+    ctx.stage_times(reset=True)
+    ctx.timestep(steps, use_graph=False)
+    ctx.synchronize()
+    seg = ctx.stage_times(reset=True)
+    ctx.enable_stage_timing(False)
+    return seg
+
+
+def cpu_baseline(inputs, p, nthreads):
+    from oracle import oracle as O
+    O.set_threads(nthreads)
+    orc = O.OracleSim(p.ngridpoints, p.grid_blowup, p.conv_domain_size, p.h, p.delta, p.dt, p.n_grains)
+    orc.set_boxes(inputs["bounds"], inputs["exists"])
+    orc.init_distance(inputs["idfield"])
+    orc.set_lists(inputs["offsets"], inputs["ids"])
+    vox = 0
+    for g in range(1, p.n_grains + 1):
+        if orc.state(g) == O.ALIVE:
+            b, _ = orc.bounds(g, 1)
+            vox += int(b[3] - b[0]) ** 3
+    t0 = time.perf_counter()
+    orc.timestep(1)
+    dt = time.perf_counter() - t0
+    orc.close()
+    return vox / dt, dt, vox
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--grains", type=int, default=8000)
+    ap.add_argument("--ppg", type=int, default=20)
+    ap.add_argument("--border", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 path")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--kernel-steps", type=int, default=4, help="eager event-timed steps for the roofline")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the per-grain timestep runs every grain box on a single MI355X "
+              "(no sharding); --gpus %d is not supported." % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    p = G.derive_params(args.grains, args.ppg, args.border)
+    t = time.perf_counter()
+    idf, bounds, exists = G.generate_voronoi(p, args.seed, True, 0)
+    offs, ids = G.initial_neighbour_lists(p, bounds, exists)
+    log("setup: %d grains, grid %d^3, generator %.2fs" % (args.grains, p.ngridpoints, time.perf_counter() - t))
+    inputs = dict(idfield=idf, bounds=bounds, exists=exists, offsets=offs, ids=ids)
+
+    # ---- FP32 batched, graph replays ------------------------------------------
+    ctx32, wall32, vb32, va32 = run_path(G.FP32_BATCHED, inputs, p, args.steps, args.warmup)
+    vox_step32 = 0.5 * (vb32 + va32)
+    value = vox_step32 * args.steps / wall32
+    log("fp32: %d steps %.3fs -> %.3f ms/step, %.3e voxel-updates/s (boxes %.3e voxels)" %
+        (args.steps, wall32, 1e3 * wall32 / args.steps, value, vox_step32))
+    kbar, _ = kbar_and_voxels(ctx32, p.n_grains)
+    seg32 = kernel_pass(ctx32, args.kernel_steps)
+    vox_k = ctx32.box_voxels()
+    counts32 = ctx32.count_history()
+    ctx32.close()
+
+    # roofline of the comparison kernel (neighbour max + wall + set comparison): per voxel it
+    # reads K neighbour values and its own value and writes the new value and IDLocal
+    cmp_ms, cmp_n = seg32["comparison"]
+    cmp_avg_s = cmp_ms / max(cmp_n, 1) * 1e-3
+    cmp_bytes = vox_k * (4.0 * kbar + 12.0)
+    achieved = cmp_bytes / cmp_avg_s / 1e9 if cmp_avg_s > 0 else 0.0
+    seg_ms_per_step = {k: round(v[0] / max(v[1], 1), 4) for k, v in seg32.items() if v[1]}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "voxel-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall32 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "%d grains, seeded Voronoi (seed %d), %d points/grain, border %d, grid %d^3, "
+                        "FP32 batched path, hipGraph-replayed steps" % (args.grains, args.seed, args.ppg,
+                                                                        args.border, p.ngridpoints),
+            "grains": args.grains, "points_per_grain": args.ppg, "border": args.border,
+            "ngridpoints": p.ngridpoints, "box_voxels_per_step": vox_step32, "kbar": kbar,
+        },
+        "roofline": {
+            "kernel": "k_comparison<float>",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": cmp_bytes,
+            "avg_launch_ms": cmp_avg_s * 1e3,
+        },
+        "segment_ms_per_step_fp32": seg_ms_per_step,
+        "grain_count_trajectory_fp32": counts32[-3:].tolist(),
+    }
+
+    if not args.no_fp64:
+        ctx64, wall64, vb64, va64 = run_path(G.FP64_PLAIN, inputs, p, args.steps, args.warmup)
+        vox_step64 = 0.5 * (vb64 + va64)
+        v64 = vox_step64 * args.steps / wall64
+        seg64 = kernel_pass(ctx64, max(1, args.kernel_steps // 2))
+        ctx64.close()
+        log("fp64: %d steps %.3fs -> %.3f ms/step, %.3e voxel-updates/s" % (args.steps, wall64,
+                                                                          1e3 * wall64 / args.steps, v64))
+        out["fp64_plain"] = {"value": v64, "ms_per_step": 1e3 * wall64 / args.steps,
+                             "segment_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in seg64.items()
+                                                     if v[1]}}
+        out["fp32_speedup_vs_fp64"] = value / v64
+
+    if not args.no_cpu:
+        try:
+            v_cpu, dt_cpu, vox_cpu = cpu_baseline(inputs, p, args.cpu_threads)
+            out["cpu_baseline"] = {"value": v_cpu, "unit": "voxel-updates/s", "cores": args.cpu_threads,
+                                   "kind": "port",
+                                   "sample": "FP64 CPU oracle (g++ -O3 -fopenmp), one full timestep of the same "
+                                             "%d-grain workload (%.3e box voxels) in %.1fs" % (args.grains, vox_cpu,
+                                                                                              dt_cpu)}
+            log("cpu oracle: %.1fs for one step -> %.3e voxel-updates/s" % (dt_cpu, v_cpu))
+        except Exception as e:  # the checker may be absent on a product install
+            log("cpu baseline skipped: %s" % e)
+            out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -62,7 +62,8 @@ struct gg_ctx {
   // stage timing
   bool timing = false;
   hipEvent_t ev[12] = {};
-  double stage_ms[6] = {0, 0, 0, 0, 0, 0};
+  double seg_ms[16] = {};
+  int64_t seg_n[16] = {};
   std::vector<std::pair<int, int>> pending;  // (stage, event pair index)
   std::vector<hipEvent_t> evpool;
   int evused = 0;
@@ -206,22 +207,28 @@ hipEvent_t take_event(gg_ctx* c) {
   return c->evpool[c->evused++];
 }
 
-struct StageTimer {
+// Event-timed segments, one per kernel (or per kernel with one launch per size class).
+enum Seg {
+  kSegConvA, kSegConvB, kSegConvC, kSegIdl, kSegLists, kSegCompare, kSegHull, kSegFinalize, kSegPlan,
+  kSegRedistXY, kSegRedistZ, kSegPost, kSegCount, kNumSeg
+};
+
+struct SegTimer {
   gg_ctx* c;
-  int stage;
+  int seg;
   hipEvent_t a = nullptr;
-  StageTimer(gg_ctx* ctx, int s) : c(ctx), stage(s) {
+  SegTimer(gg_ctx* ctx, int s) : c(ctx), seg(s) {
     if (c->timing) {
       a = take_event(c);
       if (a) hipEventRecord(a, c->stream);
     }
   }
-  ~StageTimer() {
+  ~SegTimer() {
     if (c->timing && a) {
       hipEvent_t b = take_event(c);
       if (b) {
         hipEventRecord(b, c->stream);
-        c->pending.push_back({stage, c->evused - 2});
+        c->pending.push_back({seg, c->evused - 2});
       }
     }
   }
@@ -230,8 +237,10 @@ struct StageTimer {
 void collect_timing(gg_ctx* c) {
   for (auto& pr : c->pending) {
     float ms = 0;
-    if (hipEventElapsedTime(&ms, c->evpool[pr.second], c->evpool[pr.second + 1]) == hipSuccess)
-      c->stage_ms[pr.first] += ms;
+    if (hipEventElapsedTime(&ms, c->evpool[pr.second], c->evpool[pr.second + 1]) == hipSuccess) {
+      c->seg_ms[pr.first] += ms;
+      c->seg_n[pr.first] += 1;
+    }
   }
   c->pending.clear();
   c->evused = 0;
@@ -242,43 +251,66 @@ int require_boxes(gg_ctx* c) {
   return GG_OK;
 }
 
-// one run_sim iteration body, enqueued on the stream
-void enqueue_step(gg_ctx* c) {
+// ---- stages (each enqueues its kernels on the context stream) -------------
+void stage_convolution(gg_ctx* c) {
   Launch L = launch_of(c);
-  c->sel ^= 1;  // switchDistancebuffer (gridCoarsement's else branch)
-  {
-    StageTimer t(c, 0);
-    launch_conv(L, c->sp, c->P, din(c), dout(c), c->TB, c->d_spec_off, c->d_spec, c->fp32);
-    launch_idl_resize(L, c->sp, c->d_state, dout(c), c->IL[c->il_cur ^ 1]);
+  { SegTimer t(c, kSegConvA); launch_conv(L, 0, c->sp, c->P, din(c), dout(c), c->TB, c->d_spec_off, c->d_spec, c->fp32); }
+  { SegTimer t(c, kSegConvB); launch_conv(L, 1, c->sp, c->P, din(c), dout(c), c->TB, c->d_spec_off, c->d_spec, c->fp32); }
+  if (!c->fp32) {
+    SegTimer t(c, kSegConvC);
+    launch_conv(L, 2, c->sp, c->P, din(c), dout(c), c->TB, c->d_spec_off, c->d_spec, c->fp32);
   }
-  c->sel ^= 1;
+  { SegTimer t(c, kSegIdl); launch_idl_resize(L, c->sp, c->d_state, dout(c), c->IL[c->il_cur ^ 1]); }
+}
+
+void stage_lists(gg_ctx* c) {
+  SegTimer t(c, kSegLists);
+  launch_lists(launch_of(c), c->sp, c->d_state, din(c), dout(c), c->LD, c->d_newb, c->d_err);
+}
+
+void stage_comparison(gg_ctx* c) {
   {
-    StageTimer t(c, 1);
-    launch_lists(L, c->sp, c->d_state, din(c), dout(c), c->LD, c->d_newb, c->d_err);
-  }
-  {
-    StageTimer t(c, 2);
-    launch_comparison(L, c->sp, c->P, c->d_state, din(c), dout(c), c->LD, c->IL[c->il_cur], c->IL[c->il_cur ^ 1],
-                      c->d_newb, c->fp32);
+    SegTimer t(c, kSegCompare);
+    launch_comparison(launch_of(c), c->sp, c->P, c->d_state, din(c), dout(c), c->LD, c->IL[c->il_cur],
+                      c->IL[c->il_cur ^ 1], c->d_newb, c->fp32);
   }
   c->il_cur ^= 1;
-  c->sel ^= 1;
+}
+
+void stage_level_set(gg_ctx* c) {
+  Launch L = launch_of(c);
+  { SegTimer t(c, kSegHull); launch_hull(L, c->sp, c->P, din(c), c->d_exflag, c->d_partials, c->fp32); }
   {
-    StageTimer t(c, 3);
-    launch_hull(L, c->sp, c->P, din(c), c->d_exflag, c->d_partials, c->fp32);
+    SegTimer t(c, kSegFinalize);
     launch_finalize(L, c->sp, c->P, c->d_state, c->LD, c->d_exflag, c->d_partials, c->d_newb, din(c), dout(c),
                     c->d_volume, c->d_area);
-    plan_layout(c, dout(c));
   }
-  {
-    StageTimer t(c, 4);
-    launch_redist(L, c->sp, c->P, c->d_state, din(c), dout(c), c->d_err, c->fp32);
-    launch_post_redist(L, c->sp, c->d_state, din(c), dout(c));
-  }
-  {
-    StageTimer t(c, 5);
-    launch_count(L, c->sp, c->d_state, c->d_step, c->d_hist, c->hist_cap, c->d_err);
-  }
+  { SegTimer t(c, kSegPlan); plan_layout(c, dout(c)); }
+}
+
+void stage_redistancing(gg_ctx* c) {
+  Launch L = launch_of(c);
+  { SegTimer t(c, kSegRedistXY); launch_redist(L, 0, c->sp, c->P, din(c), dout(c), c->d_err, c->fp32); }
+  { SegTimer t(c, kSegRedistZ); launch_redist(L, 1, c->sp, c->P, din(c), dout(c), c->d_err, c->fp32); }
+  { SegTimer t(c, kSegPost); launch_post_redist(L, c->sp, c->d_state, din(c), dout(c)); }
+}
+
+void stage_count(gg_ctx* c) {
+  SegTimer t(c, kSegCount);
+  launch_count(launch_of(c), c->sp, c->d_state, c->d_step, c->d_hist, c->hist_cap, c->d_err);
+}
+
+// one run_sim iteration body (src/grainhdl.cpp:836-919 without coarsening or output)
+void enqueue_step(gg_ctx* c) {
+  c->sel ^= 1;  // switchDistancebuffer in gridCoarsement's else branch
+  stage_convolution(c);
+  c->sel ^= 1;
+  stage_lists(c);
+  stage_comparison(c);
+  c->sel ^= 1;
+  stage_level_set(c);
+  stage_redistancing(c);
+  stage_count(c);
 }
 
 }  // namespace
@@ -451,10 +483,8 @@ int gg_init_distance(gg_ctx* c, const int32_t* idfield) {
     hipFree(d_id);
     return fail(c, GG_ERR_HIP, std::string("hipMemcpy idfield: ") + hipGetErrorString(e));
   }
-  Launch Lc = launch_of(c);
-  launch_init_phi(Lc, c->sp, c->P, din(c), d_id, c->fp32);
-  launch_redist(Lc, c->sp, c->P, c->d_state, din(c), dout(c), c->d_err, c->fp32);
-  launch_post_redist(Lc, c->sp, c->d_state, din(c), dout(c));
+  launch_init_phi(launch_of(c), c->sp, c->P, din(c), d_id, c->fp32);
+  stage_redistancing(c);
   e = hipStreamSynchronize(c->stream);
   hipFree(d_id);
   if (e != hipSuccess) return fail(c, GG_ERR_HIP, std::string("init distance: ") + hipGetErrorString(e));
@@ -494,9 +524,7 @@ int gg_convolution(gg_ctx* c) {
   int rc = require_boxes(c);
   if (rc) return rc;
   hipSetDevice(c->p.device);
-  StageTimer t(c, 0);
-  launch_conv(launch_of(c), c->sp, c->P, din(c), dout(c), c->TB, c->d_spec_off, c->d_spec, c->fp32);
-  launch_idl_resize(launch_of(c), c->sp, c->d_state, dout(c), c->IL[c->il_cur ^ 1]);
+  stage_convolution(c);
   return GG_OK;
 }
 
@@ -505,8 +533,7 @@ int gg_update_neighbour_lists(gg_ctx* c) {
   int rc = require_boxes(c);
   if (rc) return rc;
   hipSetDevice(c->p.device);
-  StageTimer t(c, 1);
-  launch_lists(launch_of(c), c->sp, c->d_state, din(c), dout(c), c->LD, c->d_newb, c->d_err);
+  stage_lists(c);
   return GG_OK;
 }
 
@@ -515,10 +542,7 @@ int gg_comparison(gg_ctx* c) {
   int rc = require_boxes(c);
   if (rc) return rc;
   hipSetDevice(c->p.device);
-  StageTimer t(c, 2);
-  launch_comparison(launch_of(c), c->sp, c->P, c->d_state, din(c), dout(c), c->LD, c->IL[c->il_cur],
-                    c->IL[c->il_cur ^ 1], c->d_newb, c->fp32);
-  c->il_cur ^= 1;
+  stage_comparison(c);
   return GG_OK;
 }
 
@@ -527,12 +551,7 @@ int gg_level_set(gg_ctx* c) {
   int rc = require_boxes(c);
   if (rc) return rc;
   hipSetDevice(c->p.device);
-  StageTimer t(c, 3);
-  Launch L = launch_of(c);
-  launch_hull(L, c->sp, c->P, din(c), c->d_exflag, c->d_partials, c->fp32);
-  launch_finalize(L, c->sp, c->P, c->d_state, c->LD, c->d_exflag, c->d_partials, c->d_newb, din(c), dout(c),
-                  c->d_volume, c->d_area);
-  plan_layout(c, dout(c));
+  stage_level_set(c);
   return GG_OK;
 }
 
@@ -541,19 +560,14 @@ int gg_redistancing(gg_ctx* c) {
   int rc = require_boxes(c);
   if (rc) return rc;
   hipSetDevice(c->p.device);
-  StageTimer t(c, 4);
-  launch_redist(launch_of(c), c->sp, c->P, c->d_state, din(c), dout(c), c->d_err, c->fp32);
-  launch_post_redist(launch_of(c), c->sp, c->d_state, din(c), dout(c));
+  stage_redistancing(c);
   return GG_OK;
 }
 
 int gg_count_grains(gg_ctx* c, int32_t* count) {
   if (!c || !count) return GG_ERR_ARG;
   hipSetDevice(c->p.device);
-  {
-    StageTimer t(c, 5);
-    launch_count(launch_of(c), c->sp, c->d_state, c->d_step, c->d_hist, c->hist_cap, c->d_err);
-  }
+  stage_count(c);
   int rc = gg_synchronize(c);
   if (rc) return rc;
   int32_t st = 0;
@@ -724,13 +738,19 @@ int gg_enable_stage_timing(gg_ctx* c, int32_t enable) {
   return GG_OK;
 }
 
-int gg_get_stage_times(gg_ctx* c, double* ms6, int32_t reset) {
-  if (!c || !ms6) return GG_ERR_ARG;
+int gg_get_stage_times(gg_ctx* c, double* ms, int64_t* launches, int32_t n, int32_t reset) {
+  if (!c || !ms || n < 0) return GG_ERR_ARG;
   int rc = gg_synchronize(c);
   if (rc) return rc;
-  for (int i = 0; i < 6; ++i) ms6[i] = c->stage_ms[i];
+  for (int i = 0; i < n && i < kNumSeg; ++i) {
+    ms[i] = c->seg_ms[i];
+    if (launches) launches[i] = c->seg_n[i];
+  }
   if (reset)
-    for (int i = 0; i < 6; ++i) c->stage_ms[i] = 0;
+    for (int i = 0; i < kNumSeg; ++i) {
+      c->seg_ms[i] = 0;
+      c->seg_n[i] = 0;
+    }
   return GG_OK;
 }
 

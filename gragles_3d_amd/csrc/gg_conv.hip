@@ -326,31 +326,35 @@ __global__ __launch_bounds__(kT) void k_circ_z(PlanDev P, int cls, DescDev dout,
   }
 }
 
+// part 0: (x,y) forward / x-then-y circulant; part 1: z; part 2: (y,x) inverse (FP64 only)
 template <int NMAX>
-void conv_class(const Launch& L, const PlanDev& P, int cls, const DescDev& din, const DescDev& dout,
+void conv_class(const Launch& L, int part, const PlanDev& P, int cls, const DescDev& din, const DescDev& dout,
                 const TablesDev& tb, const int64_t* spec_off, void* spec, bool fp32) {
   if (fp32) {
-    hipLaunchKernelGGL(k_circ_xy<NMAX>, dim3(L.grid), dim3(kT), 0, L.stream, P, cls, din, dout, tb);
-    hipLaunchKernelGGL(k_circ_z<NMAX>, dim3(L.grid), dim3(kT), 0, L.stream, P, cls, dout, tb);
+    if (part == 0) hipLaunchKernelGGL(k_circ_xy<NMAX>, dim3(L.grid), dim3(kT), 0, L.stream, P, cls, din, dout, tb);
+    if (part == 1) hipLaunchKernelGGL(k_circ_z<NMAX>, dim3(L.grid), dim3(kT), 0, L.stream, P, cls, dout, tb);
   } else {
-    hipLaunchKernelGGL(k_dft_xy_fwd<NMAX>, dim3(L.grid), dim3(kT), 0, L.stream, P, cls, din, tb, spec_off,
-                       (double2*)spec);
-    hipLaunchKernelGGL(k_dft_z<NMAX>, dim3(L.grid), dim3(kT), 0, L.stream, P, cls, dout, tb, spec_off,
-                       (double2*)spec);
-    hipLaunchKernelGGL(k_dft_xy_inv<NMAX>, dim3(L.grid), dim3(kT), 0, L.stream, P, cls, dout, tb, spec_off,
-                       (const double2*)spec);
+    if (part == 0)
+      hipLaunchKernelGGL(k_dft_xy_fwd<NMAX>, dim3(L.grid), dim3(kT), 0, L.stream, P, cls, din, tb, spec_off,
+                         (double2*)spec);
+    if (part == 1)
+      hipLaunchKernelGGL(k_dft_z<NMAX>, dim3(L.grid), dim3(kT), 0, L.stream, P, cls, dout, tb, spec_off,
+                         (double2*)spec);
+    if (part == 2)
+      hipLaunchKernelGGL(k_dft_xy_inv<NMAX>, dim3(L.grid), dim3(kT), 0, L.stream, P, cls, dout, tb, spec_off,
+                         (const double2*)spec);
   }
 }
 
 }  // namespace
 
-void launch_conv(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, const DescDev& dout,
-                 const TablesDev& tb, const int64_t* spec_off, void* spec, bool fp32) {
+void launch_conv(const Launch& L, int part, const SimParams& sp, const PlanDev& P, const DescDev& din,
+                 const DescDev& dout, const TablesDev& tb, const int64_t* spec_off, void* spec, bool fp32) {
   (void)sp;
-  conv_class<32>(L, P, 0, din, dout, tb, spec_off, spec, fp32);
-  conv_class<48>(L, P, 1, din, dout, tb, spec_off, spec, fp32);
-  conv_class<64>(L, P, 2, din, dout, tb, spec_off, spec, fp32);
-  conv_class<96>(L, P, 3, din, dout, tb, spec_off, spec, fp32);
+  conv_class<32>(L, part, P, 0, din, dout, tb, spec_off, spec, fp32);
+  conv_class<48>(L, part, P, 1, din, dout, tb, spec_off, spec, fp32);
+  conv_class<64>(L, part, P, 2, din, dout, tb, spec_off, spec, fp32);
+  conv_class<96>(L, part, P, 3, din, dout, tb, spec_off, spec, fp32);
 }
 
 }  // namespace gg

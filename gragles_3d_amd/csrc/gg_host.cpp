@@ -89,6 +89,13 @@ int gg_derive_params(int32_t n_grains, int32_t points_per_grain, int32_t border,
 
 void gg_resize_to_cube(int32_t* bounds6, int32_t ngridpoints) { cube_bounds(bounds6, ngridpoints); }
 
+int gg_voronoi_seeds(int32_t n_grains, uint64_t seed, double* xyz) {
+  if (n_grains < 1 || !xyz) return GG_ERR_ARG;
+  SplitMix64 rng{seed};
+  for (int i = 0; i < 3 * n_grains; ++i) xyz[i] = rng.uniform();
+  return GG_OK;
+}
+
 int gg_generate_voronoi(const gg_params* p, uint64_t seed, int32_t* idfield, int32_t* bounds, uint8_t* exists,
                         int32_t nthreads) {
   if (!p || !bounds || !exists) return GG_ERR_ARG;

@@ -151,7 +151,8 @@ void launch_plan(const Launch& L, const SimParams& sp, const int32_t* state, con
                  int64_t spec_cap, int32_t* err, int64_t* voxels);
 void launch_init_phi(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din,
                      const int32_t* idfield, bool fp32);
-void launch_conv(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din,
+// part 0: (x,y) forward DFT | x,y circulant; part 1: z; part 2: (y,x) inverse DFT (FP64 only)
+void launch_conv(const Launch& L, int part, const SimParams& sp, const PlanDev& P, const DescDev& din,
                  const DescDev& dout, const TablesDev& tb, const int64_t* spec_off, void* spec, bool fp32);
 void launch_idl_resize(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& dout,
                        const IdlDev& il);
@@ -165,8 +166,9 @@ void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const D
 void launch_finalize(const Launch& L, const SimParams& sp, const PlanDev& P, int32_t* state,
                      const ListDev& ld, int32_t* exflag, const double* partials, const int32_t* newb,
                      const DescDev& din, const DescDev& dout, double* volume, double* area);
-void launch_redist(const Launch& L, const SimParams& sp, const PlanDev& P, const int32_t* state,
-                   const DescDev& din, const DescDev& dout, int32_t* err, bool fp32);
+// part 0: x/y sweeps; part 1: z sweeps and clamp
+void launch_redist(const Launch& L, int part, const SimParams& sp, const PlanDev& P, const DescDev& din,
+                   const DescDev& dout, int32_t* err, bool fp32);
 void launch_post_redist(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& din,
                         const DescDev& dout);
 void launch_count(const Launch& L, const SimParams& sp, const int32_t* state, int32_t* step,

@@ -221,23 +221,26 @@ void xy_class(const Launch& L, const SimParams& sp, const PlanDev& P, int cls, c
   hipLaunchKernelGGL((k_redist_xy<T, NMAX>), dim3(L.grid), dim3(128), 0, L.stream, sp, P, cls, din, dout, err);
 }
 
+// part 0: x/y sweeps per layer (one launch per size class); part 1: z sweeps + clamp
 template <class T>
-void redist_all(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, const DescDev& dout,
-                int32_t* err) {
-  xy_class<T, 32>(L, sp, P, 0, din, dout, err);
-  xy_class<T, 48>(L, sp, P, 1, din, dout, err);
-  xy_class<T, 64>(L, sp, P, 2, din, dout, err);
-  xy_class<T, 96>(L, sp, P, 3, din, dout, err);
-  hipLaunchKernelGGL(k_redist_z<T>, dim3(L.grid), dim3(64), 0, L.stream, sp, P, din, dout);
+void redist_all(const Launch& L, int part, const SimParams& sp, const PlanDev& P, const DescDev& din,
+                const DescDev& dout, int32_t* err) {
+  if (part == 0) {
+    xy_class<T, 32>(L, sp, P, 0, din, dout, err);
+    xy_class<T, 48>(L, sp, P, 1, din, dout, err);
+    xy_class<T, 64>(L, sp, P, 2, din, dout, err);
+    xy_class<T, 96>(L, sp, P, 3, din, dout, err);
+  } else {
+    hipLaunchKernelGGL(k_redist_z<T>, dim3(L.grid), dim3(64), 0, L.stream, sp, P, din, dout);
+  }
 }
 
 }  // namespace
 
-void launch_redist(const Launch& L, const SimParams& sp, const PlanDev& P, const int32_t* state, const DescDev& din,
+void launch_redist(const Launch& L, int part, const SimParams& sp, const PlanDev& P, const DescDev& din,
                    const DescDev& dout, int32_t* err, bool fp32) {
-  (void)state;
-  if (fp32) redist_all<float>(L, sp, P, din, dout, err);
-  else redist_all<double>(L, sp, P, din, dout, err);
+  if (fp32) redist_all<float>(L, part, sp, P, din, dout, err);
+  else redist_all<double>(L, part, sp, P, din, dout, err);
 }
 
 }  // namespace gg

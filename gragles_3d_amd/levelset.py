@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(HERE, "libgg_levelset.so")
 GG_OK, GG_ERR_ARG, GG_ERR_HIP, GG_ERR_CAPACITY, GG_ERR_STATE, GG_ERR_DEVICE = range(6)
 FP64_PLAIN, FP32_BATCHED = 0, 1
 DELETED, ALIVE, VANISHED = 0, 1, 2
-STAGES = ("convolution", "lists", "comparison", "level_set", "redistancing", "count")
+SEGMENTS = ("conv_xy", "conv_z", "conv_xy_inv", "idl_resize", "lists", "comparison", "hull", "finalize", "plan",
+            "redist_xy", "redist_z", "post_redist", "count")
 
 
 class GGParams(ctypes.Structure):
@@ -78,10 +79,11 @@ SIGNATURES = {
     "gg_get_new_bounds": (ctypes.c_int, [_vp, _i32p]),
     "gg_box_voxels": (ctypes.c_int, [_vp, _i64p]),
     "gg_enable_stage_timing": (ctypes.c_int, [_vp, _i32]),
-    "gg_get_stage_times": (ctypes.c_int, [_vp, _f64p, _i32]),
+    "gg_get_stage_times": (ctypes.c_int, [_vp, _f64p, _i64p, _i32, _i32]),
     "gg_derive_params": (ctypes.c_int, [_i32, _i32, _i32, ctypes.c_double, _P(GGParams)]),
     "gg_resize_to_cube": (None, [_i32p, _i32]),
     "gg_generate_voronoi": (ctypes.c_int, [_P(GGParams), ctypes.c_uint64, _i32p, _i32p, _u8p, _i32]),
+    "gg_voronoi_seeds": (ctypes.c_int, [_i32, ctypes.c_uint64, _f64p]),
     "gg_initial_neighbour_lists": (ctypes.c_int, [_P(GGParams), _i32p, _u8p, _i32p, _i32p, ctypes.c_int64,
                                                   _i64p]),
 }
@@ -135,6 +137,14 @@ def generate_voronoi(params, seed, with_idfield=True, nthreads=0):
     if rc != GG_OK:
         raise GGError("gg_generate_voronoi failed (%d)" % rc)
     return idf, bounds, exists
+
+
+def voronoi_seeds(n_grains, seed):
+    xyz = np.zeros((n_grains, 3))
+    rc = lib().gg_voronoi_seeds(int(n_grains), ctypes.c_uint64(seed), _p(xyz, ctypes.c_double))
+    if rc != GG_OK:
+        raise GGError("gg_voronoi_seeds failed (%d)" % rc)
+    return xyz
 
 
 def initial_neighbour_lists(params, bounds, exists):
@@ -302,9 +312,12 @@ class LevelSetContext:
         self._chk(lib().gg_enable_stage_timing(self.h, 1 if on else 0))
 
     def stage_times(self, reset=True):
-        ms = np.zeros(6)
-        self._chk(lib().gg_get_stage_times(self.h, _p(ms, ctypes.c_double), 1 if reset else 0))
-        return dict(zip(STAGES, ms.tolist()))
+        """{segment: (total ms, timed passes)} since the last reset."""
+        ms = np.zeros(len(SEGMENTS))
+        n = np.zeros(len(SEGMENTS), dtype=np.int64)
+        self._chk(lib().gg_get_stage_times(self.h, _p(ms, ctypes.c_double), _p(n, ctypes.c_int64), len(SEGMENTS),
+                                           1 if reset else 0))
+        return {k: (float(a), int(b)) for k, a, b in zip(SEGMENTS, ms, n)}
 
 
 def setup_simulation(n_grains, points_per_grain, border, seed, precision=FP64_PLAIN, time_step_size=0.2,

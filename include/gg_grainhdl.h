@@ -44,6 +44,10 @@ void gg_resize_to_cube(int32_t* bounds6, int32_t ngridpoints);
 int gg_generate_voronoi(const gg_params* params, uint64_t seed, int32_t* idfield, int32_t* bounds,
                         uint8_t* exists, int32_t nthreads);
 
+/* The n_grains seed points gg_generate_voronoi uses for `seed`, as
+ * xyz[3*i .. 3*i+2] for grain id i+1. */
+int gg_voronoi_seeds(int32_t n_grains, uint64_t seed, double* xyz);
+
 /* Initial neighbour lists, as grainhdl::find_neighbors builds them with an
  * R-tree over the box bounds (src/grainhdl.cpp:1029-1054,
  * src/box.cpp:2871-2887): for every existing grain, the other existing grains

@@ -144,12 +144,19 @@ int gg_get_new_bounds(gg_ctx* ctx, int32_t* bounds);
 /* Sum of n^3 over the boxes of alive grains (the voxel-updates of one step). */
 int gg_box_voxels(gg_ctx* ctx, int64_t* voxels);
 
-/* Kernel timing of the dominant kernels: enable HIP-event timing of each
- * stage on the context stream (adds an event pair around each stage). */
+/* HIP-event timing on the context stream: with timing enabled every kernel
+ * segment below is bracketed by an event pair (and gg_timestep runs eagerly,
+ * without the graph).  Segments:
+ *  0 convolution (x,y) forward DFT | x,y circulant   1 convolution z
+ *  2 convolution (y,x) inverse DFT (FP64)            3 IDLocal resize
+ *  4 comparison lists      5 comparison              6 hull reductions
+ *  7 level-set finalize    8 work plan / layout      9 redistancing x/y sweeps
+ * 10 redistancing z sweeps + clamp   11 input-buffer resize   12 grain count */
 int gg_enable_stage_timing(gg_ctx* ctx, int32_t enable);
-/* Accumulated milliseconds per stage since the last reset:
- * [0] convolution [1] lists [2] comparison [3] level set [4] redistancing [5] count. */
-int gg_get_stage_times(gg_ctx* ctx, double* ms6, int32_t reset);
+#define GG_NUM_SEGMENTS 13
+/* Accumulated milliseconds and number of timed passes per segment since the
+ * last reset; fills up to n entries (launches may be NULL). */
+int gg_get_stage_times(gg_ctx* ctx, double* ms, int64_t* launches, int32_t n, int32_t reset);
 
 #ifdef __cplusplus
 }
