@@ -135,6 +135,26 @@ def main():
     log("fp32: %d steps %.3fs -> %.3f ms/step, %.3e voxel-updates/s (boxes %.3e voxels)" %
         (args.steps, wall32, 1e3 * wall32 / args.steps, value, vox_step32))
     kbar, _ = kbar_and_voxels(ctx32, p.n_grains)
+
+    # ---- FP64 plain, same steps; then the FP32 error against it ------------------
+    fp64 = None
+    err = None
+    if not args.no_fp64:
+        ctx64, wall64, vb64, va64 = run_path(G.FP64_PLAIN, inputs, p, args.steps, args.warmup)
+        vox_step64 = 0.5 * (vb64 + va64)
+        v64 = vox_step64 * args.steps / wall64
+        log("fp64: %d steps %.3fs -> %.3f ms/step, %.3e voxel-updates/s" % (args.steps, wall64,
+                                                                          1e3 * wall64 / args.steps, v64))
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from fp32_error import phi_volume_errors, trajectory_error
+        err = phi_volume_errors(ctx32, ctx64, p.delta, grains=np.arange(1, p.n_grains + 1, 16))
+        err.update(trajectory_error(ctx32.count_history(), ctx64.count_history()))
+        err["grains_sampled"] = "every 16th grain id"
+        seg64 = kernel_pass(ctx64, max(1, args.kernel_steps // 2))
+        ctx64.close()
+        fp64 = {"value": v64, "ms_per_step": 1e3 * wall64 / args.steps,
+                "segment_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in seg64.items() if v[1]}}
+
     seg32 = kernel_pass(ctx32, args.kernel_steps)
     vox_k = ctx32.box_voxels()
     counts32 = ctx32.count_history()
@@ -183,18 +203,10 @@ def main():
         "grain_count_trajectory_fp32": counts32[-3:].tolist(),
     }
 
-    if not args.no_fp64:
-        ctx64, wall64, vb64, va64 = run_path(G.FP64_PLAIN, inputs, p, args.steps, args.warmup)
-        vox_step64 = 0.5 * (vb64 + va64)
-        v64 = vox_step64 * args.steps / wall64
-        seg64 = kernel_pass(ctx64, max(1, args.kernel_steps // 2))
-        ctx64.close()
-        log("fp64: %d steps %.3fs -> %.3f ms/step, %.3e voxel-updates/s" % (args.steps, wall64,
-                                                                          1e3 * wall64 / args.steps, v64))
-        out["fp64_plain"] = {"value": v64, "ms_per_step": 1e3 * wall64 / args.steps,
-                             "segment_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in seg64.items()
-                                                     if v[1]}}
-        out["fp32_speedup_vs_fp64"] = value / v64
+    if fp64 is not None:
+        out["fp64_plain"] = fp64
+        out["fp32_speedup_vs_fp64"] = value / fp64["value"]
+        out["fp32_error_vs_fp64"] = err
 
     if not args.no_cpu:
         try:

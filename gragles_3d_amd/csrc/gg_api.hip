@@ -622,6 +622,7 @@ int gg_synchronize(gg_ctx* c) {
   if (!c) return GG_ERR_ARG;
   hipSetDevice(c->p.device);
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipGetLastError());  // a kernel launch that failed (e.g. resources) since the last check
   if (c->timing) collect_timing(c);
   if (!c->boxes_set) return GG_OK;
   return check_device_error(c);

@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kT) void k_dft_xy_fwd(PlanDev P, int cls, DescDev d
     const int to = tb.tw_off[n];
     for (int i = tid; i < n; i += kT) { tc[i] = tb.twc[to + i]; ts[i] = tb.tws[to + i]; }
     const double* src = arena + din.off[g] + (int64_t)z * n * n;
-    for (int i = tid; i < n * n; i += kT) L[i] = src[i];
+    stage_to_lds(L, src, n * n, tid, kT);
     __syncthreads();
     double re[R], im[R];
 #pragma unroll
@@ -114,10 +114,7 @@ __global__ __launch_bounds__(kT) void k_dft_z(PlanDev P, int cls, DescDev dref, 
     const double* G = tb.gtab + tb.g_off[n];
     for (int i = tid; i < n; i += kT) { tc[i] = tb.twc[to + i]; ts[i] = tb.tws[to + i]; }
     double2* base = spec + spec_off[g];
-    for (int e = tid; e < n * CB; e += kT) {
-      const int z = e / CB, c = c0 + (e - z * CB);
-      B[e] = c < nc ? base[(int64_t)z * nc + c] : make_double2(0.0, 0.0);
-    }
+    stage_rows_to_lds(B, CB, base + c0, nc, n, min(CB, nc - c0), tid, kT);  // columns past nc are never read
     __syncthreads();
     double fr[R], fi[R];
 #pragma unroll
@@ -176,7 +173,7 @@ __global__ __launch_bounds__(kT) void k_dft_xy_inv(PlanDev P, int cls, DescDev d
                                                    const int64_t* __restrict__ spec_off, const double2* spec) {
   constexpr int R = (NMAX * (NMAX / 2 + 1) + kT - 1) / kT;
   constexpr int R3 = (NMAX * NMAX + kT - 1) / kT;
-  __shared__ double L[NMAX * (NMAX + 2)];
+  __shared__ __align__(16) double L[NMAX * (NMAX + 2)];
   __shared__ double tc[NMAX], ts[NMAX];
   const int total = P.totals[kQLayers * kClasses + cls];
   double* arena = (double*)dout.arena;
@@ -188,11 +185,7 @@ __global__ __launch_bounds__(kT) void k_dft_xy_inv(PlanDev P, int cls, DescDev d
     const int to = tb.tw_off[n];
     for (int i = tid; i < n; i += kT) { tc[i] = tb.twc[to + i]; ts[i] = tb.tws[to + i]; }
     const double2* src = spec + spec_off[g] + (int64_t)z * nc;
-    for (int i = tid; i < nc; i += kT) {
-      const double2 v = src[i];
-      L[2 * i] = v.x;
-      L[2 * i + 1] = v.y;
-    }
+    stage_to_lds((double2*)L, src, nc, tid, kT);
     __syncthreads();
     double re[R], im[R];
 #pragma unroll
@@ -262,7 +255,7 @@ __global__ __launch_bounds__(kT) void k_circ_xy(PlanDev P, int cls, DescDev din,
     const int n = box_n(din.bounds, g), nn = n * n;
     for (int i = tid; i < n; i += kT) cr[i] = tb.circ[tb.tw_off[n] + i];
     const float* src = ain + din.off[g] + (int64_t)z * nn;
-    for (int i = tid; i < nn; i += kT) A[i] = src[i];
+    stage_to_lds(A, src, nn, tid, kT);
     __syncthreads();
     for (int o = tid; o < nn; o += kT) {
       const int y = o / n, x = o - y * n;
@@ -306,10 +299,7 @@ __global__ __launch_bounds__(kT) void k_circ_z(PlanDev P, int cls, DescDev dout,
     const int c0 = blk * CB;
     for (int i = tid; i < n; i += kT) cr[i] = tb.circ[tb.tw_off[n] + i];
     float* base = aout + dout.off[g];
-    for (int e = tid; e < n * CB; e += kT) {
-      const int z = e / CB, c = c0 + (e - z * CB);
-      Lz[e] = c < nn ? base[(int64_t)z * nn + c] : 0.f;
-    }
+    stage_rows_to_lds(Lz, CB, base + c0, nn, n, min(CB, nn - c0), tid, kT);  // columns past nn are never read
     __syncthreads();
     for (int o = tid; o < n * CB; o += kT) {
       const int z = o / CB, cb = o - z * CB, c = c0 + cb;

@@ -46,100 +46,138 @@ struct Acc {
 };
 
 constexpr int kHT = 256;
+constexpr int kChunk = 4 * kHT;  // cells classified per pass
 
-template <class T>
-__global__ __launch_bounds__(kHT) void k_hull(SimParams sp, PlanDev P, DescDev din, int32_t* exflag,
+// VertexInterp (src/marchingCubes.cpp:484-499) at iso level 0 on cube edge e of
+// the cell with lower corner (x,y,z).
+__device__ inline void edge_vertex(int e, const double* val, int x, int y, int z, double& px, double& py,
+                                   double& pz) {
+  const int a = c_edge[e][0], b = c_edge[e][1];
+  const double p1x = x + c_corner[a][0], p1y = y + c_corner[a][1], p1z = z + c_corner[a][2];
+  const double p2x = x + c_corner[b][0], p2y = y + c_corner[b][1], p2z = z + c_corner[b][2];
+  const double v1 = val[a], v2 = val[b];
+  if (fabs(v1) < 0.00001) {
+    px = p1x; py = p1y; pz = p1z;
+  } else if (fabs(v2) < 0.00001) {
+    px = p2x; py = p2y; pz = p2z;
+  } else if (fabs(v1 - v2) < 0.00001) {
+    px = p1x; py = p1y; pz = p1z;
+  } else {
+    const double mu = v1 / (v1 - v2);
+    px = p1x + mu * (p2x - p1x);
+    py = p1y + mu * (p2y - p1y);
+    pz = p1z + mu * (p2z - p1z);
+  }
+}
+
+// One (grain, z-layer) tile: layers kl and kl+1 are staged in LDS; the cells of
+// the layer are classified 1024 at a time, the cut ones (some corners inside,
+// some outside) compacted in a fixed order, and then every lane works on cut
+// cells, so the triangle work is not left to a few lanes of each wavefront.
+template <class T, int NMAX>
+__global__ __launch_bounds__(kHT) void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag,
                                               double* partials) {
+  __shared__ T L0[NMAX * NMAX];
+  __shared__ T L1[NMAX * NMAX];
+  __shared__ uint32_t q[kChunk];
+  __shared__ int s_wave[kHT / 64];
   __shared__ double s_red[4][kHT / 64];
   __shared__ int s_any[kHT / 64];
-  const int total = plan_total(P, kQLayers);
+  const int total = P.totals[kQLayers * kClasses + cls];
   const T* arena = (const T*)din.arena;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
-    int c, g, kl;
-    plan_tile_any(P, kQLayers, tile, c, g, kl);
-    // slot of this tile: class base + tile index inside the class
-    int tin = tile;
-    for (int cc = 0; cc < c; ++cc) tin -= P.totals[kQLayers * kClasses + cc];
-    const int64_t slot = P.layer_base[c] + tin;
+    int g, kl;
+    plan_tile(P, cls, kQLayers, tile, g, kl);
+    const int64_t slot = P.layer_base[cls] + tile;
     const int32_t* b = din.bounds + 6 * g;
     const int x0 = b[0], y0 = b[1], z0 = b[2];
-    const int n = b[3] - b[0];
-    const T* box = arena + din.off[g];
+    const int n = b[3] - b[0], nn2 = n * n, m1 = n - 1;
+    const int z = z0 + kl;
     Acc vol{0.0, 0.0}, area{0.0, 0.0};
     int any = 0;
-    if (kl < n - 1) {
-      const int z = z0 + kl;
-      const int m1 = n - 1;
-      for (int v = tid; v < m1 * m1; v += kHT) {
-        const int yl = v / m1, xl = v - yl * m1;
-        double val[8];
-        unsigned ci = 0;
+    if (kl < m1) {
+      const T* box = arena + din.off[g] + (int64_t)kl * nn2;
+      stage_to_lds(L0, box, nn2, tid, kHT);
+      stage_to_lds(L1, box + nn2, nn2, tid, kHT);
+      __syncthreads();
+      for (int c0 = 0; c0 < m1 * m1; c0 += kChunk) {
+        int nq = 0;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          val[q] = (double)box[lin3(n, kl + c_corner[q][2], yl + c_corner[q][1], xl + c_corner[q][0])];
-          if (val[q] >= 0) ci |= 1u << q;
+        for (int r = 0; r < kChunk / kHT; ++r) {
+          const int c = c0 + r * kHT + tid;
+          uint32_t code = 0;
+          bool cut = false;
+          if (c < m1 * m1) {
+            const int yl = c / m1, xl = c - yl * m1;
+            unsigned ci = 0;
+#pragma unroll
+            for (int qq = 0; qq < 8; ++qq) {
+              const int o = (yl + c_corner[qq][1]) * n + xl + c_corner[qq][0];
+              const T v = c_corner[qq][2] ? L1[o] : L0[o];
+              if (v >= T(0)) ci |= 1u << qq;
+            }
+            cut = ci != 0u && ci != 255u;
+            code = ci | ((uint32_t)yl << 8) | ((uint32_t)xl << 16);
+          }
+          const uint64_t m = __ballot(cut);
+          if (lane == 0) s_wave[wv] = __popcll(m);
+          __syncthreads();
+          int pos = nq;
+          for (int w = 0; w < wv; ++w) pos += s_wave[w];
+          pos += __popcll(m & ((1ull << lane) - 1ull));
+          if (cut) q[pos] = code;
+          for (int w = 0; w < kHT / 64; ++w) nq += s_wave[w];
+          __syncthreads();
         }
-        const int mask = c_emask[ci];
-        if (mask == 0) continue;
-        any = 1;
-        const int x = x0 + xl, y = y0 + yl;
-        double vx[12], vy[12], vz[12];
+        if (nq) any = 1;
+        for (int w = tid; w < nq; w += kHT) {
+          const uint32_t code = q[w];
+          const unsigned ci = code & 0xFFu;
+          const int yl = (code >> 8) & 0xFF, xl = code >> 16;
+          double val[8];
 #pragma unroll
-        for (int e = 0; e < 12; ++e) {
-          if (!(mask & (1 << e))) continue;
-          const int a = c_edge[e][0], bq = c_edge[e][1];
-          const double p1x = x + c_corner[a][0], p1y = y + c_corner[a][1], p1z = z + c_corner[a][2];
-          const double p2x = x + c_corner[bq][0], p2y = y + c_corner[bq][1], p2z = z + c_corner[bq][2];
-          const double v1 = val[a], v2 = val[bq];
-          if (fabs(v1) < 0.00001) {
-            vx[e] = p1x; vy[e] = p1y; vz[e] = p1z;
-          } else if (fabs(v2) < 0.00001) {
-            vx[e] = p2x; vy[e] = p2y; vz[e] = p2z;
-          } else if (fabs(v1 - v2) < 0.00001) {
-            vx[e] = p1x; vy[e] = p1y; vz[e] = p1z;
-          } else {
-            const double mu = v1 / (v1 - v2);
-            vx[e] = p1x + mu * (p2x - p1x);
-            vy[e] = p1y + mu * (p2y - p1y);
-            vz[e] = p1z + mu * (p2z - p1z);
+          for (int qq = 0; qq < 8; ++qq) {
+            const int o = (yl + c_corner[qq][1]) * n + xl + c_corner[qq][0];
+            val[qq] = (double)(c_corner[qq][2] ? L1[o] : L0[o]);
+          }
+          const int x = x0 + xl, y = y0 + yl;
+          for (int t = 0; t < 16 && c_tri[ci][t] >= 0; t += 3) {
+            double p0x, p0y, p0z, p1x, p1y, p1z, p2x, p2y, p2z;
+            edge_vertex(c_tri[ci][t], val, x, y, z, p0x, p0y, p0z);
+            edge_vertex(c_tri[ci][t + 1], val, x, y, z, p1x, p1y, p1z);
+            edge_vertex(c_tri[ci][t + 2], val, x, y, z, p2x, p2y, p2z);
+            const double v321 = p2x * p1y * p0z;
+            const double v231 = p1x * p2y * p0z;
+            const double v312 = p2x * p0y * p1z;
+            const double v132 = p0x * p2y * p1z;
+            const double v213 = p1x * p0y * p2z;
+            const double v123 = p0x * p1y * p2z;
+            vol.add((double)(1.0f / 6.0f) * (-v321 + v231 + v312 - v132 - v213 + v123));
+            const double abx = p0x - p1x, aby = p0y - p1y, abz = p0z - p1z;
+            const double acx = p0x - p2x, acy = p0y - p2y, acz = p0z - p2z;
+            const double cx = aby * acz - abz * acy;
+            const double cy = abz * acx - abx * acz;
+            const double cz = abx * acy - aby * acx;
+            area.add(sqrt(cx * cx + cy * cy + cz * cz) / 2.0);
           }
         }
-        for (int t = 0; t < 16 && c_tri[ci][t] >= 0; t += 3) {
-          const int e0 = c_tri[ci][t], e1 = c_tri[ci][t + 1], e2 = c_tri[ci][t + 2];
-          const double p0x = vx[e0], p0y = vy[e0], p0z = vz[e0];
-          const double p1x = vx[e1], p1y = vy[e1], p1z = vz[e1];
-          const double p2x = vx[e2], p2y = vy[e2], p2z = vz[e2];
-          const double v321 = p2x * p1y * p0z;
-          const double v231 = p1x * p2y * p0z;
-          const double v312 = p2x * p0y * p1z;
-          const double v132 = p0x * p2y * p1z;
-          const double v213 = p1x * p0y * p2z;
-          const double v123 = p0x * p1y * p2z;
-          vol.add((double)(1.0f / 6.0f) * (-v321 + v231 + v312 - v132 - v213 + v123));
-          const double abx = p0x - p1x, aby = p0y - p1y, abz = p0z - p1z;
-          const double acx = p0x - p2x, acy = p0y - p2y, acz = p0z - p2z;
-          const double cx = aby * acz - abz * acy;
-          const double cy = abz * acx - abx * acz;
-          const double cz = abx * acy - aby * acx;
-          area.add(sqrt(cx * cx + cy * cy + cz * cz) / 2.0);
-        }
+        __syncthreads();
       }
     }
     // wavefront then workgroup reduction in a fixed order
-    double r[4] = {vol.s, vol.c, area.s, area.c};
+    double r4[4] = {vol.s, vol.c, area.s, area.c};
     for (int off = 32; off > 0; off >>= 1) {
-      double s2 = __shfl_down(r[0], off), c2 = __shfl_down(r[1], off);
-      double a2 = __shfl_down(r[2], off), d2 = __shfl_down(r[3], off);
-      Acc va{r[0], r[1]}, aa{r[2], r[3]};
+      double s2 = __shfl_down(r4[0], off), c2 = __shfl_down(r4[1], off);
+      double a2 = __shfl_down(r4[2], off), d2 = __shfl_down(r4[3], off);
+      Acc va{r4[0], r4[1]}, aa{r4[2], r4[3]};
       va.merge(s2, c2);
       aa.merge(a2, d2);
-      r[0] = va.s; r[1] = va.c; r[2] = aa.s; r[3] = aa.c;
+      r4[0] = va.s; r4[1] = va.c; r4[2] = aa.s; r4[3] = aa.c;
     }
-    const uint64_t anym = __ballot(any);
     if (lane == 0) {
-      for (int q = 0; q < 4; ++q) s_red[q][wv] = r[q];
-      s_any[wv] = anym != 0ull;
+      for (int qq = 0; qq < 4; ++qq) s_red[qq][wv] = r4[qq];
+      s_any[wv] = any;
     }
     __syncthreads();
     if (tid == 0) {
@@ -245,10 +283,17 @@ void upload_mc_tables() {
 
 void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, int32_t* exflag,
                  double* partials, bool fp32) {
-  if (fp32)
-    hipLaunchKernelGGL(k_hull<float>, dim3(L.grid), dim3(kHT), 0, L.stream, sp, P, din, exflag, partials);
-  else
-    hipLaunchKernelGGL(k_hull<double>, dim3(L.grid), dim3(kHT), 0, L.stream, sp, P, din, exflag, partials);
+  if (fp32) {
+    hipLaunchKernelGGL((k_hull<float, 32>), dim3(L.grid), dim3(kHT), 0, L.stream, sp, P, 0, din, exflag, partials);
+    hipLaunchKernelGGL((k_hull<float, 48>), dim3(L.grid), dim3(kHT), 0, L.stream, sp, P, 1, din, exflag, partials);
+    hipLaunchKernelGGL((k_hull<float, 64>), dim3(L.grid), dim3(kHT), 0, L.stream, sp, P, 2, din, exflag, partials);
+    hipLaunchKernelGGL((k_hull<float, 96>), dim3(L.grid), dim3(kHT), 0, L.stream, sp, P, 3, din, exflag, partials);
+  } else {
+    hipLaunchKernelGGL((k_hull<double, 32>), dim3(L.grid), dim3(kHT), 0, L.stream, sp, P, 0, din, exflag, partials);
+    hipLaunchKernelGGL((k_hull<double, 48>), dim3(L.grid), dim3(kHT), 0, L.stream, sp, P, 1, din, exflag, partials);
+    hipLaunchKernelGGL((k_hull<double, 64>), dim3(L.grid), dim3(kHT), 0, L.stream, sp, P, 2, din, exflag, partials);
+    hipLaunchKernelGGL((k_hull<double, 96>), dim3(L.grid), dim3(kHT), 0, L.stream, sp, P, 3, din, exflag, partials);
+  }
 }
 
 void launch_finalize(const Launch& L, const SimParams& sp, const PlanDev& P, int32_t* state, const ListDev& ld,

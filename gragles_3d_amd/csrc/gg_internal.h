@@ -134,6 +134,54 @@ __device__ inline int plan_total(const PlanDev& P, int q) {
 template <class T>
 __device__ inline int sgnv(T v) { return (T(0) < v) - (v < T(0)); }
 
+// Copy count elements from global memory to LDS with kStageBatch loads in
+// flight per lane before their LDS writes (a plain element loop waits for each
+// load before its write).
+constexpr int kStageBatch = 8;
+template <class TD, class TS>
+__device__ inline void stage_to_lds(TD* __restrict__ lds, const TS* __restrict__ src, int count, int tid, int nt) {
+  for (int base = 0; base < count; base += nt * kStageBatch) {
+    TS v[kStageBatch];
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      const int e = base + u * nt + tid;
+      if (e < count) v[u] = src[e];
+    }
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      const int e = base + u * nt + tid;
+      if (e < count) lds[e] = (TD)v[u];
+    }
+  }
+}
+
+// Same for a rows x width block whose rows are `stride` elements apart in
+// global memory; element (r, c) goes to lds[r * lds_stride + c].
+template <class TD, class TS>
+__device__ inline void stage_rows_to_lds(TD* __restrict__ lds, int lds_stride, const TS* __restrict__ src,
+                                         int64_t stride, int rows, int width, int tid, int nt) {
+  const int count = rows * width;
+  for (int base = 0; base < count; base += nt * kStageBatch) {
+    TS v[kStageBatch];
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      const int e = base + u * nt + tid;
+      if (e < count) {
+        const int r = e / width, c = e - r * width;
+        v[u] = src[(int64_t)r * stride + c];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      const int e = base + u * nt + tid;
+      if (e < count) {
+        const int r = e / width, c = e - r * width;
+        lds[r * lds_stride + c] = (TD)v[u];
+      }
+    }
+  }
+}
+
 __device__ inline int64_t lin3(int n, int z, int y, int x) { return ((int64_t)z * n + y) * n + x; }
 
 }  // namespace gg

@@ -65,10 +65,8 @@ __global__ __launch_bounds__(128) void k_redist_xy(SimParams sp, PlanDev P, int 
     if (do_xy) {
       const int wI = ixmax - ixmin, hI = iymax - iymin;
       const T* src = ain + din.off[g] + (int64_t)(k - ib[2]) * ni * ni;
-      for (int e = tid; e < wI * hI; e += nt) {
-        const int i = iymin + e / wI, j = ixmin + e % wI;
-        I[(i - oy0) * S + (j - ox0)] = src[(int64_t)(i - ib[1]) * ni + (j - ib[0])];
-      }
+      stage_rows_to_lds(I + (iymin - oy0) * S + (ixmin - ox0), S, src + (int64_t)(iymin - ib[1]) * ni + (ixmin - ib[0]),
+                        ni, hI, wI, tid, nt);
     }
     __syncthreads();
     if (do_xy) {
@@ -145,73 +143,102 @@ __global__ __launch_bounds__(128) void k_redist_xy(SimParams sp, PlanDev P, int 
   }
 }
 
-template <class T>
-__global__ __launch_bounds__(64) void k_redist_z(SimParams sp, PlanDev P, DescDev din, DescDev dout) {
-  const int total = plan_total(P, kQCols);
-  const T* ain = (const T*)din.arena;
-  T* aout = (T*)dout.arena;
+// z-forward, z-backward and the clamp for 64 columns (one per lane).  Each lane
+// stages its own out column and the in values it reads in LDS (only its own
+// slots, so no barrier is needed): all global loads of the column are
+// independent and coalesced across the wavefront, the sequential sweep then
+// runs on LDS, and the clamped column is written back once.
+template <class T, int NMAX>
+__global__ __launch_bounds__(64) void k_redist_z(SimParams sp, PlanDev P, int cls, DescDev din, DescDev dout) {
+  __shared__ T O[NMAX * 64];
+  __shared__ T I[NMAX * 64];
+  const int total = P.totals[kQCols * kClasses + cls];
+  const T* __restrict__ ain = (const T*)din.arena;
+  T* __restrict__ aout = (T*)dout.arena;
   const T h = (T)sp.h;
   const T dl = (T)sp.delta;
+  const int lane = threadIdx.x;
+  T* Oc = O + lane;  // Oc[(k - oz0) * 64] = out(i, j, k)
+  T* Ic = I + lane;  // Ic[(k - oz0) * 64] = in(i, j, k) where defined
   for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
-    int c, g, blk;
-    plan_tile_any(P, kQCols, tile, c, g, blk);
+    int g, blk;
+    plan_tile(P, cls, kQCols, tile, g, blk);
     const int32_t* ob = dout.bounds + 6 * g;
     const int32_t* ib = din.bounds + 6 * g;
     const int ox0 = ob[0], oy0 = ob[1], oz0 = ob[2], oz1 = ob[5];
     const int no = ob[3] - ox0;
     const int ni = ib[3] - ib[0];
-    const int col = blk * kColBlock + threadIdx.x;
+    const int col = blk * kColBlock + lane;
     if (col >= no * no) continue;
     const int i = oy0 + col / no, j = ox0 + col % no;
     const int ixmin = max(ib[0], ox0), iymin = max(ib[1], oy0), izmin = max(ib[2], oz0);
     const int ixmax = min(ib[3], ob[3]), iymax = min(ib[4], ob[4]), izmax = min(ib[5], oz1);
     const bool empty = ixmin >= ixmax || iymin >= iymax || izmin >= izmax;
     const int64_t ls = (int64_t)no * no;
-    // out(i, j, k) = aout[ob_i + (k - oz0) * ls], in(i, j, k) = ain[ib_i + (k - iz0) * lsi]
     const int64_t ob_i = dout.off[g] + col;
-    const int iz0 = ib[2];
-#define OUTZ(k) aout[ob_i + (int64_t)((k) - oz0) * ls]
-#define INZ(k) ain[ib_i + (int64_t)((k) - iz0) * lsi]
+    // batches of kStageBatch independent loads before their LDS writes
+    for (int k0 = 0; k0 < no; k0 += kStageBatch) {
+      T v[kStageBatch];
+#pragma unroll
+      for (int u = 0; u < kStageBatch; ++u)
+        if (k0 + u < no) v[u] = aout[ob_i + (k0 + u) * ls];
+#pragma unroll
+      for (int u = 0; u < kStageBatch; ++u)
+        if (k0 + u < no) Oc[(k0 + u) * 64] = v[u];
+    }
     if (!empty && i >= iymin && j >= ixmin) {
       const bool inside = i < iymax && j < ixmax;
-      const int64_t lsi = (int64_t)ni * ni;
-      const int64_t ib_i = din.off[g] + (int64_t)(i - ib[1]) * ni + (j - ib[0]);
+      if (inside) {
+        const int64_t lsi = (int64_t)ni * ni;
+        const int64_t ib_i = din.off[g] + (int64_t)(i - ib[1]) * ni + (j - ib[0]);
+        for (int k0 = izmin; k0 < izmax; k0 += kStageBatch) {
+          T v[kStageBatch];
+#pragma unroll
+          for (int u = 0; u < kStageBatch; ++u)
+            if (k0 + u < izmax) v[u] = ain[ib_i + (int64_t)(k0 + u - ib[2]) * lsi];
+#pragma unroll
+          for (int u = 0; u < kStageBatch; ++u)
+            if (k0 + u < izmax) Ic[(k0 + u - oz0) * 64] = v[u];
+        }
+      }
+#define OZ(k) Oc[((k) - oz0) * 64]
+#define IZ(k) Ic[((k) - oz0) * 64]
       if (izmin < oz1 - 1) {
-        T cur = OUTZ(izmin);
+        T cur = OZ(izmin);
         for (int k = izmin; k < oz1 - 1; ++k) {
-          T nxt = OUTZ(k + 1);
+          T nxt = OZ(k + 1);
           T cand;
-          if (k < izmax - 1 && inside) cand = step_interp(INZ(k), INZ(k + 1), cur, h);
+          if (k < izmax - 1 && inside) cand = step_interp(IZ(k), IZ(k + 1), cur, h);
           else cand = cur + ((T)sgnv(nxt) * h);
           if (absv(cand) < absv(nxt)) nxt = cand;
-          OUTZ(k) = cur;
+          OZ(k) = cur;
           cur = nxt;
         }
-        OUTZ(oz1 - 1) = cur;
+        OZ(oz1 - 1) = cur;
       }
       if (izmax - 1 > oz0) {
-        T cur = OUTZ(izmax - 1);
+        T cur = OZ(izmax - 1);
         for (int k = izmax - 1; k > oz0; --k) {
-          T prv = OUTZ(k - 1);
+          T prv = OZ(k - 1);
           T cand;
-          if (k > izmin && inside) cand = step_interp(INZ(k), INZ(k - 1), cur, h);
+          if (k > izmin && inside) cand = step_interp(IZ(k), IZ(k - 1), cur, h);
           else cand = cur + ((T)sgnv(prv) * h);
           if (absv(cand) < absv(prv)) prv = cand;
-          OUTZ(k) = cur;
+          OZ(k) = cur;
           cur = prv;
         }
-        OUTZ(oz0) = cur;
+        OZ(oz0) = cur;
       }
+#undef OZ
+#undef IZ
     }
     // clampValues(-delta, delta) (src/dimensionalBufferReal.h:44-57)
-    for (int k = oz0; k < oz1; ++k) {
-      T v = OUTZ(k);
+    for (int k = 0; k < no; ++k) {
+      T v = Oc[k * 64];
       if (v < -dl) v = -dl;
       else if (v > dl) v = dl;
-      OUTZ(k) = v;
+      aout[ob_i + k * ls] = v;
     }
-#undef OUTZ
-#undef INZ
   }
 }
 
@@ -231,7 +258,10 @@ void redist_all(const Launch& L, int part, const SimParams& sp, const PlanDev& P
     xy_class<T, 64>(L, sp, P, 2, din, dout, err);
     xy_class<T, 96>(L, sp, P, 3, din, dout, err);
   } else {
-    hipLaunchKernelGGL(k_redist_z<T>, dim3(L.grid), dim3(64), 0, L.stream, sp, P, din, dout);
+    hipLaunchKernelGGL((k_redist_z<T, 32>), dim3(L.grid), dim3(64), 0, L.stream, sp, P, 0, din, dout);
+    hipLaunchKernelGGL((k_redist_z<T, 48>), dim3(L.grid), dim3(64), 0, L.stream, sp, P, 1, din, dout);
+    hipLaunchKernelGGL((k_redist_z<T, 64>), dim3(L.grid), dim3(64), 0, L.stream, sp, P, 2, din, dout);
+    hipLaunchKernelGGL((k_redist_z<T, 96>), dim3(L.grid), dim3(64), 0, L.stream, sp, P, 3, din, dout);
   }
 }
 
