@@ -366,7 +366,8 @@ int gg_create(const gg_params* params, gg_ctx** out) {
   c->LD.max_list = c->p.max_list;
   if ((rc = dalloc(c, &c->LD.sel, N + 1)) || (rc = dalloc(c, &c->P.cls_count, kClasses)) ||
       (rc = dalloc(c, &c->P.cls_start, kClasses)) || (rc = dalloc(c, &c->P.grains, N + 1)) ||
-      (rc = dalloc(c, &c->P.grain_pos, N + 1)) || (rc = dalloc(c, &c->P.totals, kNumQ * kClasses)) ||
+      (rc = dalloc(c, &c->P.grain_pos, N + 1)) || (rc = dalloc(c, &c->P.order, N)) ||
+      (rc = dalloc(c, &c->P.totals, kNumQ * kClasses)) ||
       (rc = dalloc(c, &c->P.layer_base, kClasses)) || (rc = dalloc(c, &c->d_partials, (size_t)4 * (N + 1) * kMaxN))) {
     gg_destroy(c);
     return rc;
@@ -448,6 +449,29 @@ int gg_set_boxes(gg_ctx* c, const int32_t* bounds, const uint8_t* exists) {
     if (rc) return rc;
   }
   HIPCHK(c, hipMemcpy(c->d_state, st.data(), 4 * (N + 1), hipMemcpyHostToDevice));
+  {
+    // spatial order: Morton code of the cubed box centres (10 bits per axis)
+    auto spread = [](uint64_t v) {
+      v &= 0x3FF;
+      v = (v | (v << 16)) & 0x30000FFull;
+      v = (v | (v << 8)) & 0x300F00Full;
+      v = (v | (v << 4)) & 0x30C30C3ull;
+      v = (v | (v << 2)) & 0x9249249ull;
+      return v;
+    };
+    std::vector<std::pair<uint64_t, int32_t>> key(N);
+    for (int g = 1; g <= N; ++g) {
+      const int32_t* r = &b[6 * g];
+      uint64_t cx = (uint64_t)std::max(0, (r[0] + r[3]) / 2), cy = (uint64_t)std::max(0, (r[1] + r[4]) / 2),
+               cz = (uint64_t)std::max(0, (r[2] + r[5]) / 2);
+      uint64_t code = spread(cx) | (spread(cy) << 1) | (spread(cz) << 2);
+      key[g - 1] = {st[g] == kAlive ? code : ~0ull, g};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int32_t> order(N);
+    for (int i = 0; i < N; ++i) order[i] = key[i].second;
+    HIPCHK(c, hipMemcpy(c->P.order, order.data(), 4 * N, hipMemcpyHostToDevice));
+  }
   for (int r = 0; r < 2; ++r) {
     HIPCHK(c, hipMemcpy(c->D[r].bounds, b.data(), 4 * b.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->IL[r].bounds, b.data(), 4 * b.size(), hipMemcpyHostToDevice));

@@ -240,11 +240,39 @@ __global__ __launch_bounds__(kT) void k_dft_xy_inv(PlanDev P, int cls, DescDev d
 }
 
 // ---- FP32 separable circulant form ----------------------------------------
+// Each lane produces 4 consecutive outputs of a line.  The kernel row is held
+// twice over (crr[i] = c[i mod n], i < 2n+4) so (o - m) mod n needs no wrap, and
+// the four coefficients slide through registers: per source value, one LDS
+// read of the data, one of the kernel row, four FMAs.
+constexpr int kRB = 4;
+
+__device__ inline void load_circ_row(float* crr, const TablesDev& tb, int n, int tid, int nt) {
+  const float* c = tb.circ + tb.tw_off[n];
+  for (int i = tid; i < 2 * n + kRB; i += nt) crr[i] = c[i % n];
+}
+
+// out[o] = sum_m c[(o - m) mod n] * src[m*stride], o = o0 .. o0+3
+__device__ inline void circ4(const float* crr, const float* src, int stride, int n, int o0, float acc[kRB]) {
+  float w0 = crr[o0 + n], w1 = crr[o0 + 1 + n], w2 = crr[o0 + 2 + n], w3 = crr[o0 + 3 + n];
+  acc[0] = acc[1] = acc[2] = acc[3] = 0.f;
+  for (int m = 0; m < n; ++m) {
+    const float v = src[m * stride];
+    acc[0] = __builtin_fmaf(w0, v, acc[0]);
+    acc[1] = __builtin_fmaf(w1, v, acc[1]);
+    acc[2] = __builtin_fmaf(w2, v, acc[2]);
+    acc[3] = __builtin_fmaf(w3, v, acc[3]);
+    w3 = w2;
+    w2 = w1;
+    w1 = w0;
+    w0 = crr[o0 + n - m - 1];
+  }
+}
+
 template <int NMAX>
 __global__ __launch_bounds__(kT) void k_circ_xy(PlanDev P, int cls, DescDev din, DescDev dout, TablesDev tb) {
   __shared__ float A[NMAX * NMAX];
   __shared__ float B[NMAX * NMAX];
-  __shared__ float cr[NMAX];
+  __shared__ float crr[2 * NMAX + kRB];
   const int total = P.totals[kQLayers * kClasses + cls];
   const float* ain = (const float*)din.arena;
   float* aout = (float*)dout.arena;
@@ -253,32 +281,29 @@ __global__ __launch_bounds__(kT) void k_circ_xy(PlanDev P, int cls, DescDev din,
     int g, z;
     plan_tile(P, cls, kQLayers, tile, g, z);
     const int n = box_n(din.bounds, g), nn = n * n;
-    for (int i = tid; i < n; i += kT) cr[i] = tb.circ[tb.tw_off[n] + i];
-    const float* src = ain + din.off[g] + (int64_t)z * nn;
-    stage_to_lds(A, src, nn, tid, kT);
+    const int G = (n + kRB - 1) / kRB;  // groups of 4 outputs per line
+    load_circ_row(crr, tb, n, tid, kT);
+    stage_to_lds(A, ain + din.off[g] + (int64_t)z * nn, nn, tid, kT);
     __syncthreads();
-    for (int o = tid; o < nn; o += kT) {
-      const int y = o / n, x = o - y * n;
-      const float* row = A + y * n;
-      float acc = 0.f;
-      int idx = x;
-      for (int m = 0; m < n; ++m) {
-        acc = __builtin_fmaf(cr[idx], row[m], acc);
-        idx = idx == 0 ? n - 1 : idx - 1;
-      }
-      B[o] = acc;
+    // along x: item = (y, group)
+    for (int it = tid; it < n * G; it += kT) {
+      const int y = it / G, x0 = (it - y * G) * kRB;
+      float acc[kRB];
+      circ4(crr, A + y * n, 1, n, x0, acc);
+#pragma unroll
+      for (int u = 0; u < kRB; ++u)
+        if (x0 + u < n) B[y * n + x0 + u] = acc[u];
     }
     __syncthreads();
+    // along y: item = (group, x), consecutive lanes on consecutive x
     float* dst = aout + dout.off[g] + (int64_t)z * nn;
-    for (int o = tid; o < nn; o += kT) {
-      const int y = o / n, x = o - y * n;
-      float acc = 0.f;
-      int idx = y;
-      for (int m = 0; m < n; ++m) {
-        acc = __builtin_fmaf(cr[idx], B[m * n + x], acc);
-        idx = idx == 0 ? n - 1 : idx - 1;
-      }
-      dst[o] = acc;
+    for (int it = tid; it < n * G; it += kT) {
+      const int gy = it / n, x = it - gy * n, y0 = gy * kRB;
+      float acc[kRB];
+      circ4(crr, B + x, n, n, y0, acc);
+#pragma unroll
+      for (int u = 0; u < kRB; ++u)
+        if (y0 + u < n) dst[(y0 + u) * n + x] = acc[u];
     }
     __syncthreads();
   }
@@ -288,7 +313,7 @@ template <int NMAX>
 __global__ __launch_bounds__(kT) void k_circ_z(PlanDev P, int cls, DescDev dout, TablesDev tb) {
   constexpr int CB = kColBlock;
   __shared__ float Lz[NMAX * CB];
-  __shared__ float cr[NMAX];
+  __shared__ float crr[2 * NMAX + kRB];
   const int total = P.totals[kQCols * kClasses + cls];
   float* aout = (float*)dout.arena;
   const int tid = threadIdx.x;
@@ -297,20 +322,20 @@ __global__ __launch_bounds__(kT) void k_circ_z(PlanDev P, int cls, DescDev dout,
     plan_tile(P, cls, kQCols, tile, g, blk);
     const int n = box_n(dout.bounds, g), nn = n * n;
     const int c0 = blk * CB;
-    for (int i = tid; i < n; i += kT) cr[i] = tb.circ[tb.tw_off[n] + i];
+    const int G = (n + kRB - 1) / kRB;
+    load_circ_row(crr, tb, n, tid, kT);
     float* base = aout + dout.off[g];
     stage_rows_to_lds(Lz, CB, base + c0, nn, n, min(CB, nn - c0), tid, kT);  // columns past nn are never read
     __syncthreads();
-    for (int o = tid; o < n * CB; o += kT) {
-      const int z = o / CB, cb = o - z * CB, c = c0 + cb;
+    // item = (group of 4 z, column), consecutive lanes on consecutive columns
+    for (int it = tid; it < G * CB; it += kT) {
+      const int gz = it / CB, cb = it - gz * CB, c = c0 + cb, zz = gz * kRB;
       if (c >= nn) continue;
-      float acc = 0.f;
-      int idx = z;
-      for (int m = 0; m < n; ++m) {
-        acc = __builtin_fmaf(cr[idx], Lz[m * CB + cb], acc);
-        idx = idx == 0 ? n - 1 : idx - 1;
-      }
-      base[(int64_t)z * nn + c] = acc;
+      float acc[kRB];
+      circ4(crr, Lz + cb, CB, n, zz, acc);
+#pragma unroll
+      for (int u = 0; u < kRB; ++u)
+        if (zz + u < n) base[(int64_t)(zz + u) * nn + c] = acc[u];
     }
     __syncthreads();
   }

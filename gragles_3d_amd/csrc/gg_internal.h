@@ -69,8 +69,9 @@ enum { kQLayers = 0, kQCols = 1, kQSpecCols = 2, kNumQ = 3 };
 struct PlanDev {
   int32_t* cls_count;   // [kClasses]
   int32_t* cls_start;   // [kClasses] first index into grains[]
-  int32_t* grains;      // [N] alive grain ids grouped by class, ascending id inside a class
+  int32_t* grains;      // [N] alive grain ids grouped by class, in `order` inside a class
   int32_t* grain_pos;   // [N+1] index of a grain in grains[] (-1 if not alive)
+  int32_t* order;       // [N] permutation of 1..N: the spatial (Morton) order of the initial box centres
   int32_t* pre[kNumQ];  // [N + kClasses] per-class exclusive prefixes (segment c at cls_start[c]+c)
   int32_t* totals;      // [kNumQ * kClasses]
   int64_t* layer_base;  // [kClasses] first hull-partial slot of each class

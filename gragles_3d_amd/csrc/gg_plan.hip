@@ -48,7 +48,10 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32
   int64_t loc[NQ];
   for (int q = 0; q < NQ; ++q) loc[q] = 0;
   int bad = 0;
-  for (int g = gb; g < ge; ++g) {
+  // grains are visited in the spatial order, so class lists, tiles and the
+  // arena layout keep neighbouring boxes close together
+  for (int idx = gb; idx < ge; ++idx) {
+    const int g = P.order[idx - 1];
     P.grain_pos[g] = -1;
     if (state[g] != kAlive) continue;
     const int32_t* b = bounds + 6 * g;
@@ -90,7 +93,8 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32
     if (off_out && tot[16] > arena_cap) atomicOr(err, kErrArena);
     if (spec_off && tot[17] > spec_cap) atomicOr(err, kErrSpec);
   }
-  for (int g = gb; g < ge; ++g) {
+  for (int idx = gb; idx < ge; ++idx) {
+    const int g = P.order[idx - 1];
     if (state[g] != kAlive) continue;
     const int32_t* b = bounds + 6 * g;
     int n = b[3] - b[0];

@@ -70,35 +70,49 @@ __global__ __launch_bounds__(128) void k_redist_xy(SimParams sp, PlanDev P, int 
     }
     __syncthreads();
     if (do_xy) {
-      // x-forward then x-backward, one lane per row i in [iymin, oy1)
+      // x-forward then x-backward, one lane per row i in [iymin, oy1).  The next
+      // step's LDS values are read one step ahead (software pipelining); each
+      // step still updates exactly the reference's values in its order.
       for (int r = tid; r < oy1 - iymin; r += nt) {
         const int i = iymin + r;
         T* orow = O + (i - oy0) * S;  // orow[j - ox0] = out(i, j, k)
         const T* irow = I + (i - oy0) * S;
         const bool inrow = i < iymax;
         if (ixmin < ox1 - 1) {
-          T cur = orow[ixmin - ox0];
-          for (int j = ixmin; j < ox1 - 1; ++j) {
-            T nxt = orow[j + 1 - ox0];
+          int jj = ixmin - ox0;
+          T cur = orow[jj], nxt = orow[jj + 1];
+          T ia = irow[jj], ibv = irow[jj + 1];
+          for (int j = ixmin; j < ox1 - 1; ++j, ++jj) {
+            const bool more = j + 2 < ox1;
+            const T nxt2 = more ? orow[jj + 2] : T(0);
+            const T ib2 = more ? irow[jj + 2] : T(0);
             T cand;
-            if (j < ixmax - 1 && inrow) cand = step_interp(irow[j - ox0], irow[j + 1 - ox0], cur, h);
+            if (j < ixmax - 1 && inrow) cand = step_interp(ia, ibv, cur, h);
             else cand = cur + ((T)sgnv(nxt) * h);
             if (absv(cand) < absv(nxt)) nxt = cand;
-            orow[j - ox0] = cur;
+            orow[jj] = cur;
             cur = nxt;
+            nxt = nxt2;
+            ia = ibv;
+            ibv = ib2;
           }
           orow[ox1 - 1 - ox0] = cur;
         }
         if (ixmax - 1 > ox0) {
-          T cur = orow[ixmax - 1 - ox0];
-          for (int j = ixmax - 1; j > ox0; --j) {
-            T prv = orow[j - 1 - ox0];
+          int jj = ixmax - 1 - ox0;
+          T cur = orow[jj], prv = orow[jj - 1], ip = irow[jj - 1];
+          for (int j = ixmax - 1; j > ox0; --j, --jj) {
+            const bool more = j - 2 >= ox0;
+            const T prv2 = more ? orow[jj - 2] : T(0);
+            const T ip2 = more ? irow[jj - 2] : T(0);
             T cand;
-            if (j > ixmin && inrow) cand = cur + ((T)sgnv(irow[j - 1 - ox0]) * h);
+            if (j > ixmin && inrow) cand = cur + ((T)sgnv(ip) * h);
             else cand = cur + (T)sgnv(prv) * h;
             if (absv(cand) < absv(prv)) prv = cand;
             cur = prv;
-            orow[j - 1 - ox0] = prv;
+            orow[jj - 1] = prv;
+            prv = prv2;
+            ip = ip2;
           }
         }
       }
@@ -110,28 +124,40 @@ __global__ __launch_bounds__(128) void k_redist_xy(SimParams sp, PlanDev P, int 
         const T* icol = I + (j - ox0);
         const bool incol = j < ixmax;
         if (iymin < oy1 - 1) {
-          T cur = ocol[(iymin - oy0) * S];
-          for (int i = iymin; i < oy1 - 1; ++i) {
-            T nxt = ocol[(i + 1 - oy0) * S];
+          int ii = iymin - oy0;
+          T cur = ocol[ii * S], nxt = ocol[(ii + 1) * S];
+          T ia = icol[ii * S], ibv = icol[(ii + 1) * S];
+          for (int i = iymin; i < oy1 - 1; ++i, ++ii) {
+            const bool more = i + 2 < oy1;
+            const T nxt2 = more ? ocol[(ii + 2) * S] : T(0);
+            const T ib2 = more ? icol[(ii + 2) * S] : T(0);
             T cand;
-            if (incol && i < iymax - 1) cand = step_interp(icol[(i - oy0) * S], icol[(i + 1 - oy0) * S], cur, h);
+            if (incol && i < iymax - 1) cand = step_interp(ia, ibv, cur, h);
             else cand = cur + ((T)sgnv(nxt) * h);
             if (absv(cand) < absv(nxt)) nxt = cand;
-            ocol[(i - oy0) * S] = cur;
+            ocol[ii * S] = cur;
             cur = nxt;
+            nxt = nxt2;
+            ia = ibv;
+            ibv = ib2;
           }
           ocol[(oy1 - 1 - oy0) * S] = cur;
         }
         if (iymax - 1 > oy0) {
-          T cur = ocol[(iymax - 1 - oy0) * S];
-          for (int i = iymax - 1; i > oy0; --i) {
-            T prv = ocol[(i - 1 - oy0) * S];
+          int ii = iymax - 1 - oy0;
+          T cur = ocol[ii * S], prv = ocol[(ii - 1) * S], ip = icol[(ii - 1) * S];
+          for (int i = iymax - 1; i > oy0; --i, --ii) {
+            const bool more = i - 2 >= oy0;
+            const T prv2 = more ? ocol[(ii - 2) * S] : T(0);
+            const T ip2 = more ? icol[(ii - 2) * S] : T(0);
             T cand;
-            if (incol && i > iymin) cand = cur + ((T)sgnv(icol[(i - 1 - oy0) * S]) * h);
+            if (incol && i > iymin) cand = cur + ((T)sgnv(ip) * h);
             else cand = cur + ((T)sgnv(prv) * h);
             if (absv(cand) < absv(prv)) prv = cand;
             cur = prv;
-            ocol[(i - 1 - oy0) * S] = prv;
+            ocol[(ii - 1) * S] = prv;
+            prv = prv2;
+            ip = ip2;
           }
         }
       }

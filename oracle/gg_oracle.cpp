@@ -942,6 +942,63 @@ void orc_redistance_box(const int32_t* ib, const double* in, const int32_t* ob, 
   std::memcpy(out, O.v.data(), sizeof(double) * O.v.size());
 }
 
+// executeComparison + boundaryCondition + executeSetComparison for one grain
+// with explicit neighbour boxes (for hand-derived test cases).  own_b is the
+// grain's box (input = output bounds), phi_in its input field; nb_b/nb_ids/
+// nb_phi the neighbours in comparison-list order (fields concatenated).
+// idl holds the grain's IDLocal on entry (its stale contents) and on exit;
+// out receives the new field, newb the set-comparison bounds.
+int orc_comparison_box(int ngp, int gb, int tube, double h, double delta, const int32_t* own_b,
+                       const double* phi_in, int n_nb, const int32_t* nb_b, const int32_t* nb_ids,
+                       const double* nb_phi, uint32_t* idl, double* out, int32_t* newb) {
+  orc_params p{};
+  p.ngridpoints = ngp;
+  p.grid_blowup = gb;
+  p.boundary_grain_tube = tube;
+  p.h = h;
+  p.delta = delta;
+  p.n_grains = 0;
+  orc_sim S;
+  S.p = p;
+  int maxid = 0;
+  for (int a = 0; a < n_nb; ++a) maxid = std::max(maxid, nb_ids[a]);
+  S.g.resize(maxid + 2);
+  const unsigned self = (unsigned)maxid + 1;
+  try {
+    Grain& G = S.g[self];
+    G.id = self;
+    G.state = kAlive;
+    G.in.reset(new Buf<double>);
+    G.out.reset(new Buf<double>);
+    G.in->resize(own_b[0], own_b[1], own_b[2], own_b[3], own_b[4], own_b[5]);
+    G.out->resize(own_b[0], own_b[1], own_b[2], own_b[3], own_b[4], own_b[5]);
+    G.idl.resize(own_b[0], own_b[1], own_b[2], own_b[3], own_b[4], own_b[5]);
+    std::memcpy(G.in->v.data(), phi_in, sizeof(double) * G.in->v.size());
+    std::memcpy(G.idl.v.data(), idl, sizeof(uint32_t) * G.idl.v.size());
+    size_t off = 0;
+    for (int a = 0; a < n_nb; ++a) {
+      Grain& N = S.g[nb_ids[a]];
+      N.id = (unsigned)nb_ids[a];
+      N.state = kAlive;
+      N.in.reset(new Buf<double>);
+      const int32_t* q = nb_b + 6 * a;
+      N.in->resize(q[0], q[1], q[2], q[3], q[4], q[5]);
+      std::memcpy(N.in->v.data(), nb_phi + off, sizeof(double) * N.in->v.size());
+      off += N.in->v.size();
+      G.cmp.push_back(N.id);
+    }
+    comparison(S, G);
+    set_comparison(S, G);
+    std::memcpy(out, G.out->v.data(), sizeof(double) * G.out->v.size());
+    std::memcpy(idl, G.idl.v.data(), sizeof(uint32_t) * G.idl.v.size());
+    for (int a = 0; a < 6; ++a) newb[a] = G.nb[a];
+  } catch (std::exception& e) {
+    g_static_err = e.what();
+    return 1;
+  }
+  return 0;
+}
+
 // Triangle table row access for the table-consistency test.
 int orc_tri_row(int c, int32_t* row16) {
   for (int i = 0; i < 16; ++i) row16[i] = g_tri[c][i];

@@ -75,6 +75,9 @@ def lib():
                                             ctypes.c_double, f64p, f64p]),
             "orc_redistance_box": (None, [i32p, f64p, i32p, f64p, ctypes.c_double, ctypes.c_double]),
             "orc_tri_row": (ctypes.c_int, [ctypes.c_int, i32p]),
+            "orc_comparison_box": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                  ctypes.c_double, i32p, f64p, ctypes.c_int, i32p, i32p, f64p,
+                                                  u32p, f64p, i32p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -132,6 +135,29 @@ def redistance_box(in_bounds, phi_in, out_bounds, h, delta):
     lib().orc_redistance_box(_p(ib, ctypes.c_int32), _p(phi_in, ctypes.c_double), _p(ob, ctypes.c_int32),
                              _p(out, ctypes.c_double), float(h), float(delta))
     return out
+
+
+def comparison_box(ngp, gb, h, delta, own_bounds, phi_in, neighbours, idl_stale, tube=0):
+    """Comparison + wall + set-comparison of one grain.
+
+    neighbours: list of (id, bounds6, phi) in comparison-list order.
+    Returns (phi_out, idlocal, new_bounds)."""
+    ob = np.ascontiguousarray(own_bounds, dtype=np.int32)
+    phi_in = np.ascontiguousarray(phi_in, dtype=np.float64)
+    nbb = np.ascontiguousarray([b for _, b, _ in neighbours] or [[0] * 6], dtype=np.int32)
+    ids = np.ascontiguousarray([i for i, _, _ in neighbours] or [0], dtype=np.int32)
+    phis = np.ascontiguousarray(np.concatenate([np.ravel(p) for _, _, p in neighbours]) if neighbours
+                                else np.zeros(1), dtype=np.float64)
+    idl = np.ascontiguousarray(idl_stale, dtype=np.uint32).copy()
+    out = np.empty_like(phi_in)
+    nb = np.zeros(6, dtype=np.int32)
+    rc = lib().orc_comparison_box(int(ngp), int(gb), int(tube), float(h), float(delta), _p(ob, ctypes.c_int32),
+                                  _p(phi_in, ctypes.c_double), len(neighbours), _p(nbb, ctypes.c_int32),
+                                  _p(ids, ctypes.c_int32), _p(phis, ctypes.c_double), _p(idl, ctypes.c_uint32),
+                                  _p(out, ctypes.c_double), _p(nb, ctypes.c_int32))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error(None).decode())
+    return out, idl, nb
 
 
 def tri_row(c):
