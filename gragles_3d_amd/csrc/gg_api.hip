@@ -524,9 +524,14 @@ int gg_set_neighbour_lists(gg_ctx* c, const int32_t* offsets, const int32_t* ids
     int a = offsets[g], b = offsets[g + 1];
     if (b - a > M) return fail(c, GG_ERR_CAPACITY, "initial neighbour list longer than max_list");
     for (int i = a; i < b; ++i) {
-      if (ids[i] < 1 || ids[i] > N) return fail(c, GG_ERR_ARG, "neighbour id out of range");
+      if (ids[i] < 1 || ids[i] > N || ids[i] == g) return fail(c, GG_ERR_ARG, "neighbour id out of range");
       buf[(size_t)g * M + (i - a)] = ids[i];
     }
+    // the list builder relies on a list holding no id twice (as the reference's lists never do)
+    std::vector<int32_t> sorted(ids + a, ids + b);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+      return fail(c, GG_ERR_ARG, "initial neighbour list holds an id twice");
     len[g] = b - a;
   }
   std::vector<int32_t> zero(N + 1, 0);

@@ -78,21 +78,18 @@ __global__ __launch_bounds__(64) void k_lists(SimParams sp, const int32_t* __res
             p = !(gx0 > cb[3] || gx1 < cb[0] || gy0 > cb[4] || gy1 < cb[1] || gz0 > cb[5] || gz1 < cb[2]);
           }
         }
-        uint64_t m = __ballot(p);
-        while (m) {
-          const int l = __ffsll((unsigned long long)m) - 1;
-          m &= m - 1;
-          const int c = __shfl(cid, l);
-          const bool fresh = ((bitmap[c >> 5] >> (c & 31)) & 1u) == 0u;
-          if (fresh) {
-            if (lane == 0) {
-              bitmap[c >> 5] |= 1u << (c & 31);
-              if (outlen < maxL) dst[outlen] = c;
-            }
-            ++outlen;
-          }
-          __syncthreads();
+        // A list holds no id twice, so the ids of one chunk are distinct and
+        // "first occurrence" reduces to "not yet in the bitmap"; the ballot
+        // prefix appends the fresh ones in lane (= traversal) order.
+        const bool fresh = p && ((bitmap[cid >> 5] >> (cid & 31)) & 1u) == 0u;
+        const uint64_t m = __ballot(fresh);
+        const int pos = outlen + __popcll(m & ((1ull << lane) - 1ull));
+        if (fresh) {
+          if (pos < maxL) dst[pos] = cid;
+          atomicOr(&bitmap[cid >> 5], 1u << (cid & 31));
         }
+        outlen += __popcll(m);
+        __syncthreads();
       }
     }
     if (lane == 0) {

@@ -229,29 +229,42 @@ __global__ __launch_bounds__(64) void k_redist_z(SimParams sp, PlanDev P, int cl
       }
 #define OZ(k) Oc[((k) - oz0) * 64]
 #define IZ(k) Ic[((k) - oz0) * 64]
+      // software-pipelined: the next step's LDS values are read one step ahead
       if (izmin < oz1 - 1) {
-        T cur = OZ(izmin);
+        T cur = OZ(izmin), nxt = OZ(izmin + 1);
+        T ia = IZ(izmin), ibv = IZ(izmin + 1);
         for (int k = izmin; k < oz1 - 1; ++k) {
-          T nxt = OZ(k + 1);
+          const bool more = k + 2 < oz1;
+          const T nxt2 = more ? OZ(k + 2) : T(0);
+          const T ib2 = more ? IZ(k + 2) : T(0);
           T cand;
-          if (k < izmax - 1 && inside) cand = step_interp(IZ(k), IZ(k + 1), cur, h);
+          if (k < izmax - 1 && inside) cand = step_interp(ia, ibv, cur, h);
           else cand = cur + ((T)sgnv(nxt) * h);
           if (absv(cand) < absv(nxt)) nxt = cand;
           OZ(k) = cur;
           cur = nxt;
+          nxt = nxt2;
+          ia = ibv;
+          ibv = ib2;
         }
         OZ(oz1 - 1) = cur;
       }
       if (izmax - 1 > oz0) {
-        T cur = OZ(izmax - 1);
+        T cur = OZ(izmax - 1), prv = OZ(izmax - 2);
+        T ic = IZ(izmax - 1), ip = IZ(izmax - 2);
         for (int k = izmax - 1; k > oz0; --k) {
-          T prv = OZ(k - 1);
+          const bool more = k - 2 >= oz0;
+          const T prv2 = more ? OZ(k - 2) : T(0);
+          const T ip2 = more ? IZ(k - 2) : T(0);
           T cand;
-          if (k > izmin && inside) cand = step_interp(IZ(k), IZ(k - 1), cur, h);
+          if (k > izmin && inside) cand = step_interp(ic, ip, cur, h);
           else cand = cur + ((T)sgnv(prv) * h);
           if (absv(cand) < absv(prv)) prv = cand;
           OZ(k) = cur;
           cur = prv;
+          prv = prv2;
+          ic = ip;
+          ip = ip2;
         }
         OZ(oz0) = cur;
       }
