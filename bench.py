@@ -35,24 +35,37 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kbar_and_voxels(ctx, n_grains):
+CLASS_NMAX = (32, 48, 64, 96)  # box-size classes of the work plan, one comparison launch each
+
+
+def size_class(n):
+    for c, m in enumerate(CLASS_NMAX):
+        if n <= m:
+            return c
+    return len(CLASS_NMAX) - 1
+
+
+def kbar_and_voxels(ctx):
     """Exact mean number of comparison-list boxes covering a voxel, K = sum_i sum_{j in list(i)}
-    |B_i cap B_j| / sum_i |B_i|, over the grains alive now, from their box bounds."""
+    |B_i cap B_j| / sum_i |B_i|, over the grains alive now, from their box bounds; overall and
+    per box-size class (numerator and voxel count per class)."""
     st, _, _ = ctx.stats()
     alive = np.nonzero(st == G.ALIVE)[0]
     B = {int(g): ctx.bounds(int(g), 1) for g in alive}
-    num = 0
-    den = 0
+    num = np.zeros(len(CLASS_NMAX))
+    den = np.zeros(len(CLASS_NMAX))
     for g in alive:
         b = B[int(g)]
-        den += int(b[3] - b[0]) ** 3
+        n = int(b[3] - b[0])
+        c = size_class(n)
+        den[c] += n ** 3
         for j in ctx.neighbour_list(int(g), 0):
-            c = B.get(int(j))
-            if c is None:
+            o = B.get(int(j))
+            if o is None:
                 continue
-            w = [max(0, min(b[a + 3], c[a + 3]) - max(b[a], c[a])) for a in range(3)]
-            num += w[0] * w[1] * w[2]
-    return num / max(den, 1), den
+            w = [max(0, min(b[a + 3], o[a + 3]) - max(b[a], o[a])) for a in range(3)]
+            num[c] += w[0] * w[1] * w[2]
+    return num.sum() / max(den.sum(), 1), num, den
 
 
 def run_path(precision, inputs, p, steps, warmup, use_graph=True):
@@ -134,7 +147,7 @@ def main():
     value = vox_step32 * args.steps / wall32
     log("fp32: %d steps %.3fs -> %.3f ms/step, %.3e voxel-updates/s (boxes %.3e voxels)" %
         (args.steps, wall32, 1e3 * wall32 / args.steps, value, vox_step32))
-    kbar, _ = kbar_and_voxels(ctx32, p.n_grains)
+    kbar, knum, kden = kbar_and_voxels(ctx32)
 
     # ---- FP64 plain, same steps; then the FP32 error against it ------------------
     fp64 = None
@@ -156,16 +169,26 @@ def main():
                 "segment_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in seg64.items() if v[1]}}
 
     seg32 = kernel_pass(ctx32, args.kernel_steps)
-    vox_k = ctx32.box_voxels()
     counts32 = ctx32.count_history()
     ctx32.close()
 
-    # roofline of the comparison kernel (neighbour max + wall + set comparison): per voxel it
-    # reads K neighbour values and its own value and writes the new value and IDLocal
-    cmp_ms, cmp_n = seg32["comparison"]
-    cmp_avg_s = cmp_ms / max(cmp_n, 1) * 1e-3
-    cmp_bytes = vox_k * (4.0 * kbar + 12.0)
-    achieved = cmp_bytes / cmp_avg_s / 1e9 if cmp_avg_s > 0 else 0.0
+    # Roofline of the comparison kernel (neighbour max + wall + set comparison), one launch per
+    # box-size class: per voxel it reads K neighbour values and its own value (4 B each) and its
+    # previous IDLocal (4 B), and writes the new value and IDLocal (8 B) -> (4K + 12) B per voxel.
+    # The dominant kernel is the class launch with the largest event-timed duration.
+    cls_rows = []
+    for c, m in enumerate(CLASS_NMAX):
+        ms, nl = seg32["comparison_n%d" % m]
+        avg_s = ms / max(nl, 1) * 1e-3
+        kb = knum[c] / kden[c] if kden[c] else 0.0
+        bts = kden[c] * (4.0 * kb + 12.0)
+        cls_rows.append({"kernel": "k_comparison<float, %d>" % m, "box_voxels": int(kden[c]), "kbar": kb,
+                         "bytes_per_launch": bts, "avg_launch_ms": avg_s * 1e3,
+                         "achieved_GBs": bts / avg_s / 1e9 if avg_s > 0 and bts > 0 else 0.0})
+    dom = max(cls_rows, key=lambda r: r["avg_launch_ms"])
+    achieved = dom["achieved_GBs"]
+    cmp_total_ms = sum(r["avg_launch_ms"] for r in cls_rows)
+    cmp_total_bytes = sum(r["bytes_per_launch"] for r in cls_rows)
     seg_ms_per_step = {k: round(v[0] / max(v[1], 1), 4) for k, v in seg32.items() if v[1]}
 
     out = {
@@ -189,16 +212,20 @@ def main():
             "ngridpoints": p.ngridpoints, "box_voxels_per_step": vox_step32, "kbar": kbar,
         },
         "roofline": {
-            "kernel": "k_comparison<float>",
+            "kernel": dom["kernel"],
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "bytes_per_launch": cmp_bytes,
-            "avg_launch_ms": cmp_avg_s * 1e3,
+            "bytes_per_launch": dom["bytes_per_launch"],
+            "avg_launch_ms": dom["avg_launch_ms"],
+            "kbar": dom["kbar"],
         },
+        "comparison_by_class": cls_rows,
+        "comparison_all_classes": {"ms_per_step": cmp_total_ms, "bytes_per_step": cmp_total_bytes,
+                                   "GBs": cmp_total_bytes / (cmp_total_ms * 1e-3) / 1e9 if cmp_total_ms else 0.0},
         "segment_ms_per_step_fp32": seg_ms_per_step,
         "grain_count_trajectory_fp32": counts32[-3:].tolist(),
     }

@@ -210,7 +210,7 @@ hipEvent_t take_event(gg_ctx* c) {
 // Event-timed segments, one per kernel (or per kernel with one launch per size class).
 enum Seg {
   kSegConvA, kSegConvB, kSegConvC, kSegIdl, kSegLists, kSegCompare, kSegHull, kSegFinalize, kSegPlan,
-  kSegRedistXY, kSegRedistZ, kSegPost, kSegCount, kNumSeg
+  kSegRedistXY, kSegRedistZ, kSegPost, kSegCount, kSegCompare48, kSegCompare64, kSegCompare96, kNumSeg
 };
 
 struct SegTimer {
@@ -269,9 +269,10 @@ void stage_lists(gg_ctx* c) {
 }
 
 void stage_comparison(gg_ctx* c) {
-  {
-    SegTimer t(c, kSegCompare);
-    launch_comparison(launch_of(c), c->sp, c->P, c->d_state, din(c), dout(c), c->LD, c->IL[c->il_cur],
+  static const int seg[kClasses] = {kSegCompare, kSegCompare48, kSegCompare64, kSegCompare96};
+  for (int cls = 0; cls < kClasses; ++cls) {
+    SegTimer t(c, seg[cls]);
+    launch_comparison(launch_of(c), cls, c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur],
                       c->IL[c->il_cur ^ 1], c->d_newb, c->fp32);
   }
   c->il_cur ^= 1;

@@ -3,30 +3,23 @@
 // executeSetComparison (:814-878), fused per (grain, z-layer) tile, one launch
 // per box-size class.
 //
-// Per tile the workgroup first compacts, in comparison-list order, the
-// neighbours whose input box covers this layer and overlaps the grain's box
-// in x and y.  Each lane owns a fixed set of voxels of the layer (registers)
-// and walks the compacted neighbours; for one neighbour all its voxel loads
-// are independent, so they are in flight together.  Per voxel the updates
-// still happen in list order with a strict '>' (ties keep the earlier
-// neighbour), which is the voxel-wise result of the reference's
-// neighbour-outer loop.  Then the wall distance, IDLocal (new id, or the
-// previous step's id at the same linear index when nothing covers the voxel),
-// 0.5*(in - max) inside 0.7*delta, and the bounding box of phi >= 0.
+// Per tile the neighbours' values on the rectangles they share with the
+// grain's layer are copied into LDS with independent, coalesced loads, then
+// every lane takes the maximum over the neighbours for its voxels in
+// comparison-list order with a strict '>' (ties keep the earlier neighbour),
+// which is the voxel-wise result of the reference's neighbour-outer loop.
+// Then the wall distance, IDLocal (new id, or the previous step's id at the
+// same linear index when nothing covers the voxel), 0.5*(in - max) inside
+// 0.7*delta, and the bounding box of phi >= 0.
 #include "gg_internal.h"
 
 namespace gg {
 namespace {
 
-constexpr int kMaxAct = 1024;  // >= the largest comparison-list capacity gg_create accepts
 constexpr int kCT = 256;
-
-struct ActNb {
-  int32_t id;
-  int16_t x0, x1, y0, y1;
-  int32_t nn;
-  int64_t base;
-};
+constexpr int kWaves = kCT / 64;
+constexpr int kLC = 128;            // comparison-list entries compacted per pass
+constexpr int kStageBytes = 24576;  // LDS bytes of staged neighbour values per workgroup
 
 // Value of boundaryCondition's `dist` at voxel (j,i,k): the reference assigns
 // it in every branch except when all three axis distances are >= 0 and one is
@@ -67,17 +60,47 @@ __device__ double bc_dist(int j, int i, int k, int x0, int y0, int z0, int x1, i
   }
 }
 
+// One tile = one z-layer of one grain.
+//
+// Active entries.  The comparison list is taken in chunks of kLC entries; the
+// entries of a chunk whose box covers the layer and overlaps the grain's box in
+// x and y are compacted in list order with the rectangle they share with the
+// grain's layer, the rectangle's first arena index, and per wavefront a mask of
+// the rectangle's 8x8 voxel blocks that wavefront owns.
+//
+// Staging.  The rectangles are laid end to end (a_pre = running element count)
+// and copied into LDS in batches of BUF elements, every lane issuing SB
+// independent loads at a time; a lane finds the entry of its element by one
+// binary search per batch and then walks forward.
+//
+// Maximum.  The layer is cut into 8x8 blocks, block b owned by wavefront b % 4,
+// one voxel per lane and block, so a lane's voxels stay in its registers for
+// the whole tile.  For each entry of the batch, in list order, a wavefront
+// visits only the blocks the entry's rectangle touches (one scalar bit test per
+// block) and updates with a strict '>'.  A rectangle may straddle two batches:
+// a voxel's staged index grows with the entry's list position, so each voxel
+// still sees its neighbours in list order.
 template <class T, int NMAX>
 __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int cls, DescDev din, DescDev dout,
                                                     ListDev ld, IdlDev ilo, IdlDev iln, int32_t* newb) {
-  constexpr int VPT = (NMAX * NMAX + kCT - 1) / kCT;  // voxels of one layer per lane
-  __shared__ ActNb act[kMaxAct];
-  __shared__ int s_wave[kCT / 64];
-  __shared__ int s_red[6][kCT / 64];
+  constexpr int NB = (NMAX + 7) / 8;                  // blocks per side, at most
+  constexpr int VPT = (NB * NB + kWaves - 1) / kWaves;  // blocks per wavefront = voxels per lane
+  static_assert(VPT <= 64, "a wavefront's block mask is one 64-bit word");
+  constexpr int BUF = kStageBytes / (int)sizeof(T);
+  constexpr int SB = sizeof(T) == 4 ? 16 : 8;
+  __shared__ T sbuf[BUF];
+  __shared__ int64_t a_base[kLC];  // arena index of the rectangle's first element
+  __shared__ uint64_t a_wm[kWaves][kLC];  // blocks of the rectangle owned by each wavefront
+  __shared__ int32_t a_id[kLC], a_nn[kLC], a_pre[kLC + 1];
+  __shared__ uint32_t a_rect[kLC];   // local rectangle x0 | x1 << 8 | y0 << 16 | y1 << 24
+  __shared__ uint32_t a_magic[kLC];  // ceil(2^32 / width) for the division by the width
+  __shared__ int s_cnt[kLC / 64], s_area[kLC / 64];
+  __shared__ int s_red[6][kWaves];
   const int total = P.totals[kQLayers * kClasses + cls];
   const T* __restrict__ ain = (const T*)din.arena;
   T* __restrict__ aout = (T*)dout.arena;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const T lim = (T)(0.7 * sp.delta);
   const int maxL = ld.max_list;
   for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
@@ -86,78 +109,144 @@ __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int
     const int32_t* b = din.bounds + 6 * g;
     const int x0 = b[0], y0 = b[1], z0 = b[2], x1 = b[3], y1 = b[4];
     const int n = x1 - x0, nn2 = n * n;
+    const int nbx = (n + 7) >> 3;
     const int k = z0 + kl;
     const int lsel = ld.sel[g] ^ 1;  // m_comparisonList
     const int32_t* lst = ld.buf[lsel] + (int64_t)g * maxL;
     const int len = ld.len[lsel][g];
-    // compact the neighbours covering this layer, keeping list order
-    int cnt = 0;
-    for (int base = 0; base < len; base += kCT) {
-      const int idx = base + tid;
-      bool keep = false;
-      ActNb e;
-      if (idx < len) {
-        const int nb = lst[idx];
-        const int32_t* q = din.bounds + 6 * nb;
-        if (q[2] <= k && k < q[5] && max(x0, q[0]) < min(x1, q[3]) && max(y0, q[1]) < min(y1, q[4])) {
-          keep = true;
-          e.id = nb;
-          e.x0 = (int16_t)q[0]; e.x1 = (int16_t)q[3]; e.y0 = (int16_t)q[1]; e.y1 = (int16_t)q[4];
-          e.nn = q[3] - q[0];
-          e.base = din.off[nb] + (int64_t)(k - q[2]) * e.nn * e.nn;
-        }
-      }
-      const uint64_t m = __ballot(keep);
-      if (lane == 0) s_wave[wv] = __popcll(m);
-      __syncthreads();
-      int before = cnt;
-      for (int w = 0; w < wv; ++w) before += s_wave[w];
-      const int pos = before + __popcll(m & ((1ull << lane) - 1ull));
-      if (keep && pos < kMaxAct) act[pos] = e;
-      for (int w = 0; w < kCT / 64; ++w) cnt += s_wave[w];
-      __syncthreads();
-    }
-    if (cnt > kMaxAct) cnt = kMaxAct;
 
-    // The lane's voxels: wavefront wv owns the VPT consecutive 64-voxel segments
-    // starting at wv*VPT*64, i.e. a band of rows, so most neighbours miss most
-    // segments and are skipped with one uniform test.  Packed (i - y0) << 16 | (j - x0).
+    // The lane's voxels, packed (i - y0) << 16 | (j - x0); -1 outside the layer.
     int ij[VPT];
     T best[VPT];
     uint32_t bid[VPT];
-    int seg_lo[VPT], seg_hi[VPT];  // local row range of each segment (uniform)
 #pragma unroll
     for (int r = 0; r < VPT; ++r) {
-      const int s0 = (wv * VPT + r) * 64;
-      const int v = s0 + lane;
-      ij[r] = v < nn2 ? ((v / n) << 16) | (v % n) : -1;
+      const int bb = wv + kWaves * r;
+      const int by = bb / nbx, bx = bb - by * nbx;
+      const int il = by * 8 + (lane >> 3), jl = bx * 8 + (lane & 7);
+      ij[r] = (bb < nbx * nbx && il < n && jl < n) ? (il << 16) | jl : -1;
       best[r] = (T)(-1.0);
       bid[r] = 0xFFFFFFFFu;  // not covered
-      seg_lo[r] = s0 < nn2 ? s0 / n : n;
-      seg_hi[r] = min(s0 + 63, nn2 - 1) / n;
     }
-    for (int a = 0; a < cnt; ++a) {
-      const ActNb e = act[a];
-      const int ex0 = e.x0 - x0, ex1 = e.x1 - x0, ey0 = e.y0 - y0, ey1 = e.y1 - y0;
-      const int64_t eb = e.base - (int64_t)ey0 * e.nn - ex0;  // value at local (il, jl): eb + il*nn + jl
-      // all loads of this neighbour first, then the ordered updates
-      T val[VPT];
-      bool cov[VPT];
-#pragma unroll
-      for (int r = 0; r < VPT; ++r) {
-        cov[r] = false;
-        if (seg_lo[r] < ey1 && seg_hi[r] >= ey0) {
-          const int il = ij[r] >> 16, jl = ij[r] & 0xFFFF;
-          cov[r] = ij[r] >= 0 && jl >= ex0 && jl < ex1 && il >= ey0 && il < ey1;
-          if (cov[r]) val[r] = ain[eb + (int64_t)il * e.nn + jl];
+
+    for (int lbase = 0; lbase < len; lbase += kLC) {
+      // compact the active entries of this chunk, keeping list order
+      const int idx = lbase + tid;
+      bool keep = false;
+      int area = 0, nb = 0, nn = 0, rx0 = 0, rx1 = 0, ry0 = 0, ry1 = 0;
+      int64_t gbase = 0;
+      if (tid < kLC && idx < len) {
+        nb = lst[idx];
+        const int32_t* q = din.bounds + 6 * nb;
+        const int q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5];
+        const int cx0 = max(x0, q0), cx1 = min(x1, q3), cy0 = max(y0, q1), cy1 = min(y1, q4);
+        if (q2 <= k && k < q5 && cx0 < cx1 && cy0 < cy1) {
+          keep = true;
+          nn = q3 - q0;
+          rx0 = cx0 - x0; rx1 = cx1 - x0; ry0 = cy0 - y0; ry1 = cy1 - y0;
+          area = (rx1 - rx0) * (ry1 - ry0);
+          gbase = din.off[nb] + ((int64_t)(k - q2) * nn + (cy0 - q1)) * nn + (cx0 - q0);
         }
       }
-#pragma unroll
-      for (int r = 0; r < VPT; ++r) {
-        if (cov[r] && val[r] > best[r]) {
-          best[r] = val[r];
-          bid[r] = (uint32_t)e.id;
+      const uint64_t m = __ballot(keep);
+      int sc = area;  // inclusive scan of the areas over the wavefront
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(sc, off);
+        if (lane >= off) sc += y;
+      }
+      if (lane == 63 && wv < kLC / 64) {
+        s_cnt[wv] = __popcll(m);
+        s_area[wv] = sc;
+      }
+      __syncthreads();
+      if (keep) {
+        int pos = __popcll(m & ((1ull << lane) - 1ull)), ab = sc - area;
+        for (int w = 0; w < wv; ++w) {
+          pos += s_cnt[w];
+          ab += s_area[w];
         }
+        uint64_t wm[kWaves] = {};
+        for (int by = ry0 >> 3; by <= (ry1 - 1) >> 3; ++by)
+          for (int bx = rx0 >> 3; bx <= (rx1 - 1) >> 3; ++bx) {
+            const int bb = by * nbx + bx;
+            wm[bb % kWaves] |= 1ull << (bb / kWaves);
+          }
+        for (int w = 0; w < kWaves; ++w) a_wm[w][pos] = wm[w];
+        const int rw = rx1 - rx0;
+        a_id[pos] = nb;
+        a_nn[pos] = nn;
+        a_rect[pos] = (uint32_t)rx0 | (uint32_t)rx1 << 8 | (uint32_t)ry0 << 16 | (uint32_t)ry1 << 24;
+        a_magic[pos] = rw > 1 ? (uint32_t)(0xFFFFFFFFu / (uint32_t)rw + 1u) : 0u;
+        a_base[pos] = gbase;
+        a_pre[pos] = ab;
+      }
+      int cnt = 0, E = 0;
+      for (int w = 0; w < kLC / 64; ++w) {
+        cnt += s_cnt[w];
+        E += s_area[w];
+      }
+      if (tid == 0) a_pre[cnt] = E;
+      __syncthreads();
+
+      int a_lo = 0;
+      for (int s = 0; s < E; s += BUF) {
+        const int e = min(s + BUF, E);
+        int a = 0;  // entry of this lane's first element: last a with a_pre[a] <= s + tid
+        {
+          const int t = min(s + tid, e - 1);
+#pragma unroll
+          for (int st = kLC / 2; st > 0; st >>= 1)
+            if (a + st < cnt && a_pre[a + st] <= t) a += st;
+        }
+        for (int t0 = s; t0 < e; t0 += kCT * SB) {
+          T v[SB];
+#pragma unroll
+          for (int u = 0; u < SB; ++u) {
+            const int t = t0 + u * kCT + tid;
+            if (t < e) {
+              while (a_pre[a + 1] <= t) ++a;  // a_pre[cnt] = E > t ends the walk
+              const int r = t - a_pre[a];
+              const uint32_t rc = a_rect[a];
+              const int rw = (int)((rc >> 8) & 0xFFu) - (int)(rc & 0xFFu);
+              const int qr = rw > 1 ? (int)__umulhi((uint32_t)r, a_magic[a]) : r;  // r / rw
+              v[u] = ain[a_base[a] + (int64_t)qr * a_nn[a] + (r - qr * rw)];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < SB; ++u) {
+            const int t = t0 + u * kCT + tid;
+            if (t < e) sbuf[t - s] = v[u];
+          }
+        }
+        __syncthreads();
+        int aa = a_lo;
+        for (; aa < cnt && a_pre[aa] < e; ++aa) {
+          const uint64_t wmask = ((uint64_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a_wm[wv][aa] >> 32)) << 32) |
+                                 (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a_wm[wv][aa]);
+          const uint32_t rc = a_rect[aa];
+          const int rx0 = rc & 0xFF, ry0 = (rc >> 16) & 0xFF;
+          const int rw = (int)((rc >> 8) & 0xFFu) - rx0, rh = (int)(rc >> 24) - ry0;
+          const int pre = a_pre[aa] - s;  // sbuf index of the rectangle's first element (may be < 0)
+          const uint32_t lim_e = (uint32_t)(e - s);
+          const uint32_t id = (uint32_t)a_id[aa];
+#pragma unroll
+          for (int r = 0; r < VPT; ++r) {
+            if ((wmask >> r) & 1ull) {
+              const int dx = (ij[r] & 0xFFFF) - rx0, dy = (ij[r] >> 16) - ry0;
+              const int el = pre + dy * rw + dx;
+              if ((uint32_t)dx < (uint32_t)rw && (uint32_t)dy < (uint32_t)rh && (uint32_t)el < lim_e) {
+                const T val = sbuf[el];
+                if (val > best[r]) {
+                  best[r] = val;
+                  bid[r] = id;
+                }
+              }
+            }
+          }
+        }
+        // the next batch starts with the entry that straddles e, if any
+        a_lo = a_pre[aa] > e ? aa - 1 : aa;
+        __syncthreads();
       }
     }
     // BoundaryIntersection on the output box (equal to the input box here)
@@ -214,7 +303,7 @@ __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int
       int rr[6];
       for (int q = 0; q < 6; ++q) {
         rr[q] = s_red[q][0];
-        for (int w = 1; w < kCT / 64; ++w) rr[q] = min(rr[q], s_red[q][w]);
+        for (int w = 1; w < kWaves; ++w) rr[q] = min(rr[q], s_red[q][w]);
       }
       if (rr[0] != INT_MAX) {
         int32_t* nb = newb + 6 * g;
@@ -231,26 +320,35 @@ __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int
 }
 
 template <class T>
-void comparison_all(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din,
-                    const DescDev& dout, const ListDev& ld, const IdlDev& ilo, const IdlDev& iln, int32_t* newb) {
-  hipLaunchKernelGGL((k_comparison<T, 32>), dim3(L.grid), dim3(kCT), 0, L.stream, sp, P, 0, din, dout, ld, ilo,
-                     iln, newb);
-  hipLaunchKernelGGL((k_comparison<T, 48>), dim3(L.grid), dim3(kCT), 0, L.stream, sp, P, 1, din, dout, ld, ilo,
-                     iln, newb);
-  hipLaunchKernelGGL((k_comparison<T, 64>), dim3(L.grid), dim3(kCT), 0, L.stream, sp, P, 2, din, dout, ld, ilo,
-                     iln, newb);
-  hipLaunchKernelGGL((k_comparison<T, 96>), dim3(L.grid), dim3(kCT), 0, L.stream, sp, P, 3, din, dout, ld, ilo,
-                     iln, newb);
+void comparison_class(const Launch& L, int cls, const SimParams& sp, const PlanDev& P, const DescDev& din,
+                      const DescDev& dout, const ListDev& ld, const IdlDev& ilo, const IdlDev& iln, int32_t* newb) {
+  switch (cls) {
+    case 0:
+      hipLaunchKernelGGL((k_comparison<T, 32>), dim3(L.grid), dim3(kCT), 0, L.stream, sp, P, 0, din, dout, ld, ilo,
+                         iln, newb);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_comparison<T, 48>), dim3(L.grid), dim3(kCT), 0, L.stream, sp, P, 1, din, dout, ld, ilo,
+                         iln, newb);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_comparison<T, 64>), dim3(L.grid), dim3(kCT), 0, L.stream, sp, P, 2, din, dout, ld, ilo,
+                         iln, newb);
+      break;
+    default:
+      hipLaunchKernelGGL((k_comparison<T, 96>), dim3(L.grid), dim3(kCT), 0, L.stream, sp, P, 3, din, dout, ld, ilo,
+                         iln, newb);
+  }
 }
 
 }  // namespace
 
-void launch_comparison(const Launch& L, const SimParams& sp, const PlanDev& P, const int32_t* state,
-                       const DescDev& din, const DescDev& dout, const ListDev& ld, const IdlDev& il_old,
-                       const IdlDev& il_new, int32_t* newb, bool fp32) {
-  (void)state;
-  if (fp32) comparison_all<float>(L, sp, P, din, dout, ld, il_old, il_new, newb);
-  else comparison_all<double>(L, sp, P, din, dout, ld, il_old, il_new, newb);
+// One size class per launch (boxes n <= 32, 48, 64, 96 for cls 0..3).
+void launch_comparison(const Launch& L, int cls, const SimParams& sp, const PlanDev& P, const DescDev& din,
+                       const DescDev& dout, const ListDev& ld, const IdlDev& il_old, const IdlDev& il_new,
+                       int32_t* newb, bool fp32) {
+  if (fp32) comparison_class<float>(L, cls, sp, P, din, dout, ld, il_old, il_new, newb);
+  else comparison_class<double>(L, cls, sp, P, din, dout, ld, il_old, il_new, newb);
 }
 
 }  // namespace gg

@@ -18,15 +18,16 @@ namespace gg {
 namespace {
 #include "mc_cases.inc"
 
-__constant__ int8_t c_tri[256][16];
-__constant__ uint16_t c_emask[256];
+// Triangle table row of each case: edge indices in nibbles, first triangle in
+// the low nibbles, 0xF after the last entry.
+__constant__ uint64_t c_tri[256];
 
-// cube corner q -> (dx,dy,dz) (src/marchingCubes.cpp:378-386)
-__constant__ int8_t c_corner[8][3] = {{0, 0, 1}, {1, 0, 1}, {1, 0, 0}, {0, 0, 0},
-                                      {0, 1, 1}, {1, 1, 1}, {1, 1, 0}, {0, 1, 0}};
-// edge e -> corner pair (src/marchingCubes.cpp:393-464)
-__constant__ int8_t c_edge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
-                                     {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+// Cube corner q -> (dx,dy,dz) (src/marchingCubes.cpp:378-386) as bit masks
+// over q: {0,0,1} {1,0,1} {1,0,0} {0,0,0} {0,1,1} {1,1,1} {1,1,0} {0,1,0}.
+constexpr unsigned kCornerX = 0x66u, kCornerY = 0xF0u, kCornerZ = 0x33u;
+// Edge e -> corner pair (src/marchingCubes.cpp:393-464) in nibbles:
+// {0,1} {1,2} {2,3} {3,0} {4,5} {5,6} {6,7} {7,4} {0,4} {1,5} {2,6} {3,7}.
+constexpr uint64_t kEdgeA = 0x321076543210ull, kEdgeB = 0x765447650321ull;
 
 struct Acc {
   double s, c;
@@ -46,16 +47,28 @@ struct Acc {
 };
 
 constexpr int kHT = 256;
-constexpr int kChunk = 4 * kHT;  // cells classified per pass
+constexpr int kCPT = 8;              // cells classified per lane and pass
+constexpr int kChunk = kCPT * kHT;  // cells classified per pass
+
+// Offset of corner q of the cell (xl, yl) in the two staged layers L[0..2*NMAX^2).
+template <int NMAX>
+__device__ inline int corner_off(int q, int xl, int yl, int n) {
+  return (int)((kCornerZ >> q) & 1u) * (NMAX * NMAX) + (yl + (int)((kCornerY >> q) & 1u)) * n + xl +
+         (int)((kCornerX >> q) & 1u);
+}
 
 // VertexInterp (src/marchingCubes.cpp:484-499) at iso level 0 on cube edge e of
-// the cell with lower corner (x,y,z).
-__device__ inline void edge_vertex(int e, const double* val, int x, int y, int z, double& px, double& py,
-                                   double& pz) {
-  const int a = c_edge[e][0], b = c_edge[e][1];
-  const double p1x = x + c_corner[a][0], p1y = y + c_corner[a][1], p1z = z + c_corner[a][2];
-  const double p2x = x + c_corner[b][0], p2y = y + c_corner[b][1], p2z = z + c_corner[b][2];
-  const double v1 = val[a], v2 = val[b];
+// the cell with lower corner (x,y,z); the corner values come from LDS.
+template <class T, int NMAX>
+__device__ inline void edge_vertex(int e, const T* L, int xl, int yl, int n, int x, int y, int z, double& px,
+                                   double& py, double& pz) {
+  const int a = (int)((kEdgeA >> (4 * e)) & 15u), b = (int)((kEdgeB >> (4 * e)) & 15u);
+  const double v1 = (double)L[corner_off<NMAX>(a, xl, yl, n)];
+  const double v2 = (double)L[corner_off<NMAX>(b, xl, yl, n)];
+  const double p1x = x + (int)((kCornerX >> a) & 1u), p1y = y + (int)((kCornerY >> a) & 1u),
+               p1z = z + (int)((kCornerZ >> a) & 1u);
+  const double p2x = x + (int)((kCornerX >> b) & 1u), p2y = y + (int)((kCornerY >> b) & 1u),
+               p2z = z + (int)((kCornerZ >> b) & 1u);
   if (fabs(v1) < 0.00001) {
     px = p1x; py = p1y; pz = p1z;
   } else if (fabs(v2) < 0.00001) {
@@ -70,22 +83,25 @@ __device__ inline void edge_vertex(int e, const double* val, int x, int y, int z
   }
 }
 
-// One (grain, z-layer) tile: layers kl and kl+1 are staged in LDS; the cells of
-// the layer are classified 1024 at a time, the cut ones (some corners inside,
-// some outside) compacted in a fixed order, and then every lane works on cut
-// cells, so the triangle work is not left to a few lanes of each wavefront.
+// One (grain, z-layer) tile: layers kl and kl+1 are staged in LDS; each lane
+// classifies kCPT cells, the cut ones (some corners inside, some outside) are
+// compacted in a fixed order by one workgroup scan, and then every lane works
+// on cut cells, so the triangle work is not left to a few lanes of each
+// wavefront.  The triangle table sits in LDS; corner and edge tables are bit
+// masks, so no lookup goes to memory with a lane-dependent address.
 template <class T, int NMAX>
 __global__ __launch_bounds__(kHT) void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag,
                                               double* partials) {
-  __shared__ T L0[NMAX * NMAX];
-  __shared__ T L1[NMAX * NMAX];
+  __shared__ T L[2 * NMAX * NMAX];
   __shared__ uint32_t q[kChunk];
+  __shared__ uint64_t s_tri[256];
   __shared__ int s_wave[kHT / 64];
   __shared__ double s_red[4][kHT / 64];
   __shared__ int s_any[kHT / 64];
   const int total = P.totals[kQLayers * kClasses + cls];
   const T* arena = (const T*)din.arena;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int i = tid; i < 256; i += kHT) s_tri[i] = c_tri[i];
   for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
     int g, kl;
     plan_tile(P, cls, kQLayers, tile, g, kl);
@@ -98,55 +114,58 @@ __global__ __launch_bounds__(kHT) void k_hull(SimParams sp, PlanDev P, int cls, 
     int any = 0;
     if (kl < m1) {
       const T* box = arena + din.off[g] + (int64_t)kl * nn2;
-      stage_to_lds(L0, box, nn2, tid, kHT);
-      stage_to_lds(L1, box + nn2, nn2, tid, kHT);
+      stage_to_lds(L, box, nn2, tid, kHT);
+      stage_to_lds(L + NMAX * NMAX, box + nn2, nn2, tid, kHT);
       __syncthreads();
       for (int c0 = 0; c0 < m1 * m1; c0 += kChunk) {
-        int nq = 0;
+        uint32_t code[kCPT];
+        int mine = 0;
 #pragma unroll
-        for (int r = 0; r < kChunk / kHT; ++r) {
+        for (int r = 0; r < kCPT; ++r) {
           const int c = c0 + r * kHT + tid;
-          uint32_t code = 0;
-          bool cut = false;
+          code[r] = 0xFFFFFFFFu;
           if (c < m1 * m1) {
             const int yl = c / m1, xl = c - yl * m1;
             unsigned ci = 0;
 #pragma unroll
-            for (int qq = 0; qq < 8; ++qq) {
-              const int o = (yl + c_corner[qq][1]) * n + xl + c_corner[qq][0];
-              const T v = c_corner[qq][2] ? L1[o] : L0[o];
-              if (v >= T(0)) ci |= 1u << qq;
+            for (int qq = 0; qq < 8; ++qq)
+              if (L[corner_off<NMAX>(qq, xl, yl, n)] >= T(0)) ci |= 1u << qq;
+            if (ci != 0u && ci != 255u) {
+              code[r] = ci | ((uint32_t)yl << 8) | ((uint32_t)xl << 16);
+              ++mine;
             }
-            cut = ci != 0u && ci != 255u;
-            code = ci | ((uint32_t)yl << 8) | ((uint32_t)xl << 16);
           }
-          const uint64_t m = __ballot(cut);
-          if (lane == 0) s_wave[wv] = __popcll(m);
-          __syncthreads();
-          int pos = nq;
-          for (int w = 0; w < wv; ++w) pos += s_wave[w];
-          pos += __popcll(m & ((1ull << lane) - 1ull));
-          if (cut) q[pos] = code;
-          for (int w = 0; w < kHT / 64; ++w) nq += s_wave[w];
-          __syncthreads();
         }
+        // exclusive scan of the per-lane counts: wavefront, then workgroup
+        int sc = mine;
+        for (int off = 1; off < 64; off <<= 1) {
+          const int y = __shfl_up(sc, off);
+          if (lane >= off) sc += y;
+        }
+        if (lane == 63) s_wave[wv] = sc;
+        __syncthreads();
+        int pos = sc - mine, nq = 0;
+        for (int w = 0; w < kHT / 64; ++w) {
+          if (w < wv) pos += s_wave[w];
+          nq += s_wave[w];
+        }
+#pragma unroll
+        for (int r = 0; r < kCPT; ++r)
+          if (code[r] != 0xFFFFFFFFu) q[pos++] = code[r];
+        __syncthreads();
         if (nq) any = 1;
         for (int w = tid; w < nq; w += kHT) {
-          const uint32_t code = q[w];
-          const unsigned ci = code & 0xFFu;
-          const int yl = (code >> 8) & 0xFF, xl = code >> 16;
-          double val[8];
-#pragma unroll
-          for (int qq = 0; qq < 8; ++qq) {
-            const int o = (yl + c_corner[qq][1]) * n + xl + c_corner[qq][0];
-            val[qq] = (double)(c_corner[qq][2] ? L1[o] : L0[o]);
-          }
+          const uint32_t cd = q[w];
+          const unsigned ci = cd & 0xFFu;
+          const int yl = (cd >> 8) & 0xFF, xl = cd >> 16;
           const int x = x0 + xl, y = y0 + yl;
-          for (int t = 0; t < 16 && c_tri[ci][t] >= 0; t += 3) {
+          uint64_t row = s_tri[ci];
+          while ((row & 15u) != 15u) {
             double p0x, p0y, p0z, p1x, p1y, p1z, p2x, p2y, p2z;
-            edge_vertex(c_tri[ci][t], val, x, y, z, p0x, p0y, p0z);
-            edge_vertex(c_tri[ci][t + 1], val, x, y, z, p1x, p1y, p1z);
-            edge_vertex(c_tri[ci][t + 2], val, x, y, z, p2x, p2y, p2z);
+            edge_vertex<T, NMAX>((int)(row & 15u), L, xl, yl, n, x, y, z, p0x, p0y, p0z);
+            edge_vertex<T, NMAX>((int)((row >> 4) & 15u), L, xl, yl, n, x, y, z, p1x, p1y, p1z);
+            edge_vertex<T, NMAX>((int)((row >> 8) & 15u), L, xl, yl, n, x, y, z, p2x, p2y, p2z);
+            row >>= 12;
             const double v321 = p2x * p1y * p0z;
             const double v231 = p1x * p2y * p0z;
             const double v312 = p2x * p0y * p1z;
@@ -261,24 +280,18 @@ __global__ void k_finalize(SimParams sp, PlanDev P, DescDev din, int32_t* state,
 }  // namespace
 
 void upload_mc_tables() {
-  int8_t tri[256][16];
-  uint16_t em[256];
-  static const int edge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
-                                  {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+  uint64_t tri[256];
   for (int c = 0; c < 256; ++c) {
-    int m = 0;
-    for (int e = 0; e < 12; ++e)
-      if (((c >> edge[e][0]) & 1) != ((c >> edge[e][1]) & 1)) m |= 1 << e;
-    em[c] = (uint16_t)m;
-    for (int i = 0; i < 16; ++i) tri[c][i] = -1;
-    const char* s = kTriCases[c];
-    for (int i = 0; s[i]; ++i) {
-      char ch = s[i];
-      tri[c][i] = (int8_t)((ch >= '0' && ch <= '9') ? ch - '0' : ch - 'a' + 10);
+    uint64_t row = ~0ull;
+    const char* t = kTriCases[c];
+    for (int i = 0; t[i]; ++i) {
+      const char ch = t[i];
+      const uint64_t e = (uint64_t)((ch >= '0' && ch <= '9') ? ch - '0' : ch - 'a' + 10);
+      row = (row & ~(15ull << (4 * i))) | e << (4 * i);
     }
+    tri[c] = row;
   }
   (void)hipMemcpyToSymbol(HIP_SYMBOL(c_tri), tri, sizeof(tri));
-  (void)hipMemcpyToSymbol(HIP_SYMBOL(c_emask), em, sizeof(em));
 }
 
 void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, int32_t* exflag,

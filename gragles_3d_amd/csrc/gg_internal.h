@@ -207,9 +207,9 @@ void launch_idl_resize(const Launch& L, const SimParams& sp, const int32_t* stat
                        const IdlDev& il);
 void launch_lists(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& din,
                   const DescDev& dout, const ListDev& ld, int32_t* newb, int32_t* err);
-void launch_comparison(const Launch& L, const SimParams& sp, const PlanDev& P, const int32_t* state,
-                       const DescDev& din, const DescDev& dout, const ListDev& ld, const IdlDev& il_old,
-                       const IdlDev& il_new, int32_t* newb, bool fp32);
+void launch_comparison(const Launch& L, int cls, const SimParams& sp, const PlanDev& P, const DescDev& din,
+                       const DescDev& dout, const ListDev& ld, const IdlDev& il_old, const IdlDev& il_new,
+                       int32_t* newb, bool fp32);
 void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, int32_t* exflag,
                  double* partials, bool fp32);
 void launch_finalize(const Launch& L, const SimParams& sp, const PlanDev& P, int32_t* state,
