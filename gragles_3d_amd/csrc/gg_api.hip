@@ -324,7 +324,7 @@ int gg_create(const gg_params* params, gg_ctx** out) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return GG_ERR_DEVICE;
   if (params->device < 0 || params->device >= ndev) return GG_ERR_ARG;
-  if (params->n_grains < 1 || params->ngridpoints < 2 || params->grid_blowup < 0 ||
+  if (params->n_grains < 1 || params->n_grains > kMaxGrains || params->ngridpoints < 2 || params->grid_blowup < 0 ||
       (params->precision != GG_FP64_PLAIN && params->precision != GG_FP32_BATCHED))
     return GG_ERR_ARG;
   gg_ctx* c = new gg_ctx;
@@ -369,12 +369,14 @@ int gg_create(const gg_params* params, gg_ctx** out) {
       (rc = dalloc(c, &c->P.cls_start, kClasses)) || (rc = dalloc(c, &c->P.grains, N + 1)) ||
       (rc = dalloc(c, &c->P.grain_pos, N + 1)) || (rc = dalloc(c, &c->P.order, N)) ||
       (rc = dalloc(c, &c->P.totals, kNumQ * kClasses)) ||
-      (rc = dalloc(c, &c->P.layer_base, kClasses)) || (rc = dalloc(c, &c->d_partials, (size_t)4 * (N + 1) * kMaxN))) {
+      (rc = dalloc(c, &c->P.layer_base, kClasses)) || (rc = dalloc(c, &c->P.tile_base, kNumQ * kClasses)) ||
+      (rc = dalloc(c, &c->d_partials, (size_t)4 * (N + 1) * kMaxN))) {
     gg_destroy(c);
     return rc;
   }
   for (int q = 0; q < kNumQ; ++q)
-    if ((rc = dalloc(c, &c->P.pre[q], N + 1 + kClasses))) {
+    if ((rc = dalloc(c, &c->P.pre[q], N + 1 + kClasses)) ||
+        (rc = dalloc(c, &c->P.tmap[q], (size_t)kMaxTilesPerGrain * N))) {
       gg_destroy(c);
       return rc;
     }

@@ -75,7 +75,15 @@ struct PlanDev {
   int32_t* pre[kNumQ];  // [N + kClasses] per-class exclusive prefixes (segment c at cls_start[c]+c)
   int32_t* totals;      // [kNumQ * kClasses]
   int64_t* layer_base;  // [kClasses] first hull-partial slot of each class
+  int32_t* tile_base;   // [kNumQ * kClasses] first entry of each class in tmap[q]
+  uint32_t* tmap[kNumQ];  // [kMaxTilesPerGrain * N] tile -> grain << 8 | tile index inside the grain
 };
+
+// Tiles of one grain for any quantity: n layers <= 96, ceil(n*n/64) <= 144
+// column blocks, ceil(n*(n/2+1)/32) <= 147 spectral blocks; tile indices fit 8
+// bits and grain ids 24 bits of a tmap entry, and tile counts stay below 2^31.
+constexpr int kMaxTilesPerGrain = 160;
+constexpr int kMaxGrains = 8000000;
 
 struct SimParams {
   int32_t N;
@@ -101,16 +109,9 @@ __device__ inline int box_n(const int32_t* b, int g) { return b[6 * g + 3] - b[6
 
 // Locate tile t of class c for quantity q: grain id and local tile index.
 __device__ inline void plan_tile(const PlanDev& P, int c, int q, int t, int& g, int& local) {
-  const int start = P.cls_start[c];
-  const int cnt = P.cls_count[c];
-  const int32_t* pre = P.pre[q] + start + c;
-  int lo = 0, hi = cnt - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (pre[mid] <= t) lo = mid; else hi = mid - 1;
-  }
-  g = P.grains[start + lo];
-  local = t - pre[lo];
+  const uint32_t v = P.tmap[q][P.tile_base[q * kClasses + c] + t];
+  g = (int)(v >> 8);
+  local = (int)(v & 0xFFu);
 }
 
 // Flattened tile lookup over all classes for one quantity.

@@ -89,6 +89,13 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32
       for (int q = 0; q < kNumQ; ++q)
         P.pre[q][cls_start[c] + c + tot[c]] = (int32_t)tot[4 + q * kClasses + c];
     }
+    for (int q = 0; q < kNumQ; ++q) {
+      int64_t tb = 0;
+      for (int c = 0; c < kClasses; ++c) {
+        P.tile_base[q * kClasses + c] = (int32_t)tb;
+        tb += overflow ? 0 : tot[4 + q * kClasses + c];
+      }
+    }
     if (voxels) *voxels = tot[16];
     if (off_out && tot[16] > arena_cap) atomicOr(err, kErrArena);
     if (spec_off && tot[17] > spec_cap) atomicOr(err, kErrSpec);
@@ -115,6 +122,24 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32
     if (spec_off) spec_off[g] = ex[17];
     ex[16] += (int64_t)n * n * n;
     ex[17] += (int64_t)n * n * (n / 2 + 1);
+  }
+}
+
+// Tile map of the plan: for every alive grain and quantity, one entry per tile
+// (grain id << 8 | tile index inside the grain), so a kernel finds its tile's
+// grain with one load.
+__global__ __launch_bounds__(256) void k_tilemap(SimParams sp, PlanDev P) {
+  for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < sp.N; pos += gridDim.x * blockDim.x) {
+    int c = 0;
+    while (c < kClasses && !(pos >= P.cls_start[c] && pos < P.cls_start[c] + P.cls_count[c])) ++c;
+    if (c == kClasses) continue;
+    const uint32_t g = (uint32_t)P.grains[pos];
+    const int seg = pos + c;  // P.pre segment of class c starts at cls_start[c] + c
+    for (int q = 0; q < kNumQ; ++q) {
+      const int b0 = P.pre[q][seg], b1 = P.pre[q][seg + 1];
+      uint32_t* m = P.tmap[q] + P.tile_base[q * kClasses + c];
+      for (int l = b0; l < b1; ++l) m[l] = g << 8 | (uint32_t)(l - b0);
+    }
   }
 }
 
@@ -195,6 +220,8 @@ void launch_plan(const Launch& L, const SimParams& sp, const int32_t* state, con
                  int32_t* err, int64_t* voxels) {
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(kPlanThreads), 0, L.stream, sp, state, bounds, off_out, spec_off, P,
                      arena_cap, spec_cap, err, voxels);
+  const int blocks = (sp.N + 255) / 256 < 1024 ? (sp.N + 255) / 256 : 1024;
+  hipLaunchKernelGGL(k_tilemap, dim3(blocks), dim3(256), 0, L.stream, sp, P);
 }
 
 void launch_init_phi(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din,

@@ -67,7 +67,7 @@ typedef struct gg_params {
   double delta;                /* grainhdl::delta = DomainBorderSize/realDomainSize */
   double dt;                   /* grainhdl::dt = TimeStepSize/realDomainSize^2 */
   int32_t precision;           /* enum gg_precision */
-  int32_t n_grains;            /* Settings::NumberOfParticles */
+  int32_t n_grains;            /* Settings::NumberOfParticles (at most 8000000) */
   int32_t max_list;            /* capacity of one comparison list (0 = 512, at most 1024) */
   int32_t device;              /* HIP device ordinal */
   int64_t arena_voxels;        /* voxels per distance arena (0 = 2x the initial sum of box volumes) */
