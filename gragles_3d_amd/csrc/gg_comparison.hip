@@ -18,8 +18,23 @@ namespace {
 
 constexpr int kCT = 256;
 constexpr int kWaves = kCT / 64;
-constexpr int kLC = 128;            // comparison-list entries compacted per pass
-constexpr int kStageBytes = 24576;  // LDS bytes of staged neighbour values per workgroup
+constexpr int kStageBytes = 20480;  // LDS bytes of staged neighbour values per workgroup
+
+typedef __attribute__((address_space(1))) const void* gptr_t;
+typedef __attribute__((address_space(3))) void* lptr_t;
+
+// Staged value at element el: one 32-bit plane for float, low and high words
+// in two planes (plane stride ps) for double.
+template <class T>
+__device__ inline T staged(const uint32_t* sbuf, int el, int ps);
+template <>
+__device__ inline float staged<float>(const uint32_t* sbuf, int el, int) {
+  return __uint_as_float(sbuf[el]);
+}
+template <>
+__device__ inline double staged<double>(const uint32_t* sbuf, int el, int ps) {
+  return __hiloint2double((int)sbuf[ps + el], (int)sbuf[el]);
+}
 
 // Value of boundaryCondition's `dist` at voxel (j,i,k): the reference assigns
 // it in every branch except when all three axis distances are >= 0 and one is
@@ -62,39 +77,45 @@ __device__ double bc_dist(int j, int i, int k, int x0, int y0, int z0, int x1, i
 
 // One tile = one z-layer of one grain.
 //
-// Active entries.  The comparison list is taken in chunks of kLC entries; the
-// entries of a chunk whose box covers the layer and overlaps the grain's box in
-// x and y are compacted in list order with the rectangle they share with the
-// grain's layer, the rectangle's first arena index, and per wavefront a mask of
-// the rectangle's 8x8 voxel blocks that wavefront owns.
+// Items.  The comparison list is taken in passes of ITEMS entries.  An entry
+// whose box covers the layer and overlaps the grain's box in x and y becomes
+// an item; items are compacted in list order with the rectangle the entry
+// shares with the grain's layer, the rectangle's first arena index, and per
+// wavefront a mask of the rectangle's 8x8 voxel blocks that wavefront owns.
 //
-// Staging.  The rectangles are laid end to end (a_pre = running element count)
-// and copied into LDS in batches of BUF elements, every lane issuing SB
-// independent loads at a time; a lane finds the entry of its element by one
-// binary search per batch and then walks forward.
+// Staging.  Each item's rectangle is staged row-major in LDS, padded to whole
+// 64-element segments, the items following each other (a_c0 = first segment
+// of each).  The segments are copied in batches of NCH by direct global-to-LDS
+// loads: one wavefront load fills one segment (one 32-bit plane of it), so the
+// item, its arena index and its width are uniform across the wavefront, and
+// the loads hold no registers while in flight.
 //
 // Maximum.  The layer is cut into 8x8 blocks, block b owned by wavefront b % 4,
 // one voxel per lane and block, so a lane's voxels stay in its registers for
-// the whole tile.  For each entry of the batch, in list order, a wavefront
-// visits only the blocks the entry's rectangle touches (one scalar bit test per
-// block) and updates with a strict '>'.  A rectangle may straddle two batches:
-// a voxel's staged index grows with the entry's list position, so each voxel
-// still sees its neighbours in list order.
+// the whole tile.  For each item of the batch, in list order, a wavefront
+// visits only the blocks the rectangle touches (one scalar bit test per block)
+// and updates with a strict '>'.  A rectangle may
+// straddle two batches: a voxel's staged index grows with the entry's list
+// position, so each voxel still sees its neighbours in list order.
 template <class T, int NMAX>
 __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int cls, DescDev din, DescDev dout,
                                                     ListDev ld, IdlDev ilo, IdlDev iln, int32_t* newb) {
-  constexpr int NB = (NMAX + 7) / 8;                  // blocks per side, at most
+  constexpr int NB = (NMAX + 7) / 8;                    // blocks per side, at most
   constexpr int VPT = (NB * NB + kWaves - 1) / kWaves;  // blocks per wavefront = voxels per lane
   static_assert(VPT <= 64, "a wavefront's block mask is one 64-bit word");
-  constexpr int BUF = kStageBytes / (int)sizeof(T);
-  constexpr int SB = sizeof(T) == 4 ? 16 : 8;
-  __shared__ T sbuf[BUF];
-  __shared__ int64_t a_base[kLC];  // arena index of the rectangle's first element
-  __shared__ uint64_t a_wm[kWaves][kLC];  // blocks of the rectangle owned by each wavefront
-  __shared__ int32_t a_id[kLC], a_nn[kLC], a_pre[kLC + 1];
-  __shared__ uint32_t a_rect[kLC];   // local rectangle x0 | x1 << 8 | y0 << 16 | y1 << 24
-  __shared__ uint32_t a_magic[kLC];  // ceil(2^32 / width) for the division by the width
-  __shared__ int s_cnt[kLC / 64], s_area[kLC / 64];
+  constexpr int ITEMS = 128;                            // list entries per pass (at most 256: a_of is 8-bit)
+  static_assert(ITEMS % 64 == 0 && ITEMS <= kCT, "whole wavefronts compact the items");
+  constexpr int NCH = kStageBytes / (int)sizeof(T) / 64;  // staged segments per batch
+  constexpr int SB = NCH / kWaves;                          // segments per wavefront and batch
+  constexpr int PL = (int)sizeof(T) / 4;                    // 32-bit planes of a staged value
+  static_assert(SB * kWaves == NCH, "a batch is whole segments for every wavefront");
+  __shared__ uint32_t sbuf[PL * NCH * 64];  // plane p of segment c, lane l at [p*NCH*64 + c*64 + l]
+  __shared__ int4 a_stg[ITEMS];  // arena index (lo, hi), ceil(2^32 / width), nn | width << 8 | height << 16
+  __shared__ uint64_t a_wm[kWaves][ITEMS];  // blocks of the rectangle owned by each wavefront
+  __shared__ int32_t a_id[ITEMS], a_c0[ITEMS + 1];
+  __shared__ uint32_t a_rect[ITEMS];  // local rectangle x0 | x1 << 8 | y0 << 16 | y1 << 24
+  __shared__ uint8_t a_of[NCH];       // item of each segment of the batch
+  __shared__ int s_cnt[ITEMS / 64], s_seg[ITEMS / 64];
   __shared__ int s_red[6][kWaves];
   const int total = P.totals[kQLayers * kClasses + cls];
   const T* __restrict__ ain = (const T*)din.arena;
@@ -129,13 +150,13 @@ __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int
       bid[r] = 0xFFFFFFFFu;  // not covered
     }
 
-    for (int lbase = 0; lbase < len; lbase += kLC) {
-      // compact the active entries of this chunk, keeping list order
+    for (int lbase = 0; lbase < len; lbase += ITEMS) {
+      // compact the items of this pass, keeping list order
       const int idx = lbase + tid;
       bool keep = false;
-      int area = 0, nb = 0, nn = 0, rx0 = 0, rx1 = 0, ry0 = 0, ry1 = 0;
+      int nseg = 0, nb = 0, nn = 0, rx0 = 0, rx1 = 0, ry0 = 0, ry1 = 0;
       int64_t gbase = 0;
-      if (tid < kLC && idx < len) {
+      if (tid < ITEMS && idx < len) {
         nb = lst[idx];
         const int32_t* q = din.bounds + 6 * nb;
         const int q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5];
@@ -144,26 +165,26 @@ __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int
           keep = true;
           nn = q3 - q0;
           rx0 = cx0 - x0; rx1 = cx1 - x0; ry0 = cy0 - y0; ry1 = cy1 - y0;
-          area = (rx1 - rx0) * (ry1 - ry0);
+          nseg = ((rx1 - rx0) * (ry1 - ry0) + 63) >> 6;
           gbase = din.off[nb] + ((int64_t)(k - q2) * nn + (cy0 - q1)) * nn + (cx0 - q0);
         }
       }
       const uint64_t m = __ballot(keep);
-      int sc = area;  // inclusive scan of the areas over the wavefront
+      int sc = nseg;  // inclusive scan of the segment counts over the wavefront
       for (int off = 1; off < 64; off <<= 1) {
         const int y = __shfl_up(sc, off);
         if (lane >= off) sc += y;
       }
-      if (lane == 63 && wv < kLC / 64) {
+      if (lane == 63 && wv < ITEMS / 64) {
         s_cnt[wv] = __popcll(m);
-        s_area[wv] = sc;
+        s_seg[wv] = sc;
       }
       __syncthreads();
       if (keep) {
-        int pos = __popcll(m & ((1ull << lane) - 1ull)), ab = sc - area;
+        int pos = __popcll(m & ((1ull << lane) - 1ull)), c0 = sc - nseg;
         for (int w = 0; w < wv; ++w) {
           pos += s_cnt[w];
-          ab += s_area[w];
+          c0 += s_seg[w];
         }
         uint64_t wm[kWaves] = {};
         for (int by = ry0 >> 3; by <= (ry1 - 1) >> 3; ++by)
@@ -172,80 +193,81 @@ __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int
             wm[bb % kWaves] |= 1ull << (bb / kWaves);
           }
         for (int w = 0; w < kWaves; ++w) a_wm[w][pos] = wm[w];
-        const int rw = rx1 - rx0;
+        const int rw = rx1 - rx0, rh = ry1 - ry0;
         a_id[pos] = nb;
-        a_nn[pos] = nn;
         a_rect[pos] = (uint32_t)rx0 | (uint32_t)rx1 << 8 | (uint32_t)ry0 << 16 | (uint32_t)ry1 << 24;
-        a_magic[pos] = rw > 1 ? (uint32_t)(0xFFFFFFFFu / (uint32_t)rw + 1u) : 0u;
-        a_base[pos] = gbase;
-        a_pre[pos] = ab;
+        a_stg[pos] = make_int4((int)(uint32_t)(uint64_t)gbase, (int)(uint32_t)((uint64_t)gbase >> 32),
+                               rw > 1 ? (int)(0xFFFFFFFFu / (uint32_t)rw + 1u) : 0, nn | rw << 8 | rh << 16);
+        a_c0[pos] = c0;
       }
-      int cnt = 0, E = 0;
-      for (int w = 0; w < kLC / 64; ++w) {
+      int cnt = 0, C = 0;  // items and staged segments of this pass
+      for (int w = 0; w < ITEMS / 64; ++w) {
         cnt += s_cnt[w];
-        E += s_area[w];
+        C += s_seg[w];
       }
-      if (tid == 0) a_pre[cnt] = E;
+      if (tid == 0) a_c0[cnt] = C;
       __syncthreads();
 
-      int a_lo = 0;
-      for (int s = 0; s < E; s += BUF) {
-        const int e = min(s + BUF, E);
-        int a = 0;  // entry of this lane's first element: last a with a_pre[a] <= s + tid
-        {
-          const int t = min(s + tid, e - 1);
+      for (int cs = 0; cs < C; cs += NCH) {
+        const int nc = min(NCH, C - cs);
+        if (tid < nc) {  // item of each segment: last a with a_c0[a] <= cs + tid
+          int a = 0;
 #pragma unroll
-          for (int st = kLC / 2; st > 0; st >>= 1)
-            if (a + st < cnt && a_pre[a + st] <= t) a += st;
-        }
-        for (int t0 = s; t0 < e; t0 += kCT * SB) {
-          T v[SB];
-#pragma unroll
-          for (int u = 0; u < SB; ++u) {
-            const int t = t0 + u * kCT + tid;
-            if (t < e) {
-              while (a_pre[a + 1] <= t) ++a;  // a_pre[cnt] = E > t ends the walk
-              const int r = t - a_pre[a];
-              const uint32_t rc = a_rect[a];
-              const int rw = (int)((rc >> 8) & 0xFFu) - (int)(rc & 0xFFu);
-              const int qr = rw > 1 ? (int)__umulhi((uint32_t)r, a_magic[a]) : r;  // r / rw
-              v[u] = ain[a_base[a] + (int64_t)qr * a_nn[a] + (r - qr * rw)];
-            }
-          }
-#pragma unroll
-          for (int u = 0; u < SB; ++u) {
-            const int t = t0 + u * kCT + tid;
-            if (t < e) sbuf[t - s] = v[u];
-          }
+          for (int st = ITEMS / 2; st > 0; st >>= 1)
+            if (a + st < cnt && a_c0[a + st] <= cs + tid) a += st;
+          a_of[tid] = (uint8_t)a;
         }
         __syncthreads();
-        int aa = a_lo;
-        for (; aa < cnt && a_pre[aa] < e; ++aa) {
+        // one direct global-to-LDS load per segment and plane: lane l of the
+        // wavefront writes word l of the segment, lanes past the rectangle's end
+        // load its first element into the padding
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+          const int c = u * kWaves + wv;  // this wavefront's segment (uniform)
+          if (c < nc) {
+            const int a = __builtin_amdgcn_readfirstlane((int)a_of[c]);
+            const int4 e4 = a_stg[a];
+            const uint32_t blo = (uint32_t)__builtin_amdgcn_readfirstlane(e4.x);
+            const uint32_t bhi = (uint32_t)__builtin_amdgcn_readfirstlane(e4.y);
+            const uint32_t mg = (uint32_t)__builtin_amdgcn_readfirstlane(e4.z);
+            const int wd = __builtin_amdgcn_readfirstlane(e4.w);
+            const int c0 = __builtin_amdgcn_readfirstlane(a_c0[a]);
+            const int nn_ = wd & 0xFF, rw = (wd >> 8) & 0xFF, rh = wd >> 16;
+            const int r = ((cs + c - c0) << 6) + lane;  // element of the rectangle
+            const int qr = rw > 1 ? (int)__umulhi((uint32_t)r, mg) : r;  // r / rw
+            const int64_t off = qr < rh ? (int64_t)qr * nn_ + (r - qr * rw) : 0;
+            const char* src = (const char*)(ain + ((int64_t)((uint64_t)bhi << 32 | blo) + off));
+#pragma unroll
+            for (int pl = 0; pl < PL; ++pl)
+              __builtin_amdgcn_global_load_lds((gptr_t)(src + 4 * pl), (lptr_t)(sbuf + pl * NCH * 64 + c * 64), 4,
+                                               0, 0);
+          }
+        }
+        __builtin_amdgcn_s_waitcnt(0);  // this wavefront's LDS writes are done before the barrier
+        __syncthreads();
+        const uint32_t lim_e = (uint32_t)(nc << 6);
+        // the items with segments in the batch, from the item of its first segment
+        for (int aa = a_of[0]; aa < cnt && a_c0[aa] < cs + nc; ++aa) {
           const uint64_t wmask = ((uint64_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a_wm[wv][aa] >> 32)) << 32) |
                                  (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a_wm[wv][aa]);
           const uint32_t rc = a_rect[aa];
           const int rx0 = rc & 0xFF, ry0 = (rc >> 16) & 0xFF;
           const int rw = (int)((rc >> 8) & 0xFFu) - rx0, rh = (int)(rc >> 24) - ry0;
-          const int pre = a_pre[aa] - s;  // sbuf index of the rectangle's first element (may be < 0)
-          const uint32_t lim_e = (uint32_t)(e - s);
+          const int pre = (a_c0[aa] - cs) << 6;  // sbuf index of the rectangle's first element (may be < 0)
           const uint32_t id = (uint32_t)a_id[aa];
 #pragma unroll
           for (int r = 0; r < VPT; ++r) {
             if ((wmask >> r) & 1ull) {
               const int dx = (ij[r] & 0xFFFF) - rx0, dy = (ij[r] >> 16) - ry0;
               const int el = pre + dy * rw + dx;
-              if ((uint32_t)dx < (uint32_t)rw && (uint32_t)dy < (uint32_t)rh && (uint32_t)el < lim_e) {
-                const T val = sbuf[el];
-                if (val > best[r]) {
-                  best[r] = val;
-                  bid[r] = id;
-                }
-              }
+              const bool ok = (uint32_t)dx < (uint32_t)rw && (uint32_t)dy < (uint32_t)rh && (uint32_t)el < lim_e;
+              const T val = staged<T>(sbuf, ok ? el : 0, NCH * 64);
+              const bool up = ok && val > best[r];
+              best[r] = up ? val : best[r];
+              bid[r] = up ? id : bid[r];
             }
           }
         }
-        // the next batch starts with the entry that straddles e, if any
-        a_lo = a_pre[aa] > e ? aa - 1 : aa;
         __syncthreads();
       }
     }

@@ -43,8 +43,9 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32
   const int gb = 1 + t * chunk;
   const int ge = min(N + 1, gb + chunk);
 
-  // quantities per thread: [0..3] counts, [4..15] tile sums (q*4+c), [16] voxels, [17] spectrum
-  constexpr int NQ = 4 + kNumQ * kClasses + 2;
+  // quantities per thread: [0..3] counts, [4..4+4*kNumQ) tile sums (q*4+c), then voxels and spectrum
+  constexpr int kVox = 4 + kNumQ * kClasses, kSpec = kVox + 1;  // loc/ex/tot slots of voxels, spectrum
+  constexpr int NQ = kSpec + 1;
   int64_t loc[NQ];
   for (int q = 0; q < NQ; ++q) loc[q] = 0;
   int bad = 0;
@@ -62,8 +63,8 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32
     loc[4 + kQLayers * kClasses + c] += n;
     loc[4 + kQCols * kClasses + c] += ceil_div((int64_t)n * n, kColBlock);
     loc[4 + kQSpecCols * kClasses + c] += ceil_div((int64_t)n * (n / 2 + 1), kSpecBlock);
-    loc[16] += (int64_t)n * n * n;
-    loc[17] += (int64_t)n * n * (n / 2 + 1);
+    loc[kVox] += (int64_t)n * n * n;
+    loc[kSpec] += (int64_t)n * n * (n / 2 + 1);
   }
   if (bad) atomicOr(err, kErrBoxSize);
   int64_t ex[NQ], tot[NQ];
@@ -76,7 +77,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32
   }
   // A layout that does not fit its arena leaves an empty plan, so no kernel of
   // this step touches memory beyond the arenas; the host reports the error bit.
-  const bool overflow = (off_out && tot[16] > arena_cap) || (spec_off && tot[17] > spec_cap);
+  const bool overflow = (off_out && tot[kVox] > arena_cap) || (spec_off && tot[kSpec] > spec_cap);
   if (t == 0) {
     int64_t lb = 0;
     for (int c = 0; c < kClasses; ++c) {
@@ -96,9 +97,9 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32
         tb += overflow ? 0 : tot[4 + q * kClasses + c];
       }
     }
-    if (voxels) *voxels = tot[16];
-    if (off_out && tot[16] > arena_cap) atomicOr(err, kErrArena);
-    if (spec_off && tot[17] > spec_cap) atomicOr(err, kErrSpec);
+    if (voxels) *voxels = tot[kVox];
+    if (off_out && tot[kVox] > arena_cap) atomicOr(err, kErrArena);
+    if (spec_off && tot[kSpec] > spec_cap) atomicOr(err, kErrSpec);
   }
   for (int idx = gb; idx < ge; ++idx) {
     const int g = P.order[idx - 1];
@@ -118,10 +119,10 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32
     ex[4 + kQLayers * kClasses + c] += n;
     ex[4 + kQCols * kClasses + c] += ceil_div((int64_t)n * n, kColBlock);
     ex[4 + kQSpecCols * kClasses + c] += ceil_div((int64_t)n * (n / 2 + 1), kSpecBlock);
-    if (off_out) off_out[g] = ex[16];
-    if (spec_off) spec_off[g] = ex[17];
-    ex[16] += (int64_t)n * n * n;
-    ex[17] += (int64_t)n * n * (n / 2 + 1);
+    if (off_out) off_out[g] = ex[kVox];
+    if (spec_off) spec_off[g] = ex[kSpec];
+    ex[kVox] += (int64_t)n * n * n;
+    ex[kSpec] += (int64_t)n * n * (n / 2 + 1);
   }
 }
 
