@@ -50,21 +50,21 @@ def kbar_and_voxels(ctx):
     |B_i cap B_j| / sum_i |B_i|, over the grains alive now, from their box bounds; overall and
     per box-size class (numerator and voxel count per class)."""
     st, _, _ = ctx.stats()
-    alive = np.nonzero(st == G.ALIVE)[0]
-    B = {int(g): ctx.bounds(int(g), 1) for g in alive}
-    num = np.zeros(len(CLASS_NMAX))
-    den = np.zeros(len(CLASS_NMAX))
-    for g in alive:
-        b = B[int(g)]
-        n = int(b[3] - b[0])
-        c = size_class(n)
-        den[c] += n ** 3
-        for j in ctx.neighbour_list(int(g), 0):
-            o = B.get(int(j))
-            if o is None:
-                continue
-            w = [max(0, min(b[a + 3], o[a + 3]) - max(b[a], o[a])) for a in range(3)]
-            num[c] += w[0] * w[1] * w[2]
+    alive = st == G.ALIVE
+    B = ctx.all_bounds(1).astype(np.int64)
+    offs, ids = ctx.all_neighbour_lists(0)
+    n = B[:, 3] - B[:, 0]
+    cls = np.searchsorted(np.array(CLASS_NMAX), n)
+    cls = np.minimum(cls, len(CLASS_NMAX) - 1)
+    own = np.repeat(np.arange(len(offs) - 1), np.diff(offs))
+    keep = alive[own] & alive[ids]
+    own, nb = own[keep], ids[keep]
+    w = np.ones(len(own), dtype=np.int64)
+    for a in range(3):
+        w *= np.maximum(0, np.minimum(B[own, a + 3], B[nb, a + 3]) - np.maximum(B[own, a], B[nb, a]))
+    num = np.bincount(cls[own], weights=w, minlength=len(CLASS_NMAX)).astype(np.float64)
+    g = np.nonzero(alive)[0]
+    den = np.bincount(cls[g], weights=n[g].astype(np.float64) ** 3, minlength=len(CLASS_NMAX))
     return num.sum() / max(den.sum(), 1), num, den
 
 
@@ -94,6 +94,24 @@ def kernel_pass(ctx, steps):
     seg = ctx.stage_times(reset=True)
     ctx.enable_stage_timing(False)
     return seg
+
+
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """Device-to-device copies of a known size (the runtime's blit kernel), so that HBM
+    byte counters read under a profiler can be scaled: a copy of B bytes must read and
+    write B bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    for _ in range(reps):
+        hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log("copy calibration: %d device-to-device copies of %d bytes" % (reps, nbytes))
 
 
 def cpu_baseline(inputs, p, nthreads):
@@ -127,6 +145,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--kernel-steps", type=int, default=4, help="eager event-timed steps for the roofline")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run device-to-device copies of 1 GiB (scale for profiler byte counters)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -134,6 +154,8 @@ def main():
               "(no sharding); --gpus %d is not supported." % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.calibrate_copy:
+        copy_calibration()
     p = G.derive_params(args.grains, args.ppg, args.border)
     t = time.perf_counter()
     idf, bounds, exists = G.generate_voronoi(p, args.seed, True, 0)

@@ -747,6 +747,48 @@ int gg_download_list(gg_ctx* c, int32_t id, int32_t which, int32_t* dst, int32_t
   return GG_OK;
 }
 
+int gg_get_all_bounds(gg_ctx* c, int32_t which, int32_t* bounds) {
+  if (!c || !bounds) return GG_ERR_ARG;
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  const DescDev& d = which ? dout(c) : din(c);
+  HIPCHK(c, hipMemcpy(bounds, d.bounds, 24 * (size_t)(c->p.n_grains + 1), hipMemcpyDeviceToHost));
+  return GG_OK;
+}
+
+int gg_download_all_lists(gg_ctx* c, int32_t which, int32_t* offsets, int32_t* ids, int64_t cap, int64_t* n) {
+  if (!c || !offsets || !n) return GG_ERR_ARG;
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  const int N = c->p.n_grains, M = c->p.max_list;
+  std::vector<int32_t> sel(N + 1), len0(N + 1), len1(N + 1);
+  HIPCHK(c, hipMemcpy(sel.data(), c->LD.sel, 4 * (size_t)(N + 1), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(len0.data(), c->LD.len[0], 4 * (size_t)(N + 1), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(len1.data(), c->LD.len[1], 4 * (size_t)(N + 1), hipMemcpyDeviceToHost));
+  offsets[0] = 0;
+  offsets[1] = 0;
+  int64_t total = 0;
+  for (int g = 1; g <= N; ++g) {
+    const int b = which ? sel[g] ^ 1 : sel[g];
+    total += b ? len1[g] : len0[g];
+    offsets[g + 1] = (int32_t)total;
+  }
+  *n = total;
+  if (!ids) return GG_OK;
+  if (total > cap) return fail(c, GG_ERR_ARG, "destination too small");
+  std::vector<int32_t> buf((size_t)(N + 1) * M);
+  for (int b = 0; b < 2; ++b) {
+    HIPCHK(c, hipMemcpy(buf.data(), c->LD.buf[b], 4 * buf.size(), hipMemcpyDeviceToHost));
+    for (int g = 1; g <= N; ++g) {
+      const int bg = which ? sel[g] ^ 1 : sel[g];
+      if (bg != b) continue;
+      std::copy(buf.begin() + (size_t)g * M, buf.begin() + (size_t)g * M + (offsets[g + 1] - offsets[g]),
+                ids + offsets[g]);
+    }
+  }
+  return GG_OK;
+}
+
 int gg_get_new_bounds(gg_ctx* c, int32_t* bounds) {
   if (!c || !bounds) return GG_ERR_ARG;
   int rc = gg_synchronize(c);

@@ -76,6 +76,8 @@ SIGNATURES = {
     "gg_download_distance": (ctypes.c_int, [_vp, _i32, _i32, _f64p, ctypes.c_int64, _i64p]),
     "gg_download_idlocal": (ctypes.c_int, [_vp, _i32, _i32p, _u32p, ctypes.c_int64, _i64p]),
     "gg_download_list": (ctypes.c_int, [_vp, _i32, _i32, _i32p, _i32, _i32p]),
+    "gg_get_all_bounds": (ctypes.c_int, [_vp, _i32, _i32p]),
+    "gg_download_all_lists": (ctypes.c_int, [_vp, _i32, _i32p, _i32p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     "gg_get_new_bounds": (ctypes.c_int, [_vp, _i32p]),
     "gg_box_voxels": (ctypes.c_int, [_vp, _i64p]),
     "gg_enable_stage_timing": (ctypes.c_int, [_vp, _i32]),
@@ -297,6 +299,22 @@ class LevelSetContext:
         self._chk(lib().gg_download_list(self.h, int(gid), int(which), _p(buf, ctypes.c_int32), buf.size,
                                          ctypes.byref(n)))
         return buf[: n.value].copy()
+
+    def all_bounds(self, which):
+        """Rows 0..n_grains of every grain's buffer bounds (which=0 input, 1 output)."""
+        b = np.zeros((self.params.n_grains + 1, 6), dtype=np.int32)
+        self._chk(lib().gg_get_all_bounds(self.h, int(which), _p(b, ctypes.c_int32)))
+        return b
+
+    def all_neighbour_lists(self, which=0):
+        """Every grain's list as CSR (offsets[g]..offsets[g+1] = grain g)."""
+        offs = np.zeros(self.params.n_grains + 2, dtype=np.int32)
+        n = ctypes.c_int64(0)
+        self._chk(lib().gg_download_all_lists(self.h, int(which), _p(offs, ctypes.c_int32), None, 0, ctypes.byref(n)))
+        ids = np.zeros(max(n.value, 1), dtype=np.int32)
+        self._chk(lib().gg_download_all_lists(self.h, int(which), _p(offs, ctypes.c_int32), _p(ids, ctypes.c_int32),
+                                              ids.size, ctypes.byref(n)))
+        return offs, ids[:n.value]
 
     def new_bounds(self):
         b = np.zeros((self.n_grains + 1, 6), dtype=np.int32)

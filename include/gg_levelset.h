@@ -138,6 +138,14 @@ int gg_download_idlocal(gg_ctx* ctx, int32_t id, int32_t* bounds6, uint32_t* dst
 /* Copy grain id's m_secondOrderNeighbours (which=0) or m_comparisonList (which=1). */
 int gg_download_list(gg_ctx* ctx, int32_t id, int32_t which, int32_t* dst, int32_t cap, int32_t* n);
 
+/* Bulk forms of gg_get_bounds and gg_download_list for inspection (no
+ * reference counterpart: the reference reads these fields per LSbox).  Bounds:
+ * rows 0..n_grains of 6.  Lists: CSR with offsets[0..n_grains+1] (row g =
+ * grain g, row 0 empty), ids filled up to cap entries, *n = total entries;
+ * ids may be NULL to query *n. */
+int gg_get_all_bounds(gg_ctx* ctx, int32_t which, int32_t* bounds);
+int gg_download_all_lists(gg_ctx* ctx, int32_t which, int32_t* offsets, int32_t* ids, int64_t cap, int64_t* n);
+
 /* New bounds from executeSetComparison (m_new{X,Y,Z}{Min,Max}), rows of 6. */
 int gg_get_new_bounds(gg_ctx* ctx, int32_t* bounds);
 

@@ -69,3 +69,17 @@ def test_graph_replay_matches_eager():
     compare_all(ctx_b, orc, "graph after 7 steps")
     ctx_a.close()
     ctx_b.close()
+
+
+def test_bulk_getters_match_per_grain_getters():
+    _say("bulk getters n=27 ppg=12 border=3 seed=11")
+    ctx, orc, p = make_pair(27, 12, 3, 11)
+    step_both(ctx, orc)
+    for which in (0, 1):
+        allb = ctx.all_bounds(which)
+        offs, ids = ctx.all_neighbour_lists(which)
+        assert offs[0] == 0 and offs[1] == 0
+        for g in range(1, p.n_grains + 1):
+            assert np.array_equal(allb[g], ctx.bounds(g, which))
+            assert np.array_equal(ids[offs[g]:offs[g + 1]], ctx.neighbour_list(g, which))
+    ctx.close()
