@@ -281,6 +281,203 @@ __global__ __launch_bounds__(64) void k_redist_z(SimParams sp, PlanDev P, int cl
   }
 }
 
+// Register-resident forms of the two kernels (boxes up to 32 in both types,
+// up to 48 in FP32):
+// one wavefront per tile, a lane keeps its whole line (out and in values) in
+// registers, the sweeps are unrolled over the line with the reference's loop
+// ranges as predicates, and LDS is used once, to turn rows into columns.  The
+// update order along a line is the same as above, so results are identical.
+
+// x/y sweeps of one output layer.  LDS B first holds the in/out intersection of
+// the input layer (read as rows), then the swept out rows (read as columns).
+template <class T, int NMAX>
+__global__ __launch_bounds__(64) void k_redist_xy_reg(SimParams sp, PlanDev P, int cls, DescDev din, DescDev dout,
+                                                      int32_t* err) {
+  constexpr int S = NMAX + 1;
+  __shared__ T B[NMAX * S];
+  const int total = P.totals[kQLayers * kClasses + cls];
+  const T* ain = (const T*)din.arena;
+  T* aout = (T*)dout.arena;
+  const T h = (T)sp.h;
+  const int lane = threadIdx.x;
+  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    int g, kl;
+    plan_tile(P, cls, kQLayers, tile, g, kl);
+    const int32_t* ob = dout.bounds + 6 * g;
+    const int32_t* ib = din.bounds + 6 * g;
+    const int ox0 = ob[0], oy0 = ob[1], oz0 = ob[2], ox1 = ob[3], oy1 = ob[4];
+    const int no = ox1 - ox0;
+    const int ni = ib[3] - ib[0];
+    const int k = oz0 + kl;
+    const int ixmin = max(ib[0], ox0), iymin = max(ib[1], oy0), izmin = max(ib[2], oz0);
+    const int ixmax = min(ib[3], ox1), iymax = min(ib[4], oy1), izmax = min(ib[5], ob[5]);
+    const bool empty = ixmin >= ixmax || iymin >= iymax || izmin >= izmax;
+    if (empty && lane == 0 && kl == 0) atomicOr(err, kErrEmptyIO);
+    const bool do_xy = !empty && k >= izmin && k < izmax - 1;
+    const T* src = ain + din.off[g] + (int64_t)(k - ib[2]) * ni * ni;
+    // local (box) coordinates of the loop ranges
+    const int xlo = ixmin - ox0, xim = ixmax - 1 - ox0, ylo = iymin - oy0, yim = iymax - 1 - oy0;
+    T o[NMAX], in[NMAX];
+#pragma unroll
+    for (int q = 0; q < NMAX; ++q) o[q] = T(-1);
+    if (do_xy) {
+      stage_rows_to_lds(B + ylo * S + xlo, S, src + (int64_t)(iymin - ib[1]) * ni + (ixmin - ib[0]), ni,
+                        iymax - iymin, ixmax - ixmin, lane, 64);
+      __syncthreads();
+      // x-forward then x-backward along row oy0 + lane, for rows in [iymin, oy1)
+      const int r = lane;
+      if (r >= ylo && r < no) {
+        const bool inrow = r <= yim;
+#pragma unroll
+        for (int q = 0; q < NMAX; ++q) in[q] = B[r * S + q];
+#pragma unroll
+        for (int q = 0; q < NMAX - 1; ++q) {
+          if (q >= xlo && q < no - 1) {
+            T cur = o[q];
+            T cand;
+            if (q < xim && inrow) cand = step_interp(in[q], in[q + 1], cur, h);
+            else cand = cur + ((T)sgnv(o[q + 1]) * h);
+            o[q] = cur;
+            if (absv(cand) < absv(o[q + 1])) o[q + 1] = cand;
+          }
+        }
+#pragma unroll
+        for (int q = NMAX - 1; q >= 1; --q) {
+          if (q <= xim) {
+            const T cur = o[q];
+            const T cand = (q > xlo && inrow) ? cur + ((T)sgnv(in[q - 1]) * h) : cur + ((T)sgnv(o[q - 1]) * h);
+            if (absv(cand) < absv(o[q - 1])) o[q - 1] = cand;
+          }
+        }
+      }
+      __syncthreads();
+      if (r < no) {
+#pragma unroll
+        for (int q = 0; q < NMAX; ++q) B[r * S + q] = o[q];
+      }
+      __syncthreads();
+      // y-forward then y-backward along column ox0 + lane, for columns in [ixmin, ox1)
+      const int c = lane;
+      if (c < no) {
+#pragma unroll
+        for (int q = 0; q < NMAX; ++q) o[q] = B[q * S + c];
+      }
+      if (c >= xlo && c < no) {
+        const bool incol = c <= xim;
+        if (incol) {
+          const T* scol = src + (ox0 + c - ib[0]) + (int64_t)(oy0 - ib[1]) * ni;  // scol[q * ni] = in(oy0 + q, j)
+#pragma unroll
+          for (int q = 0; q < NMAX; ++q)
+            if (q >= ylo && q <= yim) in[q] = scol[(int64_t)q * ni];
+        }
+#pragma unroll
+        for (int q = 0; q < NMAX - 1; ++q) {
+          if (q >= ylo && q < no - 1) {
+            T cur = o[q];
+            T cand;
+            if (incol && q < yim) cand = step_interp(in[q], in[q + 1], cur, h);
+            else cand = cur + ((T)sgnv(o[q + 1]) * h);
+            o[q] = cur;
+            if (absv(cand) < absv(o[q + 1])) o[q + 1] = cand;
+          }
+        }
+#pragma unroll
+        for (int q = NMAX - 1; q >= 1; --q) {
+          if (q <= yim) {
+            const T cur = o[q];
+            const T cand = (incol && q > ylo) ? cur + ((T)sgnv(in[q - 1]) * h) : cur + ((T)sgnv(o[q - 1]) * h);
+            if (absv(cand) < absv(o[q - 1])) o[q - 1] = cand;
+          }
+        }
+      }
+    }
+    // lane c writes column ox0 + c of the layer (coalesced across lanes)
+    T* dst = aout + dout.off[g] + (int64_t)kl * no * no;
+    if (lane < no) {
+#pragma unroll
+      for (int q = 0; q < NMAX; ++q)
+        if (q < no) dst[q * no + lane] = o[q];
+    }
+    __syncthreads();
+  }
+}
+
+// z-forward, z-backward and the clamp for 64 columns, one per lane, the column
+// in registers.
+template <class T, int NMAX>
+__global__ __launch_bounds__(64) void k_redist_z_reg(SimParams sp, PlanDev P, int cls, DescDev din, DescDev dout) {
+  const int total = P.totals[kQCols * kClasses + cls];
+  const T* __restrict__ ain = (const T*)din.arena;
+  T* __restrict__ aout = (T*)dout.arena;
+  const T h = (T)sp.h;
+  const T dl = (T)sp.delta;
+  const int lane = threadIdx.x;
+  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    int g, blk;
+    plan_tile(P, cls, kQCols, tile, g, blk);
+    const int32_t* ob = dout.bounds + 6 * g;
+    const int32_t* ib = din.bounds + 6 * g;
+    const int ox0 = ob[0], oy0 = ob[1], oz0 = ob[2], oz1 = ob[5];
+    const int no = ob[3] - ox0;
+    const int ni = ib[3] - ib[0];
+    const int col = blk * kColBlock + lane;
+    if (col >= no * no) continue;
+    const int i = oy0 + col / no, j = ox0 + col % no;
+    const int ixmin = max(ib[0], ox0), iymin = max(ib[1], oy0), izmin = max(ib[2], oz0);
+    const int ixmax = min(ib[3], ob[3]), iymax = min(ib[4], ob[4]), izmax = min(ib[5], oz1);
+    const bool empty = ixmin >= ixmax || iymin >= iymax || izmin >= izmax;
+    const int64_t ls = (int64_t)no * no;
+    T* oc = aout + dout.off[g] + col;  // oc[q * ls] = out(i, j, oz0 + q)
+    T o[NMAX], in[NMAX];
+#pragma unroll
+    for (int q = 0; q < NMAX; ++q)
+      if (q < no) o[q] = oc[q * ls];
+    if (!empty && i >= iymin && j >= ixmin) {
+      const bool inside = i < iymax && j < ixmax;
+      const int zlo = izmin - oz0, zim = izmax - 1 - oz0;
+      if (inside) {
+        const int64_t lsi = (int64_t)ni * ni;
+        const T* ic = ain + din.off[g] + (int64_t)(i - ib[1]) * ni + (j - ib[0]) + (int64_t)(oz0 - ib[2]) * lsi;
+#pragma unroll
+        for (int q = 0; q < NMAX; ++q)
+          if (q >= zlo && q <= zim) in[q] = ic[q * lsi];
+      }
+#pragma unroll
+      for (int q = 0; q < NMAX - 1; ++q) {
+        if (q >= zlo && q < no - 1) {
+          T cur = o[q];
+          T cand;
+          if (q < zim && inside) cand = step_interp(in[q], in[q + 1], cur, h);
+          else cand = cur + ((T)sgnv(o[q + 1]) * h);
+          o[q] = cur;
+          if (absv(cand) < absv(o[q + 1])) o[q + 1] = cand;
+        }
+      }
+#pragma unroll
+      for (int q = NMAX - 1; q >= 1; --q) {
+        if (q <= zim) {
+          T cur = o[q];
+          T cand;
+          if (q > zlo && inside) cand = step_interp(in[q], in[q - 1], cur, h);
+          else cand = cur + ((T)sgnv(o[q - 1]) * h);
+          o[q] = cur;
+          if (absv(cand) < absv(o[q - 1])) o[q - 1] = cand;
+        }
+      }
+    }
+    // clampValues(-delta, delta) (src/dimensionalBufferReal.h:44-57)
+#pragma unroll
+    for (int q = 0; q < NMAX; ++q) {
+      if (q < no) {
+        T v = o[q];
+        if (v < -dl) v = -dl;
+        else if (v > dl) v = dl;
+        oc[q * ls] = v;
+      }
+    }
+  }
+}
+
 template <class T, int NMAX>
 void xy_class(const Launch& L, const SimParams& sp, const PlanDev& P, int cls, const DescDev& din,
               const DescDev& dout, int32_t* err) {
@@ -291,14 +488,22 @@ void xy_class(const Launch& L, const SimParams& sp, const PlanDev& P, int cls, c
 template <class T>
 void redist_all(const Launch& L, int part, const SimParams& sp, const PlanDev& P, const DescDev& din,
                 const DescDev& dout, int32_t* err) {
+  // register-resident lines for boxes up to 32 (both types) and up to 48 (FP32)
+  constexpr bool reg48 = sizeof(T) == 4;
   if (part == 0) {
-    xy_class<T, 32>(L, sp, P, 0, din, dout, err);
-    xy_class<T, 48>(L, sp, P, 1, din, dout, err);
+    hipLaunchKernelGGL((k_redist_xy_reg<T, 32>), dim3(L.grid), dim3(64), 0, L.stream, sp, P, 0, din, dout, err);
+    if constexpr (reg48)
+      hipLaunchKernelGGL((k_redist_xy_reg<T, 48>), dim3(L.grid), dim3(64), 0, L.stream, sp, P, 1, din, dout, err);
+    else
+      xy_class<T, 48>(L, sp, P, 1, din, dout, err);
     xy_class<T, 64>(L, sp, P, 2, din, dout, err);
     xy_class<T, 96>(L, sp, P, 3, din, dout, err);
   } else {
-    hipLaunchKernelGGL((k_redist_z<T, 32>), dim3(L.grid), dim3(64), 0, L.stream, sp, P, 0, din, dout);
-    hipLaunchKernelGGL((k_redist_z<T, 48>), dim3(L.grid), dim3(64), 0, L.stream, sp, P, 1, din, dout);
+    hipLaunchKernelGGL((k_redist_z_reg<T, 32>), dim3(L.grid), dim3(64), 0, L.stream, sp, P, 0, din, dout);
+    if constexpr (reg48)
+      hipLaunchKernelGGL((k_redist_z_reg<T, 48>), dim3(L.grid), dim3(64), 0, L.stream, sp, P, 1, din, dout);
+    else
+      hipLaunchKernelGGL((k_redist_z<T, 48>), dim3(L.grid), dim3(64), 0, L.stream, sp, P, 1, din, dout);
     hipLaunchKernelGGL((k_redist_z<T, 64>), dim3(L.grid), dim3(64), 0, L.stream, sp, P, 2, din, dout);
     hipLaunchKernelGGL((k_redist_z<T, 96>), dim3(L.grid), dim3(64), 0, L.stream, sp, P, 3, din, dout);
   }
