@@ -96,6 +96,22 @@ def kernel_pass(ctx, steps):
     return seg
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_fp32_8000grains.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed profiler summary (FETCH_SIZE and
+    WRITE_SIZE passes of rocprofv3 on this workload, scaled by a copy of known size), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch %s:" % kernel):
+                    return float(line.rsplit(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def copy_calibration(nbytes=1 << 30, reps=3):
     """Device-to-device copies of a known size (the runtime's blit kernel), so that HBM
     byte counters read under a profiler can be scaled: a copy of B bytes must read and
@@ -240,7 +256,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dom["kernel"]),
+            "traffic_source": "profiles/hbm_traffic_fp32_8000grains.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
+                              "copy-calibrated)",
             "bytes_per_launch": dom["bytes_per_launch"],
             "avg_launch_ms": dom["avg_launch_ms"],
             "kbar": dom["kbar"],
