@@ -83,3 +83,15 @@ def test_bulk_getters_match_per_grain_getters():
             assert np.array_equal(allb[g], ctx.bounds(g, which))
             assert np.array_equal(ids[offs[g]:offs[g + 1]], ctx.neighbour_list(g, which))
     ctx.close()
+
+
+def test_512_grains_50_steps_bitexact():
+    # the CPU-runnable configuration: 512 grains, 16 points per grain, border 7
+    # (grid 118^3), 50 timesteps; GPU steps replayed from the captured graph
+    _say("512 grains ppg=16 border=7 seed=1, 50 steps (graph) vs oracle")
+    ctx, orc, p = make_pair(512, 16, 7, 1)
+    ctx.timestep(50, use_graph=True)
+    counts_o = orc.timestep(50)
+    np.testing.assert_array_equal(ctx.count_history(), counts_o)
+    compare_all(ctx, orc, "after 50 steps")
+    ctx.close()
