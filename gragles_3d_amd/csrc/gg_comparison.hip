@@ -75,6 +75,20 @@ __device__ double bc_dist(int j, int i, int k, int x0, int y0, int z0, int x1, i
   }
 }
 
+// True when boundaryCondition leaves voxel (j,i,k) alone: all three axis
+// distances are negative (dist is their maximum, assigned in this voxel) and
+// dist <= -gb.  Integer form of the first lines of bc_dist; most voxels of a
+// box that touches the domain wall take this exit.
+__device__ inline bool deep_inside(int j, int i, int k, int gb, int m) {
+  const int dXm = gb - 1 - j, dXM = j - (m - 1 - gb);
+  const int dYm = gb - 1 - i, dYM = i - (m - 1 - gb);
+  const int dZm = gb - 1 - k, dZM = k - (m - 1 - gb);
+  const int dX = abs(dXm) < abs(dXM) ? dXm : dXM;
+  const int dY = abs(dYm) < abs(dYM) ? dYm : dYM;
+  const int dZ = abs(dZm) < abs(dZM) ? dZm : dZM;
+  return dX < 0 && dY < 0 && dZ < 0 && max(dX, max(dY, dZ)) <= -gb;
+}
+
 // One tile = one z-layer of one grain.
 //
 // Items.  The comparison list is taken in passes of ITEMS entries.  An entry
@@ -124,7 +138,8 @@ __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const T lim = (T)(0.7 * sp.delta);
   const int maxL = ld.max_list;
-  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+  const TileRange tr = xcd_tiles(total);
+  for (int tile = tr.first; tile < tr.end; tile += tr.step) {
     int g, kl;
     plan_tile(P, cls, kQLayers, tile, g, kl);
     const int32_t* b = din.bounds + 6 * g;
@@ -136,16 +151,21 @@ __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int
     const int32_t* lst = ld.buf[lsel] + (int64_t)g * maxL;
     const int len = ld.len[lsel][g];
 
-    // The lane's voxels, packed (i - y0) << 16 | (j - x0); -1 outside the layer.
-    int ij[VPT];
+    // The lane's voxel in its r-th block: (i - y0, j - x0) = (by*8 + ly, bx*8 + lx)
+    // with the block origin uniform across the wavefront.
+    const int ly = lane >> 3, lx = lane & 7;
+    const uint32_t nbx_inv = (65536u + (uint32_t)nbx - 1u) / (uint32_t)nbx;  // bb / nbx = bb * nbx_inv >> 16 (bb < 2^12)
+    auto block_origin = [&](int r, int& oy, int& ox) {
+      const int bb = wv + kWaves * r;
+      const int by = (int)(((uint32_t)bb * nbx_inv) >> 16);
+      oy = by * 8;
+      ox = (bb - by * nbx) * 8;
+      return bb < nbx * nbx;
+    };
     T best[VPT];
     uint32_t bid[VPT];
 #pragma unroll
     for (int r = 0; r < VPT; ++r) {
-      const int bb = wv + kWaves * r;
-      const int by = bb / nbx, bx = bb - by * nbx;
-      const int il = by * 8 + (lane >> 3), jl = bx * 8 + (lane & 7);
-      ij[r] = (bb < nbx * nbx && il < n && jl < n) ? (il << 16) | jl : -1;
       best[r] = (T)(-1.0);
       bid[r] = 0xFFFFFFFFu;  // not covered
     }
@@ -258,7 +278,9 @@ __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int
 #pragma unroll
           for (int r = 0; r < VPT; ++r) {
             if ((wmask >> r) & 1ull) {
-              const int dx = (ij[r] & 0xFFFF) - rx0, dy = (ij[r] >> 16) - ry0;
+              int oy, ox;
+              block_origin(r, oy, ox);
+              const int dx = lx + (ox - rx0), dy = ly + (oy - ry0);
               const int el = pre + dy * rw + dx;
               const bool ok = (uint32_t)dx < (uint32_t)rw && (uint32_t)dy < (uint32_t)rh && (uint32_t)el < lim_e;
               const T val = staged<T>(sbuf, ok ? el : 0, NCH * 64);
@@ -286,11 +308,13 @@ __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int
     int bx0 = INT_MAX, by0 = INT_MAX, bz0 = INT_MAX, bx1 = -INT_MAX, by1 = -INT_MAX, bz1 = -INT_MAX;
 #pragma unroll
     for (int r = 0; r < VPT; ++r) {
-      if (ij[r] < 0) continue;
-      const int il = ij[r] >> 16, jl = ij[r] & 0xFFFF;
+      int oy, ox;
+      const bool inb = block_origin(r, oy, ox);
+      const int il = oy + ly, jl = ox + lx;
+      if (!inb || il >= n || jl >= n) continue;
       const int i = y0 + il, j = x0 + jl;
       const int v = il * n + jl;
-      if (wall) {
+      if (wall && !deep_inside(j, i, k, sp.gb, sp.ngp)) {
         const double dist = bc_dist(j, i, k, x0, y0, z0, x1, y1, sp.gb, sp.ngp);
         if (dist > -sp.gb) {
           const T dh = (T)(dist * sp.h);

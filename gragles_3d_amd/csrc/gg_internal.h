@@ -114,6 +114,22 @@ __device__ inline void plan_tile(const PlanDev& P, int c, int q, int t, int& g, 
   local = (int)(v & 0xFFu);
 }
 
+// Tiles of a grid-stride loop, XCD-aware.  The GPU hands workgroups to its 8
+// XCDs round-robin (workgroup b runs on XCD b % 8) and every XCD has its own
+// L2, so XCD x takes the contiguous tile range [x*T/8, (x+1)*T/8): the tiles of
+// neighbouring grains (the plan orders grains spatially) then share one L2.
+struct TileRange {
+  int first, end, step;
+};
+constexpr int kXcds = 8;
+__device__ inline TileRange xcd_tiles(int total) {
+  const int G = gridDim.x, b = blockIdx.x;
+  if (G % kXcds) return TileRange{b, total, G};
+  const int x = b % kXcds, r = b / kXcds;
+  const int lo = (int)((int64_t)total * x / kXcds), hi = (int)((int64_t)total * (x + 1) / kXcds);
+  return TileRange{lo + r, hi, G / kXcds};
+}
+
 // Flattened tile lookup over all classes for one quantity.
 __device__ inline bool plan_tile_any(const PlanDev& P, int q, int t, int& c, int& g, int& local) {
   for (c = 0; c < kClasses; ++c) {
