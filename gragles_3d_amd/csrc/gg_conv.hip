@@ -20,9 +20,9 @@
 // FP32 batched form: the Gaussian factor is separable,
 // G(kx,ky,kz) = f(kx) f(ky) f(kz), so IDFT . diag(G) . DFT on the box equals
 // three 1-D circular convolutions with the real kernel
-// c[m] = (1/n) sum_k f(min(k,n-k)) cos(2 pi k m / n) (host FP64, stored FP32).
-// Two launches per size class: x then y per z-layer in LDS, then z per block of
-// 64 columns.
+// c[m] = (1/n) sum_k f(min(k,n-k)) cos(2 pi k m / n) (host long double, stored
+// FP32).  Two launches per size class: x then y per z-layer in LDS (vector
+// FMAs), then z per block of 64 columns (matrix cores).
 #include "gg_internal.h"
 
 namespace gg {
@@ -309,33 +309,63 @@ __global__ __launch_bounds__(kT) void k_circ_xy(PlanDev P, int cls, DescDev din,
   }
 }
 
+// Along z the convolution of a block of 64 columns is a product with the n x n
+// circulant matrix C[z][k] = c[(z - k) mod n], a small GEMM run on the matrix
+// cores (v_mfma_f32_16x16x4_f32, FP32 in and out).  n is padded to
+// NP = 16*ceil(n/16); padded k contribute zero (both operands are masked, so
+// stale LDS never meets a zero weight).  C is never stored: its entries are
+// read from the kernel row held twice over (crr), at index (z - k) + n.
+//
+// Operand layout of the 16x16x4 instruction (64 lanes): lane l supplies
+// A[l % 16][k0 + l / 16] and B[k0 + l / 16][l % 16]; it receives
+// D[4 * (l / 16) + v][l % 16] in accumulator v = 0..3.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
 template <int NMAX>
 __global__ __launch_bounds__(kT) void k_circ_z(PlanDev P, int cls, DescDev dout, TablesDev tb) {
   constexpr int CB = kColBlock;
-  __shared__ float Lz[NMAX * CB];
+  constexpr int NPM = (NMAX + 15) / 16 * 16;
+  constexpr int KSM = NPM / 4;  // k-steps of the largest box of the class
+  constexpr int SZ = CB + 1;    // row stride of the staged column block
+  __shared__ float Lz[NPM * SZ];
   __shared__ float crr[2 * NMAX + kRB];
   const int total = P.totals[kQCols * kClasses + cls];
   float* aout = (float*)dout.arena;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
   for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
     int g, blk;
     plan_tile(P, cls, kQCols, tile, g, blk);
     const int n = box_n(dout.bounds, g), nn = n * n;
-    const int c0 = blk * CB;
-    const int G = (n + kRB - 1) / kRB;
+    const int c0 = blk * CB, width = min(CB, nn - c0);
+    const int T16 = (n + 15) >> 4, KS = T16 * 4;
     load_circ_row(crr, tb, n, tid, kT);
     float* base = aout + dout.off[g];
-    stage_rows_to_lds(Lz, CB, base + c0, nn, n, min(CB, nn - c0), tid, kT);  // columns past nn are never read
+    stage_rows_to_lds(Lz, SZ, base + c0, nn, n, width, tid, kT);
     __syncthreads();
-    // item = (group of 4 z, column), consecutive lanes on consecutive columns
-    for (int it = tid; it < G * CB; it += kT) {
-      const int gz = it / CB, cb = it - gz * CB, c = c0 + cb, zz = gz * kRB;
-      if (c >= nn) continue;
-      float acc[kRB];
-      circ4(crr, Lz + cb, CB, n, zz, acc);
+    // out[z][p] = sum_k c[(z - k) mod n] Lz[k][p]; tiles of 16 z by 16 columns
+    for (int t = wv; t < T16 * (CB / 16); t += kT / 64) {
+      const int zt = t / (CB / 16), pt = t - zt * (CB / 16);
+      const int za = zt * 16 + lr, pb = pt * 16 + lr;
+      float av[KSM], bv[KSM];
 #pragma unroll
-      for (int u = 0; u < kRB; ++u)
-        if (zz + u < n) base[(int64_t)(zz + u) * nn + c] = acc[u];
+      for (int kk = 0; kk < KSM; ++kk) {
+        const int k = kk * 4 + lk;
+        const bool ca = k < n && za < n;
+        av[kk] = (kk < KS && ca) ? crr[ca ? za - k + n : 0] : 0.f;
+        bv[kk] = (kk < KS && k < n) ? Lz[k * SZ + pb] : 0.f;
+      }
+      v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kk = 0; kk < KSM; ++kk)
+        if (kk < KS) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk], bv[kk], acc, 0, 0, 0);
+      if (pb < width) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int zz = zt * 16 + 4 * lk + v;
+          if (zz < n) base[(int64_t)zz * nn + c0 + pb] = acc[v];
+        }
+      }
     }
     __syncthreads();
   }
