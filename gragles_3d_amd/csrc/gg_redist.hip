@@ -281,12 +281,38 @@ __global__ __launch_bounds__(64) void k_redist_z(SimParams sp, PlanDev P, int cl
   }
 }
 
+// One forward step of a register-resident sweep at q -> q+1 (the loop bodies
+// above): `act` = q inside the loop range, `ip` = interpolate (in values a, b
+// at q, q+1).  FP64 keeps the reference's two divisions (bit-identical); FP32
+// uses d = -a*h/(b - a) with the hardware reciprocal and no branch, which
+// rounds differently but is the same formula.
+__device__ inline void fwd_step(double& cur, double& nxt, double a, double b, bool act, bool ip, double h) {
+  if (!act) return;
+  double cand;
+  if (ip) cand = step_interp(a, b, cur, h);
+  else cand = cur + ((double)sgnv(nxt) * h);
+  if (absv(cand) < absv(nxt)) nxt = cand;
+}
+__device__ inline float sgnh(float v, float h) { return v > 0.f ? h : (v < 0.f ? -h : 0.f); }
+__device__ inline void fwd_step(float& cur, float& nxt, float a, float b, bool act, bool ip, float h) {
+  const float dba = b - a;
+  const float d = -a * h * __builtin_amdgcn_rcpf(dba);
+  const bool moved = ip & (a * b <= 0.f) & (fabsf(cur) > fabsf(d));
+  const float c1 = moved ? -d * (dba > 0.f ? 1.f : (dba < 0.f ? -1.f : 0.f)) : cur;
+  const float cand = c1 + sgnh(ip ? b : nxt, h);
+  const bool take = act & (fabsf(cand) < fabsf(nxt));
+  cur = act ? c1 : cur;
+  nxt = take ? cand : nxt;
+}
+
 // Register-resident forms of the two kernels (boxes up to 32 in both types,
-// up to 48 in FP32):
-// one wavefront per tile, a lane keeps its whole line (out and in values) in
-// registers, the sweeps are unrolled over the line with the reference's loop
-// ranges as predicates, and LDS is used once, to turn rows into columns.  The
-// update order along a line is the same as above, so results are identical.
+// up to 48 in FP32): one wavefront per tile, a lane keeps its whole line (out
+// and in values) in registers, the sweeps are unrolled over the line with the
+// reference's loop ranges as predicates, and LDS is used once, to turn rows
+// into columns.  The update order along a line is the same as above, so FP64
+// results are identical; the FP32 steps are written without branches (vector
+// selects), which a sequential recurrence needs to avoid an exec-mask round
+// trip per step.
 
 // x/y sweeps of one output layer.  LDS B first holds the in/out intersection of
 // the input layer (read as rows), then the swept out rows (read as columns).
@@ -316,10 +342,16 @@ __global__ __launch_bounds__(64) void k_redist_xy_reg(SimParams sp, PlanDev P, i
     const bool do_xy = !empty && k >= izmin && k < izmax - 1;
     const T* src = ain + din.off[g] + (int64_t)(k - ib[2]) * ni * ni;
     // local (box) coordinates of the loop ranges
-    const int xlo = ixmin - ox0, xim = ixmax - 1 - ox0, ylo = iymin - oy0, yim = iymax - 1 - oy0;
+    // loop-range bounds as per-lane values: the predicates below then compare in
+    // vector registers instead of building a scalar lane mask for every step
+    int xlo = ixmin - ox0, xim = ixmax - 1 - ox0, ylo = iymin - oy0, yim = iymax - 1 - oy0, nol = no - 1;
+    __asm__ volatile("" : "+v"(xlo), "+v"(xim), "+v"(ylo), "+v"(yim), "+v"(nol));
     T o[NMAX], in[NMAX];
 #pragma unroll
-    for (int q = 0; q < NMAX; ++q) o[q] = T(-1);
+    for (int q = 0; q < NMAX; ++q) {
+      o[q] = T(-1);
+      __asm__ volatile("" : "+v"(o[q]));  // opaque: no control flow specialised on the initial value
+    }
     if (do_xy) {
       stage_rows_to_lds(B + ylo * S + xlo, S, src + (int64_t)(iymin - ib[1]) * ni + (ixmin - ib[0]), ni,
                         iymax - iymin, ixmax - ixmin, lane, 64);
@@ -331,23 +363,12 @@ __global__ __launch_bounds__(64) void k_redist_xy_reg(SimParams sp, PlanDev P, i
 #pragma unroll
         for (int q = 0; q < NMAX; ++q) in[q] = B[r * S + q];
 #pragma unroll
-        for (int q = 0; q < NMAX - 1; ++q) {
-          if (q >= xlo && q < no - 1) {
-            T cur = o[q];
-            T cand;
-            if (q < xim && inrow) cand = step_interp(in[q], in[q + 1], cur, h);
-            else cand = cur + ((T)sgnv(o[q + 1]) * h);
-            o[q] = cur;
-            if (absv(cand) < absv(o[q + 1])) o[q + 1] = cand;
-          }
-        }
+        for (int q = 0; q < NMAX - 1; ++q)
+          fwd_step(o[q], o[q + 1], in[q], in[q + 1], (q >= xlo) & (q < nol), (q < xim) & inrow, h);
 #pragma unroll
         for (int q = NMAX - 1; q >= 1; --q) {
-          if (q <= xim) {
-            const T cur = o[q];
-            const T cand = (q > xlo && inrow) ? cur + ((T)sgnv(in[q - 1]) * h) : cur + ((T)sgnv(o[q - 1]) * h);
-            if (absv(cand) < absv(o[q - 1])) o[q - 1] = cand;
-          }
+          const T cand = o[q] + ((T)sgnv(((q > xlo) & inrow) ? in[q - 1] : o[q - 1]) * h);
+          o[q - 1] = ((q <= xim) & (absv(cand) < absv(o[q - 1]))) ? cand : o[q - 1];
         }
       }
       __syncthreads();
@@ -371,23 +392,12 @@ __global__ __launch_bounds__(64) void k_redist_xy_reg(SimParams sp, PlanDev P, i
             if (q >= ylo && q <= yim) in[q] = scol[(int64_t)q * ni];
         }
 #pragma unroll
-        for (int q = 0; q < NMAX - 1; ++q) {
-          if (q >= ylo && q < no - 1) {
-            T cur = o[q];
-            T cand;
-            if (incol && q < yim) cand = step_interp(in[q], in[q + 1], cur, h);
-            else cand = cur + ((T)sgnv(o[q + 1]) * h);
-            o[q] = cur;
-            if (absv(cand) < absv(o[q + 1])) o[q + 1] = cand;
-          }
-        }
+        for (int q = 0; q < NMAX - 1; ++q)
+          fwd_step(o[q], o[q + 1], in[q], in[q + 1], (q >= ylo) & (q < nol), incol & (q < yim), h);
 #pragma unroll
         for (int q = NMAX - 1; q >= 1; --q) {
-          if (q <= yim) {
-            const T cur = o[q];
-            const T cand = (incol && q > ylo) ? cur + ((T)sgnv(in[q - 1]) * h) : cur + ((T)sgnv(o[q - 1]) * h);
-            if (absv(cand) < absv(o[q - 1])) o[q - 1] = cand;
-          }
+          const T cand = o[q] + ((T)sgnv((incol & (q > ylo)) ? in[q - 1] : o[q - 1]) * h);
+          o[q - 1] = ((q <= yim) & (absv(cand) < absv(o[q - 1]))) ? cand : o[q - 1];
         }
       }
     }
@@ -434,7 +444,8 @@ __global__ __launch_bounds__(64) void k_redist_z_reg(SimParams sp, PlanDev P, in
       if (q < no) o[q] = oc[q * ls];
     if (!empty && i >= iymin && j >= ixmin) {
       const bool inside = i < iymax && j < ixmax;
-      const int zlo = izmin - oz0, zim = izmax - 1 - oz0;
+      int zlo = izmin - oz0, zim = izmax - 1 - oz0, nol = no - 1;
+      __asm__ volatile("" : "+v"(zlo), "+v"(zim), "+v"(nol));  // per-lane predicates (see above)
       if (inside) {
         const int64_t lsi = (int64_t)ni * ni;
         const T* ic = ain + din.off[g] + (int64_t)(i - ib[1]) * ni + (j - ib[0]) + (int64_t)(oz0 - ib[2]) * lsi;
@@ -443,27 +454,12 @@ __global__ __launch_bounds__(64) void k_redist_z_reg(SimParams sp, PlanDev P, in
           if (q >= zlo && q <= zim) in[q] = ic[q * lsi];
       }
 #pragma unroll
-      for (int q = 0; q < NMAX - 1; ++q) {
-        if (q >= zlo && q < no - 1) {
-          T cur = o[q];
-          T cand;
-          if (q < zim && inside) cand = step_interp(in[q], in[q + 1], cur, h);
-          else cand = cur + ((T)sgnv(o[q + 1]) * h);
-          o[q] = cur;
-          if (absv(cand) < absv(o[q + 1])) o[q + 1] = cand;
-        }
-      }
+      for (int q = 0; q < NMAX - 1; ++q)
+        fwd_step(o[q], o[q + 1], in[q], in[q + 1], (q >= zlo) & (q < nol), (q < zim) & inside, h);
+      // z-backward interpolates too: the same step towards q - 1
 #pragma unroll
-      for (int q = NMAX - 1; q >= 1; --q) {
-        if (q <= zim) {
-          T cur = o[q];
-          T cand;
-          if (q > zlo && inside) cand = step_interp(in[q], in[q - 1], cur, h);
-          else cand = cur + ((T)sgnv(o[q - 1]) * h);
-          o[q] = cur;
-          if (absv(cand) < absv(o[q - 1])) o[q - 1] = cand;
-        }
-      }
+      for (int q = NMAX - 1; q >= 1; --q)
+        fwd_step(o[q], o[q - 1], in[q], in[q - 1], q <= zim, (q > zlo) & inside, h);
     }
     // clampValues(-delta, delta) (src/dimensionalBufferReal.h:44-57)
 #pragma unroll
