@@ -110,6 +110,8 @@ __global__ __launch_bounds__(kHT) void k_hull(SimParams sp, PlanDev P, int cls, 
     const int x0 = b[0], y0 = b[1], z0 = b[2];
     const int n = b[3] - b[0], nn2 = n * n, m1 = n - 1;
     const int z = z0 + kl;
+    // c / m1 = c * m1_inv >> 24 for every cell index c < m1^2 (m1 < 96)
+    const uint32_t m1_inv = m1 > 0 ? ((1u << 24) + (uint32_t)m1 - 1u) / (uint32_t)m1 : 0u;
     Acc vol{0.0, 0.0}, area{0.0, 0.0};
     int any = 0;
     if (kl < m1) {
@@ -125,7 +127,7 @@ __global__ __launch_bounds__(kHT) void k_hull(SimParams sp, PlanDev P, int cls, 
           const int c = c0 + r * kHT + tid;
           code[r] = 0xFFFFFFFFu;
           if (c < m1 * m1) {
-            const int yl = c / m1, xl = c - yl * m1;
+            const int yl = (int)(((uint32_t)c * m1_inv) >> 24), xl = c - yl * m1;
             unsigned ci = 0;
 #pragma unroll
             for (int qq = 0; qq < 8; ++qq)
