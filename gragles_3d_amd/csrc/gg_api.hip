@@ -98,7 +98,7 @@ int dalloc(gg_ctx* c, T** ptr, size_t count) {
 
 const double kPi = 3.14159265358979323846;
 
-Launch launch_of(gg_ctx* c) { return Launch{c->stream, c->grid}; }
+Launch launch_of(gg_ctx* c) { return Launch{c->stream, c->grid, c->grid / 8}; }
 
 int check_device_error(gg_ctx* c) {
   int32_t e = 0;

@@ -209,8 +209,16 @@ struct gg_ctx;
 namespace gg {
 struct Launch {
   hipStream_t stream;
-  int grid;   // workgroups per grid-stride launch
+  int grid;   // workgroups per grid-stride launch (8 per compute unit)
+  int cus;    // compute units of the device
 };
+// Workgroups of a tile launch: per_cu per compute unit.  Grids of about one
+// workgroup per tile let the hardware dispatcher balance tiles of uneven cost
+// (a new workgroup starts wherever one finishes), which a persistent grid with
+// a fixed share of tiles per workgroup does not; the grid-stride loop of each
+// kernel covers tiles beyond the grid.  The values at the launch sites are the
+// fastest of a sweep on the 8000-grain workload.
+inline int tile_grid(const Launch& L, int per_cu) { return L.cus * per_cu; }
 
 void launch_plan(const Launch& L, const SimParams& sp, const int32_t* state, const int32_t* bounds,
                  int64_t* off_out, int64_t* spec_off, const PlanDev& P, int64_t arena_cap,
