@@ -20,6 +20,8 @@ constexpr int kGridPerCU = 512;  // workgroups per CU of a comparison launch (on
 constexpr int kCT = 256;
 constexpr int kWaves = kCT / 64;
 constexpr int kStageBytes = 20480;  // LDS bytes of staged neighbour values per workgroup
+constexpr int kDirectMax = 32;      // box sides up to this gather neighbour values directly
+constexpr int kDepth = 2;           // list entries whose loads are in flight together (direct gathers)
 
 typedef __attribute__((address_space(1))) const void* gptr_t;
 typedef __attribute__((address_space(3))) void* lptr_t;
@@ -98,6 +100,11 @@ __device__ inline bool deep_inside(int j, int i, int k, int gb, int m) {
 // shares with the grain's layer, the rectangle's first arena index, and per
 // wavefront a mask of the rectangle's 8x8 voxel blocks that wavefront owns.
 //
+// Boxes up to kDirectMax skip the items and the staging: their wavefronts
+// gather the neighbour values straight from the arena (described at that
+// branch), which keeps the workgroup free of LDS and of barriers until the
+// bounding-box reduction.
+//
 // Staging.  Each item's rectangle is staged row-major in LDS, padded to whole
 // 64-element segments, the items following each other (a_c0 = first segment
 // of each).  The segments are copied in batches of NCH by direct global-to-LDS
@@ -117,6 +124,7 @@ __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int
                                                     ListDev ld, IdlDev ilo, IdlDev iln, int32_t* newb) {
   constexpr int NB = (NMAX + 7) / 8;                    // blocks per side, at most
   constexpr int VPT = (NB * NB + kWaves - 1) / kWaves;  // blocks per wavefront = voxels per lane
+  constexpr bool kDirect = NMAX <= kDirectMax;          // gathers straight from the arena, no staging
   static_assert(VPT <= 64, "a wavefront's block mask is one 64-bit word");
   constexpr int ITEMS = 128;                            // list entries per pass (at most 256: a_of is 8-bit)
   static_assert(ITEMS % 64 == 0 && ITEMS <= kCT, "whole wavefronts compact the items");
@@ -171,127 +179,223 @@ __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int
       bid[r] = 0xFFFFFFFFu;  // not covered
     }
 
-    for (int lbase = 0; lbase < len; lbase += ITEMS) {
-      // compact the items of this pass, keeping list order
-      const int idx = lbase + tid;
-      bool keep = false;
-      int nseg = 0, nb = 0, nn = 0, rx0 = 0, rx1 = 0, ry0 = 0, ry1 = 0;
-      int64_t gbase = 0;
-      if (tid < ITEMS && idx < len) {
-        nb = lst[idx];
-        const int32_t* q = din.bounds + 6 * nb;
-        const int q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5];
-        const int cx0 = max(x0, q0), cx1 = min(x1, q3), cy0 = max(y0, q1), cy1 = min(y1, q4);
-        if (q2 <= k && k < q5 && cx0 < cx1 && cy0 < cy1) {
-          keep = true;
-          nn = q3 - q0;
-          rx0 = cx0 - x0; rx1 = cx1 - x0; ry0 = cy0 - y0; ry1 = cy1 - y0;
-          nseg = ((rx1 - rx0) * (ry1 - ry0) + 63) >> 6;
-          gbase = din.off[nb] + ((int64_t)(k - q2) * nn + (cy0 - q1)) * nn + (cx0 - q0);
-        }
-      }
-      const uint64_t m = __ballot(keep);
-      int sc = nseg;  // inclusive scan of the segment counts over the wavefront
-      for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(sc, off);
-        if (lane >= off) sc += y;
-      }
-      if (lane == 63 && wv < ITEMS / 64) {
-        s_cnt[wv] = __popcll(m);
-        s_seg[wv] = sc;
-      }
-      __syncthreads();
-      if (keep) {
-        int pos = __popcll(m & ((1ull << lane) - 1ull)), c0 = sc - nseg;
-        for (int w = 0; w < wv; ++w) {
-          pos += s_cnt[w];
-          c0 += s_seg[w];
-        }
-        uint64_t wm[kWaves] = {};
-        for (int by = ry0 >> 3; by <= (ry1 - 1) >> 3; ++by)
-          for (int bx = rx0 >> 3; bx <= (rx1 - 1) >> 3; ++bx) {
-            const int bb = by * nbx + bx;
-            wm[bb % kWaves] |= 1ull << (bb / kWaves);
+    if constexpr (!kDirect) {
+      for (int lbase = 0; lbase < len; lbase += ITEMS) {
+        // compact the items of this pass, keeping list order
+        const int idx = lbase + tid;
+        bool keep = false;
+        int nseg = 0, nb = 0, nn = 0, rx0 = 0, rx1 = 0, ry0 = 0, ry1 = 0;
+        int64_t gbase = 0;
+        if (tid < ITEMS && idx < len) {
+          nb = lst[idx];
+          const int32_t* q = din.bounds + 6 * nb;
+          const int q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5];
+          const int cx0 = max(x0, q0), cx1 = min(x1, q3), cy0 = max(y0, q1), cy1 = min(y1, q4);
+          if (q2 <= k && k < q5 && cx0 < cx1 && cy0 < cy1) {
+            keep = true;
+            nn = q3 - q0;
+            rx0 = cx0 - x0; rx1 = cx1 - x0; ry0 = cy0 - y0; ry1 = cy1 - y0;
+            nseg = ((rx1 - rx0) * (ry1 - ry0) + 63) >> 6;
+            gbase = din.off[nb] + ((int64_t)(k - q2) * nn + (cy0 - q1)) * nn + (cx0 - q0);
           }
-        for (int w = 0; w < kWaves; ++w) a_wm[w][pos] = wm[w];
-        const int rw = rx1 - rx0, rh = ry1 - ry0;
-        a_id[pos] = nb;
-        a_rect[pos] = (uint32_t)rx0 | (uint32_t)rx1 << 8 | (uint32_t)ry0 << 16 | (uint32_t)ry1 << 24;
-        a_stg[pos] = make_int4((int)(uint32_t)(uint64_t)gbase, (int)(uint32_t)((uint64_t)gbase >> 32),
-                               rw > 1 ? (int)(0xFFFFFFFFu / (uint32_t)rw + 1u) : 0, nn | rw << 8 | rh << 16);
-        a_c0[pos] = c0;
-      }
-      int cnt = 0, C = 0;  // items and staged segments of this pass
-      for (int w = 0; w < ITEMS / 64; ++w) {
-        cnt += s_cnt[w];
-        C += s_seg[w];
-      }
-      if (tid == 0) a_c0[cnt] = C;
-      __syncthreads();
+        }
+        const uint64_t m = __ballot(keep);
+        int sc = nseg;  // inclusive scan of the segment counts over the wavefront
+        for (int off = 1; off < 64; off <<= 1) {
+          const int y = __shfl_up(sc, off);
+          if (lane >= off) sc += y;
+        }
+        if (lane == 63 && wv < ITEMS / 64) {
+          s_cnt[wv] = __popcll(m);
+          s_seg[wv] = sc;
+        }
+        __syncthreads();
+        if (keep) {
+          int pos = __popcll(m & ((1ull << lane) - 1ull)), c0 = sc - nseg;
+          for (int w = 0; w < wv; ++w) {
+            pos += s_cnt[w];
+            c0 += s_seg[w];
+          }
+          uint64_t wm[kWaves] = {};
+          for (int by = ry0 >> 3; by <= (ry1 - 1) >> 3; ++by)
+            for (int bx = rx0 >> 3; bx <= (rx1 - 1) >> 3; ++bx) {
+              const int bb = by * nbx + bx;
+              wm[bb % kWaves] |= 1ull << (bb / kWaves);
+            }
+          for (int w = 0; w < kWaves; ++w) a_wm[w][pos] = wm[w];
+          const int rw = rx1 - rx0, rh = ry1 - ry0;
+          a_id[pos] = nb;
+          a_rect[pos] = (uint32_t)rx0 | (uint32_t)rx1 << 8 | (uint32_t)ry0 << 16 | (uint32_t)ry1 << 24;
+          a_stg[pos] = make_int4((int)(uint32_t)(uint64_t)gbase, (int)(uint32_t)((uint64_t)gbase >> 32),
+                                 rw > 1 ? (int)(0xFFFFFFFFu / (uint32_t)rw + 1u) : 0, nn | rw << 8 | rh << 16);
+          a_c0[pos] = c0;
+        }
+        int cnt = 0, C = 0;  // items and staged segments of this pass
+        for (int w = 0; w < ITEMS / 64; ++w) {
+          cnt += s_cnt[w];
+          C += s_seg[w];
+        }
+        if (tid == 0) a_c0[cnt] = C;
+        __syncthreads();
 
-      for (int cs = 0; cs < C; cs += NCH) {
-        const int nc = min(NCH, C - cs);
-        if (tid < nc) {  // item of each segment: last a with a_c0[a] <= cs + tid
-          int a = 0;
-#pragma unroll
-          for (int st = ITEMS / 2; st > 0; st >>= 1)
-            if (a + st < cnt && a_c0[a + st] <= cs + tid) a += st;
-          a_of[tid] = (uint8_t)a;
+        for (int cs = 0; cs < C; cs += NCH) {
+          const int nc = min(NCH, C - cs);
+          if (tid < nc) {  // item of each segment: last a with a_c0[a] <= cs + tid
+            int a = 0;
+  #pragma unroll
+            for (int st = ITEMS / 2; st > 0; st >>= 1)
+              if (a + st < cnt && a_c0[a + st] <= cs + tid) a += st;
+            a_of[tid] = (uint8_t)a;
+          }
+          __syncthreads();
+          // one direct global-to-LDS load per segment and plane: lane l of the
+          // wavefront writes word l of the segment, lanes past the rectangle's end
+          // load its first element into the padding
+  #pragma unroll
+          for (int u = 0; u < SB; ++u) {
+            const int c = u * kWaves + wv;  // this wavefront's segment (uniform)
+            if (c < nc) {
+              const int a = __builtin_amdgcn_readfirstlane((int)a_of[c]);
+              const int4 e4 = a_stg[a];
+              const uint32_t blo = (uint32_t)__builtin_amdgcn_readfirstlane(e4.x);
+              const uint32_t bhi = (uint32_t)__builtin_amdgcn_readfirstlane(e4.y);
+              const uint32_t mg = (uint32_t)__builtin_amdgcn_readfirstlane(e4.z);
+              const int wd = __builtin_amdgcn_readfirstlane(e4.w);
+              const int c0 = __builtin_amdgcn_readfirstlane(a_c0[a]);
+              const int nn_ = wd & 0xFF, rw = (wd >> 8) & 0xFF, rh = wd >> 16;
+              const int r = ((cs + c - c0) << 6) + lane;  // element of the rectangle
+              const int qr = rw > 1 ? (int)__umulhi((uint32_t)r, mg) : r;  // r / rw
+              const int64_t off = qr < rh ? (int64_t)qr * nn_ + (r - qr * rw) : 0;
+              const char* src = (const char*)(ain + ((int64_t)((uint64_t)bhi << 32 | blo) + off));
+  #pragma unroll
+              for (int pl = 0; pl < PL; ++pl)
+                __builtin_amdgcn_global_load_lds((gptr_t)(src + 4 * pl), (lptr_t)(sbuf + pl * NCH * 64 + c * 64), 4,
+                                                 0, 0);
+            }
+          }
+          __builtin_amdgcn_s_waitcnt(0);  // this wavefront's LDS writes are done before the barrier
+          __syncthreads();
+          const uint32_t lim_e = (uint32_t)(nc << 6);
+          // the items with segments in the batch, from the item of its first segment
+          for (int aa = a_of[0]; aa < cnt && a_c0[aa] < cs + nc; ++aa) {
+            const uint64_t wmask = ((uint64_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a_wm[wv][aa] >> 32)) << 32) |
+                                   (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a_wm[wv][aa]);
+            const uint32_t rc = a_rect[aa];
+            const int rx0 = rc & 0xFF, ry0 = (rc >> 16) & 0xFF;
+            const int rw = (int)((rc >> 8) & 0xFFu) - rx0, rh = (int)(rc >> 24) - ry0;
+            const int pre = (a_c0[aa] - cs) << 6;  // sbuf index of the rectangle's first element (may be < 0)
+            const uint32_t id = (uint32_t)a_id[aa];
+  #pragma unroll
+            for (int r = 0; r < VPT; ++r) {
+              if ((wmask >> r) & 1ull) {
+                int oy, ox;
+                block_origin(r, oy, ox);
+                const int dx = lx + (ox - rx0), dy = ly + (oy - ry0);
+                const int el = pre + dy * rw + dx;
+                const bool ok = (uint32_t)dx < (uint32_t)rw && (uint32_t)dy < (uint32_t)rh && (uint32_t)el < lim_e;
+                const T val = staged<T>(sbuf, ok ? el : 0, NCH * 64);
+                const bool up = ok && val > best[r];
+                best[r] = up ? val : best[r];
+                bid[r] = up ? id : bid[r];
+              }
+            }
+          }
+          __syncthreads();
         }
-        __syncthreads();
-        // one direct global-to-LDS load per segment and plane: lane l of the
-        // wavefront writes word l of the segment, lanes past the rectangle's end
-        // load its first element into the padding
+      }
+    } else {
+      // Direct gathers (boxes up to kDirectMax): each wavefront walks the list on
+      // its own, one entry per lane to set up a pass of 64 entries (the shared
+      // rectangle, and where the entry's layer k starts), compacts the entries
+      // that touch the layer with a ballot and takes them in list order,
+      // kDepth at a time: the loads of all their blocks are issued before the
+      // first update.  Lanes off a rectangle load its first voxel and do not update.
+      int yr[VPT], xr[VPT];  // the lane's voxel in block r, local to the box
 #pragma unroll
-        for (int u = 0; u < SB; ++u) {
-          const int c = u * kWaves + wv;  // this wavefront's segment (uniform)
-          if (c < nc) {
-            const int a = __builtin_amdgcn_readfirstlane((int)a_of[c]);
-            const int4 e4 = a_stg[a];
-            const uint32_t blo = (uint32_t)__builtin_amdgcn_readfirstlane(e4.x);
-            const uint32_t bhi = (uint32_t)__builtin_amdgcn_readfirstlane(e4.y);
-            const uint32_t mg = (uint32_t)__builtin_amdgcn_readfirstlane(e4.z);
-            const int wd = __builtin_amdgcn_readfirstlane(e4.w);
-            const int c0 = __builtin_amdgcn_readfirstlane(a_c0[a]);
-            const int nn_ = wd & 0xFF, rw = (wd >> 8) & 0xFF, rh = wd >> 16;
-            const int r = ((cs + c - c0) << 6) + lane;  // element of the rectangle
-            const int qr = rw > 1 ? (int)__umulhi((uint32_t)r, mg) : r;  // r / rw
-            const int64_t off = qr < rh ? (int64_t)qr * nn_ + (r - qr * rw) : 0;
-            const char* src = (const char*)(ain + ((int64_t)((uint64_t)bhi << 32 | blo) + off));
+      for (int r = 0; r < VPT; ++r) {
+        int oy, ox;
+        block_origin(r, oy, ox);
+        yr[r] = oy + ly;
+        xr[r] = ox + lx;
+      }
+      int boy = 0, box = 0;  // lane r < VPT: origin of block r, to test all blocks at once
+      bool bin = false;
 #pragma unroll
-            for (int pl = 0; pl < PL; ++pl)
-              __builtin_amdgcn_global_load_lds((gptr_t)(src + 4 * pl), (lptr_t)(sbuf + pl * NCH * 64 + c * 64), 4,
-                                               0, 0);
+      for (int r = 0; r < VPT; ++r)
+        if (lane == r) bin = block_origin(r, boy, box);
+      for (int pb = 0; pb < len; pb += 64) {
+        const int idx = pb + lane;
+        bool keep = false;
+        int nb = 0, nn = 0, shift = 0, first = 0;
+        uint32_t rect = 0;
+        int64_t layer = 0;
+        if (idx < len) {
+          nb = lst[idx];
+          const int32_t* q = din.bounds + 6 * nb;
+          const int q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5];
+          const int cx0 = max(x0, q0), cx1 = min(x1, q3), cy0 = max(y0, q1), cy1 = min(y1, q4);
+          if (q2 <= k && k < q5 && cx0 < cx1 && cy0 < cy1) {
+            keep = true;
+            nn = q3 - q0;
+            layer = din.off[nb] + (int64_t)(k - q2) * nn * nn;  // the neighbour's layer k
+            shift = (y0 - q1) * nn + (x0 - q0);                 // + il*nn + jl = voxel (il, jl) there
+            first = (cy0 - q1) * nn + (cx0 - q0);               // the rectangle's first voxel
+            rect = (uint32_t)(cx0 - x0) | (uint32_t)(cx1 - x0) << 8 | (uint32_t)(cy0 - y0) << 16 |
+                   (uint32_t)(cy1 - y0) << 24;
           }
         }
-        __builtin_amdgcn_s_waitcnt(0);  // this wavefront's LDS writes are done before the barrier
-        __syncthreads();
-        const uint32_t lim_e = (uint32_t)(nc << 6);
-        // the items with segments in the batch, from the item of its first segment
-        for (int aa = a_of[0]; aa < cnt && a_c0[aa] < cs + nc; ++aa) {
-          const uint64_t wmask = ((uint64_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a_wm[wv][aa] >> 32)) << 32) |
-                                 (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a_wm[wv][aa]);
-          const uint32_t rc = a_rect[aa];
-          const int rx0 = rc & 0xFF, ry0 = (rc >> 16) & 0xFF;
-          const int rw = (int)((rc >> 8) & 0xFFu) - rx0, rh = (int)(rc >> 24) - ry0;
-          const int pre = (a_c0[aa] - cs) << 6;  // sbuf index of the rectangle's first element (may be < 0)
-          const uint32_t id = (uint32_t)a_id[aa];
+        uint64_t m = __ballot(keep);
+        while (m) {
+          T val[kDepth][VPT];
+          bool ok[kDepth][VPT];
+          uint64_t ov[kDepth];
+          uint32_t id[kDepth];
 #pragma unroll
-          for (int r = 0; r < VPT; ++r) {
-            if ((wmask >> r) & 1ull) {
-              int oy, ox;
-              block_origin(r, oy, ox);
-              const int dx = lx + (ox - rx0), dy = ly + (oy - ry0);
-              const int el = pre + dy * rw + dx;
-              const bool ok = (uint32_t)dx < (uint32_t)rw && (uint32_t)dy < (uint32_t)rh && (uint32_t)el < lim_e;
-              const T val = staged<T>(sbuf, ok ? el : 0, NCH * 64);
-              const bool up = ok && val > best[r];
-              best[r] = up ? val : best[r];
-              bid[r] = up ? id : bid[r];
+          for (int d = 0; d < kDepth; ++d) {
+            ov[d] = 0;
+            id[d] = 0;
+            if (m == 0) continue;
+            const int a = (int)__builtin_ctzll(m);  // next entry in list order
+            m &= m - 1;
+            id[d] = (uint32_t)__builtin_amdgcn_readlane(nb, a);
+            const int nn_a = __builtin_amdgcn_readlane(nn, a);
+            const int shift_a = __builtin_amdgcn_readlane(shift, a);
+            const uint32_t first_a = (uint32_t)__builtin_amdgcn_readlane(first, a);
+            const uint32_t rc = (uint32_t)__builtin_amdgcn_readlane((int)rect, a);
+            const uint32_t llo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)layer, a);
+            const uint32_t lhi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)layer >> 32), a);
+            const T* __restrict__ src = ain + (int64_t)((uint64_t)lhi << 32 | llo);
+            const int rx0 = rc & 0xFF, rx1 = (rc >> 8) & 0xFF, ry0 = (rc >> 16) & 0xFF, ry1 = rc >> 24;
+            const int rw = rx1 - rx0, rh = ry1 - ry0;
+            // blocks the rectangle touches, and those it covers whole
+            ov[d] = __ballot(bin & (box < rx1) & (box + 8 > rx0) & (boy < ry1) & (boy + 8 > ry0));
+            const uint64_t fm = __ballot(bin & (box >= rx0) & (box + 8 <= rx1) & (boy >= ry0) & (boy + 8 <= ry1));
+#pragma unroll
+            for (int r = 0; r < VPT; ++r) {
+              ok[d][r] = true;
+              val[d][r] = (T)0;
+              if ((ov[d] >> r) & 1ull) {
+                uint32_t off = (uint32_t)(__mul24(yr[r], nn_a) + xr[r] + shift_a);  // 24-bit product: both < 256
+                if (!((fm >> r) & 1ull)) {
+                  ok[d][r] = ((uint32_t)(xr[r] - rx0) < (uint32_t)rw) & ((uint32_t)(yr[r] - ry0) < (uint32_t)rh);
+                  off = ok[d][r] ? off : first_a;
+                }
+                val[d][r] = src[off];
+              }
+            }
+          }
+#pragma unroll
+          for (int d = 0; d < kDepth; ++d) {
+#pragma unroll
+            for (int r = 0; r < VPT; ++r) {
+              if ((ov[d] >> r) & 1ull) {
+                const bool up = ok[d][r] & (val[d][r] > best[r]);
+                best[r] = up ? val[d][r] : best[r];
+                bid[r] = up ? id[d] : bid[r];
+              }
             }
           }
         }
-        __syncthreads();
       }
     }
     // BoundaryIntersection on the output box (equal to the input box here)
