@@ -85,10 +85,11 @@ __device__ inline void edge_vertex(int e, const T* L, int xl, int yl, int n, int
 }
 
 // One (grain, z-layer) tile: layers kl and kl+1 are staged in LDS; each lane
-// classifies kCPT cells, the cut ones (some corners inside, some outside) are
-// compacted in a fixed order by one workgroup scan, and then every lane works
-// on cut cells, so the triangle work is not left to a few lanes of each
-// wavefront.  The triangle table sits in LDS; corner and edge tables are bit
+// classifies kCPT cells, the triangles of the cut ones (some corners inside,
+// some outside) are listed in a fixed order by one workgroup scan, and then
+// every lane works on one triangle at a time, so the triangle work is neither
+// left to a few lanes of each wavefront nor held up by cells with many
+// triangles.  The triangle table sits in LDS; corner and edge tables are bit
 // masks, so no lookup goes to memory with a lane-dependent address.
 template <class T, int NMAX>
 __global__ __launch_bounds__(kHT) void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag,
@@ -96,13 +97,20 @@ __global__ __launch_bounds__(kHT) void k_hull(SimParams sp, PlanDev P, int cls, 
   __shared__ T L[2 * NMAX * NMAX];
   __shared__ uint32_t q[kChunk];
   __shared__ uint64_t s_tri[256];
+  __shared__ uint8_t s_ntri[256];  // triangles of each case
   __shared__ int s_wave[kHT / 64];
   __shared__ double s_red[4][kHT / 64];
   __shared__ int s_any[kHT / 64];
   const int total = P.totals[kQLayers * kClasses + cls];
   const T* arena = (const T*)din.arena;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  for (int i = tid; i < 256; i += kHT) s_tri[i] = c_tri[i];
+  for (int i = tid; i < 256; i += kHT) {
+    uint64_t row = c_tri[i];
+    s_tri[i] = row;
+    int cnt = 0;
+    for (; (row & 15u) != 15u; row >>= 12) ++cnt;
+    s_ntri[i] = (uint8_t)cnt;
+  }
   for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
     int g, kl;
     plan_tile(P, cls, kQLayers, tile, g, kl);
@@ -122,11 +130,11 @@ __global__ __launch_bounds__(kHT) void k_hull(SimParams sp, PlanDev P, int cls, 
       __syncthreads();
       for (int c0 = 0; c0 < m1 * m1; c0 += kChunk) {
         uint32_t code[kCPT];
-        int mine = 0;
+        int mine = 0;  // triangles of this lane's cut cells
 #pragma unroll
         for (int r = 0; r < kCPT; ++r) {
           const int c = c0 + r * kHT + tid;
-          code[r] = 0xFFFFFFFFu;
+          code[r] = 0u;
           if (c < m1 * m1) {
             const int yl = (int)(((uint32_t)c * m1_inv) >> 24), xl = c - yl * m1;
             unsigned ci = 0;
@@ -135,11 +143,11 @@ __global__ __launch_bounds__(kHT) void k_hull(SimParams sp, PlanDev P, int cls, 
               if (L[corner_off<NMAX>(qq, xl, yl, n)] >= T(0)) ci |= 1u << qq;
             if (ci != 0u && ci != 255u) {
               code[r] = ci | ((uint32_t)yl << 8) | ((uint32_t)xl << 16);
-              ++mine;
+              mine += s_ntri[ci];
             }
           }
         }
-        // exclusive scan of the per-lane counts: wavefront, then workgroup
+        // exclusive scan of the per-lane triangle counts: wavefront, then workgroup
         int sc = mine;
         for (int off = 1; off < 64; off <<= 1) {
           const int y = __shfl_up(sc, off);
@@ -152,23 +160,32 @@ __global__ __launch_bounds__(kHT) void k_hull(SimParams sp, PlanDev P, int cls, 
           if (w < wv) pos += s_wave[w];
           nq += s_wave[w];
         }
-#pragma unroll
-        for (int r = 0; r < kCPT; ++r)
-          if (code[r] != 0xFFFFFFFFu) q[pos++] = code[r];
-        __syncthreads();
         if (nq) any = 1;
-        for (int w = tid; w < nq; w += kHT) {
-          const uint32_t cd = q[w];
-          const unsigned ci = cd & 0xFFu;
-          const int yl = (cd >> 8) & 0xFF, xl = cd >> 16;
-          const int x = x0 + xl, y = y0 + yl;
-          uint64_t row = s_tri[ci];
-          while ((row & 15u) != 15u) {
+        // The triangles are listed cell by cell (entry = cell code | triangle
+        // index << 24) in windows of kChunk entries; every lane then takes one
+        // triangle at a time, so lanes do not wait on cells with more triangles.
+        for (int w0 = 0; w0 < nq; w0 += kChunk) {
+          int pp = pos;
+#pragma unroll
+          for (int r = 0; r < kCPT; ++r) {
+            if (code[r] != 0u) {
+              const int t = s_ntri[code[r] & 0xFFu];
+              for (int j = 0; j < t; ++j, ++pp)
+                if (pp >= w0 && pp < w0 + kChunk) q[pp - w0] = code[r] | (uint32_t)j << 24;
+            }
+          }
+          __syncthreads();
+          const int nw = min(kChunk, nq - w0);
+          for (int w = tid; w < nw; w += kHT) {
+            const uint32_t cd = q[w];
+            const unsigned ci = cd & 0xFFu;
+            const int yl = (cd >> 8) & 0xFF, xl = (cd >> 16) & 0xFF;
+            const uint32_t row = (uint32_t)(s_tri[ci] >> (12 * (cd >> 24)));
+            const int x = x0 + xl, y = y0 + yl;
             double p0x, p0y, p0z, p1x, p1y, p1z, p2x, p2y, p2z;
             edge_vertex<T, NMAX>((int)(row & 15u), L, xl, yl, n, x, y, z, p0x, p0y, p0z);
             edge_vertex<T, NMAX>((int)((row >> 4) & 15u), L, xl, yl, n, x, y, z, p1x, p1y, p1z);
             edge_vertex<T, NMAX>((int)((row >> 8) & 15u), L, xl, yl, n, x, y, z, p2x, p2y, p2z);
-            row >>= 12;
             const double v321 = p2x * p1y * p0z;
             const double v231 = p1x * p2y * p0z;
             const double v312 = p2x * p0y * p1z;
@@ -183,8 +200,9 @@ __global__ __launch_bounds__(kHT) void k_hull(SimParams sp, PlanDev P, int cls, 
             const double cz = abx * acy - aby * acx;
             area.add(sqrt(cx * cx + cy * cy + cz * cz) / 2.0);
           }
+          __syncthreads();
         }
-        __syncthreads();
+        __syncthreads();  // s_wave is read above before the next chunk writes it
       }
     }
     // wavefront then workgroup reduction in a fixed order
