@@ -317,19 +317,28 @@ __device__ inline void fwd_step(float& cur, float& nxt, float a, float b, bool a
 
 // x/y sweeps of one output layer.  LDS B first holds the in/out intersection of
 // the input layer (read as rows), then the swept out rows (read as columns).
+// Boxes up to 32 put two layers (two tiles) in one wavefront, 32 lanes each,
+// so no lane idles for want of a row; what was uniform for a tile is then a
+// per-lane value of its half.
 template <class T, int NMAX>
 __global__ __launch_bounds__(64) void k_redist_xy_reg(SimParams sp, PlanDev P, int cls, DescDev din, DescDev dout,
                                                       int32_t* err) {
   constexpr int S = NMAX + 1;
-  __shared__ T B[NMAX * S];
+  constexpr int PACK = NMAX <= 32 ? 2 : 1;  // tiles per wavefront
+  constexpr int LW = 64 / PACK;             // lanes per tile
+  __shared__ T Bs[PACK * NMAX * S];
   const int total = P.totals[kQLayers * kClasses + cls];
   const T* ain = (const T*)din.arena;
   T* aout = (T*)dout.arena;
   const T h = (T)sp.h;
   const int lane = threadIdx.x;
-  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
-    int g, kl;
-    plan_tile(P, cls, kQLayers, tile, g, kl);
+  const int sub = lane / LW, ln = lane % LW;  // tile of the wavefront, lane in the tile
+  T* B = Bs + sub * NMAX * S;
+  for (int tp = blockIdx.x; tp * PACK < total; tp += gridDim.x) {
+    const int tile = tp * PACK + sub;
+    const bool tv = tile < total;
+    int g = 0, kl = 0;
+    if (tv) plan_tile(P, cls, kQLayers, tile, g, kl);
     const int32_t* ob = dout.bounds + 6 * g;
     const int32_t* ib = din.bounds + 6 * g;
     const int ox0 = ob[0], oy0 = ob[1], oz0 = ob[2], ox1 = ob[3], oy1 = ob[4];
@@ -339,8 +348,8 @@ __global__ __launch_bounds__(64) void k_redist_xy_reg(SimParams sp, PlanDev P, i
     const int ixmin = max(ib[0], ox0), iymin = max(ib[1], oy0), izmin = max(ib[2], oz0);
     const int ixmax = min(ib[3], ox1), iymax = min(ib[4], oy1), izmax = min(ib[5], ob[5]);
     const bool empty = ixmin >= ixmax || iymin >= iymax || izmin >= izmax;
-    if (empty && lane == 0 && kl == 0) atomicOr(err, kErrEmptyIO);
-    const bool do_xy = !empty && k >= izmin && k < izmax - 1;
+    if (tv && empty && ln == 0 && kl == 0) atomicOr(err, kErrEmptyIO);
+    const bool do_xy = tv && !empty && k >= izmin && k < izmax - 1;
     const T* src = ain + din.off[g] + (int64_t)(k - ib[2]) * ni * ni;
     // local (box) coordinates of the loop ranges
     // loop-range bounds as per-lane values: the predicates below then compare in
@@ -353,33 +362,36 @@ __global__ __launch_bounds__(64) void k_redist_xy_reg(SimParams sp, PlanDev P, i
       o[q] = T(-1);
       __asm__ volatile("" : "+v"(o[q]));  // opaque: no control flow specialised on the initial value
     }
-    if (do_xy) {
+    // (the barriers sit outside the per-tile conditions: the two tiles of a
+    // wavefront may differ in them)
+    if (do_xy)
       stage_rows_to_lds(B + ylo * S + xlo, S, src + (int64_t)(iymin - ib[1]) * ni + (ixmin - ib[0]), ni,
-                        iymax - iymin, ixmax - ixmin, lane, 64);
-      __syncthreads();
-      // x-forward then x-backward along row oy0 + lane, for rows in [iymin, oy1)
-      const int r = lane;
-      if (r >= ylo && r < no) {
-        const bool inrow = r <= yim;
+                        iymax - iymin, ixmax - ixmin, ln, LW);
+    __syncthreads();
+    // x-forward then x-backward along row oy0 + ln, for rows in [iymin, oy1)
+    const int r = ln;
+    if (do_xy && r >= ylo && r < no) {
+      const bool inrow = r <= yim;
 #pragma unroll
-        for (int q = 0; q < NMAX; ++q) in[q] = B[r * S + q];
+      for (int q = 0; q < NMAX; ++q) in[q] = B[r * S + q];
 #pragma unroll
-        for (int q = 0; q < NMAX - 1; ++q)
-          fwd_step(o[q], o[q + 1], in[q], in[q + 1], (q >= xlo) & (q < nol), (q < xim) & inrow, h);
+      for (int q = 0; q < NMAX - 1; ++q)
+        fwd_step(o[q], o[q + 1], in[q], in[q + 1], (q >= xlo) & (q < nol), (q < xim) & inrow, h);
 #pragma unroll
-        for (int q = NMAX - 1; q >= 1; --q) {
-          const T cand = o[q] + ((T)sgnv(((q > xlo) & inrow) ? in[q - 1] : o[q - 1]) * h);
-          o[q - 1] = ((q <= xim) & (absv(cand) < absv(o[q - 1]))) ? cand : o[q - 1];
-        }
+      for (int q = NMAX - 1; q >= 1; --q) {
+        const T cand = o[q] + ((T)sgnv(((q > xlo) & inrow) ? in[q - 1] : o[q - 1]) * h);
+        o[q - 1] = ((q <= xim) & (absv(cand) < absv(o[q - 1]))) ? cand : o[q - 1];
       }
-      __syncthreads();
-      if (r < no) {
+    }
+    __syncthreads();
+    if (do_xy && r < no) {
 #pragma unroll
-        for (int q = 0; q < NMAX; ++q) B[r * S + q] = o[q];
-      }
-      __syncthreads();
-      // y-forward then y-backward along column ox0 + lane, for columns in [ixmin, ox1)
-      const int c = lane;
+      for (int q = 0; q < NMAX; ++q) B[r * S + q] = o[q];
+    }
+    __syncthreads();
+    if (do_xy) {
+      // y-forward then y-backward along column ox0 + ln, for columns in [ixmin, ox1)
+      const int c = ln;
       if (c < no) {
 #pragma unroll
         for (int q = 0; q < NMAX; ++q) o[q] = B[q * S + c];
@@ -404,10 +416,10 @@ __global__ __launch_bounds__(64) void k_redist_xy_reg(SimParams sp, PlanDev P, i
     }
     // lane c writes column ox0 + c of the layer (coalesced across lanes)
     T* dst = aout + dout.off[g] + (int64_t)kl * no * no;
-    if (lane < no) {
+    if (tv && ln < no) {
 #pragma unroll
       for (int q = 0; q < NMAX; ++q)
-        if (q < no) dst[q * no + lane] = o[q];
+        if (q < no) dst[q * no + ln] = o[q];
     }
     __syncthreads();
   }
