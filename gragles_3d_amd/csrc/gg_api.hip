@@ -272,6 +272,8 @@ void stage_comparison(gg_ctx* c) {
   static const int seg[kClasses] = {kSegCompare, kSegCompare48, kSegCompare64, kSegCompare96};
   for (int cls = 0; cls < kClasses; ++cls) {
     SegTimer t(c, seg[cls]);
+    // the first class's segment includes building the neighbour table
+    if (cls == 0) launch_comparison_table(launch_of(c), c->sp, c->d_state, din(c), c->LD);
     launch_comparison(launch_of(c), cls, c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur],
                       c->IL[c->il_cur ^ 1], c->d_newb, c->fp32);
   }
@@ -365,6 +367,11 @@ int gg_create(const gg_params* params, gg_ctx** out) {
     }
   }
   c->LD.max_list = c->p.max_list;
+  if ((rc = dalloc(c, &c->LD.tab, (size_t)(N + 1) * c->p.max_list)) ||
+      (rc = dalloc(c, &c->LD.tab_id, (size_t)(N + 1) * c->p.max_list)) || (rc = dalloc(c, &c->LD.tab_len, N + 1))) {
+    gg_destroy(c);
+    return rc;
+  }
   if ((rc = dalloc(c, &c->LD.sel, N + 1)) || (rc = dalloc(c, &c->P.cls_count, kClasses)) ||
       (rc = dalloc(c, &c->P.cls_start, kClasses)) || (rc = dalloc(c, &c->P.grains, N + 1)) ||
       (rc = dalloc(c, &c->P.grain_pos, N + 1)) || (rc = dalloc(c, &c->P.order, N)) ||

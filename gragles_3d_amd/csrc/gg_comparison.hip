@@ -156,9 +156,9 @@ __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int
     const int n = x1 - x0, nn2 = n * n;
     const int nbx = (n + 7) >> 3;
     const int k = z0 + kl;
-    const int lsel = ld.sel[g] ^ 1;  // m_comparisonList
-    const int32_t* lst = ld.buf[lsel] + (int64_t)g * maxL;
-    const int len = ld.len[lsel][g];
+    const int4* tab = ld.tab + (int64_t)g * maxL;  // neighbours overlapping the box, in list order
+    const int32_t* tid_of = ld.tab_id + (int64_t)g * maxL;
+    const int len = ld.tab_len[g];
 
     // The lane's voxel in its r-th block: (i - y0, j - x0) = (by*8 + ly, bx*8 + lx)
     // with the block origin uniform across the wavefront.
@@ -187,16 +187,16 @@ __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int
         int nseg = 0, nb = 0, nn = 0, rx0 = 0, rx1 = 0, ry0 = 0, ry1 = 0;
         int64_t gbase = 0;
         if (tid < ITEMS && idx < len) {
-          nb = lst[idx];
-          const int32_t* q = din.bounds + 6 * nb;
-          const int q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5];
-          const int cx0 = max(x0, q0), cx1 = min(x1, q3), cy0 = max(y0, q1), cy1 = min(y1, q4);
-          if (q2 <= k && k < q5 && cx0 < cx1 && cy0 < cy1) {
+          const int4 e = tab[idx];
+          nb = tid_of[idx];
+          const int q2 = e.w & 0xFFFF, q5 = (int)((uint32_t)e.w >> 16);
+          if (q2 <= k && k < q5) {
             keep = true;
-            nn = q3 - q0;
-            rx0 = cx0 - x0; rx1 = cx1 - x0; ry0 = cy0 - y0; ry1 = cy1 - y0;
+            nn = (int)((uint32_t)e.y >> 16);
+            const uint32_t rc = (uint32_t)e.z;
+            rx0 = rc & 0xFF; rx1 = (rc >> 8) & 0xFF; ry0 = (rc >> 16) & 0xFF; ry1 = rc >> 24;
             nseg = ((rx1 - rx0) * (ry1 - ry0) + 63) >> 6;
-            gbase = din.off[nb] + ((int64_t)(k - q2) * nn + (cy0 - q1)) * nn + (cx0 - q0);
+            gbase = (int64_t)((uint64_t)((uint32_t)e.y & 0xFFFFu) << 32 | (uint32_t)e.x) + (int64_t)(k - q2) * nn * nn;
           }
         }
         const uint64_t m = __ballot(keep);
@@ -330,18 +330,18 @@ __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int
         uint32_t rect = 0;
         int64_t layer = 0;
         if (idx < len) {
-          nb = lst[idx];
-          const int32_t* q = din.bounds + 6 * nb;
-          const int q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5];
-          const int cx0 = max(x0, q0), cx1 = min(x1, q3), cy0 = max(y0, q1), cy1 = min(y1, q4);
-          if (q2 <= k && k < q5 && cx0 < cx1 && cy0 < cy1) {
+          const int4 e = tab[idx];
+          nb = tid_of[idx];
+          const int q2 = e.w & 0xFFFF, q5 = (int)((uint32_t)e.w >> 16);
+          if (q2 <= k && k < q5) {
             keep = true;
-            nn = q3 - q0;
-            layer = din.off[nb] + (int64_t)(k - q2) * nn * nn;  // the neighbour's layer k
-            shift = (y0 - q1) * nn + (x0 - q0);                 // + il*nn + jl = voxel (il, jl) there
-            first = (cy0 - q1) * nn + (cx0 - q0);               // the rectangle's first voxel
-            rect = (uint32_t)(cx0 - x0) | (uint32_t)(cx1 - x0) << 8 | (uint32_t)(cy0 - y0) << 16 |
-                   (uint32_t)(cy1 - y0) << 24;
+            nn = (int)((uint32_t)e.y >> 16);
+            rect = (uint32_t)e.z;
+            // the rectangle's first voxel in the neighbour's layer k; voxel (il, jl)
+            // of the grain's layer is at il*nn + jl + shift from there
+            layer = (int64_t)((uint64_t)((uint32_t)e.y & 0xFFFFu) << 32 | (uint32_t)e.x) + (int64_t)(k - q2) * nn * nn;
+            shift = -(int)((rect >> 16) & 0xFF) * nn - (int)(rect & 0xFF);
+            first = 0;
           }
         }
         uint64_t m = __ballot(keep);
@@ -470,6 +470,61 @@ __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int
   }
 }
 
+// Neighbour table of every alive grain (layout in gg_internal.h): the entries
+// of m_comparisonList whose input box overlaps the grain's box in x, y and z,
+// in list order, with the shared x/y rectangle and where it starts in the
+// arena.  One wavefront per grain; the comparison tiles of the grain's layers
+// then read one table entry per neighbour instead of the list, the
+// neighbour's bounds and its arena offset one after the other.
+__global__ __launch_bounds__(64) void k_cmp_table(SimParams sp, const int32_t* __restrict__ state, DescDev din,
+                                                  ListDev ld) {
+  const int lane = threadIdx.x;
+  const int maxL = ld.max_list;
+  for (int g = 1 + blockIdx.x; g <= sp.N; g += gridDim.x) {
+    if (state[g] != kAlive) {
+      if (lane == 0) ld.tab_len[g] = 0;
+      continue;
+    }
+    const int32_t* b = din.bounds + 6 * g;
+    const int x0 = b[0], y0 = b[1], z0 = b[2], x1 = b[3], y1 = b[4], z1 = b[5];
+    const int lsel = ld.sel[g] ^ 1;  // m_comparisonList
+    const int32_t* lst = ld.buf[lsel] + (int64_t)g * maxL;
+    const int len = ld.len[lsel][g];
+    int4* tab = ld.tab + (int64_t)g * maxL;
+    int32_t* tid_of = ld.tab_id + (int64_t)g * maxL;
+    int out = 0;
+    for (int base = 0; base < len; base += 64) {
+      const int idx = base + lane;
+      bool keep = false;
+      int nb = 0;
+      int4 e = make_int4(0, 0, 0, 0);
+      if (idx < len) {
+        nb = lst[idx];
+        const int32_t* q = din.bounds + 6 * nb;
+        const int q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5];
+        const int cx0 = max(x0, q0), cx1 = min(x1, q3), cy0 = max(y0, q1), cy1 = min(y1, q4);
+        if (cx0 < cx1 && cy0 < cy1 && max(z0, q2) < min(z1, q5)) {
+          keep = true;
+          const int nn = q3 - q0;
+          const int64_t B = din.off[nb] + (int64_t)(cy0 - q1) * nn + (cx0 - q0);
+          e = make_int4((int)(uint32_t)B, (int)(((uint32_t)((uint64_t)B >> 32) & 0xFFFFu) | (uint32_t)nn << 16),
+                        (int)((uint32_t)(cx0 - x0) | (uint32_t)(cx1 - x0) << 8 | (uint32_t)(cy0 - y0) << 16 |
+                              (uint32_t)(cy1 - y0) << 24),
+                        (int)((uint32_t)q2 | (uint32_t)q5 << 16));
+        }
+      }
+      const uint64_t m = __ballot(keep);
+      if (keep) {
+        const int pos = out + __popcll(m & ((1ull << lane) - 1ull));
+        tab[pos] = e;
+        tid_of[pos] = nb;
+      }
+      out += __popcll(m);
+    }
+    if (lane == 0) ld.tab_len[g] = out;
+  }
+}
+
 template <class T>
 void comparison_class(const Launch& L, int cls, const SimParams& sp, const PlanDev& P, const DescDev& din,
                       const DescDev& dout, const ListDev& ld, const IdlDev& ilo, const IdlDev& iln, int32_t* newb) {
@@ -493,6 +548,12 @@ void comparison_class(const Launch& L, int cls, const SimParams& sp, const PlanD
 }
 
 }  // namespace
+
+void launch_comparison_table(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& din,
+                             const ListDev& ld) {
+  const int grid = sp.N < 65536 ? (sp.N > 0 ? sp.N : 1) : 65536;  // one wavefront per grain
+  hipLaunchKernelGGL(k_cmp_table, dim3(grid), dim3(64), 0, L.stream, sp, state, din, ld);
+}
 
 // One size class per launch (boxes n <= 32, 48, 64, 96 for cls 0..3).
 void launch_comparison(const Launch& L, int cls, const SimParams& sp, const PlanDev& P, const DescDev& din,

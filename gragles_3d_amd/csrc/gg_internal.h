@@ -61,6 +61,17 @@ struct ListDev {
   int32_t* len[2];   // [N+1]
   int32_t* sel;      // [N+1] buffer index holding m_secondOrderNeighbours
   int32_t max_list;
+  // Neighbour table of the comparison, built at the start of the comparison
+  // stage from m_comparisonList: per grain, in list order, the entries whose
+  // input box overlaps the grain's box, each as
+  //   x, y: B (48 bits: arena index of the shared rectangle's first voxel in the
+  //         neighbour's layer q2), y >> 16: the neighbour's box side nn
+  //   z: the shared x/y rectangle, local to the grain's box: x0 | x1<<8 | y0<<16 | y1<<24
+  //   w: the neighbour's z range q2 | q5 << 16
+  // so the rectangle's first voxel in layer k is B + (k - q2) * nn^2.
+  int4* tab;         // [(N+1)*max_list]
+  int32_t* tab_id;   // [(N+1)*max_list] neighbour id of each entry
+  int32_t* tab_len;  // [N+1]
 };
 
 // Tile quantities carried per class by the plan.
@@ -232,6 +243,8 @@ void launch_idl_resize(const Launch& L, const SimParams& sp, const int32_t* stat
                        const IdlDev& il);
 void launch_lists(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& din,
                   const DescDev& dout, const ListDev& ld, int32_t* newb, int32_t* err);
+void launch_comparison_table(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& din,
+                             const ListDev& ld);
 void launch_comparison(const Launch& L, int cls, const SimParams& sp, const PlanDev& P, const DescDev& din,
                        const DescDev& dout, const ListDev& ld, const IdlDev& il_old, const IdlDev& il_new,
                        int32_t* newb, bool fp32);
