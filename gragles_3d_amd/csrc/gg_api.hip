@@ -61,6 +61,9 @@ struct gg_ctx {
   int graph_sel = -1, graph_il = -1;
   // stage timing
   bool timing = false;
+  // second stream for the list build, which runs beside the convolution
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipEvent_t ev[12] = {};
   double seg_ms[16] = {};
   int64_t seg_n[16] = {};
@@ -306,9 +309,25 @@ void stage_count(gg_ctx* c) {
 // one run_sim iteration body (src/grainhdl.cpp:836-919 without coarsening or output)
 void enqueue_step(gg_ctx* c) {
   c->sel ^= 1;  // switchDistancebuffer in gridCoarsement's else branch
-  stage_convolution(c);
-  c->sel ^= 1;
-  stage_lists(c);
+  if (c->timing) {
+    stage_convolution(c);
+    c->sel ^= 1;
+    stage_lists(c);
+  } else {
+    // The list build reads the state, the box bounds and the current lists and
+    // writes the other list buffer and the set-comparison start bounds; the
+    // convolution reads and writes distance values and IDLocal bounds only. So
+    // the lists go on the side stream, forked from and joined back to the main
+    // stream (inside a captured graph too), beside the convolution.
+    hipEventRecord(c->ev_fork, c->stream);
+    hipStreamWaitEvent(c->side, c->ev_fork, 0);
+    launch_lists(Launch{c->side, c->grid, c->grid / 8}, c->sp, c->d_state, dout(c), din(c), c->LD, c->d_newb,
+                 c->d_err);  // the roles after the switch below
+    hipEventRecord(c->ev_join, c->side);
+    stage_convolution(c);
+    c->sel ^= 1;
+    hipStreamWaitEvent(c->stream, c->ev_join, 0);
+  }
   stage_comparison(c);
   c->sel ^= 1;
   stage_level_set(c);
@@ -342,6 +361,12 @@ int gg_create(const gg_params* params, gg_ctx** out) {
                     params->h, params->delta};
   if (hipSetDevice(params->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
+    return GG_ERR_HIP;
+  }
+  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    gg_destroy(c);
     return GG_ERR_HIP;
   }
   hipDeviceProp_t prop;
@@ -418,6 +443,9 @@ int gg_destroy(gg_ctx* c) {
   if (c->graph) hipGraphDestroy(c->graph);
   for (auto e : c->evpool) hipEventDestroy(e);
   for (void* p : c->allocs) hipFree(p);
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->side) hipStreamDestroy(c->side);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return GG_OK;

@@ -530,19 +530,19 @@ void comparison_class(const Launch& L, int cls, const SimParams& sp, const PlanD
                       const DescDev& dout, const ListDev& ld, const IdlDev& ilo, const IdlDev& iln, int32_t* newb) {
   switch (cls) {
     case 0:
-      hipLaunchKernelGGL((k_comparison<T, 32>), dim3(tile_grid(L, kGridPerCU)), dim3(kCT), 0, L.stream, sp, P, 0, din, dout, ld, ilo,
+      hipLaunchKernelGGL((k_comparison<T, 32>), dim3(tile_grid(L, kGridPerCU, 0)), dim3(kCT), 0, L.stream, sp, P, 0, din, dout, ld, ilo,
                          iln, newb);
       break;
     case 1:
-      hipLaunchKernelGGL((k_comparison<T, 48>), dim3(tile_grid(L, kGridPerCU)), dim3(kCT), 0, L.stream, sp, P, 1, din, dout, ld, ilo,
+      hipLaunchKernelGGL((k_comparison<T, 48>), dim3(tile_grid(L, kGridPerCU, 1)), dim3(kCT), 0, L.stream, sp, P, 1, din, dout, ld, ilo,
                          iln, newb);
       break;
     case 2:
-      hipLaunchKernelGGL((k_comparison<T, 64>), dim3(tile_grid(L, kGridPerCU)), dim3(kCT), 0, L.stream, sp, P, 2, din, dout, ld, ilo,
+      hipLaunchKernelGGL((k_comparison<T, 64>), dim3(tile_grid(L, kGridPerCU, 2)), dim3(kCT), 0, L.stream, sp, P, 2, din, dout, ld, ilo,
                          iln, newb);
       break;
     default:
-      hipLaunchKernelGGL((k_comparison<T, 96>), dim3(tile_grid(L, kGridPerCU)), dim3(kCT), 0, L.stream, sp, P, 3, din, dout, ld, ilo,
+      hipLaunchKernelGGL((k_comparison<T, 96>), dim3(tile_grid(L, kGridPerCU, 3)), dim3(kCT), 0, L.stream, sp, P, 3, din, dout, ld, ilo,
                          iln, newb);
   }
 }

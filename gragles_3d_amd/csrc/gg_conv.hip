@@ -377,17 +377,17 @@ template <int NMAX>
 void conv_class(const Launch& L, int part, const PlanDev& P, int cls, const DescDev& din, const DescDev& dout,
                 const TablesDev& tb, const int64_t* spec_off, void* spec, bool fp32) {
   if (fp32) {
-    if (part == 0) hipLaunchKernelGGL(k_circ_xy<NMAX>, dim3(tile_grid(L, kGridPerCU)), dim3(kT), 0, L.stream, P, cls, din, dout, tb);
-    if (part == 1) hipLaunchKernelGGL(k_circ_z<NMAX>, dim3(tile_grid(L, kGridPerCU)), dim3(kT), 0, L.stream, P, cls, dout, tb);
+    if (part == 0) hipLaunchKernelGGL(k_circ_xy<NMAX>, dim3(tile_grid(L, kGridPerCU, cls)), dim3(kT), 0, L.stream, P, cls, din, dout, tb);
+    if (part == 1) hipLaunchKernelGGL(k_circ_z<NMAX>, dim3(tile_grid(L, kGridPerCU, cls)), dim3(kT), 0, L.stream, P, cls, dout, tb);
   } else {
     if (part == 0)
-      hipLaunchKernelGGL(k_dft_xy_fwd<NMAX>, dim3(tile_grid(L, kGridPerCU)), dim3(kT), 0, L.stream, P, cls, din, tb, spec_off,
+      hipLaunchKernelGGL(k_dft_xy_fwd<NMAX>, dim3(tile_grid(L, kGridPerCU, cls)), dim3(kT), 0, L.stream, P, cls, din, tb, spec_off,
                          (double2*)spec);
     if (part == 1)
-      hipLaunchKernelGGL(k_dft_z<NMAX>, dim3(tile_grid(L, kGridPerCU)), dim3(kT), 0, L.stream, P, cls, dout, tb, spec_off,
+      hipLaunchKernelGGL(k_dft_z<NMAX>, dim3(tile_grid(L, kGridPerCU, cls)), dim3(kT), 0, L.stream, P, cls, dout, tb, spec_off,
                          (double2*)spec);
     if (part == 2)
-      hipLaunchKernelGGL(k_dft_xy_inv<NMAX>, dim3(tile_grid(L, kGridPerCU)), dim3(kT), 0, L.stream, P, cls, dout, tb, spec_off,
+      hipLaunchKernelGGL(k_dft_xy_inv<NMAX>, dim3(tile_grid(L, kGridPerCU, cls)), dim3(kT), 0, L.stream, P, cls, dout, tb, spec_off,
                          (const double2*)spec);
   }
 }

@@ -318,15 +318,15 @@ void upload_mc_tables() {
 void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, int32_t* exflag,
                  double* partials, bool fp32) {
   if (fp32) {
-    hipLaunchKernelGGL((k_hull<float, 32>), dim3(tile_grid(L, kGridPerCU)), dim3(kHT), 0, L.stream, sp, P, 0, din, exflag, partials);
-    hipLaunchKernelGGL((k_hull<float, 48>), dim3(tile_grid(L, kGridPerCU)), dim3(kHT), 0, L.stream, sp, P, 1, din, exflag, partials);
-    hipLaunchKernelGGL((k_hull<float, 64>), dim3(tile_grid(L, kGridPerCU)), dim3(kHT), 0, L.stream, sp, P, 2, din, exflag, partials);
-    hipLaunchKernelGGL((k_hull<float, 96>), dim3(tile_grid(L, kGridPerCU)), dim3(kHT), 0, L.stream, sp, P, 3, din, exflag, partials);
+    hipLaunchKernelGGL((k_hull<float, 32>), dim3(tile_grid(L, kGridPerCU, 0)), dim3(kHT), 0, L.stream, sp, P, 0, din, exflag, partials);
+    hipLaunchKernelGGL((k_hull<float, 48>), dim3(tile_grid(L, kGridPerCU, 1)), dim3(kHT), 0, L.stream, sp, P, 1, din, exflag, partials);
+    hipLaunchKernelGGL((k_hull<float, 64>), dim3(tile_grid(L, kGridPerCU, 2)), dim3(kHT), 0, L.stream, sp, P, 2, din, exflag, partials);
+    hipLaunchKernelGGL((k_hull<float, 96>), dim3(tile_grid(L, kGridPerCU, 3)), dim3(kHT), 0, L.stream, sp, P, 3, din, exflag, partials);
   } else {
-    hipLaunchKernelGGL((k_hull<double, 32>), dim3(tile_grid(L, kGridPerCU)), dim3(kHT), 0, L.stream, sp, P, 0, din, exflag, partials);
-    hipLaunchKernelGGL((k_hull<double, 48>), dim3(tile_grid(L, kGridPerCU)), dim3(kHT), 0, L.stream, sp, P, 1, din, exflag, partials);
-    hipLaunchKernelGGL((k_hull<double, 64>), dim3(tile_grid(L, kGridPerCU)), dim3(kHT), 0, L.stream, sp, P, 2, din, exflag, partials);
-    hipLaunchKernelGGL((k_hull<double, 96>), dim3(tile_grid(L, kGridPerCU)), dim3(kHT), 0, L.stream, sp, P, 3, din, exflag, partials);
+    hipLaunchKernelGGL((k_hull<double, 32>), dim3(tile_grid(L, kGridPerCU, 0)), dim3(kHT), 0, L.stream, sp, P, 0, din, exflag, partials);
+    hipLaunchKernelGGL((k_hull<double, 48>), dim3(tile_grid(L, kGridPerCU, 1)), dim3(kHT), 0, L.stream, sp, P, 1, din, exflag, partials);
+    hipLaunchKernelGGL((k_hull<double, 64>), dim3(tile_grid(L, kGridPerCU, 2)), dim3(kHT), 0, L.stream, sp, P, 2, din, exflag, partials);
+    hipLaunchKernelGGL((k_hull<double, 96>), dim3(tile_grid(L, kGridPerCU, 3)), dim3(kHT), 0, L.stream, sp, P, 3, din, exflag, partials);
   }
 }
 

@@ -229,7 +229,12 @@ struct Launch {
 // a fixed share of tiles per workgroup does not; the grid-stride loop of each
 // kernel covers tiles beyond the grid.  The values at the launch sites are the
 // fastest of a sweep on the 8000-grain workload.
-inline int tile_grid(const Launch& L, int per_cu) { return L.cus * per_cu; }
+// Classes 2 and 3 (boxes of side 49..96) hold few grains in the workloads
+// measured here; their launches get 1/16 of that (at least 8 per CU), so a
+// nearly empty class does not pay for dispatching a full-size grid.
+inline int tile_grid(const Launch& L, int per_cu, int cls) {
+  return L.cus * (cls >= 2 ? (per_cu / 16 > 8 ? per_cu / 16 : 8) : per_cu);
+}
 
 void launch_plan(const Launch& L, const SimParams& sp, const int32_t* state, const int32_t* bounds,
                  int64_t* off_out, int64_t* spec_off, const PlanDev& P, int64_t arena_cap,

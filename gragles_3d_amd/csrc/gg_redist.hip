@@ -490,7 +490,7 @@ __global__ __launch_bounds__(64) void k_redist_z_reg(SimParams sp, PlanDev P, in
 template <class T, int NMAX>
 void xy_class(const Launch& L, const SimParams& sp, const PlanDev& P, int cls, const DescDev& din,
               const DescDev& dout, int32_t* err) {
-  hipLaunchKernelGGL((k_redist_xy<T, NMAX>), dim3(tile_grid(L, kGridPerCU)), dim3(128), 0, L.stream, sp, P, cls, din, dout, err);
+  hipLaunchKernelGGL((k_redist_xy<T, NMAX>), dim3(tile_grid(L, kGridPerCU, cls)), dim3(128), 0, L.stream, sp, P, cls, din, dout, err);
 }
 
 // part 0: x/y sweeps per layer (one launch per size class); part 1: z sweeps + clamp
@@ -500,21 +500,21 @@ void redist_all(const Launch& L, int part, const SimParams& sp, const PlanDev& P
   // register-resident lines for boxes up to 32 (both types) and up to 48 (FP32)
   constexpr bool reg48 = sizeof(T) == 4;
   if (part == 0) {
-    hipLaunchKernelGGL((k_redist_xy_reg<T, 32>), dim3(tile_grid(L, kGridPerCU)), dim3(64), 0, L.stream, sp, P, 0, din, dout, err);
+    hipLaunchKernelGGL((k_redist_xy_reg<T, 32>), dim3(tile_grid(L, kGridPerCU, 0)), dim3(64), 0, L.stream, sp, P, 0, din, dout, err);
     if constexpr (reg48)
-      hipLaunchKernelGGL((k_redist_xy_reg<T, 48>), dim3(tile_grid(L, kGridPerCU)), dim3(64), 0, L.stream, sp, P, 1, din, dout, err);
+      hipLaunchKernelGGL((k_redist_xy_reg<T, 48>), dim3(tile_grid(L, kGridPerCU, 1)), dim3(64), 0, L.stream, sp, P, 1, din, dout, err);
     else
       xy_class<T, 48>(L, sp, P, 1, din, dout, err);
     xy_class<T, 64>(L, sp, P, 2, din, dout, err);
     xy_class<T, 96>(L, sp, P, 3, din, dout, err);
   } else {
-    hipLaunchKernelGGL((k_redist_z_reg<T, 32>), dim3(tile_grid(L, kGridPerCU)), dim3(64), 0, L.stream, sp, P, 0, din, dout);
+    hipLaunchKernelGGL((k_redist_z_reg<T, 32>), dim3(tile_grid(L, kGridPerCU, 0)), dim3(64), 0, L.stream, sp, P, 0, din, dout);
     if constexpr (reg48)
-      hipLaunchKernelGGL((k_redist_z_reg<T, 48>), dim3(tile_grid(L, kGridPerCU)), dim3(64), 0, L.stream, sp, P, 1, din, dout);
+      hipLaunchKernelGGL((k_redist_z_reg<T, 48>), dim3(tile_grid(L, kGridPerCU, 1)), dim3(64), 0, L.stream, sp, P, 1, din, dout);
     else
-      hipLaunchKernelGGL((k_redist_z<T, 48>), dim3(tile_grid(L, kGridPerCU)), dim3(64), 0, L.stream, sp, P, 1, din, dout);
-    hipLaunchKernelGGL((k_redist_z<T, 64>), dim3(tile_grid(L, kGridPerCU)), dim3(64), 0, L.stream, sp, P, 2, din, dout);
-    hipLaunchKernelGGL((k_redist_z<T, 96>), dim3(tile_grid(L, kGridPerCU)), dim3(64), 0, L.stream, sp, P, 3, din, dout);
+      hipLaunchKernelGGL((k_redist_z<T, 48>), dim3(tile_grid(L, kGridPerCU, 1)), dim3(64), 0, L.stream, sp, P, 1, din, dout);
+    hipLaunchKernelGGL((k_redist_z<T, 64>), dim3(tile_grid(L, kGridPerCU, 2)), dim3(64), 0, L.stream, sp, P, 2, din, dout);
+    hipLaunchKernelGGL((k_redist_z<T, 96>), dim3(tile_grid(L, kGridPerCU, 3)), dim3(64), 0, L.stream, sp, P, 3, din, dout);
   }
 }
 
