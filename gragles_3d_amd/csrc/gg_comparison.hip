@@ -19,7 +19,16 @@ constexpr int kGridPerCU = 512;  // workgroups per CU of a comparison launch (on
 
 constexpr int kCT = 256;
 constexpr int kWaves = kCT / 64;
-constexpr int kStageBytes = 20480;  // LDS bytes of staged neighbour values per workgroup
+// LDS bytes of staged neighbour values per workgroup: FP32 keeps it small
+// enough for six workgroups per CU; FP64 (VGPR-bound at five) gains more from
+// fewer batches per tile.
+template <class T>
+constexpr int kStageBytes = sizeof(T) == 4 ? 12288 : 20480;
+// Occupancy asked of the compiler: six workgroups per CU for the FP32 n <= 48
+// class, which the LDS of the staged form allows (measured 6 % faster than
+// the five it reaches unasked); no request elsewhere (1), where it would spill.
+template <class T, int NMAX>
+constexpr int kWavesPerEU = (sizeof(T) == 4 && NMAX == 48) ? 6 : 1;
 constexpr int kDirectMax = 32;      // box sides up to this gather neighbour values directly
 constexpr int kDepth = 2;           // list entries whose loads are in flight together (direct gathers)
 
@@ -120,7 +129,7 @@ __device__ inline bool deep_inside(int j, int i, int k, int gb, int m) {
 // straddle two batches: a voxel's staged index grows with the entry's list
 // position, so each voxel still sees its neighbours in list order.
 template <class T, int NMAX>
-__global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int cls, DescDev din, DescDev dout,
+__global__ __launch_bounds__(kCT) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<T, NMAX>))) void k_comparison(SimParams sp, PlanDev P, int cls, DescDev din, DescDev dout,
                                                     ListDev ld, IdlDev ilo, IdlDev iln, int32_t* newb) {
   constexpr int NB = (NMAX + 7) / 8;                    // blocks per side, at most
   constexpr int VPT = (NB * NB + kWaves - 1) / kWaves;  // blocks per wavefront = voxels per lane
@@ -128,7 +137,7 @@ __global__ __launch_bounds__(kCT) void k_comparison(SimParams sp, PlanDev P, int
   static_assert(VPT <= 64, "a wavefront's block mask is one 64-bit word");
   constexpr int ITEMS = 128;                            // list entries per pass (at most 256: a_of is 8-bit)
   static_assert(ITEMS % 64 == 0 && ITEMS <= kCT, "whole wavefronts compact the items");
-  constexpr int NCH = kStageBytes / (int)sizeof(T) / 64;  // staged segments per batch
+  constexpr int NCH = kStageBytes<T> / (int)sizeof(T) / 64;  // staged segments per batch
   constexpr int SB = NCH / kWaves;                          // segments per wavefront and batch
   constexpr int PL = (int)sizeof(T) / 4;                    // 32-bit planes of a staged value
   static_assert(SB * kWaves == NCH, "a batch is whole segments for every wavefront");
