@@ -91,8 +91,10 @@ __device__ inline void edge_vertex(int e, const T* L, int xl, int yl, int n, int
 // left to a few lanes of each wavefront nor held up by cells with many
 // triangles.  The triangle table sits in LDS; corner and edge tables are bit
 // masks, so no lookup goes to memory with a lane-dependent address.
+// FP32: six waves per SIMD asked of the compiler (n <= 32 reaches it with a
+// small spill; n <= 48 stays LDS-bound at five).
 template <class T, int NMAX>
-__global__ __launch_bounds__(kHT) void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag,
+__global__ __launch_bounds__(kHT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag,
                                               double* partials) {
   __shared__ T L[2 * NMAX * NMAX];
   __shared__ uint32_t q[kChunk];

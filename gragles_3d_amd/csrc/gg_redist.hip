@@ -427,8 +427,12 @@ __global__ __launch_bounds__(64) void k_redist_xy_reg(SimParams sp, PlanDev P, i
 
 // z-forward, z-backward and the clamp for 64 columns, one per lane, the column
 // in registers.
+// FP32 n <= 48: three waves per SIMD asked of the compiler (it fits the
+// columns in 168 VGPRs without spilling; 20 % faster than the two it reaches
+// unasked).
 template <class T, int NMAX>
-__global__ __launch_bounds__(64) void k_redist_z_reg(SimParams sp, PlanDev P, int cls, DescDev din, DescDev dout) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && NMAX == 48 ? 3 : 1)))
+void k_redist_z_reg(SimParams sp, PlanDev P, int cls, DescDev din, DescDev dout) {
   const int total = P.totals[kQCols * kClasses + cls];
   const T* __restrict__ ain = (const T*)din.arena;
   T* __restrict__ aout = (T*)dout.arena;
