@@ -14,20 +14,27 @@ namespace {
 
 constexpr int kPlanThreads = 1024;
 
-__device__ int64_t block_scan_excl(int64_t v, int64_t* s, int64_t* total) {
-  const int t = threadIdx.x;
-  s[t] = v;
-  __syncthreads();
-  for (int d = 1; d < kPlanThreads; d <<= 1) {
-    int64_t add = t >= d ? s[t - d] : 0;
-    __syncthreads();
-    s[t] += add;
-    __syncthreads();
+// Exclusive workgroup scan of one per-thread quantity: wavefront scan by
+// shuffles, then one exchange of the wavefront totals through LDS (two
+// barriers, instead of a barrier ladder over the workgroup).
+__device__ int64_t block_scan_excl(int64_t v, int64_t* sw, int64_t* total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int64_t x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int64_t y = __shfl_up(x, off);
+    if (lane >= off) x += y;
   }
-  int64_t incl = s[t];
-  *total = s[kPlanThreads - 1];
+  if (lane == 63) sw[wv] = x;
   __syncthreads();
-  return incl - v;
+  int64_t pre = 0, t = 0;
+  for (int w = 0; w < kPlanThreads / 64; ++w) {
+    const int64_t y = sw[w];
+    if (w < wv) pre += y;
+    t += y;
+  }
+  *total = t;
+  __syncthreads();
+  return pre + x - v;
 }
 
 __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -36,7 +43,6 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32
                                                        const int32_t* __restrict__ bounds, int64_t* off_out,
                                                        int64_t* spec_off, PlanDev P, int64_t arena_cap,
                                                        int64_t spec_cap, int32_t* err, int64_t* voxels) {
-  __shared__ int64_t s[kPlanThreads];
   const int N = sp.N;
   const int t = threadIdx.x;
   const int chunk = (N + kPlanThreads - 1) / kPlanThreads;
@@ -46,6 +52,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32
   // quantities per thread: [0..3] counts, [4..4+4*kNumQ) tile sums (q*4+c), then voxels and spectrum
   constexpr int kVox = 4 + kNumQ * kClasses, kSpec = kVox + 1;  // loc/ex/tot slots of voxels, spectrum
   constexpr int NQ = kSpec + 1;
+  __shared__ int64_t sw[kPlanThreads / 64];
   int64_t loc[NQ];
   for (int q = 0; q < NQ; ++q) loc[q] = 0;
   int bad = 0;
@@ -68,7 +75,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32
   }
   if (bad) atomicOr(err, kErrBoxSize);
   int64_t ex[NQ], tot[NQ];
-  for (int q = 0; q < NQ; ++q) ex[q] = block_scan_excl(loc[q], s, &tot[q]);
+  for (int q = 0; q < NQ; ++q) ex[q] = block_scan_excl(loc[q], sw, &tot[q]);
 
   int64_t cls_start[kClasses];
   {
