@@ -77,7 +77,16 @@ __device__ inline void edge_vertex(int e, const T* L, int xl, int yl, int n, int
   } else if (fabs(v1 - v2) < 0.00001) {
     px = p1x; py = p1y; pz = p1z;
   } else {
-    const double mu = v1 / (v1 - v2);
+    // An edge runs along one axis, so p2 - p1 is a unit step and only mu is
+    // rounded. The FP32 path takes mu in FP32 (reciprocal, ~2 ulp): the values
+    // it interpolates are FP32 already; the FP64 path divides in FP64.
+    double mu;
+    if constexpr (sizeof(T) == 4) {
+      const float f1 = (float)v1, f2 = (float)v2;
+      mu = (double)(f1 * __builtin_amdgcn_rcpf(f1 - f2));
+    } else {
+      mu = v1 / (v1 - v2);
+    }
     px = p1x + mu * (p2x - p1x);
     py = p1y + mu * (p2y - p1y);
     pz = p1z + mu * (p2z - p1z);
