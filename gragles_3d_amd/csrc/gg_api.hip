@@ -399,7 +399,7 @@ int gg_create(const gg_params* params, gg_ctx** out) {
   }
   if ((rc = dalloc(c, &c->LD.sel, N + 1)) || (rc = dalloc(c, &c->P.cls_count, kClasses)) ||
       (rc = dalloc(c, &c->P.cls_start, kClasses)) || (rc = dalloc(c, &c->P.grains, N + 1)) ||
-      (rc = dalloc(c, &c->P.grain_pos, N + 1)) || (rc = dalloc(c, &c->P.order, N)) ||
+      (rc = dalloc(c, &c->P.grain_pos, N + 1)) || (rc = dalloc(c, &c->P.order, N)) || (rc = dalloc(c, &c->P.ncls, N)) ||
       (rc = dalloc(c, &c->P.totals, kNumQ * kClasses)) ||
       (rc = dalloc(c, &c->P.layer_base, kClasses)) || (rc = dalloc(c, &c->P.tile_base, kNumQ * kClasses)) ||
       (rc = dalloc(c, &c->d_partials, (size_t)4 * (N + 1) * kMaxN))) {

@@ -39,6 +39,26 @@ __device__ int64_t block_scan_excl(int64_t v, int64_t* sw, int64_t* total) {
 
 __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Side and size class of every grain, per position in the spatial order
+// (one thread per grain), so the single-workgroup plan below reads one word
+// per grain instead of a chain of dependent loads.
+__global__ __launch_bounds__(256) void k_plan_classes(SimParams sp, const int32_t* __restrict__ state,
+                                                      const int32_t* __restrict__ bounds, PlanDev P, int32_t* err) {
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < sp.N; idx += gridDim.x * blockDim.x) {
+    const int g = P.order[idx];
+    P.grain_pos[g] = -1;
+    int v = -1;
+    if (state[g] == kAlive) {
+      const int32_t* b = bounds + 6 * g;
+      const int n = b[3] - b[0];
+      const int c = size_class(n);
+      if (n < 1 || b[4] - b[1] != n || b[5] - b[2] != n || c < 0) atomicOr(err, kErrBoxSize);
+      else v = n << 8 | c;
+    }
+    P.ncls[idx] = v;
+  }
+}
+
 __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32_t* __restrict__ state,
                                                        const int32_t* __restrict__ bounds, int64_t* off_out,
                                                        int64_t* spec_off, PlanDev P, int64_t arena_cap,
@@ -55,17 +75,12 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32
   __shared__ int64_t sw[kPlanThreads / 64];
   int64_t loc[NQ];
   for (int q = 0; q < NQ; ++q) loc[q] = 0;
-  int bad = 0;
   // grains are visited in the spatial order, so class lists, tiles and the
   // arena layout keep neighbouring boxes close together
   for (int idx = gb; idx < ge; ++idx) {
-    const int g = P.order[idx - 1];
-    P.grain_pos[g] = -1;
-    if (state[g] != kAlive) continue;
-    const int32_t* b = bounds + 6 * g;
-    int n = b[3] - b[0];
-    int c = size_class(n);
-    if (n < 1 || b[4] - b[1] != n || b[5] - b[2] != n || c < 0) { bad = 1; continue; }
+    const int v = P.ncls[idx - 1];
+    if (v < 0) continue;
+    const int n = v >> 8, c = v & 0xFF;
     loc[c] += 1;
     loc[4 + kQLayers * kClasses + c] += n;
     loc[4 + kQCols * kClasses + c] += ceil_div((int64_t)n * n, kColBlock);
@@ -73,7 +88,6 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32
     loc[kVox] += (int64_t)n * n * n;
     loc[kSpec] += (int64_t)n * n * (n / 2 + 1);
   }
-  if (bad) atomicOr(err, kErrBoxSize);
   int64_t ex[NQ], tot[NQ];
   for (int q = 0; q < NQ; ++q) ex[q] = block_scan_excl(loc[q], sw, &tot[q]);
 
@@ -109,12 +123,10 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32
     if (spec_off && tot[kSpec] > spec_cap) atomicOr(err, kErrSpec);
   }
   for (int idx = gb; idx < ge; ++idx) {
+    const int v = P.ncls[idx - 1];
+    if (v < 0) continue;
+    const int n = v >> 8, c = v & 0xFF;
     const int g = P.order[idx - 1];
-    if (state[g] != kAlive) continue;
-    const int32_t* b = bounds + 6 * g;
-    int n = b[3] - b[0];
-    int c = size_class(n);
-    if (n < 1 || b[4] - b[1] != n || b[5] - b[2] != n || c < 0) continue;
     int64_t pos = cls_start[c] + ex[c];
     P.grains[pos] = g;
     P.grain_pos[g] = (int32_t)pos;
@@ -226,6 +238,9 @@ __global__ __launch_bounds__(1024) void k_count(SimParams sp, const int32_t* __r
 void launch_plan(const Launch& L, const SimParams& sp, const int32_t* state, const int32_t* bounds,
                  int64_t* off_out, int64_t* spec_off, const PlanDev& P, int64_t arena_cap, int64_t spec_cap,
                  int32_t* err, int64_t* voxels) {
+  const int cblocks = (sp.N + 255) / 256 < 4096 ? (sp.N + 255) / 256 : 4096;
+  hipLaunchKernelGGL(k_plan_classes, dim3(cblocks > 0 ? cblocks : 1), dim3(256), 0, L.stream, sp, state, bounds, P,
+                     err);
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(kPlanThreads), 0, L.stream, sp, state, bounds, off_out, spec_off, P,
                      arena_cap, spec_cap, err, voxels);
   const int blocks = (sp.N + 255) / 256 < 1024 ? (sp.N + 255) / 256 : 1024;
