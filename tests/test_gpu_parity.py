@@ -95,3 +95,26 @@ def test_512_grains_50_steps_bitexact():
     np.testing.assert_array_equal(ctx.count_history(), counts_o)
     compare_all(ctx, orc, "after 50 steps")
     ctx.close()
+
+
+def test_list_capacity_overflow_is_reported():
+    # Initial lists cut to their first two entries and a capacity of two: the
+    # list build adds the neighbours-of-neighbours whose boxes intersect, which
+    # exceeds the capacity, and the step must report it (not truncate silently).
+    _say("list overflow n=64 ppg=10 border=3 seed=5 max_list=2")
+    p = G.derive_params(64, 10, 3)
+    idf, bounds, exists = G.generate_voronoi(p, 5)
+    offs, ids = G.initial_neighbour_lists(p, bounds, exists)
+    lens = np.diff(offs)
+    assert lens.max() > 2
+    short = [ids[offs[g]:offs[g] + min(2, lens[g])] for g in range(len(lens))]
+    soffs = np.concatenate([[0], np.cumsum([len(x) for x in short])]).astype(np.int32)
+    sids = np.concatenate(short).astype(np.int32)
+    ctx = G.LevelSetContext(p, precision=G.FP64_PLAIN, max_list=2)
+    ctx.set_boxes(bounds, exists)
+    ctx.init_distance(idf)
+    ctx.set_neighbour_lists(soffs, sids)
+    ctx.timestep(1, use_graph=False)  # asynchronous: the device error surfaces at the next synchronisation
+    with pytest.raises(G.GGError, match="comparison list capacity exceeded"):
+        ctx.synchronize()
+    ctx.close()
