@@ -25,6 +25,7 @@ struct gg_ctx {
   // per grain
   int32_t* d_state = nullptr;
   int32_t* d_newb = nullptr;
+  TopoDev topo{nullptr, nullptr, 0};  // hull topology tables (allocated when first enabled)
   int32_t* d_exflag = nullptr;
   double* d_volume = nullptr;
   double* d_area = nullptr;
@@ -114,6 +115,7 @@ int check_device_error(gg_ctx* c) {
     if (e & kErrSpec) m += " spectral scratch full;";
     if (e & kErrEmptyIO) m += " empty in/out intersection in redistancing;";
     if (e & kErrHistory) m += " count history full;";
+    if (e & kErrTopo) m += " hull topology table full;";
     return fail(c, GG_ERR_CAPACITY, m);
   }
   return GG_OK;
@@ -285,7 +287,10 @@ void stage_comparison(gg_ctx* c) {
 
 void stage_level_set(gg_ctx* c) {
   Launch L = launch_of(c);
-  { SegTimer t(c, kSegHull); launch_hull(L, c->sp, c->P, din(c), c->d_exflag, c->d_partials, c->fp32); }
+  {
+    SegTimer t(c, kSegHull);
+    launch_hull(L, c->sp, c->P, din(c), c->d_exflag, c->d_partials, c->IL[c->il_cur], c->topo, c->d_err, c->fp32);
+  }
   {
     SegTimer t(c, kSegFinalize);
     launch_finalize(L, c->sp, c->P, c->d_state, c->LD, c->d_exflag, c->d_partials, c->d_newb, din(c), dout(c),
@@ -780,6 +785,73 @@ int gg_download_list(gg_ctx* c, int32_t id, int32_t which, int32_t* dst, int32_t
   if (len > 0)
     HIPCHK(c, hipMemcpy(dst, c->LD.buf[b] + (size_t)id * c->p.max_list, 4 * len, hipMemcpyDeviceToHost));
   return GG_OK;
+}
+
+int gg_enable_hull_topology(gg_ctx* c, int32_t enable) {
+  if (!c) return GG_ERR_ARG;
+  hipSetDevice(c->p.device);
+  const int on = enable ? 1 : 0;
+  if (on && !c->topo.key) {
+    const size_t n = (size_t)(c->p.n_grains + 1) * kTopoCap;
+    int rc;
+    if ((rc = dalloc(c, &c->topo.key, n)) || (rc = dalloc(c, &c->topo.ord, n))) return rc;
+    HIPCHK(c, hipMemset(c->topo.key, 0xFF, n * sizeof(int32_t)));  // empty tables: no grain touched yet
+  }
+  if (on != c->topo.on) {
+    c->topo.on = on;
+    // a captured step graph holds the old setting
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+  }
+  return GG_OK;
+}
+
+// The grain's topology table, sorted by first appearance.
+static int topo_ids(gg_ctx* c, int32_t id, std::vector<std::pair<int32_t, int32_t>>& out) {
+  out.clear();
+  std::vector<int32_t> k(kTopoCap), o(kTopoCap);
+  HIPCHK(c, hipMemcpy(k.data(), c->topo.key + (size_t)id * kTopoCap, kTopoCap * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(o.data(), c->topo.ord + (size_t)id * kTopoCap, kTopoCap * sizeof(int32_t), hipMemcpyDeviceToHost));
+  for (int i = 0; i < kTopoCap; ++i)
+    if (k[i] != -1) out.push_back({o[i], k[i]});
+  std::sort(out.begin(), out.end());
+  return GG_OK;
+}
+
+int gg_get_hull_neighbour_counts(gg_ctx* c, int32_t* counts, uint8_t* touches_wall) {
+  if (!c || !counts || !touches_wall) return GG_ERR_ARG;
+  if (!c->topo.key) return fail(c, GG_ERR_STATE, "hull topology was never enabled");
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  const size_t n = (size_t)(c->p.n_grains + 1) * kTopoCap;
+  std::vector<int32_t> k(n);
+  HIPCHK(c, hipMemcpy(k.data(), c->topo.key, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  for (int g = 0; g <= c->p.n_grains; ++g) {
+    int cnt = 0, wall = 0;
+    for (int i = 0; i < kTopoCap; ++i) {
+      const int32_t v = k[(size_t)g * kTopoCap + i];
+      cnt += v != -1;
+      wall |= v == 0;
+    }
+    counts[g] = cnt;
+    touches_wall[g] = (uint8_t)wall;
+  }
+  return GG_OK;
+}
+
+int gg_download_hull_neighbours(gg_ctx* c, int32_t id, int32_t* dst, int32_t cap, int32_t* n) {
+  if (!c || !n || id < 0 || id > c->p.n_grains) return GG_ERR_ARG;
+  if (!c->topo.key) return fail(c, GG_ERR_STATE, "hull topology was never enabled");
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  std::vector<std::pair<int32_t, int32_t>> v;
+  if ((rc = topo_ids(c, id, v))) return rc;
+  *n = (int32_t)v.size();
+  if (dst)
+    for (int i = 0; i < (int)v.size() && i < cap; ++i) dst[i] = v[i].second;
+  return (int)v.size() > cap && dst ? fail(c, GG_ERR_CAPACITY, "hull neighbours: cap too small") : GG_OK;
 }
 
 int gg_get_all_bounds(gg_ctx* c, int32_t which, int32_t* bounds) {

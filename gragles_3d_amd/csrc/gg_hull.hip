@@ -12,6 +12,8 @@
 // some cell is cut.  Each tile keeps Neumaier-compensated FP64 sums and writes
 // them to its own slot; k_finalize adds a grain's slots in layer order, so the
 // result does not depend on scheduling.
+#include <algorithm>
+
 #include "gg_internal.h"
 
 namespace gg {
@@ -48,6 +50,9 @@ struct Acc {
 };
 
 constexpr int kHT = 256;
+constexpr int kTopoLdsBits = 7, kTopoLds = 1 << kTopoLdsBits;  // per-tile topology table
+constexpr int kTopoCapBits = 6;
+static_assert((1 << kTopoCapBits) == kTopoCap, "grain topology table size");
 constexpr int kCPT = 8;              // cells classified per lane and pass
 constexpr int kChunk = kCPT * kHT;  // cells classified per pass
 
@@ -102,9 +107,15 @@ __device__ inline void edge_vertex(int e, const T* L, int xl, int yl, int n, int
 // masks, so no lookup goes to memory with a lane-dependent address.
 // FP32: six waves per SIMD asked of the compiler (n <= 32 reaches it with a
 // small spill; n <= 48 stays LDS-bound at five).
-template <class T, int NMAX>
-__global__ __launch_bounds__(kHT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag,
-                                              double* partials) {
+//
+// TOPO: each cut cell also names its grains (the grain itself at inside
+// corners, the IDLocal id at the others), a per-tile LDS table keeps the first
+// appearance of each id, and the tile adds it to the grain's table.  A
+// separate instantiation, so the registers it needs cost nothing when off.
+template <class T, int NMAX, bool TOPO>
+__global__ __launch_bounds__(kHT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1)))
+void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, double* partials, IdlDev il,
+            TopoDev topo, int32_t* err) {
   __shared__ T L[2 * NMAX * NMAX];
   __shared__ uint32_t q[kChunk];
   __shared__ uint64_t s_tri[256];
@@ -112,6 +123,7 @@ __global__ __launch_bounds__(kHT) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
   __shared__ int s_wave[kHT / 64];
   __shared__ double s_red[4][kHT / 64];
   __shared__ int s_any[kHT / 64];
+  __shared__ int s_tk[kTopoLds], s_to[kTopoLds];  // per-tile topology table (id, first appearance)
   const int total = P.totals[kQLayers * kClasses + cls];
   const T* arena = (const T*)din.arena;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -121,6 +133,12 @@ __global__ __launch_bounds__(kHT) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     int cnt = 0;
     for (; (row & 15u) != 15u; row >>= 12) ++cnt;
     s_ntri[i] = (uint8_t)cnt;
+  }
+  if constexpr (TOPO) {
+    for (int i = tid; i < kTopoLds; i += kHT) {
+      s_tk[i] = -1;
+      s_to[i] = INT_MAX;
+    }
   }
   for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
     int g, kl;
@@ -155,6 +173,61 @@ __global__ __launch_bounds__(kHT) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
             if (ci != 0u && ci != 255u) {
               code[r] = ci | ((uint32_t)yl << 8) | ((uint32_t)xl << 16);
               mine += s_ntri[ci];
+            }
+          }
+        }
+        if constexpr (TOPO) {
+          const int32_t* ib = il.bounds + 6 * g;
+          const int ix0 = ib[0], iy0 = ib[1], iz0 = ib[2], inx = ib[3] - ib[0], iny = ib[4] - ib[1],
+                    inz = ib[5] - ib[2];
+          const uint32_t* idl = il.arena + il.off[g];
+#pragma unroll
+          for (int r = 0; r < kCPT; ++r) {
+            if (code[r] == 0u) continue;
+            const unsigned ci = code[r] & 0xFFu;
+            const int yl = (code[r] >> 8) & 0xFF, xl = code[r] >> 16;
+            uint32_t ids[8];
+#pragma unroll
+            for (int qq = 0; qq < 8; ++qq) {
+              uint32_t id = (uint32_t)g;
+              if (!((ci >> qq) & 1u)) {  // outside corner: the IDLocal id there (none outside its box)
+                const int cx = x0 + xl + (int)((kCornerX >> qq) & 1u) - ix0;
+                const int cy = y0 + yl + (int)((kCornerY >> qq) & 1u) - iy0;
+                const int cz = z + (int)((kCornerZ >> qq) & 1u) - iz0;
+                id = ((unsigned)cx < (unsigned)inx && (unsigned)cy < (unsigned)iny && (unsigned)cz < (unsigned)inz)
+                         ? idl[((int64_t)cz * iny + cy) * inx + cx]
+                         : 0xFFFFFFFFu;
+              }
+              ids[qq] = id;
+            }
+            // the cell's distinct ids in ascending order (insertDistinctInteger):
+            // each first occurrence with its rank among the distinct ids
+            bool first[8];
+#pragma unroll
+            for (int qq = 0; qq < 8; ++qq) {
+              first[qq] = true;
+#pragma unroll
+              for (int q2 = 0; q2 < qq; ++q2) first[qq] &= ids[q2] != ids[qq];
+            }
+            const int cell = (kl * m1 + yl) * m1 + xl;
+#pragma unroll
+            for (int qq = 0; qq < 8; ++qq) {
+              if (!first[qq] || ids[qq] == 0xFFFFFFFFu) continue;
+              int rank = 0;
+#pragma unroll
+              for (int q2 = 0; q2 < 8; ++q2) rank += (first[q2] & (ids[q2] < ids[qq])) ? 1 : 0;
+              const int key = (int)ids[qq], o = cell * 8 + rank;
+              uint32_t hsl = ((uint32_t)key * 2654435761u) >> (32 - kTopoLdsBits);
+              bool done = false;
+              for (int pr = 0; pr < kTopoLds && !done; ++pr) {
+                const int old = atomicCAS(&s_tk[hsl], -1, key);
+                if (old == -1 || old == key) {
+                  atomicMin(&s_to[hsl], o);
+                  done = true;
+                }
+                hsl = (hsl + 1) & (kTopoLds - 1);
+              }
+              if (!done) atomicOr(err, kErrTopo);
             }
           }
         }
@@ -214,6 +287,30 @@ __global__ __launch_bounds__(kHT) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
           __syncthreads();
         }
         __syncthreads();  // s_wave is read above before the next chunk writes it
+      }
+    }
+    if constexpr (TOPO) {
+      // add the tile's grains to the grain's table, then clear the tile table
+      __syncthreads();
+      for (int i = tid; i < kTopoLds; i += kHT) {
+        const int key = s_tk[i];
+        if (key != -1) {
+          int32_t* gk = topo.key + (int64_t)g * kTopoCap;
+          int32_t* go = topo.ord + (int64_t)g * kTopoCap;
+          uint32_t hsl = ((uint32_t)key * 2654435761u) >> (32 - kTopoCapBits);
+          bool done = false;
+          for (int pr = 0; pr < kTopoCap && !done; ++pr) {
+            const int old = atomicCAS(&gk[hsl], -1, key);
+            if (old == -1 || old == key) {
+              atomicMin(&go[hsl], s_to[i]);
+              done = true;
+            }
+            hsl = (hsl + 1) & (kTopoCap - 1);
+          }
+          if (!done) atomicOr(err, kErrTopo);
+        }
+        s_tk[i] = -1;
+        s_to[i] = INT_MAX;
       }
     }
     // wavefront then workgroup reduction in a fixed order
@@ -326,18 +423,41 @@ void upload_mc_tables() {
   (void)hipMemcpyToSymbol(HIP_SYMBOL(c_tri), tri, sizeof(tri));
 }
 
+template <class T, bool TOPO>
+void hull_classes(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, int32_t* exflag,
+                  double* partials, const IdlDev& il, const TopoDev& topo, int32_t* err) {
+  hipLaunchKernelGGL((k_hull<T, 32, TOPO>), dim3(tile_grid(L, kGridPerCU, 0)), dim3(kHT), 0, L.stream, sp, P, 0, din,
+                     exflag, partials, il, topo, err);
+  hipLaunchKernelGGL((k_hull<T, 48, TOPO>), dim3(tile_grid(L, kGridPerCU, 1)), dim3(kHT), 0, L.stream, sp, P, 1, din,
+                     exflag, partials, il, topo, err);
+  hipLaunchKernelGGL((k_hull<T, 64, TOPO>), dim3(tile_grid(L, kGridPerCU, 2)), dim3(kHT), 0, L.stream, sp, P, 2, din,
+                     exflag, partials, il, topo, err);
+  hipLaunchKernelGGL((k_hull<T, 96, TOPO>), dim3(tile_grid(L, kGridPerCU, 3)), dim3(kHT), 0, L.stream, sp, P, 3, din,
+                     exflag, partials, il, topo, err);
+}
+
+// Clears the grains' topology tables before a hull with topology on.
+__global__ __launch_bounds__(256) void k_topo_clear(SimParams sp, TopoDev topo) {
+  const int64_t n = (int64_t)(sp.N + 1) * kTopoCap;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    topo.key[i] = -1;
+    topo.ord[i] = INT_MAX;
+  }
+}
+
 void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, int32_t* exflag,
-                 double* partials, bool fp32) {
+                 double* partials, const IdlDev& il, const TopoDev& topo, int32_t* err, bool fp32) {
+  if (topo.on) {
+    const int64_t n = (int64_t)(sp.N + 1) * kTopoCap;
+    const int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_topo_clear, dim3(blocks), dim3(256), 0, L.stream, sp, topo);
+  }
   if (fp32) {
-    hipLaunchKernelGGL((k_hull<float, 32>), dim3(tile_grid(L, kGridPerCU, 0)), dim3(kHT), 0, L.stream, sp, P, 0, din, exflag, partials);
-    hipLaunchKernelGGL((k_hull<float, 48>), dim3(tile_grid(L, kGridPerCU, 1)), dim3(kHT), 0, L.stream, sp, P, 1, din, exflag, partials);
-    hipLaunchKernelGGL((k_hull<float, 64>), dim3(tile_grid(L, kGridPerCU, 2)), dim3(kHT), 0, L.stream, sp, P, 2, din, exflag, partials);
-    hipLaunchKernelGGL((k_hull<float, 96>), dim3(tile_grid(L, kGridPerCU, 3)), dim3(kHT), 0, L.stream, sp, P, 3, din, exflag, partials);
+    if (topo.on) hull_classes<float, true>(L, sp, P, din, exflag, partials, il, topo, err);
+    else hull_classes<float, false>(L, sp, P, din, exflag, partials, il, topo, err);
   } else {
-    hipLaunchKernelGGL((k_hull<double, 32>), dim3(tile_grid(L, kGridPerCU, 0)), dim3(kHT), 0, L.stream, sp, P, 0, din, exflag, partials);
-    hipLaunchKernelGGL((k_hull<double, 48>), dim3(tile_grid(L, kGridPerCU, 1)), dim3(kHT), 0, L.stream, sp, P, 1, din, exflag, partials);
-    hipLaunchKernelGGL((k_hull<double, 64>), dim3(tile_grid(L, kGridPerCU, 2)), dim3(kHT), 0, L.stream, sp, P, 2, din, exflag, partials);
-    hipLaunchKernelGGL((k_hull<double, 96>), dim3(tile_grid(L, kGridPerCU, 3)), dim3(kHT), 0, L.stream, sp, P, 3, din, exflag, partials);
+    if (topo.on) hull_classes<double, true>(L, sp, P, din, exflag, partials, il, topo, err);
+    else hull_classes<double, false>(L, sp, P, din, exflag, partials, il, topo, err);
   }
 }
 

@@ -41,7 +41,20 @@ enum {
   kErrBoxSize = 4,    // box side above kMaxN or box not cubic
   kErrSpec = 8,       // spectral scratch capacity exceeded
   kErrEmptyIO = 16,   // redistancing with an empty in/out intersection
-  kErrHistory = 32    // count history full
+  kErrHistory = 32,   // count history full
+  kErrTopo = 64       // a grain's hull touches more grains than its topology table holds
+};
+
+// Hull topology (optional output of the level set): per grain, the grains its
+// marching-cubes hull touches, as an open-addressed table of kTopoCap slots:
+// key = grain id (-1 empty), ord = smallest first-appearance key of that id
+// (cell index in the (z, y, x) loop * 8 + rank of the id among the cell's
+// ascending ids).
+constexpr int kTopoCap = 64;
+struct TopoDev {
+  int32_t* key;  // [(N+1) * kTopoCap]
+  int32_t* ord;  // [(N+1) * kTopoCap]
+  int on;        // 0: not computed
 };
 
 struct DescDev {
@@ -255,7 +268,7 @@ void launch_comparison(const Launch& L, int cls, const SimParams& sp, const Plan
                        const DescDev& dout, const ListDev& ld, const IdlDev& il_old, const IdlDev& il_new,
                        int32_t* newb, bool fp32);
 void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, int32_t* exflag,
-                 double* partials, bool fp32);
+                 double* partials, const IdlDev& il, const TopoDev& topo, int32_t* err, bool fp32);
 void launch_finalize(const Launch& L, const SimParams& sp, const PlanDev& P, int32_t* state,
                      const ListDev& ld, int32_t* exflag, const double* partials, const int32_t* newb,
                      const DescDev& din, const DescDev& dout, double* volume, double* area);

@@ -77,6 +77,9 @@ SIGNATURES = {
     "gg_download_idlocal": (ctypes.c_int, [_vp, _i32, _i32p, _u32p, ctypes.c_int64, _i64p]),
     "gg_download_list": (ctypes.c_int, [_vp, _i32, _i32, _i32p, _i32, _i32p]),
     "gg_get_all_bounds": (ctypes.c_int, [_vp, _i32, _i32p]),
+    "gg_enable_hull_topology": (ctypes.c_int, [_vp, _i32]),
+    "gg_get_hull_neighbour_counts": (ctypes.c_int, [_vp, _i32p, _u8p]),
+    "gg_download_hull_neighbours": (ctypes.c_int, [_vp, _i32, _i32p, _i32, _i32p]),
     "gg_download_all_lists": (ctypes.c_int, [_vp, _i32, _i32p, _i32p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     "gg_get_new_bounds": (ctypes.c_int, [_vp, _i32p]),
     "gg_box_voxels": (ctypes.c_int, [_vp, _i64p]),
@@ -298,6 +301,26 @@ class LevelSetContext:
         buf = np.zeros(max(n.value, 1), dtype=np.int32)
         self._chk(lib().gg_download_list(self.h, int(gid), int(which), _p(buf, ctypes.c_int32), buf.size,
                                          ctypes.byref(n)))
+        return buf[: n.value].copy()
+
+    def enable_hull_topology(self, on=True):
+        """Collect, at every level set, the grains each hull touches (off by default)."""
+        self._chk(lib().gg_enable_hull_topology(self.h, 1 if on else 0))
+
+    def hull_neighbour_counts(self):
+        """Per grain: number of grains its last hull touched (itself included) and
+        whether the domain wall was among them."""
+        cnt = np.zeros(self.params.n_grains + 1, dtype=np.int32)
+        wall = np.zeros(self.params.n_grains + 1, dtype=np.uint8)
+        self._chk(lib().gg_get_hull_neighbour_counts(self.h, _p(cnt, ctypes.c_int32), _p(wall, ctypes.c_uint8)))
+        return cnt, wall.astype(bool)
+
+    def hull_neighbours(self, gid):
+        """The grains the last hull of gid touched, in the reference's order."""
+        buf = np.zeros(64, dtype=np.int32)
+        n = ctypes.c_int32(0)
+        self._chk(lib().gg_download_hull_neighbours(self.h, int(gid), _p(buf, ctypes.c_int32), buf.size,
+                                                    ctypes.byref(n)))
         return buf[: n.value].copy()
 
     def all_bounds(self, which):

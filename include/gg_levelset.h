@@ -146,6 +146,26 @@ int gg_download_list(gg_ctx* ctx, int32_t id, int32_t which, int32_t* dst, int32
 int gg_get_all_bounds(gg_ctx* ctx, int32_t which, int32_t* bounds);
 int gg_download_all_lists(gg_ctx* ctx, int32_t which, int32_t* offsets, int32_t* ids, int64_t cap, int64_t* n);
 
+/* Hull topology: the grains each grain's marching-cubes hull touches
+ * (MarchingCubesAlgorithm::generateAdditionalInformation / recordNewGrains,
+ * src/marchingCubes.cpp:501-602, with LSbox::getNeighbourAt, src/box.cpp:3000-3006):
+ * every cut cell names the grain itself at its inside corners and the IDLocal id
+ * at the others.  Off by default; while on, every level set (gg_level_set,
+ * gg_timestep) collects it, at most 64 grains per hull (more sets a device
+ * error reported at the next synchronisation).  Enabling drops a captured
+ * timestep graph. */
+int gg_enable_hull_topology(gg_ctx* ctx, int32_t enable);
+/* Per grain after the last level set: the number of grains its hull touched,
+ * itself included (GrainHull::getAllNeighborsCount, which LSbox::extractContour
+ * stores in m_neighborCount, src/box.cpp:1154) and whether the domain wall
+ * (id 0) is among them (GrainHull::IsNeighbor(0)).  Arrays of n_grains+1
+ * entries; 0 / 0 for grains whose hull has no cut cell. */
+int gg_get_hull_neighbour_counts(gg_ctx* ctx, int32_t* counts, uint8_t* touches_wall);
+/* The grains the hull of grain id touched, in the order the reference records
+ * them (first appearance in the (z, y, x) cell loop, ascending within a
+ * cell): up to cap ids into dst, *n = their number. */
+int gg_download_hull_neighbours(gg_ctx* ctx, int32_t id, int32_t* dst, int32_t cap, int32_t* n);
+
 /* New bounds from executeSetComparison (m_new{X,Y,Z}{Min,Max}), rows of 6. */
 int gg_get_new_bounds(gg_ctx* ctx, int32_t* bounds);
 

@@ -450,6 +450,51 @@ void redistance(const Buf<double>& in, Buf<double>& out, double h, double delta)
   }
 }
 
+// Grains the hull touches (MarchingCubesAlgorithm::generateAdditionalInformation
+// + insertDistinctInteger + recordNewGrains, src/marchingCubes.cpp:501-602, with
+// LSbox::getNeighbourAt, src/box.cpp:3000-3006): for every cut cell in the
+// (z, y, x) loop order of generateHull, each corner names the grain itself if
+// it is inside (phi >= 0) and otherwise the IDLocal id there (-1, i.e.
+// 0xFFFFFFFF, outside the IDLocal box); the cell's distinct ids, ascending, are
+// appended to the result in that order where not yet present, 0xFFFFFFFF
+// excepted.  The size of the result is LSbox::m_neighborCount
+// (GrainHull::getAllNeighborsCount) and IsNeighbor(0) asks whether 0 (the
+// domain wall) is in it.
+void hull_neighbours(const Buf<double>& B, const Buf<uint32_t>& idl, unsigned self, std::vector<unsigned>& out) {
+  out.clear();
+  for (int z = B.z0; z < B.z1 - 1; ++z)
+    for (int y = B.y0; y < B.y1 - 1; ++y)
+      for (int x = B.x0; x < B.x1 - 1; ++x) {
+        unsigned c = 0, ids[8];
+        for (int q = 0; q < 8; ++q) {
+          int cx = x + kCorner[q][0], cy = y + kCorner[q][1], cz = z + kCorner[q][2];
+          const bool inside = B.at(cy, cx, cz) >= 0;
+          if (inside) c |= 1u << q;
+          if (inside) ids[q] = self;
+          else if (cx >= idl.x0 && cx < idl.x1 && cy >= idl.y0 && cy < idl.y1 && cz >= idl.z0 && cz < idl.z1)
+            ids[q] = idl.at(cy, cx, cz);
+          else
+            ids[q] = 0xFFFFFFFFu;
+        }
+        if (g_edge_mask[c] == 0) continue;
+        // generateAdditionalInformation visits the corners in the order
+        // (0,0,0) (1,0,0) (1,1,0) (0,1,0) (0,0,1) (1,0,1) (1,1,1) (0,1,1); the
+        // distinct set it builds is sorted, so only the set matters here
+        unsigned list[8];
+        int cnt = 0;
+        for (int q = 0; q < 8; ++q) {
+          bool found = false;
+          for (int i = 0; i < cnt; ++i) found |= list[i] == ids[q];
+          if (!found) list[cnt++] = ids[q];
+        }
+        std::sort(list, list + cnt);
+        for (int i = 0; i < cnt; ++i) {
+          if (list[i] == 0xFFFFFFFFu) continue;
+          if (std::find(out.begin(), out.end(), list[i]) == out.end()) out.push_back(list[i]);
+        }
+      }
+}
+
 // ---------------------------------------------------------------------------
 enum { kDeleted = 0, kAlive = 1, kVanished = 2 };
 
@@ -463,6 +508,7 @@ struct Grain {
   int nb[6] = {0, 0, 0, 0, 0, 0}; // m_new{X,Y,Z}{Min,Max}: xmin,ymin,zmin,xmax,ymax,zmax
   bool touches_wall = false;
   double volume = 0, area = 0;
+  std::vector<unsigned> hull_nb;  // grains the last hull touched, in first-appearance order
 };
 
 }  // namespace
@@ -803,10 +849,12 @@ int orc_level_set(orc_sim* S) {  // level_set, src/grainhdl.cpp:669-684 + extrac
         G.state = kDeleted;
         G.in.reset();
         G.out.reset();
+        G.hull_nb.clear();
         return;
       }
       double vr, ar;
       bool ex = hull_reduce(*G.in, &vr, &ar);
+      hull_neighbours(*G.in, G.idl, G.id, G.hull_nb);
       if (!ex) {
         G.state = kVanished;
         return;
@@ -877,6 +925,34 @@ long orc_get_idlocal(orc_sim* S, int id, int32_t* b, uint32_t* dst, long cap) {
   if (n > cap) return -2;
   std::memcpy(dst, G.idl.v.data(), sizeof(uint32_t) * n);
   return n;
+}
+
+// Grains the last hull of `id` touched, in first-appearance order.
+long orc_get_hull_neighbours(orc_sim* S, int id, int32_t* dst, long cap) {
+  const Grain& G = S->g.at(id);
+  long n = (long)G.hull_nb.size();
+  if (!dst) return n;
+  if (n > cap) return -2;
+  for (long i = 0; i < n; ++i) dst[i] = (int32_t)G.hull_nb[i];
+  return n;
+}
+
+// Hull neighbours of one cube box at origin (x0,y0,z0) for grain `self`, with
+// its IDLocal over the box bounds (b6) given as idl: the ids in
+// first-appearance order; returns their number (-2 if cap is too small).
+long orc_hull_neighbours_box(int n, int x0, int y0, int z0, const double* phi, const int32_t* b6,
+                             const uint32_t* idl, int self, int32_t* dst, long cap) {
+  Buf<double> B;
+  B.resize(x0, y0, z0, x0 + n, y0 + n, z0 + n);
+  std::memcpy(B.v.data(), phi, sizeof(double) * B.v.size());
+  Buf<uint32_t> I;
+  I.resize(b6[0], b6[1], b6[2], b6[3], b6[4], b6[5]);
+  std::memcpy(I.v.data(), idl, sizeof(uint32_t) * I.v.size());
+  std::vector<unsigned> out;
+  hull_neighbours(B, I, (unsigned)self, out);
+  if ((long)out.size() > cap) return -2;
+  for (size_t i = 0; i < out.size(); ++i) dst[i] = (int32_t)out[i];
+  return (long)out.size();
 }
 
 // which: 0 = m_secondOrderNeighbours, 1 = m_comparisonList

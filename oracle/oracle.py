@@ -67,6 +67,9 @@ def lib():
             "orc_get_phi": (ctypes.c_long, [vp, ctypes.c_int, ctypes.c_int, f64p, ctypes.c_long]),
             "orc_get_idlocal": (ctypes.c_long, [vp, ctypes.c_int, i32p, u32p, ctypes.c_long]),
             "orc_get_list": (ctypes.c_long, [vp, ctypes.c_int, ctypes.c_int, i32p, ctypes.c_long]),
+            "orc_get_hull_neighbours": (ctypes.c_long, [vp, ctypes.c_int, i32p, ctypes.c_long]),
+            "orc_hull_neighbours_box": (ctypes.c_long, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, f64p,
+                                                        i32p, u32p, ctypes.c_int, i32p, ctypes.c_long]),
             "orc_get_stats": (None, [vp, f64p, f64p, i32p]),
             "orc_resize_to_cube": (None, [i32p, ctypes.c_int]),
             "orc_convolve_box": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_double, f64p, f64p]),
@@ -125,6 +128,22 @@ def hull_box(phi, origin, h):
     ex = lib().orc_hull_box(n, int(origin[0]), int(origin[1]), int(origin[2]), _p(phi, ctypes.c_double),
                             float(h), _p(v, ctypes.c_double), _p(a, ctypes.c_double))
     return bool(ex), float(v[0]), float(a[0])
+
+
+def hull_neighbours_box(phi, origin, idl_bounds, idl, self_id):
+    """Grains the hull of a cube box touches (own id for inside corners, IDLocal
+    ids elsewhere), in first-appearance order."""
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    n = phi.shape[0]
+    b6 = np.ascontiguousarray(idl_bounds, dtype=np.int32)
+    idl = np.ascontiguousarray(idl, dtype=np.uint32)
+    assert idl.size == (b6[3] - b6[0]) * (b6[4] - b6[1]) * (b6[5] - b6[2])
+    out = np.empty(1024, dtype=np.int32)
+    k = lib().orc_hull_neighbours_box(n, int(origin[0]), int(origin[1]), int(origin[2]), _p(phi, ctypes.c_double),
+                                      _p(b6, ctypes.c_int32), _p(idl, ctypes.c_uint32), int(self_id),
+                                      _p(out, ctypes.c_int32), out.size)
+    assert k >= 0
+    return out[:k].copy()
 
 
 def redistance_box(in_bounds, phi_in, out_bounds, h, delta):
@@ -260,6 +279,13 @@ class OracleSim:
     def neighbour_list(self, gid, which=0):
         buf = np.empty(4096, dtype=np.int32)
         n = lib().orc_get_list(self.h, int(gid), int(which), _p(buf, ctypes.c_int32), buf.size)
+        assert n >= 0
+        return buf[:n].copy()
+
+    def hull_neighbours(self, gid):
+        """Grains the last hull of gid touched, in first-appearance order."""
+        buf = np.empty(4096, dtype=np.int32)
+        n = lib().orc_get_hull_neighbours(self.h, int(gid), _p(buf, ctypes.c_int32), buf.size)
         assert n >= 0
         return buf[:n].copy()
 
