@@ -25,7 +25,7 @@ struct gg_ctx {
   // per grain
   int32_t* d_state = nullptr;
   int32_t* d_newb = nullptr;
-  TopoDev topo{nullptr, nullptr, 0};  // hull topology tables (allocated when first enabled)
+  TopoDev topo{nullptr, nullptr, nullptr, 0};  // hull topology tables (allocated when first enabled)
   int32_t* d_exflag = nullptr;
   double* d_volume = nullptr;
   double* d_area = nullptr;
@@ -794,8 +794,12 @@ int gg_enable_hull_topology(gg_ctx* c, int32_t enable) {
   if (on && !c->topo.key) {
     const size_t n = (size_t)(c->p.n_grains + 1) * kTopoCap;
     int rc;
-    if ((rc = dalloc(c, &c->topo.key, n)) || (rc = dalloc(c, &c->topo.ord, n))) return rc;
-    HIPCHK(c, hipMemset(c->topo.key, 0xFF, n * sizeof(int32_t)));  // empty tables: no grain touched yet
+    const size_t nj = (size_t)(c->p.n_grains + 1) * kJuncCap;
+    if ((rc = dalloc(c, &c->topo.key, n)) || (rc = dalloc(c, &c->topo.ord, n)) || (rc = dalloc(c, &c->topo.jkey, nj)))
+      return rc;
+    // empty tables: no grain touched yet
+    HIPCHK(c, hipMemset(c->topo.key, 0xFF, n * sizeof(int32_t)));
+    HIPCHK(c, hipMemset(c->topo.jkey, 0, nj * sizeof(unsigned long long)));
   }
   if (on != c->topo.on) {
     c->topo.on = on;
@@ -837,6 +841,27 @@ int gg_get_hull_neighbour_counts(gg_ctx* c, int32_t* counts, uint8_t* touches_wa
     }
     counts[g] = cnt;
     touches_wall[g] = (uint8_t)wall;
+  }
+  return GG_OK;
+}
+
+int gg_get_hull_junction_counts(gg_ctx* c, int32_t* counts4) {
+  if (!c || !counts4) return GG_ERR_ARG;
+  if (!c->topo.key) return fail(c, GG_ERR_STATE, "hull topology was never enabled");
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  const size_t nj = (size_t)(c->p.n_grains + 1) * kJuncCap;
+  std::vector<unsigned long long> j(nj);
+  HIPCHK(c, hipMemcpy(j.data(), c->topo.jkey, nj * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  for (int g = 0; g <= c->p.n_grains; ++g) {
+    int32_t* o = counts4 + 4 * (size_t)g;
+    o[0] = o[1] = o[2] = o[3] = 0;
+    for (int i = 0; i < kJuncCap; ++i) {
+      const unsigned long long m = j[(size_t)g * kJuncCap + i];
+      if (!m) continue;
+      const int k = __builtin_popcountll(m);
+      ++o[k == 2 ? 0 : k == 3 ? 1 : k == 4 ? 2 : 3];
+    }
   }
   return GG_OK;
 }

@@ -53,6 +53,10 @@ constexpr int kHT = 256;
 constexpr int kTopoLdsBits = 7, kTopoLds = 1 << kTopoLdsBits;  // per-tile topology table
 constexpr int kTopoCapBits = 6;
 static_assert((1 << kTopoCapBits) == kTopoCap, "grain topology table size");
+static_assert(kTopoCap <= 64, "a junction list is a 64-bit mask over the grain table's slots");
+constexpr int kJuncLdsBits = 8, kJuncLds = 1 << kJuncLdsBits;  // per-tile junction table
+constexpr int kJuncCapBits = 9;
+static_assert((1 << kJuncCapBits) == kJuncCap, "grain junction table size");
 constexpr int kCPT = 8;              // cells classified per lane and pass
 constexpr int kChunk = kCPT * kHT;  // cells classified per pass
 
@@ -124,6 +128,7 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
   __shared__ double s_red[4][kHT / 64];
   __shared__ int s_any[kHT / 64];
   __shared__ int s_tk[kTopoLds], s_to[kTopoLds];  // per-tile topology table (id, first appearance)
+  __shared__ unsigned long long s_jm[kJuncLds];    // per-tile junction lists (slot masks)
   const int total = P.totals[kQLayers * kClasses + cls];
   const T* arena = (const T*)din.arena;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -139,6 +144,7 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
       s_tk[i] = -1;
       s_to[i] = INT_MAX;
     }
+    for (int i = tid; i < kJuncLds; i += kHT) s_jm[i] = 0ull;
   }
   for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
     int g, kl;
@@ -181,16 +187,15 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
           const int ix0 = ib[0], iy0 = ib[1], iz0 = ib[2], inx = ib[3] - ib[0], iny = ib[4] - ib[1],
                     inz = ib[5] - ib[2];
           const uint32_t* idl = il.arena + il.off[g];
-#pragma unroll
-          for (int r = 0; r < kCPT; ++r) {
-            if (code[r] == 0u) continue;
-            const unsigned ci = code[r] & 0xFFu;
-            const int yl = (code[r] >> 8) & 0xFF, xl = code[r] >> 16;
-            uint32_t ids[8];
+          // the ids a cut cell names (own id inside, IDLocal id outside, none
+          // outside the IDLocal box) and which of them are first occurrences
+          auto cell_ids = [&](uint32_t cd, uint32_t (&ids)[8], bool (&first)[8]) {
+            const unsigned ci = cd & 0xFFu;
+            const int yl = (cd >> 8) & 0xFF, xl = cd >> 16;
 #pragma unroll
             for (int qq = 0; qq < 8; ++qq) {
               uint32_t id = (uint32_t)g;
-              if (!((ci >> qq) & 1u)) {  // outside corner: the IDLocal id there (none outside its box)
+              if (!((ci >> qq) & 1u)) {
                 const int cx = x0 + xl + (int)((kCornerX >> qq) & 1u) - ix0;
                 const int cy = y0 + yl + (int)((kCornerY >> qq) & 1u) - iy0;
                 const int cz = z + (int)((kCornerZ >> qq) & 1u) - iz0;
@@ -200,16 +205,23 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
               }
               ids[qq] = id;
             }
-            // the cell's distinct ids in ascending order (insertDistinctInteger):
-            // each first occurrence with its rank among the distinct ids
-            bool first[8];
 #pragma unroll
             for (int qq = 0; qq < 8; ++qq) {
               first[qq] = true;
 #pragma unroll
               for (int q2 = 0; q2 < qq; ++q2) first[qq] &= ids[q2] != ids[qq];
             }
-            const int cell = (kl * m1 + yl) * m1 + xl;
+          };
+          // 1. grains: each distinct id with its first-appearance key (cell in the
+          //    (z, y, x) loop * 8 + rank among the cell's ascending ids), per tile
+          //    in LDS, then into the grain's table
+#pragma unroll
+          for (int r = 0; r < kCPT; ++r) {
+            if (code[r] == 0u) continue;
+            uint32_t ids[8];
+            bool first[8];
+            cell_ids(code[r], ids, first);
+            const int cell = (kl * m1 + (int)((code[r] >> 8) & 0xFF)) * m1 + (int)(code[r] >> 16);
 #pragma unroll
             for (int qq = 0; qq < 8; ++qq) {
               if (!first[qq] || ids[qq] == 0xFFFFFFFFu) continue;
@@ -229,6 +241,79 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
               }
               if (!done) atomicOr(err, kErrTopo);
             }
+          }
+          __syncthreads();
+          int32_t* gk = topo.key + (int64_t)g * kTopoCap;
+          int32_t* go = topo.ord + (int64_t)g * kTopoCap;
+          for (int i = tid; i < kTopoLds; i += kHT) {
+            const int key = s_tk[i];
+            if (key != -1) {
+              uint32_t hsl = ((uint32_t)key * 2654435761u) >> (32 - kTopoCapBits);
+              bool done = false;
+              for (int pr = 0; pr < kTopoCap && !done; ++pr) {
+                const int old = atomicCAS(&gk[hsl], -1, key);
+                if (old == -1 || old == key) {
+                  atomicMin(&go[hsl], s_to[i]);
+                  done = true;
+                }
+                hsl = (hsl + 1) & (kTopoCap - 1);
+              }
+              if (!done) atomicOr(err, kErrTopo);
+            }
+            s_tk[i] = -1;
+            s_to[i] = INT_MAX;
+          }
+          __syncthreads();
+          // 2. junctions: a cell's list of distinct ids as a mask over the slots
+          //    its ids hold in the grain's table (a slot never changes once
+          //    taken, so equal lists give equal masks); distinct masks per tile in
+          //    LDS, then into the grain's junction table
+#pragma unroll
+          for (int r = 0; r < kCPT; ++r) {
+            if (code[r] == 0u) continue;
+            uint32_t ids[8];
+            bool first[8];
+            cell_ids(code[r], ids, first);
+            uint64_t mask = 0;
+#pragma unroll
+            for (int qq = 0; qq < 8; ++qq) {
+              if (!first[qq] || ids[qq] == 0xFFFFFFFFu) continue;
+              uint32_t hsl = (ids[qq] * 2654435761u) >> (32 - kTopoCapBits);
+              for (int pr = 0; pr < kTopoCap; ++pr) {
+                const int k = __hip_atomic_load(&gk[hsl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (k == (int)ids[qq]) {
+                  mask |= 1ull << hsl;
+                  break;
+                }
+                if (k == -1) break;  // not in the table (it was full: reported above)
+                hsl = (hsl + 1) & (kTopoCap - 1);
+              }
+            }
+            if (mask == 0) continue;
+            uint32_t hsl = (uint32_t)((mask * 0x9E3779B97F4A7C15ull) >> (64 - kJuncLdsBits));
+            bool done = false;
+            for (int pr = 0; pr < kJuncLds && !done; ++pr) {
+              const unsigned long long old = atomicCAS(&s_jm[hsl], 0ull, (unsigned long long)mask);
+              done = old == 0ull || old == mask;
+              hsl = (hsl + 1) & (kJuncLds - 1);
+            }
+            if (!done) atomicOr(err, kErrTopo);
+          }
+          __syncthreads();
+          unsigned long long* gj = topo.jkey + (int64_t)g * kJuncCap;
+          for (int i = tid; i < kJuncLds; i += kHT) {
+            const unsigned long long m = s_jm[i];
+            if (m != 0ull) {
+              uint32_t hsl = (uint32_t)((m * 0x9E3779B97F4A7C15ull) >> (64 - kJuncCapBits));
+              bool done = false;
+              for (int pr = 0; pr < kJuncCap && !done; ++pr) {
+                const unsigned long long old = atomicCAS(&gj[hsl], 0ull, m);
+                done = old == 0ull || old == m;
+                hsl = (hsl + 1) & (kJuncCap - 1);
+              }
+              if (!done) atomicOr(err, kErrTopo);
+            }
+            s_jm[i] = 0ull;
           }
         }
         // exclusive scan of the per-lane triangle counts: wavefront, then workgroup
@@ -287,30 +372,6 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
           __syncthreads();
         }
         __syncthreads();  // s_wave is read above before the next chunk writes it
-      }
-    }
-    if constexpr (TOPO) {
-      // add the tile's grains to the grain's table, then clear the tile table
-      __syncthreads();
-      for (int i = tid; i < kTopoLds; i += kHT) {
-        const int key = s_tk[i];
-        if (key != -1) {
-          int32_t* gk = topo.key + (int64_t)g * kTopoCap;
-          int32_t* go = topo.ord + (int64_t)g * kTopoCap;
-          uint32_t hsl = ((uint32_t)key * 2654435761u) >> (32 - kTopoCapBits);
-          bool done = false;
-          for (int pr = 0; pr < kTopoCap && !done; ++pr) {
-            const int old = atomicCAS(&gk[hsl], -1, key);
-            if (old == -1 || old == key) {
-              atomicMin(&go[hsl], s_to[i]);
-              done = true;
-            }
-            hsl = (hsl + 1) & (kTopoCap - 1);
-          }
-          if (!done) atomicOr(err, kErrTopo);
-        }
-        s_tk[i] = -1;
-        s_to[i] = INT_MAX;
       }
     }
     // wavefront then workgroup reduction in a fixed order
@@ -438,17 +499,20 @@ void hull_classes(const Launch& L, const SimParams& sp, const PlanDev& P, const 
 
 // Clears the grains' topology tables before a hull with topology on.
 __global__ __launch_bounds__(256) void k_topo_clear(SimParams sp, TopoDev topo) {
-  const int64_t n = (int64_t)(sp.N + 1) * kTopoCap;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    topo.key[i] = -1;
-    topo.ord[i] = INT_MAX;
+  const int64_t n = (int64_t)(sp.N + 1) * kTopoCap, nj = (int64_t)(sp.N + 1) * kJuncCap;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nj; i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < n) {
+      topo.key[i] = -1;
+      topo.ord[i] = INT_MAX;
+    }
+    topo.jkey[i] = 0ull;
   }
 }
 
 void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, int32_t* exflag,
                  double* partials, const IdlDev& il, const TopoDev& topo, int32_t* err, bool fp32) {
   if (topo.on) {
-    const int64_t n = (int64_t)(sp.N + 1) * kTopoCap;
+    const int64_t n = (int64_t)(sp.N + 1) * kJuncCap;
     const int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
     hipLaunchKernelGGL(k_topo_clear, dim3(blocks), dim3(256), 0, L.stream, sp, topo);
   }

@@ -50,11 +50,17 @@ enum {
 // key = grain id (-1 empty), ord = smallest first-appearance key of that id
 // (cell index in the (z, y, x) loop * 8 + rank of the id among the cell's
 // ascending ids).
+// Junction lists: each cut cell's list of distinct ids, as a mask over the
+// slots those ids hold in the grain's table; per grain an open-addressed set of
+// kJuncCap masks (0 empty), counted by popcount on the host
+// (GrainHull::computeGrainBoundaryElements' classes).
 constexpr int kTopoCap = 64;
+constexpr int kJuncCap = 512;  // ~160 lists at most per hull at 8000 grains
 struct TopoDev {
-  int32_t* key;  // [(N+1) * kTopoCap]
-  int32_t* ord;  // [(N+1) * kTopoCap]
-  int on;        // 0: not computed
+  int32_t* key;                // [(N+1) * kTopoCap]
+  int32_t* ord;                // [(N+1) * kTopoCap]
+  unsigned long long* jkey;    // [(N+1) * kJuncCap]
+  int on;                      // 0: not computed
 };
 
 struct DescDev {

@@ -80,6 +80,7 @@ SIGNATURES = {
     "gg_enable_hull_topology": (ctypes.c_int, [_vp, _i32]),
     "gg_get_hull_neighbour_counts": (ctypes.c_int, [_vp, _i32p, _u8p]),
     "gg_download_hull_neighbours": (ctypes.c_int, [_vp, _i32, _i32p, _i32, _i32p]),
+    "gg_get_hull_junction_counts": (ctypes.c_int, [_vp, _i32p]),
     "gg_download_all_lists": (ctypes.c_int, [_vp, _i32, _i32p, _i32p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     "gg_get_new_bounds": (ctypes.c_int, [_vp, _i32p]),
     "gg_box_voxels": (ctypes.c_int, [_vp, _i64p]),
@@ -314,6 +315,14 @@ class LevelSetContext:
         wall = np.zeros(self.params.n_grains + 1, dtype=np.uint8)
         self._chk(lib().gg_get_hull_neighbour_counts(self.h, _p(cnt, ctypes.c_int32), _p(wall, ctypes.c_uint8)))
         return cnt, wall.astype(bool)
+
+    def hull_junction_counts(self):
+        """Per grain (rows 0..n_grains): distinct cell id lists of its last hull with
+        2, 3, 4 and other numbers of grains (boundaries, triple lines, quadruple
+        and higher-order junctions)."""
+        c4 = np.zeros((self.params.n_grains + 1, 4), dtype=np.int32)
+        self._chk(lib().gg_get_hull_junction_counts(self.h, _p(c4, ctypes.c_int32)))
+        return c4
 
     def hull_neighbours(self, gid):
         """The grains the last hull of gid touched, in the reference's order."""

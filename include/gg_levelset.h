@@ -165,6 +165,14 @@ int gg_get_hull_neighbour_counts(gg_ctx* ctx, int32_t* counts, uint8_t* touches_
  * them (first appearance in the (z, y, x) cell loop, ascending within a
  * cell): up to cap ids into dst, *n = their number. */
 int gg_download_hull_neighbours(gg_ctx* ctx, int32_t id, int32_t* dst, int32_t cap, int32_t* n);
+/* Per grain after the last level set: the distinct per-cell id lists of its
+ * hull (MarchingCubesAlgorithm::insertNeighborList, src/marchingCubes.cpp:604-617)
+ * counted as GrainHull::computeGrainBoundaryElements classifies them
+ * (src/grainHull.cpp:187-226): counts4[4g+0] lists of 2 grains (grain
+ * boundaries), +1 of 3 (triple lines), +2 of 4 (quadruple junctions), +3 any
+ * other size (higher-order junctions); before GrainHull::mergeJunction.  At
+ * most 512 lists per hull (more sets the topology device error). */
+int gg_get_hull_junction_counts(gg_ctx* ctx, int32_t* counts4);
 
 /* New bounds from executeSetComparison (m_new{X,Y,Z}{Min,Max}), rows of 6. */
 int gg_get_new_bounds(gg_ctx* ctx, int32_t* bounds);

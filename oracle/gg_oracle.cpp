@@ -460,8 +460,18 @@ void redistance(const Buf<double>& in, Buf<double>& out, double h, double delta)
 // excepted.  The size of the result is LSbox::m_neighborCount
 // (GrainHull::getAllNeighborsCount) and IsNeighbor(0) asks whether 0 (the
 // domain wall) is in it.
-void hull_neighbours(const Buf<double>& B, const Buf<uint32_t>& idl, unsigned self, std::vector<unsigned>& out) {
+//
+// It also counts the distinct per-cell id lists (insertNeighborList,
+// :604-617; 0xFFFFFFFF pads a list, so it does not tell lists apart) by size,
+// as GrainHull::computeGrainBoundaryElements classifies them
+// (src/grainHull.cpp:187-226): junc[0] lists of 2 ids (grain boundaries),
+// junc[1] of 3 (triple lines), junc[2] of 4 (quadruple junctions), junc[3]
+// any other size (higher-order junctions).
+void hull_neighbours(const Buf<double>& B, const Buf<uint32_t>& idl, unsigned self, std::vector<unsigned>& out,
+                     int* junc) {
   out.clear();
+  std::vector<std::vector<unsigned>> lists;
+  for (int i = 0; i < 4; ++i) junc[i] = 0;
   for (int z = B.z0; z < B.z1 - 1; ++z)
     for (int y = B.y0; y < B.y1 - 1; ++y)
       for (int x = B.x0; x < B.x1 - 1; ++x) {
@@ -492,6 +502,14 @@ void hull_neighbours(const Buf<double>& B, const Buf<uint32_t>& idl, unsigned se
           if (list[i] == 0xFFFFFFFFu) continue;
           if (std::find(out.begin(), out.end(), list[i]) == out.end()) out.push_back(list[i]);
         }
+        std::vector<unsigned> key;
+        for (int i = 0; i < cnt; ++i)
+          if (list[i] != 0xFFFFFFFFu) key.push_back(list[i]);
+        if (std::find(lists.begin(), lists.end(), key) == lists.end()) {
+          lists.push_back(key);
+          const int k = (int)key.size();
+          ++junc[k == 2 ? 0 : k == 3 ? 1 : k == 4 ? 2 : 3];
+        }
       }
 }
 
@@ -509,6 +527,7 @@ struct Grain {
   bool touches_wall = false;
   double volume = 0, area = 0;
   std::vector<unsigned> hull_nb;  // grains the last hull touched, in first-appearance order
+  int hull_junc[4] = {0, 0, 0, 0};  // distinct cell id lists of 2, 3, 4 and other sizes
 };
 
 }  // namespace
@@ -850,11 +869,12 @@ int orc_level_set(orc_sim* S) {  // level_set, src/grainhdl.cpp:669-684 + extrac
         G.in.reset();
         G.out.reset();
         G.hull_nb.clear();
+        for (int& v : G.hull_junc) v = 0;
         return;
       }
       double vr, ar;
       bool ex = hull_reduce(*G.in, &vr, &ar);
-      hull_neighbours(*G.in, G.idl, G.id, G.hull_nb);
+      hull_neighbours(*G.in, G.idl, G.id, G.hull_nb, G.hull_junc);
       if (!ex) {
         G.state = kVanished;
         return;
@@ -937,11 +957,17 @@ long orc_get_hull_neighbours(orc_sim* S, int id, int32_t* dst, long cap) {
   return n;
 }
 
+// Junction counts of the last hull of `id` (see hull_neighbours).
+void orc_get_hull_junctions(orc_sim* S, int id, int32_t* c4) {
+  const Grain& G = S->g.at(id);
+  for (int i = 0; i < 4; ++i) c4[i] = G.hull_junc[i];
+}
+
 // Hull neighbours of one cube box at origin (x0,y0,z0) for grain `self`, with
 // its IDLocal over the box bounds (b6) given as idl: the ids in
 // first-appearance order; returns their number (-2 if cap is too small).
 long orc_hull_neighbours_box(int n, int x0, int y0, int z0, const double* phi, const int32_t* b6,
-                             const uint32_t* idl, int self, int32_t* dst, long cap) {
+                             const uint32_t* idl, int self, int32_t* dst, long cap, int32_t* junc) {
   Buf<double> B;
   B.resize(x0, y0, z0, x0 + n, y0 + n, z0 + n);
   std::memcpy(B.v.data(), phi, sizeof(double) * B.v.size());
@@ -949,7 +975,10 @@ long orc_hull_neighbours_box(int n, int x0, int y0, int z0, const double* phi, c
   I.resize(b6[0], b6[1], b6[2], b6[3], b6[4], b6[5]);
   std::memcpy(I.v.data(), idl, sizeof(uint32_t) * I.v.size());
   std::vector<unsigned> out;
-  hull_neighbours(B, I, (unsigned)self, out);
+  int j4[4];
+  hull_neighbours(B, I, (unsigned)self, out, j4);
+  if (junc)
+    for (int i = 0; i < 4; ++i) junc[i] = j4[i];
   if ((long)out.size() > cap) return -2;
   for (size_t i = 0; i < out.size(); ++i) dst[i] = (int32_t)out[i];
   return (long)out.size();

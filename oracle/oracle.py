@@ -69,7 +69,8 @@ def lib():
             "orc_get_list": (ctypes.c_long, [vp, ctypes.c_int, ctypes.c_int, i32p, ctypes.c_long]),
             "orc_get_hull_neighbours": (ctypes.c_long, [vp, ctypes.c_int, i32p, ctypes.c_long]),
             "orc_hull_neighbours_box": (ctypes.c_long, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, f64p,
-                                                        i32p, u32p, ctypes.c_int, i32p, ctypes.c_long]),
+                                                        i32p, u32p, ctypes.c_int, i32p, ctypes.c_long, i32p]),
+            "orc_get_hull_junctions": (None, [vp, ctypes.c_int, i32p]),
             "orc_get_stats": (None, [vp, f64p, f64p, i32p]),
             "orc_resize_to_cube": (None, [i32p, ctypes.c_int]),
             "orc_convolve_box": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_double, f64p, f64p]),
@@ -130,20 +131,22 @@ def hull_box(phi, origin, h):
     return bool(ex), float(v[0]), float(a[0])
 
 
-def hull_neighbours_box(phi, origin, idl_bounds, idl, self_id):
+def hull_neighbours_box(phi, origin, idl_bounds, idl, self_id, junctions=False):
     """Grains the hull of a cube box touches (own id for inside corners, IDLocal
-    ids elsewhere), in first-appearance order."""
+    ids elsewhere), in first-appearance order; with junctions=True also the
+    counts of distinct cell id lists of 2, 3, 4 and other sizes."""
     phi = np.ascontiguousarray(phi, dtype=np.float64)
     n = phi.shape[0]
     b6 = np.ascontiguousarray(idl_bounds, dtype=np.int32)
     idl = np.ascontiguousarray(idl, dtype=np.uint32)
     assert idl.size == (b6[3] - b6[0]) * (b6[4] - b6[1]) * (b6[5] - b6[2])
     out = np.empty(1024, dtype=np.int32)
+    j4 = np.zeros(4, dtype=np.int32)
     k = lib().orc_hull_neighbours_box(n, int(origin[0]), int(origin[1]), int(origin[2]), _p(phi, ctypes.c_double),
                                       _p(b6, ctypes.c_int32), _p(idl, ctypes.c_uint32), int(self_id),
-                                      _p(out, ctypes.c_int32), out.size)
+                                      _p(out, ctypes.c_int32), out.size, _p(j4, ctypes.c_int32))
     assert k >= 0
-    return out[:k].copy()
+    return (out[:k].copy(), j4) if junctions else out[:k].copy()
 
 
 def redistance_box(in_bounds, phi_in, out_bounds, h, delta):
@@ -288,6 +291,12 @@ class OracleSim:
         n = lib().orc_get_hull_neighbours(self.h, int(gid), _p(buf, ctypes.c_int32), buf.size)
         assert n >= 0
         return buf[:n].copy()
+
+    def hull_junctions(self, gid):
+        """Distinct cell id lists of the last hull of gid with 2, 3, 4 and other numbers of ids."""
+        c4 = np.zeros(4, dtype=np.int32)
+        lib().orc_get_hull_junctions(self.h, int(gid), _p(c4, ctypes.c_int32))
+        return c4
 
     def stats(self):
         v = np.zeros(self.n_grains + 1)

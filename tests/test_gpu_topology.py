@@ -21,9 +21,13 @@ def _say(msg):
 
 def _compare(ctx, orc, where):
     cnt, wall = ctx.hull_neighbour_counts()
+    junc = ctx.hull_junction_counts()
     st, _, _ = ctx.stats()
     checked = 0
     for g in range(1, len(st)):
+        want_j = orc.hull_junctions(g)
+        if not np.array_equal(junc[g], want_j):
+            raise AssertionError("%s: grain %d junction counts %s, oracle %s" % (where, g, list(junc[g]), list(want_j)))
         want = orc.hull_neighbours(g)
         got = ctx.hull_neighbours(g)
         if not np.array_equal(got, want):
@@ -41,11 +45,14 @@ def test_hull_topology_matches_oracle(n_grains, ppg, border, seed):
     ctx, orc, p = make_pair(n_grains, ppg, border, seed)
     ctx.enable_hull_topology(True)
     walls = 0
+    junctions = np.zeros(4, dtype=np.int64)
     for s in range(6):
         step_both(ctx, orc)
         assert _compare(ctx, orc, "step %d" % s) > 0
         walls += int(ctx.hull_neighbour_counts()[1].sum())
+        junctions += ctx.hull_junction_counts().sum(axis=0)
     assert walls > 0, "no hull touched the domain wall; the case does not exercise id 0"
+    assert junctions[0] > 0 and junctions[1] > 0 and junctions[2] > 0, junctions
     ctx.close()
 
 

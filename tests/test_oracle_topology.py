@@ -21,7 +21,9 @@ def half_space():
 
 def test_half_space_names_self_and_neighbour_ascending():
     idl = np.full(N ** 3, 7, dtype=np.uint32)
-    assert list(O.hull_neighbours_box(half_space(), ORIGIN, BOX, idl, 3)) == [3, 7]
+    ids, junc = O.hull_neighbours_box(half_space(), ORIGIN, BOX, idl, 3, junctions=True)
+    assert list(ids) == [3, 7]
+    assert list(junc) == [1, 0, 0, 0]  # one grain boundary, no junction
     assert list(O.hull_neighbours_box(half_space(), ORIGIN, BOX, idl, 9)) == [7, 9]
 
 
@@ -36,11 +38,34 @@ def test_triple_line_order_of_first_appearance():
     idl = np.empty((N, N, N), dtype=np.uint32)
     idl[:, :4, :] = 5
     idl[:, 4:, :] = 6
-    got = list(O.hull_neighbours_box(half_space(), ORIGIN, BOX, idl.ravel(), 3))
-    assert got == [3, 5, 6]
+    got, junc = O.hull_neighbours_box(half_space(), ORIGIN, BOX, idl.ravel(), 3, junctions=True)
+    assert list(got) == [3, 5, 6]
+    # lists {3,5}, {3,6} (grain boundaries) and {3,5,6} (the triple line)
+    assert list(junc) == [2, 1, 0, 0]
     # with the ids swapped the first cells see {3, 6}: order follows the cells, not the ids
     idl2 = np.where(idl == 5, 6, 5).astype(np.uint32)
     assert list(O.hull_neighbours_box(half_space(), ORIGIN, BOX, idl2.ravel(), 3)) == [3, 6, 5]
+
+
+def test_quadruple_and_higher_junctions():
+    # three outside regions meeting the grain's face: 11 for y < 4, and for
+    # y >= 4 12 below z = 4 and 13 above; the cell at the meeting point sees the
+    # grain and all three (4 ids: a quadruple junction)
+    idl = np.empty((N, N, N), dtype=np.uint32)
+    idl[:, :4, :] = 11
+    idl[:4, 4:, :] = 12
+    idl[4:, 4:, :] = 13
+    ids, junc = O.hull_neighbours_box(half_space(), ORIGIN, BOX, idl.ravel(), 3, junctions=True)
+    assert sorted(ids) == [3, 11, 12, 13]
+    assert list(junc) == [3, 3, 1, 0]  # {3,11} {3,12} {3,13}; three triple lines; the point
+    # four quadrants meeting: the centre cell sees five grains (a higher-order junction)
+    idl[:4, :4, :] = 11
+    idl[:4, 4:, :] = 12
+    idl[4:, :4, :] = 13
+    idl[4:, 4:, :] = 14
+    ids, junc = O.hull_neighbours_box(half_space(), ORIGIN, BOX, idl.ravel(), 3, junctions=True)
+    assert sorted(ids) == [3, 11, 12, 13, 14]
+    assert list(junc) == [4, 4, 0, 1]
 
 
 def test_wall_id_zero_is_a_neighbour():
@@ -55,4 +80,6 @@ def test_corners_outside_the_idlocal_box_name_nothing():
     # so their outside corners (x = 14) fall outside it and only the grain remains
     b6 = np.array([15, 20, 30, 18, 28, 38], dtype=np.int32)
     idl = np.full(3 * N * N, 7, dtype=np.uint32)
-    assert list(O.hull_neighbours_box(half_space(), ORIGIN, b6, idl, 3)) == [3]
+    ids, junc = O.hull_neighbours_box(half_space(), ORIGIN, b6, idl, 3, junctions=True)
+    assert list(ids) == [3]
+    assert list(junc) == [0, 0, 0, 1]  # the one-id list {3} falls to the reference's default branch
