@@ -256,6 +256,9 @@ __device__ inline void load_circ_row(float* crr, const TablesDev& tb, int n, int
 __device__ inline void circ4(const float* crr, const float* src, int stride, int n, int o0, float acc[kRB]) {
   float w0 = crr[o0 + n], w1 = crr[o0 + 1 + n], w2 = crr[o0 + 2 + n], w3 = crr[o0 + 3 + n];
   acc[0] = acc[1] = acc[2] = acc[3] = 0.f;
+  // unrolled by 4 so that the weight window rotates through register names
+  // instead of three moves per tap
+#pragma unroll 4
   for (int m = 0; m < n; ++m) {
     const float v = src[m * stride];
     acc[0] = __builtin_fmaf(w0, v, acc[0]);
