@@ -57,7 +57,7 @@ static_assert(kTopoCap <= 64, "a junction list is a 64-bit mask over the grain t
 constexpr int kJuncLdsBits = 8, kJuncLds = 1 << kJuncLdsBits;  // per-tile junction table
 constexpr int kJuncCapBits = 9;
 static_assert((1 << kJuncCapBits) == kJuncCap, "grain junction table size");
-constexpr int kCPT = 8;              // cells classified per lane and pass
+constexpr int kCPT = 4;              // cells classified per lane and pass (a 4 KiB queue: six workgroups per CU)
 constexpr int kChunk = kCPT * kHT;  // cells classified per pass
 
 // Offset of corner q of the cell (xl, yl) in the two staged layers L[0..2*NMAX^2).
@@ -109,8 +109,8 @@ __device__ inline void edge_vertex(int e, const T* L, int xl, int yl, int n, int
 // left to a few lanes of each wavefront nor held up by cells with many
 // triangles.  The triangle table sits in LDS; corner and edge tables are bit
 // masks, so no lookup goes to memory with a lane-dependent address.
-// FP32: six waves per SIMD asked of the compiler (n <= 32 reaches it with a
-// small spill; n <= 48 stays LDS-bound at five).
+// FP32: six waves per SIMD asked of the compiler, which the LDS of both
+// classes then allows.
 //
 // TOPO: each cut cell also names its grains (the grain itself at inside
 // corners, the IDLocal id at the others), a per-tile LDS table keeps the first
