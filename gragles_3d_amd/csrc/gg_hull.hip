@@ -129,6 +129,7 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
   __shared__ int s_any[kHT / 64];
   __shared__ int s_tk[kTopoLds], s_to[kTopoLds];  // per-tile topology table (id, first appearance)
   __shared__ unsigned long long s_jm[kJuncLds];    // per-tile junction lists (slot masks)
+  __shared__ int s_gk[kTopoCap];                   // copy of the grain's id table
   const int total = P.totals[kQLayers * kClasses + cls];
   const T* arena = (const T*)din.arena;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -264,6 +265,11 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
             s_to[i] = INT_MAX;
           }
           __syncthreads();
+          // the grain's id table as it stands now (this chunk's ids are in it), read
+          // past the L1 once, for the slot lookups below
+          for (int i = tid; i < kTopoCap; i += kHT)
+            s_gk[i] = __hip_atomic_load(&gk[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __syncthreads();
           // 2. junctions: a cell's list of distinct ids as a mask over the slots
           //    its ids hold in the grain's table (a slot never changes once
           //    taken, so equal lists give equal masks); distinct masks per tile in
@@ -280,7 +286,7 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
               if (!first[qq] || ids[qq] == 0xFFFFFFFFu) continue;
               uint32_t hsl = (ids[qq] * 2654435761u) >> (32 - kTopoCapBits);
               for (int pr = 0; pr < kTopoCap; ++pr) {
-                const int k = __hip_atomic_load(&gk[hsl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const int k = s_gk[hsl];
                 if (k == (int)ids[qq]) {
                   mask |= 1ull << hsl;
                   break;
