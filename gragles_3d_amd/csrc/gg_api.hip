@@ -585,6 +585,16 @@ int gg_set_neighbour_lists(gg_ctx* c, const int32_t* offsets, const int32_t* ids
   return GG_OK;
 }
 
+int gg_build_neighbour_lists(gg_ctx* c) {
+  if (!c) return GG_ERR_ARG;
+  if (!c->boxes_set) return fail(c, GG_ERR_STATE, "gg_set_boxes has not been called");
+  hipSetDevice(c->p.device);
+  launch_init_lists(launch_of(c), c->sp, c->d_state, din(c), c->LD, c->d_err);
+  HIPCHK(c, hipGetLastError());
+  c->lists_set = true;
+  return gg_synchronize(c);
+}
+
 int gg_switch_buffers(gg_ctx* c) {
   if (!c) return GG_ERR_ARG;
   c->sel ^= 1;

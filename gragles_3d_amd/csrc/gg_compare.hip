@@ -174,6 +174,50 @@ __global__ __launch_bounds__(kLT) void k_lists(SimParams sp, const int32_t* __re
 
 }  // namespace
 
+// Initial m_secondOrderNeighbours from the boxes alone (grainhdl::find_neighbors,
+// src/grainhdl.cpp:1029-1054, with LSbox::computeDirectNeighbours,
+// src/box.cpp:2871-2887): every other alive grain whose cubed box overlaps the
+// grain's with inclusive bounds, in ascending id (the reference's R-tree
+// traversal order is not reproduced; the host routine gg_initial_neighbour_lists
+// uses the same order).  One wavefront per grain, all grains tested 64 at a
+// time: O(N^2) tests, a one-off at set-up.
+__global__ __launch_bounds__(64) void k_init_lists(SimParams sp, const int32_t* __restrict__ state, DescDev d,
+                                                   ListDev ld, int32_t* err) {
+  const int N = sp.N, maxL = ld.max_list, lane = threadIdx.x;
+  for (int g = 1 + blockIdx.x; g <= N; g += gridDim.x) {
+    int out = 0;
+    if (state[g] == kAlive) {
+      const int32_t* a = d.bounds + 6 * g;
+      const int a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3], a4 = a[4], a5 = a[5];
+      int32_t* dst = ld.buf[0] + (int64_t)g * maxL;
+      for (int base = 1; base <= N; base += 64) {
+        const int h = base + lane;
+        bool p = false;
+        if (h <= N && h != g && state[h] == kAlive) {
+          const int32_t* b = d.bounds + 6 * h;
+          p = !(a0 > b[3] || a3 < b[0] || a1 > b[4] || a4 < b[1] || a2 > b[5] || a5 < b[2]);
+        }
+        const uint64_t m = __ballot(p);
+        const int pos = out + __popcll(m & ((1ull << lane) - 1ull));
+        if (p && pos < maxL) dst[pos] = h;
+        out += __popcll(m);
+      }
+    }
+    if (lane == 0) {
+      ld.len[0][g] = out < maxL ? out : maxL;
+      ld.sel[g] = 0;
+      if (out > maxL) atomicOr(err, kErrList);
+    }
+  }
+}
+
+void launch_init_lists(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& d,
+                       const ListDev& ld, int32_t* err) {
+  int grid = sp.N < 65536 ? sp.N : 65536;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_init_lists, dim3(grid), dim3(64), 0, L.stream, sp, state, d, ld, err);
+}
+
 void launch_lists(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& din,
                   const DescDev& dout, const ListDev& ld, int32_t* newb, int32_t* err) {
   size_t lds = (size_t)(ld.max_list + 1) * sizeof(int32_t);

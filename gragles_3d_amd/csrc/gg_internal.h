@@ -266,6 +266,8 @@ void launch_conv(const Launch& L, int part, const SimParams& sp, const PlanDev& 
                  const DescDev& dout, const TablesDev& tb, const int64_t* spec_off, void* spec, bool fp32);
 void launch_idl_resize(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& dout,
                        const IdlDev& il);
+void launch_init_lists(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& d,
+                       const ListDev& ld, int32_t* err);
 void launch_lists(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& din,
                   const DescDev& dout, const ListDev& ld, int32_t* newb, int32_t* err);
 void launch_comparison_table(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& din,

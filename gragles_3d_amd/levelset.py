@@ -78,6 +78,7 @@ SIGNATURES = {
     "gg_download_list": (ctypes.c_int, [_vp, _i32, _i32, _i32p, _i32, _i32p]),
     "gg_get_all_bounds": (ctypes.c_int, [_vp, _i32, _i32p]),
     "gg_enable_hull_topology": (ctypes.c_int, [_vp, _i32]),
+    "gg_build_neighbour_lists": (ctypes.c_int, [_vp]),
     "gg_get_hull_neighbour_counts": (ctypes.c_int, [_vp, _i32p, _u8p]),
     "gg_download_hull_neighbours": (ctypes.c_int, [_vp, _i32, _i32p, _i32, _i32p]),
     "gg_get_hull_junction_counts": (ctypes.c_int, [_vp, _i32p]),
@@ -226,6 +227,11 @@ class LevelSetContext:
         if i.size == 0:
             i = np.zeros(1, dtype=np.int32)
         self._chk(lib().gg_set_neighbour_lists(self.h, _p(o, ctypes.c_int32), _p(i, ctypes.c_int32)))
+
+    def build_neighbour_lists(self):
+        """Initial neighbour lists from the boxes, built on the device (same lists as
+        initial_neighbour_lists on the host)."""
+        self._chk(lib().gg_build_neighbour_lists(self.h))
 
     # stages
     def switch(self):

@@ -95,6 +95,16 @@ int gg_init_distance(gg_ctx* ctx, const int32_t* idfield);
  * src/grainhdl.cpp:1029-1054) as CSR: offsets[0..n_grains+1], ids. */
 int gg_set_neighbour_lists(gg_ctx* ctx, const int32_t* offsets, const int32_t* ids);
 
+/* The initial m_secondOrderNeighbours built on the device from the boxes given
+ * to gg_set_boxes (grainhdl::find_neighbors, src/grainhdl.cpp:1029-1054, and
+ * LSbox::computeDirectNeighbours, src/box.cpp:2871-2887): for every alive grain
+ * the other alive grains whose cubed boxes overlap it with inclusive bounds,
+ * in ascending id (the R-tree's traversal order is not reproduced) -- the same
+ * lists gg_initial_neighbour_lists gives on the host.  Replaces
+ * gg_set_neighbour_lists at set-up.  Tests all pairs (a one-off; about 10 ms at
+ * 50 000 grains). */
+int gg_build_neighbour_lists(gg_ctx* ctx);
+
 /* Stage entry points (see the table above). */
 int gg_switch_buffers(gg_ctx* ctx);
 int gg_convolution(gg_ctx* ctx);

@@ -118,3 +118,18 @@ def test_list_capacity_overflow_is_reported():
     with pytest.raises(G.GGError, match="comparison list capacity exceeded"):
         ctx.synchronize()
     ctx.close()
+
+
+@pytest.mark.parametrize("n_grains,ppg,border,seed", [(64, 10, 3, 5), (512, 16, 7, 1)])
+def test_device_initial_lists_match_host(n_grains, ppg, border, seed):
+    _say("device initial lists n=%d ppg=%d border=%d seed=%d" % (n_grains, ppg, border, seed))
+    p = G.derive_params(n_grains, ppg, border)
+    idf, bounds, exists = G.generate_voronoi(p, seed)
+    offs, ids = G.initial_neighbour_lists(p, bounds, exists)
+    ctx = G.LevelSetContext(p, precision=G.FP64_PLAIN)
+    ctx.set_boxes(bounds, exists)
+    ctx.build_neighbour_lists()
+    d_offs, d_ids = ctx.all_neighbour_lists(0)
+    np.testing.assert_array_equal(d_offs, offs)
+    np.testing.assert_array_equal(d_ids, ids)
+    ctx.close()
