@@ -275,21 +275,63 @@ void stage_lists(gg_ctx* c) {
 
 void stage_comparison(gg_ctx* c) {
   static const int seg[kClasses] = {kSegCompare, kSegCompare48, kSegCompare64, kSegCompare96};
-  for (int cls = 0; cls < kClasses; ++cls) {
-    SegTimer t(c, seg[cls]);
-    // the first class's segment includes building the neighbour table
-    if (cls == 0) launch_comparison_table(launch_of(c), c->sp, c->d_state, din(c), c->LD);
-    launch_comparison(launch_of(c), cls, c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur],
-                      c->IL[c->il_cur ^ 1], c->d_newb, c->fp32);
+  if (c->timing) {
+    for (int cls = 0; cls < kClasses; ++cls) {
+      SegTimer t(c, seg[cls]);
+      // the first class's segment includes building the neighbour table
+      if (cls == 0) launch_comparison_table(launch_of(c), c->sp, c->d_state, din(c), c->LD);
+      launch_comparison(launch_of(c), cls, c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur],
+                        c->IL[c->il_cur ^ 1], c->d_newb, c->fp32);
+    }
+  } else {
+    // The size classes touch disjoint grains (their bounding-box atomics and
+    // IDLocal rows included), so the n <= 32 class runs on the side stream
+    // beside the others; each launch's tail then overlaps the other's work.
+    launch_comparison_table(launch_of(c), c->sp, c->d_state, din(c), c->LD);
+    hipEventRecord(c->ev_fork, c->stream);
+    hipStreamWaitEvent(c->side, c->ev_fork, 0);
+    Launch S = launch_of(c);
+    S.stream = c->side;
+    launch_comparison(S, 0, c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur], c->IL[c->il_cur ^ 1], c->d_newb,
+                      c->fp32);
+    hipEventRecord(c->ev_join, c->side);
+    for (int cls = 1; cls < kClasses; ++cls)
+      launch_comparison(launch_of(c), cls, c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur],
+                        c->IL[c->il_cur ^ 1], c->d_newb, c->fp32);
+    hipStreamWaitEvent(c->stream, c->ev_join, 0);
   }
   c->il_cur ^= 1;
+}
+
+// Runs launch(L, c0, c1) over the size classes: outside stage timing the
+// n <= 32 class goes on the side stream beside the others (classes touch
+// disjoint grains), forked from and joined back to the main stream.
+template <class F>
+void over_classes(gg_ctx* c, F launch) {
+  Launch L = launch_of(c);
+  if (c->timing) {
+    launch(L, 0, kClasses);
+    return;
+  }
+  hipEventRecord(c->ev_fork, c->stream);
+  hipStreamWaitEvent(c->side, c->ev_fork, 0);
+  Launch S = L;
+  S.stream = c->side;
+  launch(S, 0, 1);
+  hipEventRecord(c->ev_join, c->side);
+  launch(L, 1, kClasses);
+  hipStreamWaitEvent(c->stream, c->ev_join, 0);
 }
 
 void stage_level_set(gg_ctx* c) {
   Launch L = launch_of(c);
   {
     SegTimer t(c, kSegHull);
-    launch_hull(L, c->sp, c->P, din(c), c->d_exflag, c->d_partials, c->IL[c->il_cur], c->topo, c->d_err, c->fp32);
+    if (c->topo.on) launch_topo_clear(L, c->sp, c->topo);
+    over_classes(c, [&](const Launch& LL, int c0, int c1) {
+      launch_hull(LL, c->sp, c->P, din(c), c->d_exflag, c->d_partials, c->IL[c->il_cur], c->topo, c->d_err, c->fp32,
+                  c0, c1);
+    });
   }
   {
     SegTimer t(c, kSegFinalize);
@@ -301,8 +343,12 @@ void stage_level_set(gg_ctx* c) {
 
 void stage_redistancing(gg_ctx* c) {
   Launch L = launch_of(c);
-  { SegTimer t(c, kSegRedistXY); launch_redist(L, 0, c->sp, c->P, din(c), dout(c), c->d_err, c->fp32); }
-  { SegTimer t(c, kSegRedistZ); launch_redist(L, 1, c->sp, c->P, din(c), dout(c), c->d_err, c->fp32); }
+  for (int part = 0; part < 2; ++part) {
+    SegTimer t(c, part == 0 ? kSegRedistXY : kSegRedistZ);
+    over_classes(c, [&](const Launch& LL, int c0, int c1) {
+      launch_redist(LL, part, c->sp, c->P, din(c), dout(c), c->d_err, c->fp32, c0, c1);
+    });
+  }
   { SegTimer t(c, kSegPost); launch_post_redist(L, c->sp, c->d_state, din(c), dout(c)); }
 }
 

@@ -492,15 +492,19 @@ void upload_mc_tables() {
 
 template <class T, bool TOPO>
 void hull_classes(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, int32_t* exflag,
-                  double* partials, const IdlDev& il, const TopoDev& topo, int32_t* err) {
-  hipLaunchKernelGGL((k_hull<T, 32, TOPO>), dim3(tile_grid(L, kGridPerCU, 0)), dim3(kHT), 0, L.stream, sp, P, 0, din,
-                     exflag, partials, il, topo, err);
-  hipLaunchKernelGGL((k_hull<T, 48, TOPO>), dim3(tile_grid(L, kGridPerCU, 1)), dim3(kHT), 0, L.stream, sp, P, 1, din,
-                     exflag, partials, il, topo, err);
-  hipLaunchKernelGGL((k_hull<T, 64, TOPO>), dim3(tile_grid(L, kGridPerCU, 2)), dim3(kHT), 0, L.stream, sp, P, 2, din,
-                     exflag, partials, il, topo, err);
-  hipLaunchKernelGGL((k_hull<T, 96, TOPO>), dim3(tile_grid(L, kGridPerCU, 3)), dim3(kHT), 0, L.stream, sp, P, 3, din,
-                     exflag, partials, il, topo, err);
+                  double* partials, const IdlDev& il, const TopoDev& topo, int32_t* err, int c0, int c1) {
+  if (c0 <= 0 && 0 < c1)
+    hipLaunchKernelGGL((k_hull<T, 32, TOPO>), dim3(tile_grid(L, kGridPerCU, 0)), dim3(kHT), 0, L.stream, sp, P, 0,
+                       din, exflag, partials, il, topo, err);
+  if (c0 <= 1 && 1 < c1)
+    hipLaunchKernelGGL((k_hull<T, 48, TOPO>), dim3(tile_grid(L, kGridPerCU, 1)), dim3(kHT), 0, L.stream, sp, P, 1,
+                       din, exflag, partials, il, topo, err);
+  if (c0 <= 2 && 2 < c1)
+    hipLaunchKernelGGL((k_hull<T, 64, TOPO>), dim3(tile_grid(L, kGridPerCU, 2)), dim3(kHT), 0, L.stream, sp, P, 2,
+                       din, exflag, partials, il, topo, err);
+  if (c0 <= 3 && 3 < c1)
+    hipLaunchKernelGGL((k_hull<T, 96, TOPO>), dim3(tile_grid(L, kGridPerCU, 3)), dim3(kHT), 0, L.stream, sp, P, 3,
+                       din, exflag, partials, il, topo, err);
 }
 
 // Clears the grains' topology tables before a hull with topology on.
@@ -515,19 +519,20 @@ __global__ __launch_bounds__(256) void k_topo_clear(SimParams sp, TopoDev topo) 
   }
 }
 
+void launch_topo_clear(const Launch& L, const SimParams& sp, const TopoDev& topo) {
+  const int64_t n = (int64_t)(sp.N + 1) * kJuncCap;
+  const int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_topo_clear, dim3(blocks), dim3(256), 0, L.stream, sp, topo);
+}
+
 void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, int32_t* exflag,
-                 double* partials, const IdlDev& il, const TopoDev& topo, int32_t* err, bool fp32) {
-  if (topo.on) {
-    const int64_t n = (int64_t)(sp.N + 1) * kJuncCap;
-    const int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
-    hipLaunchKernelGGL(k_topo_clear, dim3(blocks), dim3(256), 0, L.stream, sp, topo);
-  }
+                 double* partials, const IdlDev& il, const TopoDev& topo, int32_t* err, bool fp32, int c0, int c1) {
   if (fp32) {
-    if (topo.on) hull_classes<float, true>(L, sp, P, din, exflag, partials, il, topo, err);
-    else hull_classes<float, false>(L, sp, P, din, exflag, partials, il, topo, err);
+    if (topo.on) hull_classes<float, true>(L, sp, P, din, exflag, partials, il, topo, err, c0, c1);
+    else hull_classes<float, false>(L, sp, P, din, exflag, partials, il, topo, err, c0, c1);
   } else {
-    if (topo.on) hull_classes<double, true>(L, sp, P, din, exflag, partials, il, topo, err);
-    else hull_classes<double, false>(L, sp, P, din, exflag, partials, il, topo, err);
+    if (topo.on) hull_classes<double, true>(L, sp, P, din, exflag, partials, il, topo, err, c0, c1);
+    else hull_classes<double, false>(L, sp, P, din, exflag, partials, il, topo, err, c0, c1);
   }
 }
 

@@ -275,14 +275,18 @@ void launch_comparison_table(const Launch& L, const SimParams& sp, const int32_t
 void launch_comparison(const Launch& L, int cls, const SimParams& sp, const PlanDev& P, const DescDev& din,
                        const DescDev& dout, const ListDev& ld, const IdlDev& il_old, const IdlDev& il_new,
                        int32_t* newb, bool fp32);
+// Clears the grains' topology tables; before every hull with topology on.
+void launch_topo_clear(const Launch& L, const SimParams& sp, const TopoDev& topo);
 void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, int32_t* exflag,
-                 double* partials, const IdlDev& il, const TopoDev& topo, int32_t* err, bool fp32);
+                 double* partials, const IdlDev& il, const TopoDev& topo, int32_t* err, bool fp32, int c0 = 0,
+                 int c1 = kClasses);
 void launch_finalize(const Launch& L, const SimParams& sp, const PlanDev& P, int32_t* state,
                      const ListDev& ld, int32_t* exflag, const double* partials, const int32_t* newb,
                      const DescDev& din, const DescDev& dout, double* volume, double* area);
 // part 0: x/y sweeps; part 1: z sweeps and clamp
 void launch_redist(const Launch& L, int part, const SimParams& sp, const PlanDev& P, const DescDev& din,
-                   const DescDev& dout, int32_t* err, bool fp32);
+                   const DescDev& dout, int32_t* err, bool fp32, int c0 = 0,
+                   int c1 = kClasses);
 void launch_post_redist(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& din,
                         const DescDev& dout);
 void launch_count(const Launch& L, const SimParams& sp, const int32_t* state, int32_t* step,

@@ -500,34 +500,50 @@ void xy_class(const Launch& L, const SimParams& sp, const PlanDev& P, int cls, c
 // part 0: x/y sweeps per layer (one launch per size class); part 1: z sweeps + clamp
 template <class T>
 void redist_all(const Launch& L, int part, const SimParams& sp, const PlanDev& P, const DescDev& din,
-                const DescDev& dout, int32_t* err) {
+                const DescDev& dout, int32_t* err, int c0, int c1) {
   // register-resident lines for boxes up to 32 (both types) and up to 48 (FP32)
   constexpr bool reg48 = sizeof(T) == 4;
+  auto in = [&](int c) { return c0 <= c && c < c1; };
   if (part == 0) {
-    hipLaunchKernelGGL((k_redist_xy_reg<T, 32>), dim3(tile_grid(L, kGridPerCU, 0)), dim3(64), 0, L.stream, sp, P, 0, din, dout, err);
-    if constexpr (reg48)
-      hipLaunchKernelGGL((k_redist_xy_reg<T, 48>), dim3(tile_grid(L, kGridPerCU, 1)), dim3(64), 0, L.stream, sp, P, 1, din, dout, err);
-    else
-      xy_class<T, 48>(L, sp, P, 1, din, dout, err);
-    xy_class<T, 64>(L, sp, P, 2, din, dout, err);
-    xy_class<T, 96>(L, sp, P, 3, din, dout, err);
+    if (in(0))
+      hipLaunchKernelGGL((k_redist_xy_reg<T, 32>), dim3(tile_grid(L, kGridPerCU, 0)), dim3(64), 0, L.stream, sp, P, 0,
+                         din, dout, err);
+    if (in(1)) {
+      if constexpr (reg48)
+        hipLaunchKernelGGL((k_redist_xy_reg<T, 48>), dim3(tile_grid(L, kGridPerCU, 1)), dim3(64), 0, L.stream, sp, P,
+                           1, din, dout, err);
+      else
+        xy_class<T, 48>(L, sp, P, 1, din, dout, err);
+    }
+    if (in(2)) xy_class<T, 64>(L, sp, P, 2, din, dout, err);
+    if (in(3)) xy_class<T, 96>(L, sp, P, 3, din, dout, err);
   } else {
-    hipLaunchKernelGGL((k_redist_z_reg<T, 32>), dim3(tile_grid(L, kGridPerCU, 0)), dim3(64), 0, L.stream, sp, P, 0, din, dout);
-    if constexpr (reg48)
-      hipLaunchKernelGGL((k_redist_z_reg<T, 48>), dim3(tile_grid(L, kGridPerCU, 1)), dim3(64), 0, L.stream, sp, P, 1, din, dout);
-    else
-      hipLaunchKernelGGL((k_redist_z<T, 48>), dim3(tile_grid(L, kGridPerCU, 1)), dim3(64), 0, L.stream, sp, P, 1, din, dout);
-    hipLaunchKernelGGL((k_redist_z<T, 64>), dim3(tile_grid(L, kGridPerCU, 2)), dim3(64), 0, L.stream, sp, P, 2, din, dout);
-    hipLaunchKernelGGL((k_redist_z<T, 96>), dim3(tile_grid(L, kGridPerCU, 3)), dim3(64), 0, L.stream, sp, P, 3, din, dout);
+    if (in(0))
+      hipLaunchKernelGGL((k_redist_z_reg<T, 32>), dim3(tile_grid(L, kGridPerCU, 0)), dim3(64), 0, L.stream, sp, P, 0,
+                         din, dout);
+    if (in(1)) {
+      if constexpr (reg48)
+        hipLaunchKernelGGL((k_redist_z_reg<T, 48>), dim3(tile_grid(L, kGridPerCU, 1)), dim3(64), 0, L.stream, sp, P, 1,
+                           din, dout);
+      else
+        hipLaunchKernelGGL((k_redist_z<T, 48>), dim3(tile_grid(L, kGridPerCU, 1)), dim3(64), 0, L.stream, sp, P, 1,
+                           din, dout);
+    }
+    if (in(2))
+      hipLaunchKernelGGL((k_redist_z<T, 64>), dim3(tile_grid(L, kGridPerCU, 2)), dim3(64), 0, L.stream, sp, P, 2, din,
+                         dout);
+    if (in(3))
+      hipLaunchKernelGGL((k_redist_z<T, 96>), dim3(tile_grid(L, kGridPerCU, 3)), dim3(64), 0, L.stream, sp, P, 3, din,
+                         dout);
   }
 }
 
 }  // namespace
 
 void launch_redist(const Launch& L, int part, const SimParams& sp, const PlanDev& P, const DescDev& din,
-                   const DescDev& dout, int32_t* err, bool fp32) {
-  if (fp32) redist_all<float>(L, part, sp, P, din, dout, err);
-  else redist_all<double>(L, part, sp, P, din, dout, err);
+                   const DescDev& dout, int32_t* err, bool fp32, int c0, int c1) {
+  if (fp32) redist_all<float>(L, part, sp, P, din, dout, err, c0, c1);
+  else redist_all<double>(L, part, sp, P, din, dout, err, c0, c1);
 }
 
 }  // namespace gg
