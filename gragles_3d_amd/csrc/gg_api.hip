@@ -25,6 +25,7 @@ struct gg_ctx {
   // per grain
   int32_t* d_state = nullptr;
   int32_t* d_newb = nullptr;
+  int32_t* d_cnb = nullptr;  // resizeGrid's bounds of the last grid coarsening (allocated on first use)
   TopoDev topo{nullptr, nullptr, nullptr, 0};  // hull topology tables (allocated when first enabled)
   int32_t* d_exflag = nullptr;
   double* d_volume = nullptr;
@@ -671,6 +672,63 @@ int gg_comparison(gg_ctx* c) {
   if (rc) return rc;
   hipSetDevice(c->p.device);
   stage_comparison(c);
+  return GG_OK;
+}
+
+// gridCoarsement's coarsening branch (src/grainhdl.cpp:1446-1492).  Enqueued
+// on the context stream like the stages; ends with the switch that the next
+// timestep's opening switch undoes.
+int gg_coarsen_grid(gg_ctx* c, int32_t new_size, double new_dt) {
+  if (!c) return GG_ERR_ARG;
+  int rc = require_boxes(c);
+  if (rc) return rc;
+  if (!c->lists_set) return fail(c, GG_ERR_STATE, "comparison lists have not been set");
+  const int gb = c->p.grid_blowup;
+  if (new_size < 1 || new_size + 2 * gb > c->p.ngridpoints || !(new_dt > 0))
+    return fail(c, GG_ERR_ARG, "coarsening needs 1 <= new_size <= ngridpoints - 2*grid_blowup and dt > 0");
+  hipSetDevice(c->p.device);
+  if (!c->d_cnb && (rc = dalloc(c, &c->d_cnb, (size_t)6 * (c->p.n_grains + 1)))) return rc;
+  // updateGridAndTimeVariables (src/grainhdl.cpp:1416-1444, Gaussian mode: dt from the caller)
+  const double h_old = c->p.h, h_new = 1.0 / new_size;
+  c->p.ngridpoints = new_size + 2 * gb;
+  c->p.conv_domain_size = new_size;
+  c->p.h = h_new;
+  c->p.delta = gb / double(new_size);
+  c->p.dt = new_dt;
+  c->sp.ngp = c->p.ngridpoints;
+  c->sp.h = c->p.h;
+  c->sp.delta = c->p.delta;
+  // the convolution tables depend on the grid and dt; a captured step graph
+  // holds the old grid in its kernel arguments
+  if ((rc = build_tables(c))) return rc;
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  const CoarsenArgs a{gb, h_old / h_new, h_new / h_old, c->p.delta};
+  Launch L = launch_of(c);
+  // resizeGrid: the output buffer (the last redistancing's result) is
+  // resampled into the input buffer at new bounds and a new layout; then the
+  // output buffer takes the input buffer's bounds
+  launch_coarsen_bounds(L, c->sp, a, c->d_state, dout(c).bounds, din(c).bounds, c->d_cnb);
+  plan_layout(c, din(c));
+  launch_coarsen_resample(L, c->sp, a, c->d_state, dout(c), din(c), c->fp32);
+  launch_post_redist(L, c->sp, c->d_state, dout(c), din(c));
+  // recalculateIDLocal (src/box.cpp:2699-2702): IDLocal to the output bounds,
+  // then the comparison over the lists of the last comparison
+  launch_idl_resize(L, c->sp, c->d_state, dout(c), c->IL[c->il_cur ^ 1]);
+  launch_comparison_table(L, c->sp, c->d_state, din(c), c->LD, true);
+  for (int cls = 0; cls < kClasses; ++cls)
+    launch_comparison(L, cls, c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur], c->IL[c->il_cur ^ 1],
+                      c->d_newb, c->fp32);
+  c->il_cur ^= 1;
+  // extractContour on the resampled input buffer
+  if (c->topo.on) launch_topo_clear(L, c->sp, c->topo);
+  launch_hull(L, c->sp, c->P, din(c), c->d_exflag, c->d_partials, c->IL[c->il_cur], c->topo, c->d_err, c->fp32);
+  launch_finalize(L, c->sp, c->P, c->d_state, c->LD, c->d_exflag, c->d_partials, c->d_cnb, din(c), dout(c),
+                  c->d_volume, c->d_area, true);
+  plan_layout(c, dout(c));
+  c->sel ^= 1;
   return GG_OK;
 }
 

@@ -486,7 +486,7 @@ __global__ __launch_bounds__(kCT) __attribute__((amdgpu_waves_per_eu(kWavesPerEU
 // then read one table entry per neighbour instead of the list, the
 // neighbour's bounds and its arena offset one after the other.
 __global__ __launch_bounds__(64) void k_cmp_table(SimParams sp, const int32_t* __restrict__ state, DescDev din,
-                                                  ListDev ld) {
+                                                  ListDev ld, int coarsen) {
   const int lane = threadIdx.x;
   const int maxL = ld.max_list;
   for (int g = 1 + blockIdx.x; g <= sp.N; g += gridDim.x) {
@@ -496,7 +496,7 @@ __global__ __launch_bounds__(64) void k_cmp_table(SimParams sp, const int32_t* _
     }
     const int32_t* b = din.bounds + 6 * g;
     const int x0 = b[0], y0 = b[1], z0 = b[2], x1 = b[3], y1 = b[4], z1 = b[5];
-    const int lsel = ld.sel[g] ^ 1;  // m_comparisonList
+    const int lsel = ld.sel[g] ^ (coarsen ? 0 : 1);  // m_comparisonList (coarsening: m_secondOrderNeighbours)
     const int32_t* lst = ld.buf[lsel] + (int64_t)g * maxL;
     const int len = ld.len[lsel][g];
     int4* tab = ld.tab + (int64_t)g * maxL;
@@ -507,7 +507,7 @@ __global__ __launch_bounds__(64) void k_cmp_table(SimParams sp, const int32_t* _
       bool keep = false;
       int nb = 0;
       int4 e = make_int4(0, 0, 0, 0);
-      if (idx < len) {
+      if (idx < len && !(coarsen && state[lst[idx]] != kAlive)) {
         nb = lst[idx];
         const int32_t* q = din.bounds + 6 * nb;
         const int q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5];
@@ -559,9 +559,9 @@ void comparison_class(const Launch& L, int cls, const SimParams& sp, const PlanD
 }  // namespace
 
 void launch_comparison_table(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& din,
-                             const ListDev& ld) {
+                             const ListDev& ld, bool coarsen) {
   const int grid = sp.N < 65536 ? (sp.N > 0 ? sp.N : 1) : 65536;  // one wavefront per grain
-  hipLaunchKernelGGL(k_cmp_table, dim3(grid), dim3(64), 0, L.stream, sp, state, din, ld);
+  hipLaunchKernelGGL(k_cmp_table, dim3(grid), dim3(64), 0, L.stream, sp, state, din, ld, coarsen ? 1 : 0);
 }
 
 // One size class per launch (boxes n <= 32, 48, 64, 96 for cls 0..3).

@@ -411,36 +411,25 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
   }
 }
 
-__device__ void cube_bounds(int* b, int maximumLength) {  // resizeToCube, src/dimensionalBuffer.h:139-219
-  const int grid_size = maximumLength - 1;
-  const int w = b[3] - b[0], hgt = b[4] - b[1], d = b[5] - b[2];
-  if (w == hgt && w == d) return;
-  int ref = w > hgt ? w : hgt;
-  if (ref < d) ref = d;
-  if (ref > w) { int diff = ref - w; b[0] -= diff / 2; b[3] += diff / 2 + diff % 2; }
-  if (ref > hgt) { int diff = ref - hgt; b[1] -= diff / 2; b[4] += diff / 2 + diff % 2; }
-  if (ref > d) { int diff = ref - d; b[2] -= diff / 2; b[5] += diff / 2 + diff % 2; }
-  for (int a = 0; a < 3; ++a) {
-    if (b[a] < 0) { int delta = -b[a]; b[a + 3] += delta; b[a] += delta; }
-    else if (b[a + 3] > grid_size) { int delta = grid_size - b[a + 3]; b[a + 3] += delta; b[a] += delta; }
-  }
-}
-
 // Per grain: vanished -> deleted; alive -> take the comparison list as the new
 // neighbour list, decide existence, finish volume/area, resize the output box
 // to the set-comparison bounds (+-grid_blowup, clamped, src/box.cpp:857-877)
 // made cubic.
+//
+// coarsen: extractContour as a grid coarsening calls it (src/grainhdl.cpp:1481-1490)
+// rather than level_set: no deletion, the neighbour lists stay, and newb holds
+// resizeGrid's bounds, taken as they are.
 __global__ void k_finalize(SimParams sp, PlanDev P, DescDev din, int32_t* state, ListDev ld, int32_t* exflag,
                            const double* __restrict__ partials, const int32_t* __restrict__ newb, DescDev dout,
-                           double* volume, double* area) {
+                           double* volume, double* area, int coarsen) {
   for (int g = 1 + blockIdx.x * blockDim.x + threadIdx.x; g <= sp.N; g += gridDim.x * blockDim.x) {
     const int s = state[g];
     if (s == kVanished) {
-      state[g] = kDeleted;
+      if (!coarsen) state[g] = kDeleted;
       continue;
     }
     if (s != kAlive) continue;
-    ld.sel[g] ^= 1;
+    if (!coarsen) ld.sel[g] ^= 1;
     const int ex = exflag[g];
     exflag[g] = 0;
     if (!ex) {
@@ -464,9 +453,13 @@ __global__ void k_finalize(SimParams sp, PlanDev P, DescDev din, int32_t* state,
     area[g] = (aa.s + aa.c) * h * h;
     const int32_t* nb = newb + 6 * g;
     int bb[6] = {nb[0] - sp.gb, nb[1] - sp.gb, nb[2] - sp.gb, nb[3] + sp.gb, nb[4] + sp.gb, nb[5] + sp.gb};
-    for (int a = 0; a < 3; ++a) {
-      if (bb[a] < 0) bb[a] = 0;
-      if (bb[a + 3] >= sp.ngp) bb[a + 3] = sp.ngp - 1;
+    if (coarsen) {
+      for (int a = 0; a < 6; ++a) bb[a] = nb[a];
+    } else {
+      for (int a = 0; a < 3; ++a) {
+        if (bb[a] < 0) bb[a] = 0;
+        if (bb[a + 3] >= sp.ngp) bb[a + 3] = sp.ngp - 1;
+      }
     }
     cube_bounds(bb, sp.ngp);
     for (int a = 0; a < 6; ++a) dout.bounds[6 * g + a] = bb[a];
@@ -538,10 +531,10 @@ void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const D
 
 void launch_finalize(const Launch& L, const SimParams& sp, const PlanDev& P, int32_t* state, const ListDev& ld,
                      int32_t* exflag, const double* partials, const int32_t* newb, const DescDev& din,
-                     const DescDev& dout, double* volume, double* area) {
+                     const DescDev& dout, double* volume, double* area, bool coarsen) {
   int blocks = (sp.N + 255) / 256;
   hipLaunchKernelGGL(k_finalize, dim3(blocks), dim3(256), 0, L.stream, sp, P, din, state, ld, exflag, partials,
-                     newb, dout, volume, area);
+                     newb, dout, volume, area, coarsen ? 1 : 0);
 }
 
 }  // namespace gg

@@ -138,6 +138,21 @@ struct TablesDev {
 // ---- helpers --------------------------------------------------------------
 __device__ inline int box_n(const int32_t* b, int g) { return b[6 * g + 3] - b[6 * g]; }
 
+__device__ inline void cube_bounds(int* b, int maximumLength) {  // resizeToCube, src/dimensionalBuffer.h:139-219
+  const int grid_size = maximumLength - 1;
+  const int w = b[3] - b[0], hgt = b[4] - b[1], d = b[5] - b[2];
+  if (w == hgt && w == d) return;
+  int ref = w > hgt ? w : hgt;
+  if (ref < d) ref = d;
+  if (ref > w) { int diff = ref - w; b[0] -= diff / 2; b[3] += diff / 2 + diff % 2; }
+  if (ref > hgt) { int diff = ref - hgt; b[1] -= diff / 2; b[4] += diff / 2 + diff % 2; }
+  if (ref > d) { int diff = ref - d; b[2] -= diff / 2; b[5] += diff / 2 + diff % 2; }
+  for (int a = 0; a < 3; ++a) {
+    if (b[a] < 0) { int delta = -b[a]; b[a + 3] += delta; b[a] += delta; }
+    else if (b[a + 3] > grid_size) { int delta = grid_size - b[a + 3]; b[a + 3] += delta; b[a] += delta; }
+  }
+}
+
 // Locate tile t of class c for quantity q: grain id and local tile index.
 __device__ inline void plan_tile(const PlanDev& P, int c, int q, int t, int& g, int& local) {
   const uint32_t v = P.tmap[q][P.tile_base[q * kClasses + c] + t];
@@ -270,8 +285,11 @@ void launch_init_lists(const Launch& L, const SimParams& sp, const int32_t* stat
                        const ListDev& ld, int32_t* err);
 void launch_lists(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& din,
                   const DescDev& dout, const ListDev& ld, int32_t* newb, int32_t* err);
+// coarsen: the table is built from m_secondOrderNeighbours (the lists of the
+// last comparison, as recalculateIDLocal's comparison reads them) and passes
+// over neighbours that are not alive.
 void launch_comparison_table(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& din,
-                             const ListDev& ld);
+                             const ListDev& ld, bool coarsen = false);
 void launch_comparison(const Launch& L, int cls, const SimParams& sp, const PlanDev& P, const DescDev& din,
                        const DescDev& dout, const ListDev& ld, const IdlDev& il_old, const IdlDev& il_new,
                        int32_t* newb, bool fp32);
@@ -282,7 +300,20 @@ void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const D
                  int c1 = kClasses);
 void launch_finalize(const Launch& L, const SimParams& sp, const PlanDev& P, int32_t* state,
                      const ListDev& ld, int32_t* exflag, const double* partials, const int32_t* newb,
-                     const DescDev& din, const DescDev& dout, double* volume, double* area);
+                     const DescDev& din, const DescDev& dout, double* volume, double* area,
+                     bool coarsen = false);
+// Grid coarsening (LSbox::resizeGrid): up = h_old/h_new, down = h_new/h_old,
+// delta = the new delta; sp holds the new grid.
+struct CoarsenArgs {
+  int32_t gb;
+  double up, down, delta;
+};
+// New bounds of every alive grain: src bounds scaled (-> nb), cubed (-> dst).
+void launch_coarsen_bounds(const Launch& L, const SimParams& sp, const CoarsenArgs& a, const int32_t* state,
+                           const int32_t* src_bounds, int32_t* dst_bounds, int32_t* nb);
+// Trilinear resampling of src (old grid) into dst (new bounds and layout).
+void launch_coarsen_resample(const Launch& L, const SimParams& sp, const CoarsenArgs& a, const int32_t* state,
+                             const DescDev& src, const DescDev& dst, bool fp32);
 // part 0: x/y sweeps; part 1: z sweeps and clamp
 void launch_redist(const Launch& L, int part, const SimParams& sp, const PlanDev& P, const DescDev& din,
                    const DescDev& dout, int32_t* err, bool fp32, int c0 = 0,

@@ -66,6 +66,7 @@ SIGNATURES = {
     "gg_update_neighbour_lists": (ctypes.c_int, [_vp]),
     "gg_comparison": (ctypes.c_int, [_vp]),
     "gg_level_set": (ctypes.c_int, [_vp]),
+    "gg_coarsen_grid": (ctypes.c_int, [_vp, _i32, ctypes.c_double]),
     "gg_redistancing": (ctypes.c_int, [_vp]),
     "gg_count_grains": (ctypes.c_int, [_vp, _i32p]),
     "gg_timestep": (ctypes.c_int, [_vp, _i32, _i32]),
@@ -124,6 +125,22 @@ def derive_params(n_grains, points_per_grain, border, time_step_size=0.2):
     if rc != GG_OK:
         raise GGError("gg_derive_params failed (%d)" % rc)
     return p
+
+
+def coarsening_plan(current_grains, initial_grains, loop, points_per_grain, real_domain_size, enabled=True,
+                    gradient=0.95, time_step_size=0.2):
+    """grainhdl::gridCoarsement's test (src/grainhdl.cpp:1447-1451): the new
+    grid size sized for points_per_grain points per grain diameter at the
+    current grain count, used when the count has fallen below gradient times
+    the count at the last coarsening (or the start), after the first step, when
+    it is smaller than the current grid.  Returns (new_size, new_dt) with the
+    Gaussian mode's dt (src/grainhdl.cpp:1430-1434), or None."""
+    import math
+    new_size = int(math.pow(current_grains, 1 / 3.0) * math.pow(math.pi * 4 / 3, 1. / 3) / 2 * points_per_grain)
+    if (float(current_grains) / float(initial_grains) < gradient and loop != 0 and enabled
+            and new_size < real_domain_size):
+        return new_size, time_step_size / float(new_size * new_size)
+    return None
 
 
 def resize_to_cube(bounds6, ngridpoints):
@@ -248,6 +265,14 @@ class LevelSetContext:
 
     def level_set(self):
         self._chk(lib().gg_level_set(self.h))
+
+    def coarsen_grid(self, new_size, new_dt):
+        """gridCoarsement's coarsening branch (src/grainhdl.cpp:1446-1492): the
+        new grid (ngridpoints = new_size + 2*grid_blowup, h = 1/new_size,
+        delta = grid_blowup/new_size, dt = new_dt), every grain resampled onto
+        it, IDLocal recalculated and the contour extracted.  A coarsening step
+        is coarsen_grid() followed by timestep()."""
+        self._chk(lib().gg_coarsen_grid(self.h, int(new_size), float(new_dt)))
 
     def redistancing(self):
         self._chk(lib().gg_redistancing(self.h))

@@ -113,6 +113,22 @@ int gg_comparison(gg_ctx* ctx);
 int gg_level_set(gg_ctx* ctx);
 int gg_redistancing(gg_ctx* ctx);
 
+/* The coarsening branch of grainhdl::gridCoarsement (src/grainhdl.cpp:1446-1492),
+ * which a coarsening step runs in place of the opening switch: the grid and
+ * time variables of the new size (updateGridAndTimeVariables,
+ * src/grainhdl.cpp:1416-1444: ngridpoints = new_size + 2*grid_blowup,
+ * h = 1/new_size, delta = grid_blowup/new_size, the convolution's domain size
+ * new_size; dt = new_dt, which the caller computes for its convolution mode),
+ * then per grain LSbox::resizeGrid (src/box.cpp:2595-2697, trilinear
+ * resampling of the last redistancing's result), recalculateIDLocal
+ * (src/box.cpp:2699-2702) and extractContour (src/box.cpp:1144-1161).  A
+ * coarsening step is gg_coarsen_grid followed by gg_timestep: the call ends
+ * with the switch that gg_timestep's opening switch undoes.  Grains that
+ * vanished at the level set before it stay vanished and the comparison passes
+ * over them (the reference resamples their pre-comparison field and runs their
+ * hull again).  1 <= new_size <= ngridpoints - 2*grid_blowup. */
+int gg_coarsen_grid(gg_ctx* ctx, int32_t new_size, double new_dt);
+
 /* countGrains: synchronises and returns the number of grains not deleted. */
 int gg_count_grains(gg_ctx* ctx, int32_t* count);
 

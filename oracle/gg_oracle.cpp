@@ -18,7 +18,8 @@
 //                          src/marchingCubes.cpp:364-499, src/grainHull.cpp:40-56, 120-155
 //   redistancing           src/box.cpp:2322-2586
 //   grain count            src/grainhdl.cpp:936-943
-//   stage order            src/grainhdl.cpp:836-919 (no coarsening, no output)
+//   stage order            src/grainhdl.cpp:836-919 (no output)
+//   grid coarsening        src/grainhdl.cpp:1416-1492, src/box.cpp:2595-2702
 //
 // The FFT libraries the reference calls (Intel MKL DFTI / FFTW3, versions not
 // pinned by the reference) are replaced by an explicit DFT whose every sum runs
@@ -638,7 +639,10 @@ void boundary_condition(const orc_sim& S, Grain& G) {  // src/box.cpp:997-1088
       }
 }
 
-void comparison(orc_sim& S, Grain& G) {  // executeComparison, src/box.cpp:891-966
+// alive_only: neighbours that are not alive are passed over.  Only the
+// comparison of a grid coarsening asks for it: its lists still name the grains
+// that vanished at the level set before it, whose fields are not resampled.
+void comparison(orc_sim& S, Grain& G, bool alive_only = false) {  // executeComparison, src/box.cpp:891-966
   if (G.state != kAlive) return;
   Buf<double>& out = *G.out;
   const Buf<double>& in = *G.in;
@@ -646,6 +650,7 @@ void comparison(orc_sim& S, Grain& G) {  // executeComparison, src/box.cpp:891-9
   G.second = G.cmp;
   for (unsigned nbid : G.second) {
     const Grain& nb = S.g[nbid];
+    if (alive_only && nb.state != kAlive) continue;
     const Buf<double>& nin = *nb.in;
     int xa = in.x0 < nin.x0 ? nin.x0 : in.x0;
     int xb = in.x1 > nin.x1 ? nin.x1 : in.x1;
@@ -723,6 +728,48 @@ void for_grains(orc_sim& S, F f) {
     }
   }
   if (!err.empty()) throw std::runtime_error(err);
+}
+
+// LSbox::resizeGrid (src/box.cpp:2595-2697) of one grain.  The new bounds are
+// the old output buffer's scaled by h_old/h_new about the domain border
+// (truncated to int, as the reference's int members take them); the input
+// buffer takes them, cubed on the new grid, and each of its voxels is the
+// trilinear interpolation of the old output buffer at the voxel's position on
+// the old grid, or -delta (the new delta) where the interpolation cell is not
+// inside the old box less two layers at its upper faces.  nb receives the
+// scaled bounds (m_new*), xmin,ymin,zmin,xmax,ymax,zmax.
+void resize_grid(const Buf<double>& src, Buf<double>& dst, int* nb, int gb, int ngp_new, double h_old, double h_new,
+                 double delta_new) {
+  const double up = h_old / h_new, down = h_new / h_old;
+  nb[0] = (int)((src.x0 - gb) * up + 0.5 + gb);
+  nb[3] = (int)((src.x1 - gb) * up + 0.5 + gb);
+  nb[1] = (int)((src.y0 - gb) * up + 0.5 + gb);
+  nb[4] = (int)((src.y1 - gb) * up + 0.5 + gb);
+  nb[2] = (int)((src.z0 - gb) * up + 0.5 + gb);
+  nb[5] = (int)((src.z1 - gb) * up + 0.5 + gb);
+  dst.resize(nb[0], nb[1], nb[2], nb[3], nb[4], nb[5]);
+  resize_to_cube(dst, ngp_new);
+  auto V = [&](double y, double x, double z) { return src.at((int)(y + gb), (int)(x + gb), (int)(z + gb)); };
+  for (int k = dst.z0; k < dst.z1; ++k)
+    for (int i = dst.y0; i < dst.y1; ++i)
+      for (int j = dst.x0; j < dst.x1; ++j) {
+        const double px = (j - gb) * down, py = (i - gb) * down, pz = (k - gb) * down;
+        const double xl = std::floor(px), xr = std::floor(px + 1);
+        const double yu = std::floor(py), yo = std::floor(py + 1);
+        const double zu = std::floor(pz), zo = std::floor(pz + 1);
+        if (xr + gb > src.x1 - 2 || yo + gb > src.y1 - 2 || zo + gb > src.z1 - 2 || yu + gb < src.y0 ||
+            xl + gb < src.x0 || zu + gb < src.z0) {
+          dst.at(i, j, k) = -delta_new;
+          continue;
+        }
+        double ro = (xr - px) * V(yo, xl, zu) + (px - xl) * V(yo, xr, zu);
+        double ru = (xr - px) * V(yu, xl, zu) + (px - xl) * V(yu, xr, zu);
+        const double d_zu = (yo - py) * ru + (py - yu) * ro;
+        ro = (xr - px) * V(yo, xl, zo) + (px - xl) * V(yo, xr, zo);
+        ru = (xr - px) * V(yu, xl, zo) + (px - xl) * V(yu, xr, zo);
+        const double d_zo = (yo - py) * ru + (py - yu) * ro;
+        dst.at(i, j, k) = (zo - pz) * d_zu + (pz - zu) * d_zo;
+      }
 }
 
 }  // namespace
@@ -897,6 +944,61 @@ int orc_redistancing(orc_sim* S) {  // redistancing, src/grainhdl.cpp:720-748
   })
 }
 
+// gridCoarsement's coarsening branch (src/grainhdl.cpp:1446-1492) with the
+// grid and time variables of the new size (updateGridAndTimeVariables,
+// src/grainhdl.cpp:1416-1444, Gaussian mode: the caller passes the new dt):
+// resizeGrid, recalculateIDLocal (IDLocal to the output bounds, then
+// executeComparison, src/box.cpp:2699-2702) and extractContour, per grain.
+// It ends with a switch of the two buffers, so that the switch that opens the
+// next orc_timestep undoes it: a coarsening step is orc_coarsen followed by
+// orc_timestep, as gridCoarsement runs in place of the switch.
+// Grains that vanished at the level set before it are left as they are (the
+// reference resamples their output buffer, the pre-comparison field of that
+// step, and runs their hull again); the comparison passes over them.
+int orc_coarsen(orc_sim* S, int new_size, double new_dt) {
+  ORC_TRY(S, {
+    if (new_size < 1) throw std::runtime_error("coarsening to a grid size below 1");
+    const double h_old = S->p.h;
+    const int gb = S->p.grid_blowup;
+    S->p.ngridpoints = new_size + 2 * gb;
+    S->p.conv_domain_size = new_size;
+    S->p.h = 1.0 / new_size;
+    S->p.delta = gb / double(new_size);
+    S->p.dt = new_dt;
+    S->tables.clear();
+    const int L = S->p.ngridpoints;
+    const double h = S->p.h;
+    for_grains(*S, [&](Grain& G) {
+      if (G.state != kAlive) return;
+      resize_grid(*G.out, *G.in, G.nb, gb, L, h_old, h, S->p.delta);
+      G.out->resize(G.in->x0, G.in->y0, G.in->z0, G.in->x1, G.in->y1, G.in->z1);
+    });
+    for_grains(*S, [&](Grain& G) {
+      if (G.state != kAlive) return;
+      G.idl.resize(G.out->x0, G.out->y0, G.out->z0, G.out->x1, G.out->y1, G.out->z1);
+      comparison(*S, G, true);
+    });
+    for_grains(*S, [&](Grain& G) {
+      if (G.state != kAlive) return;
+      double vr, ar;
+      bool ex = hull_reduce(*G.in, &vr, &ar);
+      hull_neighbours(*G.in, G.idl, G.id, G.hull_nb, G.hull_junc);
+      if (!ex) {
+        G.state = kVanished;
+        return;
+      }
+      G.out->resize(G.nb[0], G.nb[1], G.nb[2], G.nb[3], G.nb[4], G.nb[5]);
+      resize_to_cube(*G.out, L);
+      G.volume = std::abs(vr) * h * h * h;
+      G.area = ar * h * h;
+    });
+    for (int id = 1; id <= S->p.n_grains; ++id) {
+      Grain& G = S->g[id];
+      if (G.state != kDeleted) std::swap(G.in, G.out);
+    }
+  })
+}
+
 int orc_count(orc_sim* S) {  // countGrains, src/grainhdl.cpp:936-943
   int c = 0;
   for (int id = 1; id <= S->p.n_grains; ++id)
@@ -1010,6 +1112,24 @@ void orc_resize_to_cube(int32_t* b, int L) {
   int t[6] = {b[0], b[1], b[2], b[3], b[4], b[5]};
   cube_bounds(t, L);
   for (int a = 0; a < 6; ++a) b[a] = t[a];
+}
+
+// resizeGrid on one box: src_bounds/src hold the old output buffer; dst_bounds
+// receives the cubed new bounds, nb6 the scaled ones, dst (capacity cap) the
+// resampled values.  Returns the number of values, or -1 if cap is too small.
+long orc_resize_grid_box(const int32_t* src_bounds, const double* src, int gb, int ngp_new, double h_old,
+                         double h_new, double delta_new, int32_t* nb6, int32_t* dst_bounds, double* dst, long cap) {
+  Buf<double> s, d;
+  s.resize(src_bounds[0], src_bounds[1], src_bounds[2], src_bounds[3], src_bounds[4], src_bounds[5]);
+  std::copy(src, src + s.v.size(), s.v.begin());
+  int nb[6];
+  resize_grid(s, d, nb, gb, ngp_new, h_old, h_new, delta_new);
+  for (int a = 0; a < 6; ++a) nb6[a] = nb[a];
+  dst_bounds[0] = d.x0; dst_bounds[1] = d.y0; dst_bounds[2] = d.z0;
+  dst_bounds[3] = d.x1; dst_bounds[4] = d.y1; dst_bounds[5] = d.z1;
+  if ((long)d.v.size() > cap) return -1;
+  std::copy(d.v.begin(), d.v.end(), dst);
+  return (long)d.v.size();
 }
 
 // Convolve one n^3 box with the heat kernel of (dt, nn).

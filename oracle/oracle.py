@@ -62,6 +62,10 @@ def lib():
             "orc_redistancing": (ctypes.c_int, [vp]),
             "orc_count": (ctypes.c_int, [vp]),
             "orc_timestep": (ctypes.c_int, [vp, ctypes.c_int, i32p]),
+            "orc_coarsen": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double]),
+            "orc_resize_grid_box": (ctypes.c_long, [i32p, f64p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                    ctypes.c_double, ctypes.c_double, i32p, i32p, f64p,
+                                                    ctypes.c_long]),
             "orc_get_state": (ctypes.c_int, [vp, ctypes.c_int]),
             "orc_get_bounds": (ctypes.c_long, [vp, ctypes.c_int, ctypes.c_int, i32p]),
             "orc_get_phi": (ctypes.c_long, [vp, ctypes.c_int, ctypes.c_int, f64p, ctypes.c_long]),
@@ -159,6 +163,24 @@ def redistance_box(in_bounds, phi_in, out_bounds, h, delta):
     return out
 
 
+def resize_grid_box(src_bounds, phi, gb, ngp_new, h_old, h_new, delta_new):
+    """LSbox::resizeGrid on one box (src/box.cpp:2595-2697): returns (scaled
+    bounds m_new*, cubed bounds, resampled values [z][y][x])."""
+    sb = np.ascontiguousarray(src_bounds, dtype=np.int32)
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    nb = np.zeros(6, dtype=np.int32)
+    db = np.zeros(6, dtype=np.int32)
+    cap = int(max(ngp_new, 1)) ** 3
+    out = np.empty(cap, dtype=np.float64)
+    n = lib().orc_resize_grid_box(_p(sb, ctypes.c_int32), _p(phi, ctypes.c_double), int(gb), int(ngp_new),
+                                  float(h_old), float(h_new), float(delta_new), _p(nb, ctypes.c_int32),
+                                  _p(db, ctypes.c_int32), _p(out, ctypes.c_double), cap)
+    if n < 0:
+        raise RuntimeError("resized box larger than the new grid")
+    shape = (db[5] - db[2], db[4] - db[1], db[3] - db[0])
+    return nb, db, out[:n].reshape(shape)
+
+
 def comparison_box(ngp, gb, h, delta, own_bounds, phi_in, neighbours, idl_stale, tube=0):
     """Comparison + wall + set-comparison of one grain.
 
@@ -254,6 +276,11 @@ class OracleSim:
         c = np.zeros(max(nsteps, 1), dtype=np.int32)
         self._chk(lib().orc_timestep(self.h, int(nsteps), _p(c, ctypes.c_int32)))
         return c[:nsteps]
+
+    def coarsen(self, new_size, new_dt):
+        """gridCoarsement's coarsening branch; a coarsening step is coarsen()
+        followed by timestep()."""
+        self._chk(lib().orc_coarsen(self.h, int(new_size), float(new_dt)))
 
     def state(self, gid):
         return lib().orc_get_state(self.h, int(gid))
