@@ -96,6 +96,37 @@ def kernel_pass(ctx, steps):
     return seg
 
 
+def coarsening_leg(ctx, p, frac=0.9):
+    """One grid coarsening (gridCoarsement's coarsening branch) of the context's
+    state onto a grid frac times the size, then one timestep on it.  The
+    resampling kernel reads every alive grain's old box and writes its new box:
+    4 B per voxel each way (FP32) is its algorithmic traffic."""
+    st, _, _ = ctx.stats()
+    alive = st == G.ALIVE
+    old_b = ctx.all_bounds(1)[alive]
+    src_vox = int(((old_b[:, 3] - old_b[:, 0]).astype(np.int64) ** 3).sum())
+    rds = p.ngridpoints - 2 * p.grid_blowup
+    new = int(rds * frac)
+    ctx.enable_stage_timing(True)
+    ctx.stage_times(reset=True)
+    ctx.synchronize()
+    t = time.perf_counter()
+    ctx.coarsen_grid(new, 0.2 / float(new * new))  # Gaussian mode, TimeStepSize 0.2 (derive_params' default)
+    ctx.synchronize()
+    wall = time.perf_counter() - t
+    ms, nl = ctx.stage_times(reset=True)["resample"]
+    ctx.enable_stage_timing(False)
+    new_b = ctx.all_bounds(1)[alive]
+    dst_vox = int(((new_b[:, 3] - new_b[:, 0]).astype(np.int64) ** 3).sum())
+    nbytes = 4.0 * (src_vox + dst_vox)
+    ctx.timestep(1, use_graph=False)
+    ctx.synchronize()
+    st2, _, _ = ctx.stats()
+    return {"grid_from": rds, "grid_to": new, "call_ms": wall * 1e3, "resample_ms": ms,
+            "resample_bytes": nbytes, "resample_GBs": nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0,
+            "grains_alive_after_next_step": int((st2 == G.ALIVE).sum())}
+
+
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_fp32_8000grains.md")
 
 
@@ -208,6 +239,10 @@ def main():
 
     seg32 = kernel_pass(ctx32, args.kernel_steps)
     counts32 = ctx32.count_history()
+    coarsen = coarsening_leg(ctx32, p)
+    log("coarsening %d -> %d: call %.2f ms, resampling %.3f ms (%.0f GB/s)" % (
+        coarsen["grid_from"], coarsen["grid_to"], coarsen["call_ms"], coarsen["resample_ms"],
+        coarsen["resample_GBs"]))
     ctx32.close()
 
     # Roofline of the comparison kernel (neighbour max + wall + set comparison), one launch per
@@ -268,6 +303,7 @@ def main():
                                    "GBs": cmp_total_bytes / (cmp_total_ms * 1e-3) / 1e9 if cmp_total_ms else 0.0},
         "segment_ms_per_step_fp32": seg_ms_per_step,
         "grain_count_trajectory_fp32": counts32[-3:].tolist(),
+        "grid_coarsening_fp32": coarsen,
     }
 
     if fp64 is not None:

@@ -67,8 +67,8 @@ struct gg_ctx {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipEvent_t ev[12] = {};
-  double seg_ms[16] = {};
-  int64_t seg_n[16] = {};
+  double seg_ms[GG_NUM_SEGMENTS] = {};
+  int64_t seg_n[GG_NUM_SEGMENTS] = {};
   std::vector<std::pair<int, int>> pending;  // (stage, event pair index)
   std::vector<hipEvent_t> evpool;
   int evused = 0;
@@ -156,9 +156,12 @@ int build_tables(gg_ctx* c) {
       int k2 = std::min(k, n - k);
       f[k] = exp(-sqrt(2 / kPi) * (k2 * k2) * 4.0 * dt * nsq / local_nsq * kPi * kPi);
     }
+    // cos(2 pi t / n) takes n values; (k m) mod n picks one
+    std::vector<long double> ct(n);
+    for (int t = 0; t < n; ++t) ct[t] = cosl(2.0L * 3.141592653589793238462643383279L * t / n);
     for (int m = 0; m < n; ++m) {
       long double acc = 0;
-      for (int k = 0; k < n; ++k) acc += (long double)f[k] * cosl(2.0L * 3.141592653589793238462643383279L * ((k * m) % n) / n);
+      for (int k = 0; k < n; ++k) acc += (long double)f[k] * ct[(k * m) % n];
       circ.push_back((float)(acc / n));
     }
   }
@@ -216,8 +219,10 @@ hipEvent_t take_event(gg_ctx* c) {
 // Event-timed segments, one per kernel (or per kernel with one launch per size class).
 enum Seg {
   kSegConvA, kSegConvB, kSegConvC, kSegIdl, kSegLists, kSegCompare, kSegHull, kSegFinalize, kSegPlan,
-  kSegRedistXY, kSegRedistZ, kSegPost, kSegCount, kSegCompare48, kSegCompare64, kSegCompare96, kNumSeg
+  kSegRedistXY, kSegRedistZ, kSegPost, kSegCount, kSegCompare48, kSegCompare64, kSegCompare96, kSegResample,
+  kNumSeg
 };
+static_assert(kNumSeg == GG_NUM_SEGMENTS, "segment table of the C ABI");
 
 struct SegTimer {
   gg_ctx* c;
@@ -712,7 +717,10 @@ int gg_coarsen_grid(gg_ctx* c, int32_t new_size, double new_dt) {
   // output buffer takes the input buffer's bounds
   launch_coarsen_bounds(L, c->sp, a, c->d_state, dout(c).bounds, din(c).bounds, c->d_cnb);
   plan_layout(c, din(c));
-  launch_coarsen_resample(L, c->sp, a, c->d_state, dout(c), din(c), c->fp32);
+  {
+    SegTimer t(c, kSegResample);
+    launch_coarsen_resample(L, c->sp, a, c->d_state, dout(c), din(c), c->fp32);
+  }
   launch_post_redist(L, c->sp, c->d_state, dout(c), din(c));
   // recalculateIDLocal (src/box.cpp:2699-2702): IDLocal to the output bounds,
   // then the comparison over the lists of the last comparison

@@ -60,11 +60,12 @@ __global__ __launch_bounds__(kCT) void k_coarsen_resample(SimParams sp, CoarsenA
       const int yy = (int)(y + gb) - sy0, xx = (int)(x + gb) - sx0, zz = (int)(z + gb) - sz0;
       return (double)S[((int64_t)zz * sny + yy) * snx + xx];
     };
-    const int64_t total = (int64_t)nx * ny * nz;
-    for (int64_t v = threadIdx.x; v < total; v += kCT) {
-      const int j = dx0 + (int)(v % nx);
-      const int i = dy0 + (int)((v / nx) % ny);
-      const int k = dz0 + (int)(v / ((int64_t)nx * ny));
+    const int total = nx * ny * nz;  // at most 96^3
+    for (int v = threadIdx.x; v < total; v += kCT) {
+      const int q = v / nx;
+      const int j = dx0 + (v - q * nx);
+      const int i = dy0 + q % ny;
+      const int k = dz0 + q / ny;
       const double px = (j - gb) * a.down, py = (i - gb) * a.down, pz = (k - gb) * a.down;
       const double xl = floor(px), xr = floor(px + 1);
       const double yu = floor(py), yo = floor(py + 1);

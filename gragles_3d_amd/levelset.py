@@ -18,7 +18,8 @@ GG_OK, GG_ERR_ARG, GG_ERR_HIP, GG_ERR_CAPACITY, GG_ERR_STATE, GG_ERR_DEVICE = ra
 FP64_PLAIN, FP32_BATCHED = 0, 1
 DELETED, ALIVE, VANISHED = 0, 1, 2
 SEGMENTS = ("conv_xy", "conv_z", "conv_xy_inv", "idl_resize", "lists", "comparison_n32", "hull", "finalize", "plan",
-            "redist_xy", "redist_z", "post_redist", "count", "comparison_n48", "comparison_n64", "comparison_n96")
+            "redist_xy", "redist_z", "post_redist", "count", "comparison_n48", "comparison_n64", "comparison_n96",
+            "resample")
 
 
 class GGParams(ctypes.Structure):
