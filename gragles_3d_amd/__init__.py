@@ -5,4 +5,4 @@ include/gg_levelset.h); this package only binds it.
 """
 from .levelset import (ALIVE, DELETED, FP32_BATCHED, FP64_PLAIN, GGError, GGParams, LevelSetContext,  # noqa: F401
                        VANISHED, coarsening_plan, derive_params, generate_voronoi, initial_neighbour_lists, lib,
-                       resize_to_cube, setup_simulation, voronoi_seeds)
+                       resize_to_cube, setup_simulation, uds_text, voronoi_seeds)

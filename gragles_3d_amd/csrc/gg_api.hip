@@ -26,6 +26,8 @@ struct gg_ctx {
   int32_t* d_state = nullptr;
   int32_t* d_newb = nullptr;
   int32_t* d_cnb = nullptr;  // resizeGrid's bounds of the last grid coarsening (allocated on first use)
+  uint32_t* d_cont = nullptr;  // voxel container (allocated on first use)
+  int64_t cont_cap = 0;
   TopoDev topo{nullptr, nullptr, nullptr, 0};  // hull topology tables (allocated when first enabled)
   int32_t* d_exflag = nullptr;
   double* d_volume = nullptr;
@@ -999,6 +1001,28 @@ int gg_download_hull_neighbours(gg_ctx* c, int32_t id, int32_t* dst, int32_t cap
   if (dst)
     for (int i = 0; i < (int)v.size() && i < cap; ++i) dst[i] = v[i].second;
   return (int)v.size() > cap && dst ? fail(c, GG_ERR_CAPACITY, "hull neighbours: cap too small") : GG_OK;
+}
+
+int gg_voxel_container(gg_ctx* c, uint32_t* dst, int64_t cap, int32_t* size) {
+  if (!c || !size) return GG_ERR_ARG;
+  int rc = require_boxes(c);
+  if (rc) return rc;
+  hipSetDevice(c->p.device);
+  const int L = c->p.conv_domain_size;
+  *size = L;
+  const int64_t n = (int64_t)L * L * L;
+  if (!dst) return GG_OK;
+  if (n > cap) return fail(c, GG_ERR_ARG, "destination too small");
+  if (c->cont_cap < n) {
+    // a context that once wrote a container keeps the buffer (a later
+    // coarsening only shrinks the grid)
+    if ((rc = dalloc(c, &c->d_cont, (size_t)n))) return rc;
+    c->cont_cap = n;
+  }
+  launch_container(launch_of(c), c->sp, c->d_state, dout(c), c->d_cont, L, c->fp32);
+  if ((rc = gg_synchronize(c))) return rc;
+  HIPCHK(c, hipMemcpy(dst, c->d_cont, 4 * (size_t)n, hipMemcpyDeviceToHost));
+  return GG_OK;
 }
 
 int gg_get_all_bounds(gg_ctx* c, int32_t which, int32_t* bounds) {

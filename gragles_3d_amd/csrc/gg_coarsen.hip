@@ -1,13 +1,16 @@
-// Grid coarsening: grainhdl::gridCoarsement's coarsening branch
-// (src/grainhdl.cpp:1446-1492), the per-grain LSbox::resizeGrid
-// (src/box.cpp:2595-2697).  The IDLocal recalculation and the contour that
-// follow it reuse the comparison and level-set kernels.
+// Whole-grid work on the grain boxes: grid coarsening (grainhdl::gridCoarsement's
+// coarsening branch, src/grainhdl.cpp:1446-1492, with the per-grain
+// LSbox::resizeGrid, src/box.cpp:2595-2697; the IDLocal recalculation and the
+// contour that follow it reuse the comparison and level-set kernels) and the
+// voxel container of grainhdl::saveNetworkAsVoxelContainer
+// (src/grainhdl.cpp:1074-1154).
 //
 // Work per grain is one trilinear interpolation per voxel of the new box (8
 // reads of the old box, 1 write); a coarsening runs once in many timesteps, so
 // the kernel is a plain workgroup-per-grain loop.  Every arithmetic step is
 // written in the reference's order (and the library is built without FMA
 // contraction), so FP64 results are bit-identical to the CPU oracle's.
+#include <algorithm>
 #include <cmath>
 
 #include "gg_internal.h"
@@ -86,7 +89,48 @@ __global__ __launch_bounds__(kCT) void k_coarsen_resample(SimParams sp, CoarsenA
   }
 }
 
+// saveNetworkAsVoxelContainer's copyDataToContainer (src/box.cpp:3008-3032):
+// every voxel of an alive grain's output buffer with phi >= 0 inside
+// [gb, gb + size)^3 names the grain in the size^3 container (x fastest).
+// Where several grains do, the largest id stays: the reference's serial
+// ascending-id order.  One workgroup per grain.
+template <class T>
+__global__ __launch_bounds__(kCT) void k_container(SimParams sp, const int32_t* __restrict__ state, DescDev d,
+                                                   uint32_t* cont, int size) {
+  for (int g = 1 + blockIdx.x; g <= sp.N; g += gridDim.x) {
+    if (state[g] != kAlive) continue;
+    const int32_t* b = d.bounds + 6 * g;
+    const int x0 = b[0], y0 = b[1], z0 = b[2];
+    const int nx = b[3] - x0, ny = b[4] - y0, nz = b[5] - z0;
+    const T* P = static_cast<const T*>(d.arena) + d.off[g];
+    const int gb = sp.gb, total = nx * ny * nz;
+    for (int v = threadIdx.x; v < total; v += kCT) {
+      if (!(T(0) <= P[v])) continue;
+      const int q = v / nx;
+      const int j = x0 + (v - q * nx) - gb, i = y0 + q % ny - gb, k = z0 + q / ny - gb;
+      if (i < 0 || j < 0 || k < 0 || i >= size || j >= size || k >= size) continue;
+      atomicMax(&cont[((int64_t)k * size + i) * size + j], (uint32_t)g);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kCT) void k_fill_u32(uint32_t* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)kCT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCT) p[i] = 0u;
+}
+
 }  // namespace
+
+void launch_container(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& d, uint32_t* cont,
+                      int size, bool fp32) {
+  const int64_t n = (int64_t)size * size * size;
+  const int fb = (int)std::min<int64_t>((n + kCT - 1) / kCT, 65536);
+  hipLaunchKernelGGL(k_fill_u32, dim3(fb > 0 ? fb : 1), dim3(kCT), 0, L.stream, cont, n);
+  const int grid = sp.N < 65536 ? (sp.N > 0 ? sp.N : 1) : 65536;
+  if (fp32)
+    hipLaunchKernelGGL(k_container<float>, dim3(grid), dim3(kCT), 0, L.stream, sp, state, d, cont, size);
+  else
+    hipLaunchKernelGGL(k_container<double>, dim3(grid), dim3(kCT), 0, L.stream, sp, state, d, cont, size);
+}
 
 void launch_coarsen_bounds(const Launch& L, const SimParams& sp, const CoarsenArgs& a, const int32_t* state,
                            const int32_t* src_bounds, int32_t* dst_bounds, int32_t* nb) {

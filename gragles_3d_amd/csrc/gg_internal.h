@@ -314,6 +314,9 @@ void launch_coarsen_bounds(const Launch& L, const SimParams& sp, const CoarsenAr
 // Trilinear resampling of src (old grid) into dst (new bounds and layout).
 void launch_coarsen_resample(const Launch& L, const SimParams& sp, const CoarsenArgs& a, const int32_t* state,
                              const DescDev& src, const DescDev& dst, bool fp32);
+// The size^3 voxel container (grain id per voxel, 0 for none) of the buffers d.
+void launch_container(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& d, uint32_t* cont,
+                      int size, bool fp32);
 // part 0: x/y sweeps; part 1: z sweeps and clamp
 void launch_redist(const Launch& L, int part, const SimParams& sp, const PlanDev& P, const DescDev& din,
                    const DescDev& dout, int32_t* err, bool fp32, int c0 = 0,

@@ -68,6 +68,7 @@ SIGNATURES = {
     "gg_comparison": (ctypes.c_int, [_vp]),
     "gg_level_set": (ctypes.c_int, [_vp]),
     "gg_coarsen_grid": (ctypes.c_int, [_vp, _i32, ctypes.c_double]),
+    "gg_voxel_container": (ctypes.c_int, [_vp, _u32p, ctypes.c_int64, _i32p]),
     "gg_redistancing": (ctypes.c_int, [_vp]),
     "gg_count_grains": (ctypes.c_int, [_vp, _i32p]),
     "gg_timestep": (ctypes.c_int, [_vp, _i32, _i32]),
@@ -142,6 +143,28 @@ def coarsening_plan(current_grains, initial_grains, loop, points_per_grain, real
             and new_size < real_domain_size):
         return new_size, time_step_size / float(new_size * new_size)
     return None
+
+
+def uds_text(ids, bounds, h, volume, bunge=None, stored=None):
+    """The Microstructure_<loop>.uds lines of saveNetworkAsVoxelContainer
+    (src/grainhdl.cpp:1088-1107): per grain, tab-separated, ID, the box centre
+    ((max - min) / 2 + min, integer division) times h for x, y, z, the three
+    Bunge angles, xmin xmax ymin ymax zmin zmax of the output buffer, volume and
+    stored elastic energy; doubles as C++ streams print them by default (six
+    significant digits, %g).  bounds rows are {xmin,ymin,zmin,xmax,ymax,zmax};
+    bunge (n x 3) and stored (n) default to 0, as this path carries no
+    orientations or stored energy."""
+    lines = []
+    for r, gid in enumerate(ids):
+        b = [int(v) for v in bounds[r]]
+        cx, cy, cz = ((b[3] - b[0]) // 2 + b[0]) * h, ((b[4] - b[1]) // 2 + b[1]) * h, ((b[5] - b[2]) // 2 + b[2]) * h
+        eul = (0.0, 0.0, 0.0) if bunge is None else tuple(float(x) for x in bunge[r])
+        see = 0.0 if stored is None else float(stored[r])
+        f = ["%d" % int(gid), "%g" % cx, "%g" % cy, "%g" % cz, "%g" % eul[0], "%g" % eul[1], "%g" % eul[2],
+             "%d" % b[0], "%d" % b[3], "%d" % b[1], "%d" % b[4], "%d" % b[2], "%d" % b[5], "%g" % float(volume[r]),
+             "%g" % see]
+        lines.append("\t".join(f) + "\n")
+    return "".join(lines)
 
 
 def resize_to_cube(bounds6, ngridpoints):
@@ -267,6 +290,43 @@ class LevelSetContext:
     def level_set(self):
         self._chk(lib().gg_level_set(self.h))
 
+    def voxel_container(self):
+        """saveNetworkAsVoxelContainer's id container (Container_size_*.raw
+        data), [z][y][x] of side realDomainSize; call after a whole timestep."""
+        L = ctypes.c_int32(0)
+        self._chk(lib().gg_voxel_container(self.h, None, 0, ctypes.byref(L)))
+        n = L.value
+        out = np.zeros(n ** 3, dtype=np.uint32)
+        self._chk(lib().gg_voxel_container(self.h, _p(out, ctypes.c_uint32), out.size, ctypes.byref(L)))
+        return out.reshape(n, n, n)
+
+    def save_network_as_voxel_container(self, loop, directory=".", bunge=None, stored=None):
+        """grainhdl::saveNetworkAsVoxelContainer (src/grainhdl.cpp:1074-1154):
+        writes Microstructure_<loop>.uds (alive grains, see uds_text),
+        Container_size_<size>_t_<loop>.raw (the id container, uint32, x fastest)
+        and ContainerVolumeEnergy_size_<size>_t_<loop>.raw, which the reference
+        leaves empty (its energy values go to the already closed container
+        file).  bunge / stored: per grain id (rows 0..n_grains), optional.
+        Returns the three paths."""
+        import os
+        cont = self.voxel_container()
+        size = cont.shape[0]
+        st, vol, _ = self.stats()
+        ids = np.nonzero(st == ALIVE)[0]
+        ids = ids[ids > 0]
+        b = self.all_bounds(1)[ids]
+        h = float(self.params.h)
+        text = uds_text(ids, b, h, vol[ids], None if bunge is None else np.asarray(bunge)[ids],
+                        None if stored is None else np.asarray(stored)[ids])
+        p_uds = os.path.join(directory, "Microstructure_%d.uds" % int(loop))
+        p_raw = os.path.join(directory, "Container_size_%d_t_%d.raw" % (size, int(loop)))
+        p_en = os.path.join(directory, "ContainerVolumeEnergy_size_%d_t_%d.raw" % (size, int(loop)))
+        with open(p_uds, "w") as f:
+            f.write(text)
+        cont.astype("<u4").tofile(p_raw)
+        open(p_en, "wb").close()
+        return p_uds, p_raw, p_en
+
     def coarsen_grid(self, new_size, new_dt):
         """gridCoarsement's coarsening branch (src/grainhdl.cpp:1446-1492): the
         new grid (ngridpoints = new_size + 2*grid_blowup, h = 1/new_size,
@@ -274,6 +334,13 @@ class LevelSetContext:
         it, IDLocal recalculated and the contour extracted.  A coarsening step
         is coarsen_grid() followed by timestep()."""
         self._chk(lib().gg_coarsen_grid(self.h, int(new_size), float(new_dt)))
+        # the host copy of the grid variables follows the library's
+        p = self.params
+        p.ngridpoints = int(new_size) + 2 * p.grid_blowup
+        p.conv_domain_size = int(new_size)
+        p.h = 1.0 / int(new_size)
+        p.delta = p.grid_blowup / float(int(new_size))
+        p.dt = float(new_dt)
 
     def redistancing(self):
         self._chk(lib().gg_redistancing(self.h))

@@ -129,6 +129,19 @@ int gg_redistancing(gg_ctx* ctx);
  * hull again).  1 <= new_size <= ngridpoints - 2*grid_blowup. */
 int gg_coarsen_grid(gg_ctx* ctx, int32_t new_size, double new_dt);
 
+/* The voxel container of grainhdl::saveNetworkAsVoxelContainer
+ * (src/grainhdl.cpp:1074-1154, the Container_size_<size>_t_<loop>.raw data)
+ * with LSbox::copyDataToContainer (src/box.cpp:3008-3032): size^3 uint32
+ * (x fastest), size = realDomainSize as the run holds it (the convolution's
+ * domain size); a voxel holds the id of a grain whose output buffer is >= 0
+ * there, shifted by -grid_blowup, else 0.  Call it after a whole timestep.
+ * Where several grains qualify the largest id stays (the reference's write
+ * order with one thread); only alive grains are written (the reference also
+ * writes a grain that vanished in that step, from its pre-comparison field,
+ * which the device does not keep).  *size is set first; dst may be NULL to
+ * query it; cap is in elements. */
+int gg_voxel_container(gg_ctx* ctx, uint32_t* dst, int64_t cap, int32_t* size);
+
 /* countGrains: synchronises and returns the number of grains not deleted. */
 int gg_count_grains(gg_ctx* ctx, int32_t* count);
 

@@ -20,6 +20,7 @@
 //   grain count            src/grainhdl.cpp:936-943
 //   stage order            src/grainhdl.cpp:836-919 (no output)
 //   grid coarsening        src/grainhdl.cpp:1416-1492, src/box.cpp:2595-2702
+//   voxel container        src/grainhdl.cpp:1074-1122, src/box.cpp:3008-3032
 //
 // The FFT libraries the reference calls (Intel MKL DFTI / FFTW3, versions not
 // pinned by the reference) are replaced by an explicit DFT whose every sum runs
@@ -997,6 +998,33 @@ int orc_coarsen(orc_sim* S, int new_size, double new_dt) {
       if (G.state != kDeleted) std::swap(G.in, G.out);
     }
   })
+}
+
+// saveNetworkAsVoxelContainer's container (src/grainhdl.cpp:1074-1122) with
+// LSbox::copyDataToContainer (src/box.cpp:3008-3032): size^3 ids, size =
+// conv_domain_size (realDomainSize as the run holds it), written grain by
+// grain in ascending id (the reference's order with one thread), so the
+// largest id stays where several grains have phi >= 0.  Only alive grains
+// are written (a grain that vanished in the step keeps, in the reference, its
+// pre-comparison field in the output buffer; the device does not keep it).
+// Returns size (out may be NULL to query it), or -1 if cap is too small.
+long orc_voxel_container(orc_sim* S, uint32_t* out, long cap) {
+  const int L = S->p.conv_domain_size, gb = S->p.grid_blowup;
+  const long n = (long)L * L * L;
+  if (!out) return L;
+  if (n > cap) return -1;
+  std::fill(out, out + n, 0u);
+  for (int id = 1; id <= S->p.n_grains; ++id) {
+    const Grain& G = S->g[id];
+    if (G.state != kAlive) continue;
+    const Buf<double>& B = *G.out;
+    for (int k = B.z0; k < B.z1; ++k)
+      for (int i = B.y0; i < B.y1; ++i)
+        for (int j = B.x0; j < B.x1; ++j)
+          if (0. <= B.at(i, j, k) && i >= gb && j >= gb && k >= gb && i < L + gb && j < L + gb && k < L + gb)
+            out[((long)(k - gb) * L + (i - gb)) * L + (j - gb)] = (uint32_t)id;
+  }
+  return L;
 }
 
 int orc_count(orc_sim* S) {  // countGrains, src/grainhdl.cpp:936-943

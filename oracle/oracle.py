@@ -63,6 +63,7 @@ def lib():
             "orc_count": (ctypes.c_int, [vp]),
             "orc_timestep": (ctypes.c_int, [vp, ctypes.c_int, i32p]),
             "orc_coarsen": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double]),
+            "orc_voxel_container": (ctypes.c_long, [vp, u32p, ctypes.c_long]),
             "orc_resize_grid_box": (ctypes.c_long, [i32p, f64p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                                     ctypes.c_double, ctypes.c_double, i32p, i32p, f64p,
                                                     ctypes.c_long]),
@@ -281,6 +282,16 @@ class OracleSim:
         """gridCoarsement's coarsening branch; a coarsening step is coarsen()
         followed by timestep()."""
         self._chk(lib().orc_coarsen(self.h, int(new_size), float(new_dt)))
+
+    def voxel_container(self):
+        """saveNetworkAsVoxelContainer's id container, [z][y][x] of side
+        realDomainSize (the convolution's domain size)."""
+        L = int(lib().orc_voxel_container(self.h, None, 0))
+        out = np.zeros(L ** 3, dtype=np.uint32)
+        n = lib().orc_voxel_container(self.h, _p(out, ctypes.c_uint32), out.size)
+        if n < 0:
+            raise RuntimeError("container buffer too small")
+        return out.reshape(L, L, L)
 
     def state(self, gid):
         return lib().orc_get_state(self.h, int(gid))
