@@ -96,6 +96,28 @@ def kernel_pass(ctx, steps):
     return seg
 
 
+def container_leg(ctx):
+    """One voxel container (saveNetworkAsVoxelContainer's id grid) of the
+    context's state: the kernel clears size^3 ids and scatters every alive
+    grain's phi >= 0 voxels; algorithmic bytes = 4 B per box voxel read + 4 B
+    per container voxel cleared and written once."""
+    st, _, _ = ctx.stats()
+    b = ctx.all_bounds(1)[st == G.ALIVE]
+    box_vox = int(((b[:, 3] - b[:, 0]).astype(np.int64) ** 3).sum())
+    ctx.voxel_container()  # first call: buffer allocation and the kernels' first load
+    ctx.enable_stage_timing(True)
+    ctx.stage_times(reset=True)
+    t = time.perf_counter()
+    cont = ctx.voxel_container()
+    wall = time.perf_counter() - t
+    ms, _ = ctx.stage_times(reset=True)["container"]
+    ctx.enable_stage_timing(False)
+    nbytes = 4.0 * box_vox + 8.0 * cont.size
+    return {"size": int(cont.shape[0]), "call_ms": wall * 1e3, "kernel_ms": ms, "kernel_bytes": nbytes,
+            "kernel_GBs": nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0,
+            "empty_voxel_fraction": float((cont == 0).mean())}
+
+
 def coarsening_leg(ctx, p, frac=0.9):
     """One grid coarsening (gridCoarsement's coarsening branch) of the context's
     state onto a grid frac times the size, then one timestep on it.  The
@@ -239,6 +261,9 @@ def main():
 
     seg32 = kernel_pass(ctx32, args.kernel_steps)
     counts32 = ctx32.count_history()
+    container = container_leg(ctx32)
+    log("voxel container %d^3: kernel %.3f ms (%.0f GB/s), call with copy-out %.1f ms" % (
+        container["size"], container["kernel_ms"], container["kernel_GBs"], container["call_ms"]))
     coarsen = coarsening_leg(ctx32, p)
     log("coarsening %d -> %d: call %.2f ms, resampling %.3f ms (%.0f GB/s)" % (
         coarsen["grid_from"], coarsen["grid_to"], coarsen["call_ms"], coarsen["resample_ms"],
@@ -303,6 +328,7 @@ def main():
                                    "GBs": cmp_total_bytes / (cmp_total_ms * 1e-3) / 1e9 if cmp_total_ms else 0.0},
         "segment_ms_per_step_fp32": seg_ms_per_step,
         "grain_count_trajectory_fp32": counts32[-3:].tolist(),
+        "voxel_container_fp32": container,
         "grid_coarsening_fp32": coarsen,
     }
 

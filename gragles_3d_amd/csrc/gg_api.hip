@@ -222,7 +222,7 @@ hipEvent_t take_event(gg_ctx* c) {
 enum Seg {
   kSegConvA, kSegConvB, kSegConvC, kSegIdl, kSegLists, kSegCompare, kSegHull, kSegFinalize, kSegPlan,
   kSegRedistXY, kSegRedistZ, kSegPost, kSegCount, kSegCompare48, kSegCompare64, kSegCompare96, kSegResample,
-  kNumSeg
+  kSegContainer, kNumSeg
 };
 static_assert(kNumSeg == GG_NUM_SEGMENTS, "segment table of the C ABI");
 
@@ -1019,7 +1019,10 @@ int gg_voxel_container(gg_ctx* c, uint32_t* dst, int64_t cap, int32_t* size) {
     if ((rc = dalloc(c, &c->d_cont, (size_t)n))) return rc;
     c->cont_cap = n;
   }
-  launch_container(launch_of(c), c->sp, c->d_state, dout(c), c->d_cont, L, c->fp32);
+  {
+    SegTimer t(c, kSegContainer);
+    launch_container(launch_of(c), c->sp, c->d_state, dout(c), c->d_cont, L, c->fp32);
+  }
   if ((rc = gg_synchronize(c))) return rc;
   HIPCHK(c, hipMemcpy(dst, c->d_cont, 4 * (size_t)n, hipMemcpyDeviceToHost));
   return GG_OK;

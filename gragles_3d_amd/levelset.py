@@ -19,7 +19,7 @@ FP64_PLAIN, FP32_BATCHED = 0, 1
 DELETED, ALIVE, VANISHED = 0, 1, 2
 SEGMENTS = ("conv_xy", "conv_z", "conv_xy_inv", "idl_resize", "lists", "comparison_n32", "hull", "finalize", "plan",
             "redist_xy", "redist_z", "post_redist", "count", "comparison_n48", "comparison_n64", "comparison_n96",
-            "resample")
+            "resample", "container")
 
 
 class GGParams(ctypes.Structure):

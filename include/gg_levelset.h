@@ -228,9 +228,10 @@ int gg_box_voxels(gg_ctx* ctx, int64_t* voxels);
  *  7 level-set finalize    8 work plan / layout      9 redistancing x/y sweeps
  * 10 redistancing z sweeps + clamp   11 input-buffer resize   12 grain count
  * 13 comparison, 32 < n <= 48   14 comparison, 48 < n <= 64
- * 15 comparison, 64 < n <= 96   16 grid-coarsening resampling */
+ * 15 comparison, 64 < n <= 96   16 grid-coarsening resampling
+ * 17 voxel container (clear + scatter) */
 int gg_enable_stage_timing(gg_ctx* ctx, int32_t enable);
-#define GG_NUM_SEGMENTS 17
+#define GG_NUM_SEGMENTS 18
 /* Accumulated milliseconds and number of timed passes per segment since the
  * last reset; fills up to n entries (launches may be NULL). */
 int gg_get_stage_times(gg_ctx* ctx, double* ms, int64_t* launches, int32_t n, int32_t reset);
