@@ -56,7 +56,6 @@ struct gg_ctx {
   int64_t* d_voxels = nullptr;
   // tables
   TablesDev TB{};
-  void* tb_mem[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   bool boxes_set = false;
   bool lists_set = false;
   // graph of two timesteps
@@ -101,6 +100,14 @@ int dalloc(gg_ctx* c, T** ptr, size_t count) {
   c->allocs.push_back(p);
   *ptr = (T*)p;
   return GG_OK;
+}
+
+// Frees a dalloc'd buffer before the context ends.
+void dfree(gg_ctx* c, const void* ptr) {
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), ptr);
+  if (it == c->allocs.end()) return;
+  hipFree(*it);
+  c->allocs.erase(it);
 }
 
 const double kPi = 3.14159265358979323846;
@@ -705,9 +712,16 @@ int gg_coarsen_grid(gg_ctx* c, int32_t new_size, double new_dt) {
   c->sp.ngp = c->p.ngridpoints;
   c->sp.h = c->p.h;
   c->sp.delta = c->p.delta;
-  // the convolution tables depend on the grid and dt; a captured step graph
+  // the convolution tables depend on the grid and dt (the old ones are freed
+  // once the work queued before this call is done); a captured step graph
   // holds the old grid in its kernel arguments
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->side));
+  const TablesDev old_tb = c->TB;
   if ((rc = build_tables(c))) return rc;
+  for (const void* q : {(const void*)old_tb.tw_off, (const void*)old_tb.twc, (const void*)old_tb.tws,
+                        (const void*)old_tb.g_off, (const void*)old_tb.gtab, (const void*)old_tb.circ})
+    dfree(c, q);
   if (c->gexec) hipGraphExecDestroy(c->gexec);
   if (c->graph) hipGraphDestroy(c->graph);
   c->gexec = nullptr;

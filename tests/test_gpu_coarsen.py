@@ -106,3 +106,21 @@ def test_coarsen_fp32_follows_fp64():
     np.testing.assert_allclose(v32[both], v64[both], rtol=1e-4)
     c64.close()
     c32.close()
+
+
+def test_two_coarsenings_in_one_run():
+    _say("two coarsenings n=64 ppg=12 border=3 seed=4")
+    ctx, orc, p = make_pair(64, 12, 3, 4)
+    ctx.timestep(2, use_graph=True)
+    orc.timestep(2)
+    size = p.ngridpoints - 2 * p.grid_blowup
+    for frac in (0.85, 0.8):
+        size = int(size * frac)
+        ctx.coarsen_grid(size, 0.2 / (size * size))
+        orc.coarsen(size, 0.2 / (size * size))
+        compare_all(ctx, orc, "after coarsening to %d" % size)
+        ctx.timestep(2, use_graph=True)
+        orc.timestep(2)
+        compare_all(ctx, orc, "2 graph steps on %d" % size)
+    assert ctx.params.ngridpoints == size + 2 * p.grid_blowup
+    ctx.close()
