@@ -217,6 +217,9 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run device-to-device copies of 1 GiB (scale for profiler byte counters)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the voxel-container and grid-coarsening legs (they follow the timed steps; "
+                         "profiles of the timestep leave them out so per-kernel means cover one grid)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the per-grain timestep runs every grain box on a single MI355X "
@@ -261,13 +264,15 @@ def main():
 
     seg32 = kernel_pass(ctx32, args.kernel_steps)
     counts32 = ctx32.count_history()
-    container = container_leg(ctx32)
-    log("voxel container %d^3: kernel %.3f ms (%.0f GB/s), call with copy-out %.1f ms" % (
-        container["size"], container["kernel_ms"], container["kernel_GBs"], container["call_ms"]))
-    coarsen = coarsening_leg(ctx32, p)
-    log("coarsening %d -> %d: call %.2f ms, resampling %.3f ms (%.0f GB/s)" % (
-        coarsen["grid_from"], coarsen["grid_to"], coarsen["call_ms"], coarsen["resample_ms"],
-        coarsen["resample_GBs"]))
+    container = coarsen = None
+    if not args.no_extras:
+        container = container_leg(ctx32)
+        log("voxel container %d^3: kernel %.3f ms (%.0f GB/s), call with copy-out %.1f ms" % (
+            container["size"], container["kernel_ms"], container["kernel_GBs"], container["call_ms"]))
+        coarsen = coarsening_leg(ctx32, p)
+        log("coarsening %d -> %d: call %.2f ms, resampling %.3f ms (%.0f GB/s)" % (
+            coarsen["grid_from"], coarsen["grid_to"], coarsen["call_ms"], coarsen["resample_ms"],
+            coarsen["resample_GBs"]))
     ctx32.close()
 
     # Roofline of the comparison kernel (neighbour max + wall + set comparison), one launch per
