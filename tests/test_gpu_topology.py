@@ -64,3 +64,20 @@ def test_hull_topology_in_graph_steps():
     orc.timestep(4)
     assert _compare(ctx, orc, "after 4 graph steps") > 0
     ctx.close()
+
+
+def test_hull_topology_of_a_grid_coarsening():
+    # extractContour in the coarsening branch computes the topology of the
+    # resampled field, against the IDLocal just recalculated
+    _say("hull topology coarsening n=64 ppg=10 border=3 seed=5")
+    ctx, orc, p = make_pair(64, 10, 3, 5)
+    ctx.enable_hull_topology(True)
+    for s in range(2):
+        step_both(ctx, orc)
+    new = int((p.ngridpoints - 2 * p.grid_blowup) * 0.8)
+    ctx.coarsen_grid(new, 0.2 / (new * new))
+    orc.coarsen(new, 0.2 / (new * new))
+    assert _compare(ctx, orc, "after coarsening") > 0
+    step_both(ctx, orc)
+    assert _compare(ctx, orc, "step after coarsening") > 0
+    ctx.close()
