@@ -43,7 +43,8 @@ struct gg_ctx {
   // lists
   ListDev LD{};
   // plan
-  PlanDev P{};
+  PlanDev P{};   // plan of the current boxes (input role)
+  PlanDev P2{};  // plan of the output boxes, from the level set's first part to the end of the redistancing
   int64_t* d_spec_off = nullptr;
   void* d_spec = nullptr;
   int64_t spec_cap = 0;
@@ -62,6 +63,7 @@ struct gg_ctx {
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int graph_sel = -1, graph_il = -1;
+  const int32_t* graph_plan = nullptr;  // c->P.grains when the graph was captured
   // stage timing
   bool timing = false;
   // second stream for the list build, which runs beside the convolution
@@ -210,10 +212,11 @@ void cube_bounds_host(int32_t* b, int L) {
 DescDev din(gg_ctx* c) { return c->D[c->sel]; }
 DescDev dout(gg_ctx* c) { return c->D[c->sel ^ 1]; }
 
-void plan_layout(gg_ctx* c, const DescDev& d) {
-  launch_plan(launch_of(c), c->sp, c->d_state, d.bounds, d.off, c->d_spec_off, c->P, c->arena_cap, c->spec_cap,
+void plan_into(gg_ctx* c, const DescDev& d, const PlanDev& P) {
+  launch_plan(launch_of(c), c->sp, c->d_state, d.bounds, d.off, c->d_spec_off, P, c->arena_cap, c->spec_cap,
               c->d_err, c->d_voxels);
 }
+void plan_layout(gg_ctx* c, const DescDev& d) { plan_into(c, d, c->P); }
 
 // stage timing helpers
 hipEvent_t take_event(gg_ctx* c) {
@@ -338,33 +341,84 @@ void over_classes(gg_ctx* c, F launch) {
   hipStreamWaitEvent(c->stream, c->ev_join, 0);
 }
 
-void stage_level_set(gg_ctx* c) {
+// Level set, first part: the output boxes (they follow from the
+// set-comparison bounds alone) and their plan, made before the hull so that the
+// redistancing need not wait for it.  Grains the hull then finds vanished keep
+// their planned space unused.
+void stage_output_plan(gg_ctx* c) {
   Launch L = launch_of(c);
-  {
-    SegTimer t(c, kSegHull);
-    if (c->topo.on) launch_topo_clear(L, c->sp, c->topo);
-    over_classes(c, [&](const Launch& LL, int c0, int c1) {
-      launch_hull(LL, c->sp, c->P, din(c), c->d_exflag, c->d_partials, c->IL[c->il_cur], c->topo, c->d_err, c->fp32,
-                  c0, c1);
-    });
-  }
   {
     SegTimer t(c, kSegFinalize);
     launch_finalize(L, c->sp, c->P, c->d_state, c->LD, c->d_exflag, c->d_partials, c->d_newb, din(c), dout(c),
-                    c->d_volume, c->d_area);
+                    c->d_volume, c->d_area, kFinBounds);
   }
-  { SegTimer t(c, kSegPlan); plan_layout(c, dout(c)); }
+  { SegTimer t(c, kSegPlan); plan_into(c, dout(c), c->P2); }
 }
 
-void stage_redistancing(gg_ctx* c) {
-  Launch L = launch_of(c);
+void stage_hull(gg_ctx* c) {
+  SegTimer t(c, kSegHull);
+  if (c->topo.on) launch_topo_clear(launch_of(c), c->sp, c->topo);
+  over_classes(c, [&](const Launch& LL, int c0, int c1) {
+    launch_hull(LL, c->sp, c->P, din(c), c->d_exflag, c->d_partials, c->IL[c->il_cur], c->topo, c->d_err, c->fp32,
+                c0, c1);
+  });
+}
+
+// Level set, last part: deletion, list flip, existence, volume and area.
+void stage_finalize_state(gg_ctx* c) {
+  SegTimer t(c, kSegFinalize);
+  launch_finalize(launch_of(c), c->sp, c->P, c->d_state, c->LD, c->d_exflag, c->d_partials, c->d_newb, din(c),
+                  dout(c), c->d_volume, c->d_area, kFinState);
+}
+
+void stage_level_set(gg_ctx* c) {
+  stage_output_plan(c);
+  stage_hull(c);
+  stage_finalize_state(c);
+}
+
+void redist_parts(gg_ctx* c) {
   for (int part = 0; part < 2; ++part) {
     SegTimer t(c, part == 0 ? kSegRedistXY : kSegRedistZ);
     over_classes(c, [&](const Launch& LL, int c0, int c1) {
-      launch_redist(LL, part, c->sp, c->P, din(c), dout(c), c->d_err, c->fp32, c0, c1);
+      launch_redist(LL, part, c->sp, c->P2, din(c), dout(c), c->d_err, c->fp32, c0, c1);
     });
   }
-  { SegTimer t(c, kSegPost); launch_post_redist(L, c->sp, c->d_state, din(c), dout(c)); }
+}
+
+// The redistancing over the output plan, then the input role takes the output
+// boxes and their plan becomes the current one.
+void stage_redistancing(gg_ctx* c) {
+  redist_parts(c);
+  { SegTimer t(c, kSegPost); launch_post_redist(launch_of(c), c->sp, c->d_state, din(c), dout(c)); }
+  std::swap(c->P, c->P2);
+}
+
+// Level set and redistancing of one timestep with the hull on the side stream
+// beside the redistancing: the hull reads the input role and IDLocal, the
+// redistancing reads the input role and writes the output role, and the
+// existence, volume and area finish after both.  The post-redistancing resize
+// follows the deletion marks, as in the stage-by-stage order.
+void stage_level_set_and_redistancing(gg_ctx* c) {
+  if (c->timing) {
+    stage_level_set(c);
+    stage_redistancing(c);
+    return;
+  }
+  Launch L = launch_of(c);
+  stage_output_plan(c);
+  if (c->topo.on) launch_topo_clear(L, c->sp, c->topo);
+  hipEventRecord(c->ev_fork, c->stream);
+  hipStreamWaitEvent(c->side, c->ev_fork, 0);
+  Launch S = L;
+  S.stream = c->side;
+  launch_hull(S, c->sp, c->P, din(c), c->d_exflag, c->d_partials, c->IL[c->il_cur], c->topo, c->d_err, c->fp32);
+  hipEventRecord(c->ev_join, c->side);
+  for (int part = 0; part < 2; ++part) launch_redist(L, part, c->sp, c->P2, din(c), dout(c), c->d_err, c->fp32);
+  hipStreamWaitEvent(c->stream, c->ev_join, 0);
+  stage_finalize_state(c);
+  launch_post_redist(L, c->sp, c->d_state, din(c), dout(c));
+  std::swap(c->P, c->P2);
 }
 
 void stage_count(gg_ctx* c) {
@@ -396,8 +450,7 @@ void enqueue_step(gg_ctx* c) {
   }
   stage_comparison(c);
   c->sel ^= 1;
-  stage_level_set(c);
-  stage_redistancing(c);
+  stage_level_set_and_redistancing(c);
   stage_count(c);
 }
 
@@ -475,6 +528,21 @@ int gg_create(const gg_params* params, gg_ctx** out) {
   for (int q = 0; q < kNumQ; ++q)
     if ((rc = dalloc(c, &c->P.pre[q], N + 1 + kClasses)) ||
         (rc = dalloc(c, &c->P.tmap[q], (size_t)kMaxTilesPerGrain * N))) {
+      gg_destroy(c);
+      return rc;
+    }
+  // the second plan shares the (constant) spatial grain order
+  c->P2.order = c->P.order;
+  if ((rc = dalloc(c, &c->P2.cls_count, kClasses)) || (rc = dalloc(c, &c->P2.cls_start, kClasses)) ||
+      (rc = dalloc(c, &c->P2.grains, N + 1)) || (rc = dalloc(c, &c->P2.grain_pos, N + 1)) ||
+      (rc = dalloc(c, &c->P2.ncls, N)) || (rc = dalloc(c, &c->P2.totals, kNumQ * kClasses)) ||
+      (rc = dalloc(c, &c->P2.layer_base, kClasses)) || (rc = dalloc(c, &c->P2.tile_base, kNumQ * kClasses))) {
+    gg_destroy(c);
+    return rc;
+  }
+  for (int q = 0; q < kNumQ; ++q)
+    if ((rc = dalloc(c, &c->P2.pre[q], N + 1 + kClasses)) ||
+        (rc = dalloc(c, &c->P2.tmap[q], (size_t)kMaxTilesPerGrain * N))) {
       gg_destroy(c);
       return rc;
     }
@@ -612,6 +680,8 @@ int gg_init_distance(gg_ctx* c, const int32_t* idfield) {
     return fail(c, GG_ERR_HIP, std::string("hipMemcpy idfield: ") + hipGetErrorString(e));
   }
   launch_init_phi(launch_of(c), c->sp, c->P, din(c), d_id, c->fp32);
+  // the redistancing runs over the output role's plan (same boxes here)
+  plan_into(c, dout(c), c->P2);
   stage_redistancing(c);
   e = hipStreamSynchronize(c->stream);
   hipFree(d_id);
@@ -750,7 +820,7 @@ int gg_coarsen_grid(gg_ctx* c, int32_t new_size, double new_dt) {
   if (c->topo.on) launch_topo_clear(L, c->sp, c->topo);
   launch_hull(L, c->sp, c->P, din(c), c->d_exflag, c->d_partials, c->IL[c->il_cur], c->topo, c->d_err, c->fp32);
   launch_finalize(L, c->sp, c->P, c->d_state, c->LD, c->d_exflag, c->d_partials, c->d_cnb, din(c), dout(c),
-                  c->d_volume, c->d_area, true);
+                  c->d_volume, c->d_area, kFinBounds | kFinState | kFinCoarsen);
   plan_layout(c, dout(c));
   c->sel ^= 1;
   return GG_OK;
@@ -800,7 +870,7 @@ int gg_timestep(gg_ctx* c, int32_t nsteps, int32_t use_graph) {
   int s = 0;
   // the graph holds two timesteps (an even number of role swaps) and is valid
   // for the role parity it was captured at
-  if (c->gexec && (c->graph_sel != c->sel || c->graph_il != c->il_cur)) {
+  if (c->gexec && (c->graph_sel != c->sel || c->graph_il != c->il_cur || c->graph_plan != c->P.grains)) {
     if (nsteps > 0) {
       enqueue_step(c);
       ++s;
@@ -809,6 +879,7 @@ int gg_timestep(gg_ctx* c, int32_t nsteps, int32_t use_graph) {
   if (!c->gexec && nsteps - s >= 2) {
     c->graph_sel = c->sel;
     c->graph_il = c->il_cur;
+    c->graph_plan = c->P.grains;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     enqueue_step(c);
     enqueue_step(c);
@@ -819,8 +890,10 @@ int gg_timestep(gg_ctx* c, int32_t nsteps, int32_t use_graph) {
     // capturing advanced the host-side role state by two steps; restore it
     c->sel = c->graph_sel;
     c->il_cur = c->graph_il;
+    if (c->P.grains != c->graph_plan) std::swap(c->P, c->P2);
   }
-  while (nsteps - s >= 2 && c->gexec && c->graph_sel == c->sel && c->graph_il == c->il_cur) {
+  while (nsteps - s >= 2 && c->gexec && c->graph_sel == c->sel && c->graph_il == c->il_cur &&
+         c->graph_plan == c->P.grains) {
     HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
     s += 2;
   }

@@ -414,44 +414,66 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
 // Per grain: vanished -> deleted; alive -> take the comparison list as the new
 // neighbour list, decide existence, finish volume/area, resize the output box
 // to the set-comparison bounds (+-grid_blowup, clamped, src/box.cpp:857-877)
-// made cubic.
-//
-// coarsen: extractContour as a grid coarsening calls it (src/grainhdl.cpp:1481-1490)
-// rather than level_set: no deletion, the neighbour lists stay, and newb holds
-// resizeGrid's bounds, taken as they are.
+// made cubic.  mode selects the parts:
+//   kFinBounds  the output box of every alive grain (it follows from the
+//               set-comparison bounds alone, so it can precede the hull)
+//   kFinState   deletion, list flip, existence, volume and area (after the hull)
+//   kFinCoarsen extractContour as a grid coarsening calls it
+//               (src/grainhdl.cpp:1481-1490) rather than level_set: no
+//               deletion, the neighbour lists stay, and newb holds resizeGrid's
+//               bounds, taken as they are
+// With kFinState the output box is written for grains that keep existing only.
 __global__ void k_finalize(SimParams sp, PlanDev P, DescDev din, int32_t* state, ListDev ld, int32_t* exflag,
                            const double* __restrict__ partials, const int32_t* __restrict__ newb, DescDev dout,
-                           double* volume, double* area, int coarsen) {
+                           double* volume, double* area, int mode) {
+  const bool coarsen = mode & kFinCoarsen, do_state = mode & kFinState, do_bounds = mode & kFinBounds;
   for (int g = 1 + blockIdx.x * blockDim.x + threadIdx.x; g <= sp.N; g += gridDim.x * blockDim.x) {
     const int s = state[g];
     if (s == kVanished) {
-      if (!coarsen) state[g] = kDeleted;
+      if (do_state && !coarsen) state[g] = kDeleted;
+      continue;
+    }
+    if (s == kDoomed) {
+      if (do_state) {
+        ld.sel[g] ^= 1;
+        exflag[g] = 0;
+        state[g] = kVanished;
+      }
       continue;
     }
     if (s != kAlive) continue;
-    if (!coarsen) ld.sel[g] ^= 1;
-    const int ex = exflag[g];
-    exflag[g] = 0;
-    if (!ex) {
-      state[g] = kVanished;
+    if (do_state) {
+      if (!coarsen) ld.sel[g] ^= 1;
+      const int ex = exflag[g];
+      exflag[g] = 0;
+      if (!ex) {
+        state[g] = kVanished;
+        continue;
+      }
+      const int pos = P.grain_pos[g];
+      int c = 0;
+      while (c + 1 < kClasses && pos >= P.cls_start[c + 1]) ++c;
+      const int idx = pos - P.cls_start[c];
+      const int first = P.pre[kQLayers][P.cls_start[c] + c + idx];
+      const int n = box_n(din.bounds, g);
+      const double* pp = partials + 4 * (P.layer_base[c] + first);
+      Acc va{pp[0], pp[1]}, aa{pp[2], pp[3]};
+      for (int l = 1; l < n; ++l) {
+        va.merge(pp[4 * l], pp[4 * l + 1]);
+        aa.merge(pp[4 * l + 2], pp[4 * l + 3]);
+      }
+      const double h = sp.h;
+      volume[g] = fabs(va.s + va.c) * h * h * h;
+      area[g] = (aa.s + aa.c) * h * h;
+    }
+    if (!do_bounds) continue;
+    const int32_t* nb = newb + 6 * g;
+    // Before the hull: a grain without a voxel >= 0 (empty set-comparison
+    // bounds) has no hull and vanishes; it is left out of the output plan.
+    if (!do_state && (nb[0] > nb[3] || nb[1] > nb[4] || nb[2] > nb[5])) {
+      state[g] = kDoomed;
       continue;
     }
-    const int pos = P.grain_pos[g];
-    int c = 0;
-    while (c + 1 < kClasses && pos >= P.cls_start[c + 1]) ++c;
-    const int idx = pos - P.cls_start[c];
-    const int first = P.pre[kQLayers][P.cls_start[c] + c + idx];
-    const int n = box_n(din.bounds, g);
-    const double* pp = partials + 4 * (P.layer_base[c] + first);
-    Acc va{pp[0], pp[1]}, aa{pp[2], pp[3]};
-    for (int l = 1; l < n; ++l) {
-      va.merge(pp[4 * l], pp[4 * l + 1]);
-      aa.merge(pp[4 * l + 2], pp[4 * l + 3]);
-    }
-    const double h = sp.h;
-    volume[g] = fabs(va.s + va.c) * h * h * h;
-    area[g] = (aa.s + aa.c) * h * h;
-    const int32_t* nb = newb + 6 * g;
     int bb[6] = {nb[0] - sp.gb, nb[1] - sp.gb, nb[2] - sp.gb, nb[3] + sp.gb, nb[4] + sp.gb, nb[5] + sp.gb};
     if (coarsen) {
       for (int a = 0; a < 6; ++a) bb[a] = nb[a];
@@ -531,10 +553,10 @@ void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const D
 
 void launch_finalize(const Launch& L, const SimParams& sp, const PlanDev& P, int32_t* state, const ListDev& ld,
                      int32_t* exflag, const double* partials, const int32_t* newb, const DescDev& din,
-                     const DescDev& dout, double* volume, double* area, bool coarsen) {
+                     const DescDev& dout, double* volume, double* area, int mode) {
   int blocks = (sp.N + 255) / 256;
   hipLaunchKernelGGL(k_finalize, dim3(blocks), dim3(256), 0, L.stream, sp, P, din, state, ld, exflag, partials,
-                     newb, dout, volume, area, coarsen ? 1 : 0);
+                     newb, dout, volume, area, mode);
 }
 
 }  // namespace gg

@@ -32,7 +32,10 @@ __host__ __device__ inline int size_class(int n) {
   return n <= 32 ? 0 : n <= 48 ? 1 : n <= 64 ? 2 : n <= 96 ? 3 : -1;
 }
 
-enum { kDeleted = 0, kAlive = 1, kVanished = 2 };
+enum { kDeleted = 0, kAlive = 1, kVanished = 2, kDoomed = 3 };
+// kDoomed lives only inside a level set: an alive grain without a voxel >= 0
+// (it has no hull, so it vanishes), marked before the output plan so that the
+// plan leaves it out; the level set's last part makes it vanished.
 
 // device error bits
 enum {
@@ -298,10 +301,12 @@ void launch_topo_clear(const Launch& L, const SimParams& sp, const TopoDev& topo
 void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, int32_t* exflag,
                  double* partials, const IdlDev& il, const TopoDev& topo, int32_t* err, bool fp32, int c0 = 0,
                  int c1 = kClasses);
+// Parts of the level-set finalize (k_finalize).
+constexpr int kFinBounds = 1, kFinState = 2, kFinCoarsen = 4;
 void launch_finalize(const Launch& L, const SimParams& sp, const PlanDev& P, int32_t* state,
                      const ListDev& ld, int32_t* exflag, const double* partials, const int32_t* newb,
                      const DescDev& din, const DescDev& dout, double* volume, double* area,
-                     bool coarsen = false);
+                     int mode = kFinBounds | kFinState);
 // Grid coarsening (LSbox::resizeGrid): up = h_old/h_new, down = h_new/h_old,
 // delta = the new delta; sp holds the new grid.
 struct CoarsenArgs {

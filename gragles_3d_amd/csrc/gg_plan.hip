@@ -194,9 +194,12 @@ __global__ __launch_bounds__(256) void k_init_phi(SimParams sp, PlanDev P, DescD
 
 // resizeIDLocalToDistanceBuffer (src/box.cpp:559-567): the IDLocal buffer the
 // comparison writes takes the output buffer's bounds and layout.
+// Vanished grains follow too: one that vanished at the last level set after
+// being planned is still listed by the current plan (the comparison then
+// writes its IDLocal, which must land in its own planned region).
 __global__ void k_idl_resize(SimParams sp, const int32_t* __restrict__ state, DescDev dout, IdlDev il) {
   for (int g = 1 + blockIdx.x * blockDim.x + threadIdx.x; g <= sp.N; g += gridDim.x * blockDim.x) {
-    if (state[g] != kAlive) continue;
+    if (state[g] != kAlive && state[g] != kVanished) continue;
     il.off[g] = dout.off[g];
     for (int a = 0; a < 6; ++a) il.bounds[6 * g + a] = dout.bounds[6 * g + a];
   }
@@ -204,9 +207,12 @@ __global__ void k_idl_resize(SimParams sp, const int32_t* __restrict__ state, De
 
 // End of executeRedistancing (src/box.cpp:2582-2585): the input buffer takes
 // the output buffer's bounds (and, here, its arena layout).
+// A grain that vanished at this level set takes them too: it was planned
+// (and redistanced) when its hull was still to come, and the plan that becomes
+// the current one lists it, so its input boxes must be its planned ones.
 __global__ void k_post_redist(SimParams sp, const int32_t* __restrict__ state, DescDev din, DescDev dout) {
   for (int g = 1 + blockIdx.x * blockDim.x + threadIdx.x; g <= sp.N; g += gridDim.x * blockDim.x) {
-    if (state[g] != kAlive) continue;
+    if (state[g] != kAlive && state[g] != kVanished) continue;
     din.off[g] = dout.off[g];
     for (int a = 0; a < 6; ++a) din.bounds[6 * g + a] = dout.bounds[6 * g + a];
   }
