@@ -1102,7 +1102,11 @@ int gg_voxel_container(gg_ctx* c, uint32_t* dst, int64_t cap, int32_t* size) {
   if (n > cap) return fail(c, GG_ERR_ARG, "destination too small");
   if (c->cont_cap < n) {
     // a context that once wrote a container keeps the buffer (a later
-    // coarsening only shrinks the grid)
+    // coarsening only shrinks the grid); every call ends synchronised, so a
+    // smaller old buffer is idle here
+    if (c->d_cont) dfree(c, c->d_cont);
+    c->d_cont = nullptr;
+    c->cont_cap = 0;
     if ((rc = dalloc(c, &c->d_cont, (size_t)n))) return rc;
     c->cont_cap = n;
   }
