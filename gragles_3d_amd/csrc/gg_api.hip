@@ -28,7 +28,7 @@ struct gg_ctx {
   int32_t* d_cnb = nullptr;  // resizeGrid's bounds of the last grid coarsening (allocated on first use)
   uint32_t* d_cont = nullptr;  // voxel container (allocated on first use)
   int64_t cont_cap = 0;
-  TopoDev topo{nullptr, nullptr, nullptr, 0};  // hull topology tables (allocated when first enabled)
+  TopoDev topo{nullptr, nullptr, nullptr, 0, nullptr, nullptr};  // hull topology tables (allocated when first enabled)
   int32_t* d_exflag = nullptr;
   double* d_volume = nullptr;
   double* d_area = nullptr;
@@ -1006,11 +1006,14 @@ int gg_enable_hull_topology(gg_ctx* c, int32_t enable) {
     const size_t n = (size_t)(c->p.n_grains + 1) * kTopoCap;
     int rc;
     const size_t nj = (size_t)(c->p.n_grains + 1) * kJuncCap;
-    if ((rc = dalloc(c, &c->topo.key, n)) || (rc = dalloc(c, &c->topo.ord, n)) || (rc = dalloc(c, &c->topo.jkey, nj)))
+    if ((rc = dalloc(c, &c->topo.key, n)) || (rc = dalloc(c, &c->topo.ord, n)) || (rc = dalloc(c, &c->topo.jkey, nj)) ||
+        (rc = dalloc(c, &c->topo.jpos, 3 * nj)) || (rc = dalloc(c, &c->topo.jcnt, nj)))
       return rc;
     // empty tables: no grain touched yet
     HIPCHK(c, hipMemset(c->topo.key, 0xFF, n * sizeof(int32_t)));
     HIPCHK(c, hipMemset(c->topo.jkey, 0, nj * sizeof(unsigned long long)));
+    HIPCHK(c, hipMemset(c->topo.jcnt, 0, nj * sizeof(int32_t)));
+    HIPCHK(c, hipMemset(c->topo.jpos, 0, 3 * nj * sizeof(double)));
   }
   if (on != c->topo.on) {
     c->topo.on = on;
@@ -1116,6 +1119,48 @@ int gg_voxel_container(gg_ctx* c, uint32_t* dst, int64_t cap, int32_t* size) {
   }
   if ((rc = gg_synchronize(c))) return rc;
   HIPCHK(c, hipMemcpy(dst, c->d_cont, 4 * (size_t)n, hipMemcpyDeviceToHost));
+  return GG_OK;
+}
+
+int gg_download_hull_elements(gg_ctx* c, int32_t id, int32_t* nids, int32_t* ids8, int32_t* ntri, double* pos,
+                              int32_t cap, int32_t* n) {
+  if (!c || !n || id < 1 || id > c->p.n_grains) return GG_ERR_ARG;
+  if (!c->topo.key) return fail(c, GG_ERR_STATE, "hull topology is not enabled");
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  std::vector<int32_t> key(kTopoCap), cnt(kJuncCap);
+  std::vector<unsigned long long> jk(kJuncCap);
+  std::vector<double> jp(3 * kJuncCap);
+  HIPCHK(c, hipMemcpy(key.data(), c->topo.key + (int64_t)id * kTopoCap, 4 * kTopoCap, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(jk.data(), c->topo.jkey + (int64_t)id * kJuncCap, 8 * kJuncCap, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(cnt.data(), c->topo.jcnt + (int64_t)id * kJuncCap, 4 * kJuncCap, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(jp.data(), c->topo.jpos + (int64_t)id * kJuncCap * 3, 24 * kJuncCap, hipMemcpyDeviceToHost));
+  struct Rec {
+    std::vector<int64_t> ids;
+    int32_t nt;
+    double p[3];
+  };
+  std::vector<Rec> recs;
+  for (int s = 0; s < kJuncCap; ++s) {
+    if (!jk[s] || cnt[s] <= 0) continue;
+    Rec r;
+    for (int b = 0; b < kTopoCap; ++b)
+      if ((jk[s] >> b) & 1ull) r.ids.push_back((int64_t)(uint32_t)key[b]);
+    std::sort(r.ids.begin(), r.ids.end());
+    r.nt = cnt[s];
+    for (int k = 0; k < 3; ++k) r.p[k] = jp[3 * s + k] / cnt[s];
+    recs.push_back(r);
+  }
+  std::sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) { return a.ids < b.ids; });
+  *n = (int32_t)recs.size();
+  if (!nids) return GG_OK;
+  if ((int32_t)recs.size() > cap || !ids8 || !ntri || !pos) return fail(c, GG_ERR_ARG, "destination too small");
+  for (size_t r = 0; r < recs.size(); ++r) {
+    nids[r] = (int32_t)recs[r].ids.size();
+    for (int k = 0; k < 8; ++k) ids8[8 * r + k] = k < (int)recs[r].ids.size() ? (int32_t)recs[r].ids[k] : -1;
+    ntri[r] = recs[r].nt;
+    for (int k = 0; k < 3; ++k) pos[3 * r + k] = recs[r].p[k];
+  }
   return GG_OK;
 }
 

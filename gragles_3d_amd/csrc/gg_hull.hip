@@ -274,8 +274,10 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
           //    its ids hold in the grain's table (a slot never changes once
           //    taken, so equal lists give equal masks); distinct masks per tile in
           //    LDS, then into the grain's junction table
+          uint64_t cmask[kCPT];
 #pragma unroll
           for (int r = 0; r < kCPT; ++r) {
+            cmask[r] = 0;
             if (code[r] == 0u) continue;
             uint32_t ids[8];
             bool first[8];
@@ -295,6 +297,7 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
                 hsl = (hsl + 1) & (kTopoCap - 1);
               }
             }
+            cmask[r] = mask;
             if (mask == 0) continue;
             uint32_t hsl = (uint32_t)((mask * 0x9E3779B97F4A7C15ull) >> (64 - kJuncLdsBits));
             bool done = false;
@@ -320,6 +323,49 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
               if (!done) atomicOr(err, kErrTopo);
             }
             s_jm[i] = 0ull;
+          }
+          __syncthreads();
+          // 3. interfacial elements (src/grainHull.cpp:227-270): each cut cell
+          //    adds its triangles' barycenters (src/triangle.h:12-18) and their
+          //    number to its list's slot in the grain's junction table
+          double* gp = topo.jpos + (int64_t)g * kJuncCap * 3;
+          int32_t* gc = topo.jcnt + (int64_t)g * kJuncCap;
+#pragma unroll
+          for (int r = 0; r < kCPT; ++r) {
+            const unsigned long long m = cmask[r];
+            if (m == 0ull) continue;
+            uint32_t hsl = (uint32_t)((m * 0x9E3779B97F4A7C15ull) >> (64 - kJuncCapBits));
+            int slot = -1;
+            for (int pr = 0; pr < kJuncCap; ++pr) {
+              const unsigned long long k = __hip_atomic_load(&gj[hsl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              if (k == m) {
+                slot = (int)hsl;
+                break;
+              }
+              if (k == 0ull) break;  // not in the table (it was full: reported above)
+              hsl = (hsl + 1) & (kJuncCap - 1);
+            }
+            if (slot < 0) continue;
+            const uint32_t cd = code[r];
+            const unsigned ci = cd & 0xFFu;
+            const int yl = (cd >> 8) & 0xFF, xl = (cd >> 16) & 0xFF;
+            const int x = x0 + xl, y = y0 + yl;
+            const int nt = s_ntri[ci];
+            uint64_t row = s_tri[ci];
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            for (int j = 0; j < nt; ++j, row >>= 12) {
+              double p0x, p0y, p0z, p1x, p1y, p1z, p2x, p2y, p2z;
+              edge_vertex<T, NMAX>((int)(row & 15u), L, xl, yl, n, x, y, z, p0x, p0y, p0z);
+              edge_vertex<T, NMAX>((int)((row >> 4) & 15u), L, xl, yl, n, x, y, z, p1x, p1y, p1z);
+              edge_vertex<T, NMAX>((int)((row >> 8) & 15u), L, xl, yl, n, x, y, z, p2x, p2y, p2z);
+              sx += (p0x + p1x + p2x) / 3;
+              sy += (p0y + p1y + p2y) / 3;
+              sz += (p0z + p1z + p2z) / 3;
+            }
+            atomicAdd(&gp[3 * slot], sx);
+            atomicAdd(&gp[3 * slot + 1], sy);
+            atomicAdd(&gp[3 * slot + 2], sz);
+            atomicAdd(&gc[slot], nt);
           }
         }
         // exclusive scan of the per-lane triangle counts: wavefront, then workgroup
@@ -531,6 +577,10 @@ __global__ __launch_bounds__(256) void k_topo_clear(SimParams sp, TopoDev topo) 
       topo.ord[i] = INT_MAX;
     }
     topo.jkey[i] = 0ull;
+    topo.jcnt[i] = 0;
+    topo.jpos[3 * i] = 0.0;
+    topo.jpos[3 * i + 1] = 0.0;
+    topo.jpos[3 * i + 2] = 0.0;
   }
 }
 

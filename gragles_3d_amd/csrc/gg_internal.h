@@ -64,6 +64,8 @@ struct TopoDev {
   int32_t* ord;                // [(N+1) * kTopoCap]
   unsigned long long* jkey;    // [(N+1) * kJuncCap]
   int on;                      // 0: not computed
+  double* jpos;                // [(N+1) * kJuncCap * 3] barycenter sums of each list's triangles
+  int32_t* jcnt;               // [(N+1) * kJuncCap] triangles of each list
 };
 
 struct DescDev {

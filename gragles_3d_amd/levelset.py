@@ -69,6 +69,7 @@ SIGNATURES = {
     "gg_level_set": (ctypes.c_int, [_vp]),
     "gg_coarsen_grid": (ctypes.c_int, [_vp, _i32, ctypes.c_double]),
     "gg_voxel_container": (ctypes.c_int, [_vp, _u32p, ctypes.c_int64, _i32p]),
+    "gg_download_hull_elements": (ctypes.c_int, [_vp, _i32, _i32p, _i32p, _i32p, _f64p, _i32, _i32p]),
     "gg_redistancing": (ctypes.c_int, [_vp]),
     "gg_count_grains": (ctypes.c_int, [_vp, _i32p]),
     "gg_timestep": (ctypes.c_int, [_vp, _i32, _i32]),
@@ -289,6 +290,20 @@ class LevelSetContext:
 
     def level_set(self):
         self._chk(lib().gg_level_set(self.h))
+
+    def hull_elements(self, gid):
+        """Interfacial elements of the grain's last hull (topology on): [(ids,
+        triangles, mean barycenter (x, y, z) in grid coordinates)] sorted by ids."""
+        n = ctypes.c_int32(0)
+        self._chk(lib().gg_download_hull_elements(self.h, int(gid), None, None, None, None, 0, ctypes.byref(n)))
+        k = max(n.value, 1)
+        nids, ids8, ntri = np.zeros(k, np.int32), np.zeros(8 * k, np.int32), np.zeros(k, np.int32)
+        pos = np.zeros(3 * k)
+        self._chk(lib().gg_download_hull_elements(self.h, int(gid), _p(nids, ctypes.c_int32), _p(ids8, ctypes.c_int32),
+                                                  _p(ntri, ctypes.c_int32), _p(pos, ctypes.c_double), k,
+                                                  ctypes.byref(n)))
+        return [(tuple(ids8[8 * r:8 * r + min(8, nids[r])].tolist()), int(ntri[r]), tuple(pos[3 * r:3 * r + 3]))
+                for r in range(n.value)]
 
     def voxel_container(self):
         """saveNetworkAsVoxelContainer's id container (Container_size_*.raw

@@ -452,6 +452,16 @@ void redistance(const Buf<double>& in, Buf<double>& out, double h, double delta)
   }
 }
 
+// One distinct neighbour list of a hull (an interfacial element): its ids
+// ascending, its triangles and the sum of their barycenters
+// (GrainHull::subDivideTrianglesToInterfacialElements, src/grainHull.cpp:227-270;
+// Triangle::computeBarycenter, src/triangle.h:12-18).
+struct JRec {
+  std::vector<unsigned> ids;
+  int ntri = 0;
+  double sum[3] = {0, 0, 0};
+};
+
 // Grains the hull touches (MarchingCubesAlgorithm::generateAdditionalInformation
 // + insertDistinctInteger + recordNewGrains, src/marchingCubes.cpp:501-602, with
 // LSbox::getNeighbourAt, src/box.cpp:3000-3006): for every cut cell in the
@@ -515,6 +525,57 @@ void hull_neighbours(const Buf<double>& B, const Buf<uint32_t>& idl, unsigned se
       }
 }
 
+// The interfacial elements of a hull: for each distinct cell id list (the
+// same lists hull_neighbours counts), its triangles and the sum of their
+// barycenters, vertices as in hull_reduce (global grid coordinates).  The
+// element's position (QuadrupleJunction / HighOrderJunction::computePosition,
+// src/InterfacialElement.cpp:406-469) is sum / ntri.  Sorted by ids.
+void hull_elements(const Buf<double>& B, const Buf<uint32_t>& idl, unsigned self, std::vector<JRec>& out) {
+  std::map<std::vector<unsigned>, JRec> acc;
+  V3 vert[12];
+  for (int z = B.z0; z < B.z1 - 1; ++z)
+    for (int y = B.y0; y < B.y1 - 1; ++y)
+      for (int x = B.x0; x < B.x1 - 1; ++x) {
+        double val[8];
+        V3 pos[8];
+        unsigned c = 0, ids[8];
+        for (int q = 0; q < 8; ++q) {
+          int cx = x + kCorner[q][0], cy = y + kCorner[q][1], cz = z + kCorner[q][2];
+          val[q] = B.at(cy, cx, cz);
+          pos[q] = V3{(double)cx, (double)cy, (double)cz};
+          const bool inside = val[q] >= 0;
+          if (inside) c |= 1u << q;
+          if (inside) ids[q] = self;
+          else if (cx >= idl.x0 && cx < idl.x1 && cy >= idl.y0 && cy < idl.y1 && cz >= idl.z0 && cz < idl.z1)
+            ids[q] = idl.at(cy, cx, cz);
+          else
+            ids[q] = 0xFFFFFFFFu;
+        }
+        int mask = g_edge_mask[c];
+        if (mask == 0) continue;
+        std::vector<unsigned> key;
+        for (int q = 0; q < 8; ++q)
+          if (ids[q] != 0xFFFFFFFFu && std::find(key.begin(), key.end(), ids[q]) == key.end()) key.push_back(ids[q]);
+        std::sort(key.begin(), key.end());
+        JRec& r = acc[key];
+        r.ids = key;
+        for (int e = 0; e < 12; ++e)
+          if (mask & (1 << e)) {
+            int a = kEdge[e][0], b = kEdge[e][1];
+            vert[e] = vertex_interp(pos[a], pos[b], val[a], val[b]);
+          }
+        for (int t = 0; g_tri[c][t] != -1; t += 3) {
+          V3 p0 = vert[g_tri[c][t]], p1 = vert[g_tri[c][t + 1]], p2 = vert[g_tri[c][t + 2]];
+          r.sum[0] += (p0.x + p1.x + p2.x) / 3;
+          r.sum[1] += (p0.y + p1.y + p2.y) / 3;
+          r.sum[2] += (p0.z + p1.z + p2.z) / 3;
+          ++r.ntri;
+        }
+      }
+  out.clear();
+  for (auto& kv : acc) out.push_back(kv.second);
+}
+
 // ---------------------------------------------------------------------------
 enum { kDeleted = 0, kAlive = 1, kVanished = 2 };
 
@@ -530,6 +591,7 @@ struct Grain {
   double volume = 0, area = 0;
   std::vector<unsigned> hull_nb;  // grains the last hull touched, in first-appearance order
   int hull_junc[4] = {0, 0, 0, 0};  // distinct cell id lists of 2, 3, 4 and other sizes
+  std::vector<JRec> hull_je;        // the distinct lists with their triangles, ids ascending
 };
 
 }  // namespace
@@ -917,12 +979,14 @@ int orc_level_set(orc_sim* S) {  // level_set, src/grainhdl.cpp:669-684 + extrac
         G.in.reset();
         G.out.reset();
         G.hull_nb.clear();
+        G.hull_je.clear();
         for (int& v : G.hull_junc) v = 0;
         return;
       }
       double vr, ar;
       bool ex = hull_reduce(*G.in, &vr, &ar);
       hull_neighbours(*G.in, G.idl, G.id, G.hull_nb, G.hull_junc);
+      hull_elements(*G.in, G.idl, G.id, G.hull_je);
       if (!ex) {
         G.state = kVanished;
         return;
@@ -984,6 +1048,7 @@ int orc_coarsen(orc_sim* S, int new_size, double new_dt) {
       double vr, ar;
       bool ex = hull_reduce(*G.in, &vr, &ar);
       hull_neighbours(*G.in, G.idl, G.id, G.hull_nb, G.hull_junc);
+      hull_elements(*G.in, G.idl, G.id, G.hull_je);
       if (!ex) {
         G.state = kVanished;
         return;
@@ -1114,6 +1179,28 @@ long orc_hull_neighbours_box(int n, int x0, int y0, int z0, const double* phi, c
   return (long)out.size();
 }
 
+// hull_elements of one cube box (as orc_hull_neighbours_box); output as
+// orc_get_hull_elements.
+long orc_hull_elements_box(int n, int x0, int y0, int z0, const double* phi, const int32_t* b6, const uint32_t* idl,
+                           int self, int32_t* nids, int32_t* ids8, int32_t* ntri, double* pos, long cap) {
+  Buf<double> B;
+  B.resize(x0, y0, z0, x0 + n, y0 + n, z0 + n);
+  std::memcpy(B.v.data(), phi, sizeof(double) * B.v.size());
+  Buf<uint32_t> I;
+  I.resize(b6[0], b6[1], b6[2], b6[3], b6[4], b6[5]);
+  std::memcpy(I.v.data(), idl, sizeof(uint32_t) * I.v.size());
+  std::vector<JRec> E;
+  hull_elements(B, I, (unsigned)self, E);
+  if ((long)E.size() > cap) return -1;
+  for (size_t r = 0; r < E.size(); ++r) {
+    nids[r] = (int32_t)E[r].ids.size();
+    for (int k = 0; k < 8; ++k) ids8[8 * r + k] = k < (int)E[r].ids.size() ? (int32_t)E[r].ids[k] : -1;
+    ntri[r] = E[r].ntri;
+    for (int k = 0; k < 3; ++k) pos[3 * r + k] = E[r].sum[k] / E[r].ntri;
+  }
+  return (long)E.size();
+}
+
 // which: 0 = m_secondOrderNeighbours, 1 = m_comparisonList
 long orc_get_list(orc_sim* S, int id, int which, int32_t* dst, long cap) {
   const Grain& G = S->g.at(id);
@@ -1122,6 +1209,21 @@ long orc_get_list(orc_sim* S, int id, int which, int32_t* dst, long cap) {
   if (n > cap) return -2;
   for (long i = 0; i < n; ++i) dst[i] = (int32_t)L[i];
   return n;
+}
+
+// The grain's interfacial elements, sorted by ids: nids[r] ids in ids8[8r..],
+// ntri[r] triangles, pos[3r..] the mean barycenter.  Returns the count, or -1
+// if cap is too small (lists of more than 8 ids are cut to their first 8).
+long orc_get_hull_elements(orc_sim* S, int id, int32_t* nids, int32_t* ids8, int32_t* ntri, double* pos, long cap) {
+  const std::vector<JRec>& E = S->g.at(id).hull_je;
+  if ((long)E.size() > cap) return -1;
+  for (size_t r = 0; r < E.size(); ++r) {
+    nids[r] = (int32_t)E[r].ids.size();
+    for (int k = 0; k < 8; ++k) ids8[8 * r + k] = k < (int)E[r].ids.size() ? (int32_t)E[r].ids[k] : -1;
+    ntri[r] = E[r].ntri;
+    for (int k = 0; k < 3; ++k) pos[3 * r + k] = E[r].sum[k] / E[r].ntri;
+  }
+  return (long)E.size();
 }
 
 void orc_get_stats(orc_sim* S, double* volume, double* area, int32_t* newbounds) {

@@ -64,6 +64,10 @@ def lib():
             "orc_timestep": (ctypes.c_int, [vp, ctypes.c_int, i32p]),
             "orc_coarsen": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double]),
             "orc_voxel_container": (ctypes.c_long, [vp, u32p, ctypes.c_long]),
+            "orc_get_hull_elements": (ctypes.c_long, [vp, ctypes.c_int, i32p, i32p, i32p, f64p, ctypes.c_long]),
+            "orc_hull_elements_box": (ctypes.c_long, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, f64p,
+                                                      i32p, u32p, ctypes.c_int, i32p, i32p, i32p, f64p,
+                                                      ctypes.c_long]),
             "orc_resize_grid_box": (ctypes.c_long, [i32p, f64p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                                     ctypes.c_double, ctypes.c_double, i32p, i32p, f64p,
                                                     ctypes.c_long]),
@@ -152,6 +156,25 @@ def hull_neighbours_box(phi, origin, idl_bounds, idl, self_id, junctions=False):
                                       _p(out, ctypes.c_int32), out.size, _p(j4, ctypes.c_int32))
     assert k >= 0
     return (out[:k].copy(), j4) if junctions else out[:k].copy()
+
+
+def hull_elements_box(phi, origin, idl_bounds, idl, self_id):
+    """Interfacial elements of a cube box's hull: [(ids, ntri, mean barycenter)]
+    sorted by ids."""
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    n = phi.shape[0]
+    b6 = np.ascontiguousarray(idl_bounds, dtype=np.int32)
+    idl = np.ascontiguousarray(idl, dtype=np.uint32)
+    cap = 1024
+    nids, ids8, ntri = np.zeros(cap, np.int32), np.zeros(8 * cap, np.int32), np.zeros(cap, np.int32)
+    pos = np.zeros(3 * cap)
+    k = lib().orc_hull_elements_box(n, int(origin[0]), int(origin[1]), int(origin[2]), _p(phi, ctypes.c_double),
+                                    _p(b6, ctypes.c_int32), _p(idl, ctypes.c_uint32), int(self_id),
+                                    _p(nids, ctypes.c_int32), _p(ids8, ctypes.c_int32), _p(ntri, ctypes.c_int32),
+                                    _p(pos, ctypes.c_double), cap)
+    assert k >= 0
+    return [(tuple(ids8[8 * r:8 * r + min(8, nids[r])].tolist()), int(ntri[r]), tuple(pos[3 * r:3 * r + 3]))
+            for r in range(k)]
 
 
 def redistance_box(in_bounds, phi_in, out_bounds, h, delta):
@@ -282,6 +305,21 @@ class OracleSim:
         """gridCoarsement's coarsening branch; a coarsening step is coarsen()
         followed by timestep()."""
         self._chk(lib().orc_coarsen(self.h, int(new_size), float(new_dt)))
+
+    def hull_elements(self, gid):
+        """The grain's interfacial elements sorted by ids: [(ids tuple, ntri,
+        position (x, y, z) = mean triangle barycenter)]."""
+        cap = 4096
+        nids = np.zeros(cap, np.int32)
+        ids8 = np.zeros(8 * cap, np.int32)
+        ntri = np.zeros(cap, np.int32)
+        pos = np.zeros(3 * cap)
+        n = lib().orc_get_hull_elements(self.h, int(gid), _p(nids, ctypes.c_int32), _p(ids8, ctypes.c_int32),
+                                        _p(ntri, ctypes.c_int32), _p(pos, ctypes.c_double), cap)
+        if n < 0:
+            raise RuntimeError("too many elements")
+        return [(tuple(ids8[8 * r:8 * r + min(8, nids[r])].tolist()), int(ntri[r]), tuple(pos[3 * r:3 * r + 3]))
+                for r in range(n)]
 
     def voxel_container(self):
         """saveNetworkAsVoxelContainer's id container, [z][y][x] of side

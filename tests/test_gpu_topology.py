@@ -34,6 +34,12 @@ def _compare(ctx, orc, where):
             raise AssertionError("%s: grain %d hull neighbours %s, oracle %s" % (where, g, list(got), list(want)))
         assert cnt[g] == len(want) and wall[g] == (0 in want), (where, g)
         if st[g] == G.ALIVE:
+            eg, eo = ctx.hull_elements(g), orc.hull_elements(g)
+            assert [(e[0], e[1]) for e in eg] == [(e[0], e[1]) for e in eo], (where, g, eg[:3], eo[:3])
+            for a, b in zip(eg, eo):
+                np.testing.assert_allclose(a[2], b[2], rtol=0, atol=1e-9, err_msg="%s grain %d" % (where, g))
+            assert sum(1 for e in eg if len(e[0]) == 2) == junc[g][0]
+        if st[g] == G.ALIVE:
             assert g in want, (where, g)  # a hull that exists touches its own grain
             checked += 1
     return checked

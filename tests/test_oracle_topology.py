@@ -83,3 +83,29 @@ def test_corners_outside_the_idlocal_box_name_nothing():
     ids, junc = O.hull_neighbours_box(half_space(), ORIGIN, b6, idl, 3, junctions=True)
     assert list(ids) == [3]
     assert list(junc) == [0, 0, 0, 1]  # the one-id list {3} falls to the reference's default branch
+
+
+def test_half_space_element_position_is_the_patch_centre():
+    # one boundary element; the plane x = 3.5 (local) is cut into two triangles
+    # per cell, whose barycenters average to the cell centre, so the element's
+    # position is the centre of the (N-1)^2 cells and it has 2 (N-1)^2 triangles
+    idl = np.full(N ** 3, 7, dtype=np.uint32)
+    el = O.hull_elements_box(half_space(), ORIGIN, BOX, idl, 3)
+    assert len(el) == 1
+    ids, ntri, pos = el[0]
+    assert ids == (3, 7) and ntri == 2 * (N - 1) ** 2
+    np.testing.assert_allclose(pos, (ORIGIN[0] + 3.5, ORIGIN[1] + (N - 1) / 2, ORIGIN[2] + (N - 1) / 2),
+                               rtol=0, atol=1e-12)
+
+
+def test_elements_match_the_junction_classes():
+    # the elements are the distinct lists the junction counts classify
+    phi = -np.ones((N, N, N))
+    phi[:, :, :4] = 1.0
+    idl = np.full(N ** 3, 5, dtype=np.uint32).reshape(N, N, N)
+    idl[:, 4:, :] = 6  # a second neighbour above local y = 4: a triple line along x = 3.5
+    ids, junc = O.hull_neighbours_box(phi, ORIGIN, BOX, idl.reshape(-1), 3, junctions=True)
+    el = O.hull_elements_box(phi, ORIGIN, BOX, idl.reshape(-1), 3)
+    sizes = [len(e[0]) for e in el]
+    assert [sizes.count(2), sizes.count(3)] == list(junc[:2])
+    assert sum(e[1] for e in el) == 2 * (N - 1) ** 2
