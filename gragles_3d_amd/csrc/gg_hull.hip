@@ -129,6 +129,12 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
   __shared__ int s_any[kHT / 64];
   __shared__ int s_tk[kTopoLds], s_to[kTopoLds];  // per-tile topology table (id, first appearance)
   __shared__ unsigned long long s_jm[kJuncLds];    // per-tile junction lists (slot masks)
+  // Element sums per tile list in LDS, added to the grain's table once per tile
+  // and list; the FP64 n <= 96 class has no LDS left for them and adds each
+  // cell's sums to the grain's table directly.
+  constexpr bool kElemLds = TOPO && !(sizeof(T) == 8 && NMAX == 96);
+  __shared__ double s_jp[kElemLds ? 3 * kJuncLds : 1];
+  __shared__ int s_jc[kElemLds ? kJuncLds : 1];
   __shared__ int s_gk[kTopoCap];                   // copy of the grain's id table
   const int total = P.totals[kQLayers * kClasses + cls];
   const T* arena = (const T*)din.arena;
@@ -145,7 +151,13 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
       s_tk[i] = -1;
       s_to[i] = INT_MAX;
     }
-    for (int i = tid; i < kJuncLds; i += kHT) s_jm[i] = 0ull;
+    for (int i = tid; i < kJuncLds; i += kHT) {
+      s_jm[i] = 0ull;
+      if constexpr (kElemLds) {
+        s_jc[i] = 0;
+        s_jp[3 * i] = s_jp[3 * i + 1] = s_jp[3 * i + 2] = 0.0;
+      }
+    }
   }
   for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
     int g, kl;
@@ -212,6 +224,25 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
 #pragma unroll
               for (int q2 = 0; q2 < qq; ++q2) first[qq] &= ids[q2] != ids[qq];
             }
+          };
+          // the barycenter sums (src/triangle.h:12-18) and number of a cut
+          // cell's triangles, vertices as the hull's
+          auto cell_elems = [&](uint32_t cd, double& sx, double& sy, double& sz) -> int {
+            const unsigned ci = cd & 0xFFu;
+            const int cyl = (cd >> 8) & 0xFF, cxl = (cd >> 16) & 0xFF;
+            const int nt = s_ntri[ci];
+            uint64_t row = s_tri[ci];
+            sx = sy = sz = 0.0;
+            for (int j = 0; j < nt; ++j, row >>= 12) {
+              double p0x, p0y, p0z, p1x, p1y, p1z, p2x, p2y, p2z;
+              edge_vertex<T, NMAX>((int)(row & 15u), L, cxl, cyl, n, x0 + cxl, y0 + cyl, z, p0x, p0y, p0z);
+              edge_vertex<T, NMAX>((int)((row >> 4) & 15u), L, cxl, cyl, n, x0 + cxl, y0 + cyl, z, p1x, p1y, p1z);
+              edge_vertex<T, NMAX>((int)((row >> 8) & 15u), L, cxl, cyl, n, x0 + cxl, y0 + cyl, z, p2x, p2y, p2z);
+              sx += (p0x + p1x + p2x) / 3;
+              sy += (p0y + p1y + p2y) / 3;
+              sz += (p0z + p1z + p2z) / 3;
+            }
+            return nt;
           };
           // 1. grains: each distinct id with its first-appearance key (cell in the
           //    (z, y, x) loop * 8 + rank among the cell's ascending ids), per tile
@@ -300,38 +331,62 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
             cmask[r] = mask;
             if (mask == 0) continue;
             uint32_t hsl = (uint32_t)((mask * 0x9E3779B97F4A7C15ull) >> (64 - kJuncLdsBits));
-            bool done = false;
-            for (int pr = 0; pr < kJuncLds && !done; ++pr) {
+            int jsl = -1;
+            for (int pr = 0; pr < kJuncLds && jsl < 0; ++pr) {
               const unsigned long long old = atomicCAS(&s_jm[hsl], 0ull, (unsigned long long)mask);
-              done = old == 0ull || old == mask;
+              if (old == 0ull || old == mask) jsl = (int)hsl;
               hsl = (hsl + 1) & (kJuncLds - 1);
             }
-            if (!done) atomicOr(err, kErrTopo);
+            if (jsl < 0) {
+              atomicOr(err, kErrTopo);
+              continue;
+            }
+            if constexpr (kElemLds) {
+              // 3. interfacial elements (src/grainHull.cpp:227-270), per tile list
+              double sx, sy, sz;
+              const int nt = cell_elems(code[r], sx, sy, sz);
+              atomicAdd(&s_jp[3 * jsl], sx);
+              atomicAdd(&s_jp[3 * jsl + 1], sy);
+              atomicAdd(&s_jp[3 * jsl + 2], sz);
+              atomicAdd(&s_jc[jsl], nt);
+            }
           }
           __syncthreads();
           unsigned long long* gj = topo.jkey + (int64_t)g * kJuncCap;
+          double* gp = topo.jpos + (int64_t)g * kJuncCap * 3;
+          int32_t* gc = topo.jcnt + (int64_t)g * kJuncCap;
           for (int i = tid; i < kJuncLds; i += kHT) {
             const unsigned long long m = s_jm[i];
             if (m != 0ull) {
               uint32_t hsl = (uint32_t)((m * 0x9E3779B97F4A7C15ull) >> (64 - kJuncCapBits));
-              bool done = false;
-              for (int pr = 0; pr < kJuncCap && !done; ++pr) {
+              int gsl = -1;
+              for (int pr = 0; pr < kJuncCap && gsl < 0; ++pr) {
                 const unsigned long long old = atomicCAS(&gj[hsl], 0ull, m);
-                done = old == 0ull || old == m;
+                if (old == 0ull || old == m) gsl = (int)hsl;
                 hsl = (hsl + 1) & (kJuncCap - 1);
               }
-              if (!done) atomicOr(err, kErrTopo);
+              if (gsl < 0) {
+                atomicOr(err, kErrTopo);
+              } else if constexpr (kElemLds) {
+                atomicAdd(&gp[3 * gsl], s_jp[3 * i]);
+                atomicAdd(&gp[3 * gsl + 1], s_jp[3 * i + 1]);
+                atomicAdd(&gp[3 * gsl + 2], s_jp[3 * i + 2]);
+                atomicAdd(&gc[gsl], s_jc[i]);
+              }
             }
             s_jm[i] = 0ull;
+            if constexpr (kElemLds) {
+              s_jc[i] = 0;
+              s_jp[3 * i] = s_jp[3 * i + 1] = s_jp[3 * i + 2] = 0.0;
+            }
           }
           __syncthreads();
-          // 3. interfacial elements (src/grainHull.cpp:227-270): each cut cell
-          //    adds its triangles' barycenters (src/triangle.h:12-18) and their
-          //    number to its list's slot in the grain's junction table
-          double* gp = topo.jpos + (int64_t)g * kJuncCap * 3;
-          int32_t* gc = topo.jcnt + (int64_t)g * kJuncCap;
+          // 3. interfacial elements (src/grainHull.cpp:227-270) of the class
+          //    without LDS for them: each cut cell adds its sums to its list's
+          //    slot in the grain's junction table
 #pragma unroll
           for (int r = 0; r < kCPT; ++r) {
+            if constexpr (kElemLds) break;
             const unsigned long long m = cmask[r];
             if (m == 0ull) continue;
             uint32_t hsl = (uint32_t)((m * 0x9E3779B97F4A7C15ull) >> (64 - kJuncCapBits));
@@ -346,22 +401,8 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
               hsl = (hsl + 1) & (kJuncCap - 1);
             }
             if (slot < 0) continue;
-            const uint32_t cd = code[r];
-            const unsigned ci = cd & 0xFFu;
-            const int yl = (cd >> 8) & 0xFF, xl = (cd >> 16) & 0xFF;
-            const int x = x0 + xl, y = y0 + yl;
-            const int nt = s_ntri[ci];
-            uint64_t row = s_tri[ci];
-            double sx = 0.0, sy = 0.0, sz = 0.0;
-            for (int j = 0; j < nt; ++j, row >>= 12) {
-              double p0x, p0y, p0z, p1x, p1y, p1z, p2x, p2y, p2z;
-              edge_vertex<T, NMAX>((int)(row & 15u), L, xl, yl, n, x, y, z, p0x, p0y, p0z);
-              edge_vertex<T, NMAX>((int)((row >> 4) & 15u), L, xl, yl, n, x, y, z, p1x, p1y, p1z);
-              edge_vertex<T, NMAX>((int)((row >> 8) & 15u), L, xl, yl, n, x, y, z, p2x, p2y, p2z);
-              sx += (p0x + p1x + p2x) / 3;
-              sy += (p0y + p1y + p2y) / 3;
-              sz += (p0z + p1z + p2z) / 3;
-            }
+            double sx, sy, sz;
+            const int nt = cell_elems(code[r], sx, sy, sz);
             atomicAdd(&gp[3 * slot], sx);
             atomicAdd(&gp[3 * slot + 1], sy);
             atomicAdd(&gp[3 * slot + 2], sz);
