@@ -37,7 +37,9 @@ def _run(cmd):
     return r.stdout
 
 
-def build(jobs=None, verbose=False):
+def build(jobs=None, verbose=False, force=False):
+    """Compile the HIP and C++ sources (only those newer than their objects
+    unless force) and link the library."""
     os.makedirs(BUILD, exist_ok=True)
     headers = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".h", ".inc"))]
     headers += [os.path.join(INCLUDE, f) for f in os.listdir(INCLUDE)]
@@ -48,7 +50,7 @@ def build(jobs=None, verbose=False):
         s = os.path.join(CSRC, src)
         o = os.path.join(BUILD, src + ".o")
         objs.append(o)
-        if os.path.exists(o) and os.path.getmtime(o) >= max(os.path.getmtime(s), hdr_mtime):
+        if not force and os.path.exists(o) and os.path.getmtime(o) >= max(os.path.getmtime(s), hdr_mtime):
             continue
         if src.endswith(".hip"):
             cmds.append([HIPCC] + HIP_FLAGS + ["-c", s, "-o", o])
@@ -68,5 +70,15 @@ def build(jobs=None, verbose=False):
     return LIB
 
 
+def build_abi_check(src, out):
+    """Compile a C program against include/*.h and link it to the library (the
+    run-time path of the library is recorded relative to the program)."""
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    rel = os.path.relpath(HERE, os.path.dirname(out))
+    _run(["gcc", "-std=c99", "-O2", "-Wall", "-Werror", "-I" + INCLUDE, src, "-o", out, "-L" + HERE, "-lgg_levelset",
+          "-Wl,-rpath,$ORIGIN/" + rel])
+    return out
+
+
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, force="-f" in sys.argv))

@@ -45,6 +45,7 @@ struct gg_ctx {
   // plan
   PlanDev P{};   // plan of the current boxes (input role)
   PlanDev P2{};  // plan of the output boxes, from the level set's first part to the end of the redistancing
+  bool p2_valid = false;  // P2 holds the plan of the current output boxes (the redistancing runs over it)
   int64_t* d_spec_off = nullptr;
   void* d_spec = nullptr;
   int64_t spec_cap = 0;
@@ -353,6 +354,7 @@ void stage_output_plan(gg_ctx* c) {
                     c->d_volume, c->d_area, kFinBounds);
   }
   { SegTimer t(c, kSegPlan); plan_into(c, dout(c), c->P2); }
+  c->p2_valid = true;
 }
 
 void stage_hull(gg_ctx* c) {
@@ -392,6 +394,7 @@ void stage_redistancing(gg_ctx* c) {
   redist_parts(c);
   { SegTimer t(c, kSegPost); launch_post_redist(launch_of(c), c->sp, c->d_state, din(c), dout(c)); }
   std::swap(c->P, c->P2);
+  c->p2_valid = false;
 }
 
 // Level set and redistancing of one timestep with the hull on the side stream
@@ -419,6 +422,7 @@ void stage_level_set_and_redistancing(gg_ctx* c) {
   stage_finalize_state(c);
   launch_post_redist(L, c->sp, c->d_state, din(c), dout(c));
   std::swap(c->P, c->P2);
+  c->p2_valid = false;
 }
 
 void stage_count(gg_ctx* c) {
@@ -650,6 +654,7 @@ int gg_set_boxes(gg_ctx* c, const int32_t* bounds, const uint8_t* exists) {
   }
   c->sel = 0;
   c->il_cur = 0;
+  c->p2_valid = false;
   plan_layout(c, c->D[0]);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   int rc = check_device_error(c);
@@ -682,6 +687,7 @@ int gg_init_distance(gg_ctx* c, const int32_t* idfield) {
   launch_init_phi(launch_of(c), c->sp, c->P, din(c), d_id, c->fp32);
   // the redistancing runs over the output role's plan (same boxes here)
   plan_into(c, dout(c), c->P2);
+  c->p2_valid = true;
   stage_redistancing(c);
   e = hipStreamSynchronize(c->stream);
   hipFree(d_id);
@@ -822,6 +828,7 @@ int gg_coarsen_grid(gg_ctx* c, int32_t new_size, double new_dt) {
   launch_finalize(L, c->sp, c->P, c->d_state, c->LD, c->d_exflag, c->d_partials, c->d_cnb, din(c), dout(c),
                   c->d_volume, c->d_area, kFinBounds | kFinState | kFinCoarsen);
   plan_layout(c, dout(c));
+  c->p2_valid = false;
   c->sel ^= 1;
   return GG_OK;
 }
@@ -839,6 +846,10 @@ int gg_redistancing(gg_ctx* c) {
   if (!c) return GG_ERR_ARG;
   int rc = require_boxes(c);
   if (rc) return rc;
+  // The redistancing writes the output boxes the level set's first part laid
+  // out (their plan); a second call, or one after a coarsening, has none.
+  if (!c->p2_valid)
+    return fail(c, GG_ERR_STATE, "gg_redistancing needs the output boxes of a gg_level_set first");
   hipSetDevice(c->p.device);
   stage_redistancing(c);
   return GG_OK;
@@ -880,6 +891,7 @@ int gg_timestep(gg_ctx* c, int32_t nsteps, int32_t use_graph) {
     c->graph_sel = c->sel;
     c->graph_il = c->il_cur;
     c->graph_plan = c->P.grains;
+    const bool p2v = c->p2_valid;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     enqueue_step(c);
     enqueue_step(c);
@@ -891,6 +903,7 @@ int gg_timestep(gg_ctx* c, int32_t nsteps, int32_t use_graph) {
     c->sel = c->graph_sel;
     c->il_cur = c->graph_il;
     if (c->P.grains != c->graph_plan) std::swap(c->P, c->P2);
+    c->p2_valid = p2v;
   }
   while (nsteps - s >= 2 && c->gexec && c->graph_sel == c->sel && c->graph_il == c->il_cur &&
          c->graph_plan == c->P.grains) {
@@ -962,6 +975,31 @@ int gg_download_distance(gg_ctx* c, int32_t id, int32_t which, double* dst, int6
     for (int64_t i = 0; i < cnt; ++i) dst[i] = tmp[i];
   } else {
     HIPCHK(c, hipMemcpy(dst, (double*)d.arena + off, 8 * cnt, hipMemcpyDeviceToHost));
+  }
+  return GG_OK;
+}
+
+int gg_upload_phi(gg_ctx* c, int32_t id, int32_t which, const double* src, int64_t n) {
+  if (!c || !src || id < 1 || id > c->p.n_grains || (which != 0 && which != 1)) return GG_ERR_ARG;
+  int rc = require_boxes(c);
+  if (rc) return rc;
+  if ((rc = gg_synchronize(c))) return rc;  // the stream's work on the buffer is done before it is overwritten
+  int32_t st = 0;
+  HIPCHK(c, hipMemcpy(&st, c->d_state + id, 4, hipMemcpyDeviceToHost));
+  if (st != kAlive) return fail(c, GG_ERR_STATE, "gg_upload_phi: grain is not alive");
+  const DescDev& d = which ? dout(c) : din(c);
+  int32_t b[6];
+  int64_t off = 0;
+  HIPCHK(c, hipMemcpy(b, d.bounds + 6 * id, 24, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(&off, d.off + id, 8, hipMemcpyDeviceToHost));
+  const int64_t cnt = (int64_t)(b[3] - b[0]) * (b[4] - b[1]) * (b[5] - b[2]);
+  if (n != cnt) return fail(c, GG_ERR_ARG, "gg_upload_phi: element count differs from the buffer's box volume");
+  if (c->fp32) {
+    std::vector<float> tmp(cnt);
+    for (int64_t i = 0; i < cnt; ++i) tmp[i] = (float)src[i];
+    HIPCHK(c, hipMemcpy((float*)d.arena + off, tmp.data(), 4 * cnt, hipMemcpyHostToDevice));
+  } else {
+    HIPCHK(c, hipMemcpy((double*)d.arena + off, src, 8 * cnt, hipMemcpyHostToDevice));
   }
   return GG_OK;
 }

@@ -79,6 +79,7 @@ SIGNATURES = {
     "gg_get_bounds": (ctypes.c_int, [_vp, _i32, _i32, _i32p]),
     "gg_download_distance": (ctypes.c_int, [_vp, _i32, _i32, _f64p, ctypes.c_int64, _i64p]),
     "gg_download_idlocal": (ctypes.c_int, [_vp, _i32, _i32p, _u32p, ctypes.c_int64, _i64p]),
+    "gg_upload_phi": (ctypes.c_int, [_vp, _i32, _i32, _f64p, ctypes.c_int64]),
     "gg_download_list": (ctypes.c_int, [_vp, _i32, _i32, _i32p, _i32, _i32p]),
     "gg_get_all_bounds": (ctypes.c_int, [_vp, _i32, _i32p]),
     "gg_enable_hull_topology": (ctypes.c_int, [_vp, _i32]),
@@ -400,6 +401,12 @@ class LevelSetContext:
         self._chk(lib().gg_download_distance(self.h, int(gid), int(which), _p(buf, ctypes.c_double), buf.size,
                                              ctypes.byref(n)))
         return self.bounds(gid, which), buf[: n.value]
+
+    def upload_phi(self, gid, which, values):
+        """Overwrite grain gid's distance buffer (which: 0 input, 1 output) with
+        values (its box volume of doubles, x fastest)."""
+        v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        self._chk(lib().gg_upload_phi(self.h, int(gid), int(which), _p(v, ctypes.c_double), v.size))
 
     def idlocal(self, gid):
         b = np.zeros(6, dtype=np.int32)

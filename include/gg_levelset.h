@@ -105,7 +105,10 @@ int gg_set_neighbour_lists(gg_ctx* ctx, const int32_t* offsets, const int32_t* i
  * 50 000 grains). */
 int gg_build_neighbour_lists(gg_ctx* ctx);
 
-/* Stage entry points (see the table above). */
+/* Stage entry points (see the table above).  Order: gg_redistancing writes
+ * the output boxes that the preceding gg_level_set laid out, so it follows a
+ * gg_level_set (or gg_init_distance's own); called again before the next
+ * gg_level_set, or after gg_coarsen_grid, it returns GG_ERR_STATE. */
 int gg_switch_buffers(gg_ctx* ctx);
 int gg_convolution(gg_ctx* ctx);
 int gg_update_neighbour_lists(gg_ctx* ctx);
@@ -183,6 +186,14 @@ int gg_get_bounds(gg_ctx* ctx, int32_t id, int32_t which, int32_t* bounds6);
 /* Copy grain id's distance buffer (which: 0 input, 1 output), converted to
  * double, into dst (capacity cap elements); *n receives the element count. */
 int gg_download_distance(gg_ctx* ctx, int32_t id, int32_t which, double* dst, int64_t cap, int64_t* n);
+
+/* Overwrite grain id's distance buffer (which: 0 input, 1 output) with the n
+ * doubles at src, x fastest, n = the buffer's box volume (rounded to float in
+ * the FP32 path); replaces writing an LSbox's DimensionalBufferReal through
+ * setValueAt (src/dimensionalBuffer.h:87-92), e.g. as
+ * LSbox::calculateDistanceFunction does (src/box.cpp:374-397).  Synchronises
+ * the context stream first; the grain must be alive. */
+int gg_upload_phi(gg_ctx* ctx, int32_t id, int32_t which, const double* src, int64_t n);
 
 /* Copy grain id's IDLocal buffer (LSbox::m_IDLocal) and its bounds. */
 int gg_download_idlocal(gg_ctx* ctx, int32_t id, int32_t* bounds6, uint32_t* dst, int64_t cap, int64_t* n);

@@ -133,3 +133,49 @@ def test_device_initial_lists_match_host(n_grains, ppg, border, seed):
     np.testing.assert_array_equal(d_offs, offs)
     np.testing.assert_array_equal(d_ids, ids)
     ctx.close()
+
+
+def test_redistancing_order_is_enforced():
+    # gg_redistancing writes the output boxes the last level set laid out; a
+    # second call (or one after a coarsening) has none and is refused
+    _say("redistancing order n=27 ppg=12 border=3 seed=11")
+    ctx, orc, p = make_pair(27, 12, 3, 11)
+    step_both(ctx, orc)
+    with pytest.raises(G.GGError, match="needs the output boxes"):
+        ctx.redistancing()
+    new = int((p.ngridpoints - 2 * p.grid_blowup) * 0.9)
+    ctx.coarsen_grid(new, 0.2 / (new * new))
+    with pytest.raises(G.GGError, match="needs the output boxes"):
+        ctx.redistancing()
+    ctx.timestep(1)  # the normal order still runs
+    ctx.synchronize()
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", [G.FP64_PLAIN, G.FP32_BATCHED])
+def test_upload_phi_round_trip(precision):
+    _say("upload phi n=27 ppg=12 border=3 seed=11 precision=%d" % precision)
+    ctx, orc, p = make_pair(27, 12, 3, 11, precision=precision)
+    rng = np.random.default_rng(5)
+    for g in (1, 7, 27):
+        b, phi = ctx.phi(g, 1)
+        new = rng.uniform(-p.delta, p.delta, phi.size)
+        ctx.upload_phi(g, 1, new)
+        _, back = ctx.phi(g, 1)
+        want = new.astype(np.float32).astype(np.float64) if precision == G.FP32_BATCHED else new
+        assert np.array_equal(back, want)
+        with pytest.raises(G.GGError, match="element count"):
+            ctx.upload_phi(g, 1, new[:-1])
+    # the uploaded fields are the ones the next step convolves: a context that
+    # received the other's fields continues bit-identically
+    ctx2, _, _ = make_pair(27, 12, 3, 11, precision=precision)
+    for g in range(1, 28):
+        if ctx.stats()[0][g] == G.ALIVE:
+            ctx2.upload_phi(g, 1, ctx.phi(g, 1)[1])
+    ctx.timestep(2)
+    ctx2.timestep(2)
+    for g in range(1, 28):
+        if ctx.stats()[0][g] == G.ALIVE:
+            assert np.array_equal(ctx.phi(g, 1)[1], ctx2.phi(g, 1)[1]), g
+    ctx.close()
+    ctx2.close()

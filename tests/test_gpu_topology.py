@@ -87,3 +87,52 @@ def test_hull_topology_of_a_grid_coarsening():
     step_both(ctx, orc)
     assert _compare(ctx, orc, "step after coarsening") > 0
     ctx.close()
+
+
+def test_hull_topology_large_boxes():
+    # box sides 39-82: the hull's n <= 64 and n <= 96 instantiations (the FP64
+    # n <= 96 class adds element sums straight to the grain's table)
+    _say("hull topology large boxes n=27 ppg=40 border=3 seed=11")
+    ctx, orc, p = make_pair(27, 40, 3, 11)
+    ctx.enable_hull_topology(True)
+    for s in range(3):
+        step_both(ctx, orc)
+        assert _compare(ctx, orc, "step %d" % s) > 0
+    ctx.close()
+
+
+@pytest.mark.parametrize("n_grains,ppg,border,seed", [(64, 10, 3, 5), (27, 40, 3, 11)])
+def test_hull_topology_fp32_matches_oracle_on_the_same_field(n_grains, ppg, border, seed):
+    # The FP32 path's topology against the oracle's topology routines run on
+    # the FP32 field the hull read (its values are exact doubles): the same
+    # inside/outside corners, so ids, their order and the junction counts
+    # must be equal; element positions differ by the FP32 interpolation factor
+    # only (atol 1e-4 grid units).
+    from oracle import oracle as O
+    _say("hull topology FP32 n=%d ppg=%d border=%d seed=%d" % (n_grains, ppg, border, seed))
+    ctx, _, p = make_pair(n_grains, ppg, border, seed, precision=G.FP32_BATCHED)
+    ctx.enable_hull_topology(True)
+    checked = 0
+    for s in range(3):
+        for name in ("switch", "convolution", "switch", "update_lists", "comparison", "switch", "level_set"):
+            getattr(ctx, name)()
+        st, _, _ = ctx.stats()
+        junc = ctx.hull_junction_counts()
+        for g in range(1, n_grains + 1):
+            if st[g] != G.ALIVE:
+                continue
+            b, phi = ctx.phi(g, 0)
+            n = int(b[3] - b[0])
+            ib, idl = ctx.idlocal(g)
+            args = (phi.reshape(n, n, n), b[:3], ib, idl, g)
+            want, want_j = O.hull_neighbours_box(*args, junctions=True)
+            assert np.array_equal(ctx.hull_neighbours(g), want), (s, g)
+            assert np.array_equal(junc[g], want_j), (s, g, junc[g], want_j)
+            eg, eo = ctx.hull_elements(g), O.hull_elements_box(*args)
+            assert [(e[0], e[1]) for e in eg] == [(e[0], e[1]) for e in eo], (s, g)
+            for a, c in zip(eg, eo):
+                np.testing.assert_allclose(a[2], c[2], rtol=0, atol=1e-4, err_msg="step %d grain %d" % (s, g))
+            checked += 1
+        ctx.redistancing()
+    assert checked > 0
+    ctx.close()
