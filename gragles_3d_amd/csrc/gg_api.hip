@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -12,6 +13,17 @@
 #include "gg_internal.h"
 
 using namespace gg;
+
+// Scratch of gg_init_distance_noalloc: the ID field of a run of grid layers,
+// staged from host memory.  4 Mi int32 entries (16 MiB) per GPU, held until the
+// process ends; a grid needs ngridpoints^2 <= kNoAllocScratch (ngridpoints <=
+// 2048).  Host code uses the device address from hipGetSymbolAddress, taken
+// once per device when the first context on it is created.
+constexpr int64_t kNoAllocScratch = int64_t(1) << 22;
+static __device__ __align__(256) int32_t scratch[kNoAllocScratch];
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu[kScratchDevices];          // one call at a time per GPU
+static int32_t* g_scratch_ptr[kScratchDevices] = {};      // device address of scratch, per GPU
 
 struct gg_ctx {
   gg_params p{};
@@ -467,7 +479,7 @@ int gg_create(const gg_params* params, gg_ctx** out) {
   *out = nullptr;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return GG_ERR_DEVICE;
-  if (params->device < 0 || params->device >= ndev) return GG_ERR_ARG;
+  if (params->device < 0 || params->device >= ndev || params->device >= kScratchDevices) return GG_ERR_ARG;
   if (params->n_grains < 1 || params->n_grains > kMaxGrains || params->ngridpoints < 2 || params->grid_blowup < 0 ||
       (params->precision != GG_FP64_PLAIN && params->precision != GG_FP32_BATCHED))
     return GG_ERR_ARG;
@@ -491,6 +503,18 @@ int gg_create(const gg_params* params, gg_ctx** out) {
       hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
     gg_destroy(c);
     return GG_ERR_HIP;
+  }
+  {
+    // this GPU's copy of the no-allocation scratch (see kNoAllocScratch)
+    std::lock_guard<std::mutex> lk(g_scratch_mu[params->device]);
+    if (!g_scratch_ptr[params->device]) {
+      void* ptr = nullptr;
+      if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+        gg_destroy(c);
+        return GG_ERR_HIP;
+      }
+      g_scratch_ptr[params->device] = (int32_t*)ptr;
+    }
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, params->device) == hipSuccess) c->grid = std::max(256, prop.multiProcessorCount * 8);
@@ -684,7 +708,7 @@ int gg_init_distance(gg_ctx* c, const int32_t* idfield) {
     hipFree(d_id);
     return fail(c, GG_ERR_HIP, std::string("hipMemcpy idfield: ") + hipGetErrorString(e));
   }
-  launch_init_phi(launch_of(c), c->sp, c->P, din(c), d_id, c->fp32);
+  launch_init_phi(launch_of(c), c->sp, c->P, din(c), d_id, c->fp32, 0, (int)L);
   // the redistancing runs over the output role's plan (same boxes here)
   plan_into(c, dout(c), c->P2);
   c->p2_valid = true;
@@ -692,6 +716,41 @@ int gg_init_distance(gg_ctx* c, const int32_t* idfield) {
   e = hipStreamSynchronize(c->stream);
   hipFree(d_id);
   if (e != hipSuccess) return fail(c, GG_ERR_HIP, std::string("init distance: ") + hipGetErrorString(e));
+  return check_device_error(c);
+}
+
+int gg_init_distance_noalloc(gg_ctx* c, const int32_t* idfield) {
+  if (!c || !idfield) return GG_ERR_ARG;
+  int rc = require_boxes(c);
+  if (rc) return rc;
+  hipSetDevice(c->p.device);
+  int32_t* const scr = g_scratch_ptr[c->p.device];
+  if (!scr) return fail(c, GG_ERR_STATE, "gg_init_distance_noalloc: no scratch on this device");
+  const int64_t L = c->p.ngridpoints, layer = L * L;
+  if (layer > kNoAllocScratch)
+    return fail(c, GG_ERR_CAPACITY, "gg_init_distance_noalloc: ngridpoints^2 above the scratch's 4194304 entries");
+  std::lock_guard<std::mutex> lk(g_scratch_mu[c->p.device]);
+  const bool timing = c->timing;
+  c->timing = false;  // stage timing would create events
+  // slabs of whole grid layers: copy one into the scratch, write the initial
+  // field of the box layers it holds; the next copy is ordered after that
+  // kernel on the stream
+  const int64_t per = kNoAllocScratch / layer;
+  hipError_t e = hipSuccess;
+  for (int64_t k0 = 0; k0 < L && e == hipSuccess; k0 += per) {
+    const int64_t k1 = std::min(L, k0 + per);
+    e = hipMemcpyAsync(scr, idfield + k0 * layer, sizeof(int32_t) * (size_t)((k1 - k0) * layer),
+                       hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) launch_init_phi(launch_of(c), c->sp, c->P, din(c), scr, c->fp32, (int)k0, (int)k1);
+  }
+  if (e == hipSuccess) {
+    plan_into(c, dout(c), c->P2);
+    c->p2_valid = true;
+    stage_redistancing(c);
+    e = hipStreamSynchronize(c->stream);
+  }
+  c->timing = timing;
+  if (e != hipSuccess) return fail(c, GG_ERR_HIP, std::string("init distance (no allocation): ") + hipGetErrorString(e));
   return check_device_error(c);
 }
 

@@ -279,8 +279,9 @@ inline int tile_grid(const Launch& L, int per_cu, int cls) {
 void launch_plan(const Launch& L, const SimParams& sp, const int32_t* state, const int32_t* bounds,
                  int64_t* off_out, int64_t* spec_off, const PlanDev& P, int64_t arena_cap,
                  int64_t spec_cap, int32_t* err, int64_t* voxels);
+// Grid layers [k_lo, k_hi) of the initial field; idfield holds those layers.
 void launch_init_phi(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din,
-                     const int32_t* idfield, bool fp32);
+                     const int32_t* idfield, bool fp32, int k_lo, int k_hi);
 // part 0: (x,y) forward DFT | x,y circulant; part 1: z; part 2: (y,x) inverse DFT (FP64 only)
 void launch_conv(const Launch& L, int part, const SimParams& sp, const PlanDev& P, const DescDev& din,
                  const DescDev& dout, const TablesDev& tb, const int64_t* spec_off, void* spec, bool fp32);

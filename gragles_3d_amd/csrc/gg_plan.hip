@@ -164,10 +164,12 @@ __global__ __launch_bounds__(256) void k_tilemap(SimParams sp, PlanDev P) {
 }
 
 // calculateDistanceFunction (src/box.cpp:374-397): +h inside the grain's
-// labelled voxels, -h elsewhere and on the border shell.
+// labelled voxels, -h elsewhere and on the border shell.  Only the grid layers
+// k in [k_lo, k_hi) are written; idfield holds those layers (idfield[0] is
+// voxel (0, 0, k_lo)).
 template <class T>
 __global__ __launch_bounds__(256) void k_init_phi(SimParams sp, PlanDev P, DescDev din,
-                                                  const int32_t* __restrict__ idfield) {
+                                                  const int32_t* __restrict__ idfield, int k_lo, int k_hi) {
   const int total = plan_total(P, kQLayers);
   T* arena = (T*)din.arena;
   const int L = sp.ngp, lo = sp.gb, hi = sp.ngp - sp.gb - 1;
@@ -177,13 +179,14 @@ __global__ __launch_bounds__(256) void k_init_phi(SimParams sp, PlanDev P, DescD
     const int32_t* b = din.bounds + 6 * g;
     const int n = b[3] - b[0];
     const int k = b[2] + kl;
+    if (k < k_lo || k >= k_hi) continue;
     T* dst = arena + din.off[g] + (int64_t)kl * n * n;
     for (int v = threadIdx.x; v < n * n; v += blockDim.x) {
       int i = b[1] + v / n, j = b[0] + v % n;
       T val;
       if (i < lo || i > hi || j < lo || j > hi || k < lo || k > hi)
         val = (T)(-sp.h);
-      else if (idfield[((int64_t)k * L + i) * L + j] == g)
+      else if (idfield[((int64_t)(k - k_lo) * L + i) * L + j] == g)
         val = (T)sp.h;
       else
         val = (T)(-sp.h);
@@ -254,11 +257,11 @@ void launch_plan(const Launch& L, const SimParams& sp, const int32_t* state, con
 }
 
 void launch_init_phi(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din,
-                     const int32_t* idfield, bool fp32) {
+                     const int32_t* idfield, bool fp32, int k_lo, int k_hi) {
   if (fp32)
-    hipLaunchKernelGGL(k_init_phi<float>, dim3(L.grid), dim3(256), 0, L.stream, sp, P, din, idfield);
+    hipLaunchKernelGGL(k_init_phi<float>, dim3(L.grid), dim3(256), 0, L.stream, sp, P, din, idfield, k_lo, k_hi);
   else
-    hipLaunchKernelGGL(k_init_phi<double>, dim3(L.grid), dim3(256), 0, L.stream, sp, P, din, idfield);
+    hipLaunchKernelGGL(k_init_phi<double>, dim3(L.grid), dim3(256), 0, L.stream, sp, P, din, idfield, k_lo, k_hi);
 }
 
 void launch_idl_resize(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& dout,

@@ -61,6 +61,7 @@ SIGNATURES = {
     "gg_last_error": (ctypes.c_char_p, [_vp]),
     "gg_set_boxes": (ctypes.c_int, [_vp, _i32p, _u8p]),
     "gg_init_distance": (ctypes.c_int, [_vp, _i32p]),
+    "gg_init_distance_noalloc": (ctypes.c_int, [_vp, _i32p]),
     "gg_set_neighbour_lists": (ctypes.c_int, [_vp, _i32p, _i32p]),
     "gg_switch_buffers": (ctypes.c_int, [_vp]),
     "gg_convolution": (ctypes.c_int, [_vp]),
@@ -263,6 +264,11 @@ class LevelSetContext:
     def init_distance(self, idfield):
         f = np.ascontiguousarray(idfield, dtype=np.int32)
         self._chk(lib().gg_init_distance(self.h, _p(f, ctypes.c_int32)))
+
+    def init_distance_noalloc(self, idfield):
+        """init_distance through the library's static scratch: no device allocation."""
+        f = np.ascontiguousarray(idfield, dtype=np.int32)
+        self._chk(lib().gg_init_distance_noalloc(self.h, _p(f, ctypes.c_int32)))
 
     def set_neighbour_lists(self, offsets, ids):
         o = np.ascontiguousarray(offsets, dtype=np.int32)

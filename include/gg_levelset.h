@@ -91,6 +91,17 @@ int gg_set_boxes(gg_ctx* ctx, const int32_t* bounds, const uint8_t* exists);
  * driven by grainhdl::distanceInitialisation (src/grainhdl.cpp:580-597). */
 int gg_init_distance(gg_ctx* ctx, const int32_t* idfield);
 
+/* gg_init_distance without any device allocation (it has no counterpart in the
+ * reference): for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The ID
+ * field is staged from idfield in runs of grid layers through a 16 MiB device
+ * array per GPU that the library holds until the process ends; the result is
+ * gg_init_distance's.  Needs ngridpoints^2 <= 4194304 (ngridpoints <= 2048),
+ * else GG_ERR_CAPACITY.  Every context on one GPU shares that GPU's array,
+ * which keeps the last call's contents: calls on one GPU are serialised by a
+ * host lock (each returns synchronised), so they never overlap. */
+int gg_init_distance_noalloc(gg_ctx* ctx, const int32_t* idfield);
+
 /* Initial m_secondOrderNeighbours of every grain (grainhdl::find_neighbors,
  * src/grainhdl.cpp:1029-1054) as CSR: offsets[0..n_grains+1], ids. */
 int gg_set_neighbour_lists(gg_ctx* ctx, const int32_t* offsets, const int32_t* ids);
