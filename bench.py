@@ -12,11 +12,25 @@ reported beside it.
 value = voxel updates per second = sum over steps of sum_boxes n_b^3 / wall
 time of the timed steps, with every input resident in HBM.
 
-Output: one JSON line on stdout.
+Order of the run: set-up; an event-timed pass of --kernel-steps eager steps
+from the initial state (every kernel segment bracketed by HIP events on the
+library's stream, all launches on that one stream, so each kernel runs alone:
+the per-kernel roofline rows come from it, and a profiler run of
+`--steps 0 --warmup 0 --no-fp64 --no-cpu --no-extras --no-accuracy` sees
+exactly these dispatches); warm-up and timed graph steps; the FP64 path; the
+200-step accuracy leg; the CPU baseline.
+
+Output: one JSON line on stdout.  `--summarize-rocprof DIR` instead prints the
+markdown tables of a rocprofv3 output directory (kernel stats, and FETCH_SIZE /
+WRITE_SIZE passes scaled by the copy calibration).
 """
 import argparse
+import csv
+import glob
 import json
 import os
+import platform
+import subprocess
 import sys
 import time
 
@@ -29,33 +43,23 @@ import gragles_3d_amd as G  # noqa: E402
 
 METRIC = "grain-box voxel updates/sec per timestep, FP32-batched vs FP64-plain; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+CLASS_NMAX = (32, 48, 64, 96)  # box-size classes of the work plan, one launch each per stage
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_fp32_8000grains.md")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-CLASS_NMAX = (32, 48, 64, 96)  # box-size classes of the work plan, one comparison launch each
-
-
-def size_class(n):
-    for c, m in enumerate(CLASS_NMAX):
-        if n <= m:
-            return c
-    return len(CLASS_NMAX) - 1
-
-
-def kbar_and_voxels(ctx):
-    """Exact mean number of comparison-list boxes covering a voxel, K = sum_i sum_{j in list(i)}
-    |B_i cap B_j| / sum_i |B_i|, over the grains alive now, from their box bounds; overall and
-    per box-size class (numerator and voxel count per class)."""
-    st, _, _ = ctx.stats()
-    alive = st == G.ALIVE
-    B = ctx.all_bounds(1).astype(np.int64)
-    offs, ids = ctx.all_neighbour_lists(0)
+def kbar_and_voxels(state, B, offs, ids):
+    """Exact mean number of comparison-list boxes covering a voxel,
+    K = sum_i sum_{j in list(i)} |B_i cap B_j| / sum_i |B_i|, over the alive grains, from box
+    bounds B and lists (CSR offs, ids); overall and per box-size class (numerator and voxel count
+    per class)."""
+    alive = state == G.ALIVE
+    B = B.astype(np.int64)
     n = B[:, 3] - B[:, 0]
-    cls = np.searchsorted(np.array(CLASS_NMAX), n)
-    cls = np.minimum(cls, len(CLASS_NMAX) - 1)
+    cls = np.minimum(np.searchsorted(np.array(CLASS_NMAX), n), len(CLASS_NMAX) - 1)
     own = np.repeat(np.arange(len(offs) - 1), np.diff(offs))
     keep = alive[own] & alive[ids]
     own, nb = own[keep], ids[keep]
@@ -68,12 +72,16 @@ def kbar_and_voxels(ctx):
     return num.sum() / max(den.sum(), 1), num, den
 
 
-def run_path(precision, inputs, p, steps, warmup, use_graph=True):
+def new_context(precision, inputs, p):
     ctx = G.LevelSetContext(p, precision=precision)
     ctx.set_boxes(inputs["bounds"], inputs["exists"])
     ctx.init_distance(inputs["idfield"])
     ctx.set_neighbour_lists(inputs["offsets"], inputs["ids"])
     ctx.synchronize()
+    return ctx
+
+
+def timed_steps(ctx, steps, warmup, use_graph=True):
     ctx.timestep(warmup, use_graph=use_graph)
     ctx.synchronize()
     vox_before = ctx.box_voxels()
@@ -81,26 +89,434 @@ def run_path(precision, inputs, p, steps, warmup, use_graph=True):
     ctx.timestep(steps, use_graph=use_graph)
     ctx.synchronize()
     t1 = time.perf_counter()
-    vox_after = ctx.box_voxels()
-    return ctx, t1 - t0, vox_before, vox_after
+    return t1 - t0, 0.5 * (vox_before + ctx.box_voxels())
 
 
-def kernel_pass(ctx, steps):
-    """Eager steps with every kernel segment bracketed by HIP events on the library's stream."""
+def kernel_pass(ctx, steps, fp32):
+    """`steps` eager steps, every kernel segment bracketed by HIP events on the library's stream
+    (all launches then go to that stream: each kernel runs alone).  Per step the state before it
+    gives the algorithmic bytes of each stage.  Returns the per-segment times, the per-stage bytes
+    and the per-class comparison bytes, summed over the steps."""
+    esz = 4 if fp32 else 8
     ctx.enable_stage_timing(True)
     ctx.stage_times(reset=True)
-    ctx.timestep(steps, use_graph=False)
+    by = {"convolution": 0.0, "comparison": 0.0, "hull": 0.0, "redistancing": 0.0}
+    cls_bytes = np.zeros(len(CLASS_NMAX))
+    cls_vox = np.zeros(len(CLASS_NMAX))
+    cls_num = np.zeros(len(CLASS_NMAX))
+    for _ in range(steps):
+        # The convolution, comparison and hull of a step run on the boxes it starts with, the
+        # redistancing on the new ones (nearly the same volume); the comparison reads the boxes of
+        # its comparison list, which is the neighbour list after the step.
+        state = ctx.stats()[0]
+        B = ctx.all_bounds(1)
+        ctx.timestep(1, use_graph=False)
+        offs, ids = ctx.all_neighbour_lists(0)
+        _, num, den = kbar_and_voxels(state, B, offs, ids)
+        vox = den.sum()
+        by["convolution"] += 2 * esz * vox
+        by["hull"] += esz * vox
+        by["redistancing"] += 2 * esz * vox
+        # K neighbour values and the own value read, the value and the IDLocal id written
+        cb = esz * num + (2 * esz + 4) * den
+        by["comparison"] += cb.sum()
+        cls_bytes += cb
+        cls_vox += den
+        cls_num += num
     ctx.synchronize()
     seg = ctx.stage_times(reset=True)
     ctx.enable_stage_timing(False)
-    return seg
+    return seg, by, cls_bytes, cls_vox, cls_num
+
+
+STAGE_SEGMENTS = {
+    "convolution": ("conv_xy", "conv_z", "conv_xy_inv"),
+    "comparison": ("comparison_n32", "comparison_n48", "comparison_n64", "comparison_n96"),
+    "hull": ("hull",),
+    "redistancing": ("redist_xy", "redist_z"),
+}
+
+
+def roofline_rows(seg, by, steps):
+    rows = {}
+    for st, segs in STAGE_SEGMENTS.items():
+        ms = sum(seg[s][0] for s in segs) / max(steps, 1)
+        b = by[st] / max(steps, 1)
+        rows[st] = {"ms_per_step": ms, "algorithmic_bytes_per_step": b,
+                    "GBs": b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0,
+                    "frac_of_hbm_peak": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS if ms > 0 else 0.0}
+    return rows
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed profiler summary (FETCH_SIZE and
+    WRITE_SIZE passes of rocprofv3 on the kernel pass of this workload, scaled by a copy of known
+    size), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch %s:" % kernel):
+                    return float(line.rsplit(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """Device-to-device copies of a known size (the runtime's blit kernel), so that HBM byte
+    counters read under a profiler can be scaled: a copy of B bytes must read and write B bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b),
+                                                                                 ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    for _ in range(reps):
+        hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log("copy calibration: %d device-to-device copies of %d bytes" % (reps, nbytes))
+
+
+def cpu_description():
+    """CPU model and physical core count from lscpu, and the CPUs this process may use."""
+    model, sockets, cores_per_socket = platform.processor() or "unknown", 1, None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            v = v.strip()
+            if k.strip() == "Model name":
+                model = v
+            elif k.strip() == "Socket(s)":
+                sockets = int(v)
+            elif k.strip() == "Core(s) per socket":
+                cores_per_socket = int(v)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    physical = sockets * cores_per_socket if cores_per_socket else None
+    usable = len(os.sched_getaffinity(0))
+    return {"model": model, "physical_cores": physical, "usable_cpus": usable}
+
+
+def cpu_baseline(inputs, p, threads, full_steps, sample_stride):
+    """The FP64 CPU oracle (g++ -O3 -fopenmp) on the same workload: `full_steps` whole timesteps
+    with `threads` OpenMP threads, and one timestep with 1 thread over the grains whose id is a
+    multiple of sample_stride (a bounded sample).  OMP_PROC_BIND / OMP_PLACES are those of the
+    environment (set before the process starts)."""
+    from oracle import oracle as O
+
+    def sim():
+        orc = O.OracleSim(p.ngridpoints, p.grid_blowup, p.conv_domain_size, p.h, p.delta, p.dt, p.n_grains)
+        orc.set_boxes(inputs["bounds"], inputs["exists"])
+        orc.init_distance(inputs["idfield"])
+        orc.set_lists(inputs["offsets"], inputs["ids"])
+        return orc
+
+    def box_voxels(orc, ids):
+        v = 0
+        for g in ids:
+            if orc.state(g) == O.ALIVE:
+                b, _ = orc.bounds(g, 1)
+                v += int(b[3] - b[0]) ** 3
+        return v
+
+    O.set_sample_stride(1)
+    O.set_threads(threads)
+    orc = sim()
+    vox = 0
+    t0 = time.perf_counter()
+    for _ in range(full_steps):
+        vox += box_voxels(orc, range(1, p.n_grains + 1))
+        orc.timestep(1)
+    t_full = time.perf_counter() - t0
+    orc.close()
+    O.set_threads(1)
+    orc = sim()
+    O.set_sample_stride(sample_stride)
+    sample_ids = range(sample_stride, p.n_grains + 1, sample_stride)
+    vox1 = box_voxels(orc, sample_ids)
+    t0 = time.perf_counter()
+    orc.timestep(1)
+    t_one = time.perf_counter() - t0
+    O.set_sample_stride(1)
+    orc.close()
+    return vox / t_full, t_full, vox, vox1 / t_one, t_one, vox1, len(sample_ids)
+
+
+# ---- rocprofv3 output summaries ------------------------------------------------
+def _rows(path):
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def short_name(name):
+    """k_comparison<float, 48> from the demangled 'void gg::(anonymous namespace)::k_comparison<float, 48>(...)'."""
+    n = name.replace("void ", "", 1) if name.startswith("void ") else name
+    n = n.replace("gg::(anonymous namespace)::", "")
+    depth = 0
+    for i, ch in enumerate(n):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            return n[:i]
+    return n
+
+
+def summarize_rocprof(d):
+    """Markdown tables of a rocprofv3 output directory: kernel stats (from *kernel_stats.csv) and,
+    if present, FETCH_SIZE / WRITE_SIZE counter passes (*counter_collection.csv) per kernel,
+    scaled by the device-to-device copy kernel of the --calibrate-copy run (1 GiB per copy)."""
+    stats = sorted(glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True))
+    for s in stats:
+        print("kernel stats (%s)\n" % os.path.relpath(s, d))
+        print("| kernel | calls | avg (us) | min (us) | max (us) | total (ms) |")
+        print("|---|---|---|---|---|---|")
+        for r in sorted(_rows(s), key=lambda r: -float(r["TotalDurationNs"])):
+            print("| %s | %s | %.1f | %.1f | %.1f | %.3f |" % (
+                short_name(r["Name"]), r["Calls"], float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3,
+                float(r["MaxNs"]) / 1e3, float(r["TotalDurationNs"]) / 1e6))
+        print()
+    counters = sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True))
+    per = {}
+    for c in counters:
+        for r in _rows(c):
+            name = short_name(r.get("Kernel_Name") or r.get("Kernel-Name") or r.get("Name"))
+            cn = r.get("Counter_Name") or r.get("Counter-Name")
+            val = float(r.get("Counter_Value") or r.get("Counter-Value") or 0.0)
+            per.setdefault((name, cn), []).append(val)
+    if not per:
+        return
+    cal = {}
+    for (name, cn), vals in per.items():
+        if "copyBuffer" in name:
+            big = [v for v in vals if v > 1e5]
+            if big:
+                cal[cn] = float(np.median(big)) / (1 << 20)  # KiB counted per KiB copied
+    print("calibration (KiB counted per KiB of a 1 GiB device-to-device copy): %s\n" % cal)
+    print("| kernel | dispatches | FETCH_SIZE KiB/dispatch (raw) | read GB (scaled) | "
+          "WRITE_SIZE KiB/dispatch (raw) | write GB (scaled) | traffic GB/dispatch |")
+    print("|---|---|---|---|---|---|---|")
+    names = sorted({n for n, _ in per}, key=lambda n: -np.mean(per.get((n, "FETCH_SIZE"), [0])))
+    for n in names:
+        f = per.get((n, "FETCH_SIZE"), [])
+        w = per.get((n, "WRITE_SIZE"), [])
+        fr = np.mean(f) if f else 0.0
+        wr = np.mean(w) if w else 0.0
+        rb = fr * 1024 / cal.get("FETCH_SIZE", 1.0) / 1e9
+        wb = wr * 1024 / cal.get("WRITE_SIZE", 1.0) / 1e9
+        print("| %s | %d | %.0f | %.3f | %.0f | %.3f | %.3f |" % (n, max(len(f), len(w)), fr, rb, wr, wb, rb + wb))
+        if fr and wr and n.startswith("k_"):
+            print("traffic_bytes_per_launch %s: %.6e" % (n, (rb + wb) * 1e9), file=sys.stderr)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--grains", type=int, default=8000)
+    ap.add_argument("--ppg", type=int, default=20)
+    ap.add_argument("--border", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--kernel-steps", type=int, default=4, help="eager event-timed steps for the roofline rows")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 path")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the 200-step FP32-vs-FP64 accuracy leg")
+    ap.add_argument("--accuracy-steps", type=int, default=200)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run device-to-device copies of 1 GiB (scale for profiler byte counters)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: the CPUs this process may use")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="whole timesteps of the multi-thread CPU baseline")
+    ap.add_argument("--cpu-sample-stride", type=int, default=16,
+                    help="the 1-thread CPU baseline times one step over every n-th grain")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the voxel-container and grid-coarsening legs")
+    ap.add_argument("--summarize-rocprof", metavar="DIR", help="print markdown tables of a rocprofv3 directory")
+    args = ap.parse_args()
+    if args.summarize_rocprof:
+        summarize_rocprof(args.summarize_rocprof)
+        return
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the per-grain timestep runs every grain box on a single MI355X "
+              "(no sharding); --gpus %d is not supported." % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    if args.calibrate_copy:
+        copy_calibration()
+    p = G.derive_params(args.grains, args.ppg, args.border)
+    t = time.perf_counter()
+    idf, bounds, exists = G.generate_voronoi(p, args.seed, True, 0)
+    offs, ids = G.initial_neighbour_lists(p, bounds, exists)
+    log("setup: %d grains, grid %d^3, generator %.2fs" % (args.grains, p.ngridpoints, time.perf_counter() - t))
+    inputs = dict(idfield=idf, bounds=bounds, exists=exists, offsets=offs, ids=ids)
+
+    # ---- FP32 batched: event-timed kernel pass from the initial state, then graph replays ----
+    ctx32 = new_context(G.FP32_BATCHED, inputs, p)
+    seg32, by32, cls_bytes, cls_vox, cls_num = kernel_pass(ctx32, args.kernel_steps, True)
+    ks = max(args.kernel_steps, 1)
+    rows32 = roofline_rows(seg32, by32, ks)
+    wall32, vox_step32 = timed_steps(ctx32, args.steps, args.warmup)
+    value = vox_step32 * args.steps / wall32 if args.steps else 0.0
+    log("fp32: %d steps %.3fs -> %.3f ms/step, %.3e voxel-updates/s (boxes %.3e voxels)" %
+        (args.steps, wall32, 1e3 * wall32 / max(args.steps, 1), value, vox_step32))
+
+    cls_rows = []
+    for c, m in enumerate(CLASS_NMAX):
+        ms, nl = seg32["comparison_n%d" % m]
+        avg_s = ms / max(nl, 1) * 1e-3
+        bts = cls_bytes[c] / ks
+        cls_rows.append({"kernel": "k_comparison<float, %d>" % m, "box_voxels": int(cls_vox[c] / ks),
+                         "kbar": cls_num[c] / cls_vox[c] if cls_vox[c] else 0.0, "bytes_per_launch": bts,
+                         "avg_launch_ms": avg_s * 1e3,
+                         "achieved_GBs": bts / avg_s / 1e9 if avg_s > 0 and bts > 0 else 0.0})
+    dom = max(cls_rows, key=lambda r: r["avg_launch_ms"])
+    cmp_red_ms = rows32["comparison"]["ms_per_step"] + rows32["redistancing"]["ms_per_step"]
+    cmp_red_b = rows32["comparison"]["algorithmic_bytes_per_step"] + rows32["redistancing"][
+        "algorithmic_bytes_per_step"]
+    seg_ms_per_step = {k: round(v[0] / ks, 4) for k, v in seg32.items() if v[1]}
+
+    # ---- FP64 plain, same schedule ------------------------------------------------------------
+    fp64 = None
+    ctx64 = None
+    if not args.no_fp64:
+        ctx64 = new_context(G.FP64_PLAIN, inputs, p)
+        seg64, by64, _, _, _ = kernel_pass(ctx64, max(1, args.kernel_steps // 2), False)
+        wall64, vox_step64 = timed_steps(ctx64, args.steps, args.warmup)
+        v64 = vox_step64 * args.steps / wall64 if args.steps else 0.0
+        log("fp64: %d steps %.3fs -> %.3f ms/step, %.3e voxel-updates/s" % (
+            args.steps, wall64, 1e3 * wall64 / max(args.steps, 1), v64))
+        k64 = max(1, args.kernel_steps // 2)
+        fp64 = {"value": v64, "ms_per_step": 1e3 * wall64 / max(args.steps, 1),
+                "segment_ms_per_step": {k: round(v[0] / k64, 4) for k, v in seg64.items() if v[1]},
+                "stages": roofline_rows(seg64, by64, k64)}
+        ctx64.close()
+
+    container = coarsen = None
+    if not args.no_extras:
+        container = container_leg(ctx32)
+        log("voxel container %d^3: kernel %.3f ms (%.0f GB/s), call with copy-out %.1f ms" % (
+            container["size"], container["kernel_ms"], container["kernel_GBs"], container["call_ms"]))
+        coarsen = coarsening_leg(ctx32, p)
+        log("coarsening %d -> %d: call %.2f ms, resampling %.3f ms (%.0f GB/s)" % (
+            coarsen["grid_from"], coarsen["grid_to"], coarsen["call_ms"], coarsen["resample_ms"],
+            coarsen["resample_GBs"]))
+    ctx32.close()
+
+    # ---- accuracy: FP32 beside FP64 (bit-identical to the oracle) over 200 steps ---------------
+    accuracy = None
+    if not args.no_accuracy and not args.no_fp64:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from fp32_error import phi_volume_errors, trajectories, trajectory_error
+        a32 = new_context(G.FP32_BATCHED, inputs, p)
+        a64 = new_context(G.FP64_PLAIN, inputs, p)
+        tr = trajectories(a32, a64, args.accuracy_steps, every=10)
+        c32, c64 = a32.count_history(), a64.count_history()
+        e = phi_volume_errors(a32, a64, p.delta, grains=np.arange(1, p.n_grains + 1, 16))
+        e.update(trajectory_error(c32, c64))
+        e["grains_sampled_for_phi"] = "every 16th grain id"
+        rel = np.abs(np.array(tr["mean_volume_fp32"]) / np.array(tr["mean_volume_fp64"]) - 1)
+        accuracy = {"steps": args.accuracy_steps, "reference": "FP64 GPU path (bit-identical to the FP64 oracle)",
+                    "after_last_step": e, "count_fp32_every_10": c32[9::10].tolist(),
+                    "count_fp64_every_10": c64[9::10].tolist(),
+                    "mean_radius_fp32": tr["mean_radius_fp32"], "mean_radius_fp64": tr["mean_radius_fp64"],
+                    "mean_volume_rel_err_max": float(rel.max()),
+                    "volume_rel_err_max_every_10": tr["volume_rel_err_max"],
+                    "volume_rel_err_median_every_10": tr["volume_rel_err_median"],
+                    "grains_above_1e-3_every_10": tr["grains_above_1e-3"],
+                    "largest_voxels_above_1e-3_every_10": tr["largest_voxels_above_1e-3"],
+                    "volume_rel_err_max_from_100_voxels_every_10": tr["volume_rel_err_max_from_100_voxels"]}
+        log("accuracy %d steps: counts %d/%d, max count error %.3g, mean volume error %.2e" % (
+            args.accuracy_steps, c32[-1], c64[-1], e["count_rel_err_max"], rel.max()))
+        a32.close()
+        a64.close()
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "voxel-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall32 / max(args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "%d grains, seeded Voronoi (seed %d), %d points/grain, border %d, grid %d^3, "
+                        "FP32 batched path, hipGraph-replayed steps" % (args.grains, args.seed, args.ppg,
+                                                                        args.border, p.ngridpoints),
+            "grains": args.grains, "points_per_grain": args.ppg, "border": args.border,
+            "ngridpoints": p.ngridpoints, "box_voxels_per_step": vox_step32,
+        },
+        "roofline": {
+            "kernel": dom["kernel"],
+            "bound": "hbm",
+            "achieved": dom["achieved_GBs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": dom["achieved_GBs"] / HBM_PEAK_GBS,
+            "traffic": measured_traffic(dom["kernel"]),
+            "bytes_per_launch": dom["bytes_per_launch"],
+            "avg_launch_ms": dom["avg_launch_ms"],
+            "kbar": dom["kbar"],
+            "timing": "HIP events around each launch, %d eager steps from the initial state, one stream" % ks,
+            "traffic_source": "profiles/hbm_traffic_fp32_8000grains.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                              "copy-calibrated)",
+        },
+        "stages_fp32": rows32,
+        "comparison_plus_redistancing": {"ms_per_step": cmp_red_ms, "algorithmic_bytes_per_step": cmp_red_b,
+                                         "GBs": cmp_red_b / (cmp_red_ms * 1e-3) / 1e9 if cmp_red_ms else 0.0,
+                                         "frac_of_hbm_peak": cmp_red_b / (cmp_red_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+                                         if cmp_red_ms else 0.0, "target_frac": 0.6},
+        "comparison_by_class": cls_rows,
+        "segment_ms_per_step_fp32": seg_ms_per_step,
+        "voxel_container_fp32": container,
+        "grid_coarsening_fp32": coarsen,
+    }
+    if fp64 is not None:
+        out["fp64_plain"] = fp64
+        out["fp32_speedup_vs_fp64"] = value / fp64["value"] if fp64["value"] else None
+    if accuracy is not None:
+        out["fp32_accuracy_vs_fp64"] = accuracy
+
+    if not args.no_cpu:
+        try:
+            cpu = cpu_description()
+            threads = args.cpu_threads or min(cpu["usable_cpus"], int(os.environ.get("OMP_NUM_THREADS", "64")))
+            v_cpu, t_full, vox_full, v1, t1, vox1, nsample = cpu_baseline(inputs, p, threads, args.cpu_steps,
+                                                                         args.cpu_sample_stride)
+            out["cpu_baseline"] = {
+                "value": v_cpu, "unit": "voxel-updates/s", "cores": threads, "kind": "port",
+                "sample": "FP64 CPU oracle (g++ -O3 -fopenmp -ffp-contract=off; explicit DFT in place of the "
+                          "reference's MKL/FFTW, hull topology every step as the reference), %d whole timesteps "
+                          "of the same %d-grain workload from its initial state (%.3e box voxels) in %.1f s with "
+                          "%d threads" % (args.cpu_steps, args.grains, vox_full, t_full, threads),
+                "cpu_model": cpu["model"], "physical_cores": cpu["physical_cores"],
+                "usable_cpus": cpu["usable_cpus"],
+                "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "omp_places": os.environ.get("OMP_PLACES"),
+                "one_thread": {"value": v1, "unit": "voxel-updates/s", "seconds": t1, "box_voxels": vox1,
+                               "sample": "one timestep over the %d grains whose id is a multiple of %d" % (
+                                   nsample, args.cpu_sample_stride)},
+            }
+            log("cpu oracle: %.1fs for %d steps (%d threads) -> %.3e; 1 thread sample %.1fs -> %.3e voxel-updates/s"
+                % (t_full, args.cpu_steps, threads, v_cpu, t1, v1))
+        except Exception as e:  # the checker may be absent on a product install
+            log("cpu baseline skipped: %s" % e)
+            out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
 
 
 def container_leg(ctx):
-    """One voxel container (saveNetworkAsVoxelContainer's id grid) of the
-    context's state: the kernel clears size^3 ids and scatters every alive
-    grain's phi >= 0 voxels; algorithmic bytes = 4 B per box voxel read + 4 B
-    per container voxel cleared and written once."""
+    """One voxel container (saveNetworkAsVoxelContainer's id grid) of the context's state: the
+    kernel clears size^3 ids and scatters every alive grain's phi >= 0 voxels; algorithmic bytes =
+    4 B per box voxel read + 4 B per container voxel cleared and written once."""
     st, _, _ = ctx.stats()
     b = ctx.all_bounds(1)[st == G.ALIVE]
     box_vox = int(((b[:, 3] - b[:, 0]).astype(np.int64) ** 3).sum())
@@ -119,10 +535,9 @@ def container_leg(ctx):
 
 
 def coarsening_leg(ctx, p, frac=0.9):
-    """One grid coarsening (gridCoarsement's coarsening branch) of the context's
-    state onto a grid frac times the size, then one timestep on it.  The
-    resampling kernel reads every alive grain's old box and writes its new box:
-    4 B per voxel each way (FP32) is its algorithmic traffic."""
+    """One grid coarsening (gridCoarsement's coarsening branch) of the context's state onto a grid
+    frac times the size, then one timestep on it.  The resampling kernel reads every alive grain's
+    old box and writes its new box: 4 B per voxel each way (FP32) is its algorithmic traffic."""
     st, _, _ = ctx.stats()
     alive = st == G.ALIVE
     old_b = ctx.all_bounds(1)[alive]
@@ -147,214 +562,6 @@ def coarsening_leg(ctx, p, frac=0.9):
     return {"grid_from": rds, "grid_to": new, "call_ms": wall * 1e3, "resample_ms": ms,
             "resample_bytes": nbytes, "resample_GBs": nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0,
             "grains_alive_after_next_step": int((st2 == G.ALIVE).sum())}
-
-
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_fp32_8000grains.md")
-
-
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed profiler summary (FETCH_SIZE and
-    WRITE_SIZE passes of rocprofv3 on this workload, scaled by a copy of known size), or None."""
-    try:
-        with open(TRAFFIC_PROFILE) as f:
-            for line in f:
-                if line.startswith("traffic_bytes_per_launch %s:" % kernel):
-                    return float(line.rsplit(":", 1)[1])
-    except OSError:
-        pass
-    return None
-
-
-def copy_calibration(nbytes=1 << 30, reps=3):
-    """Device-to-device copies of a known size (the runtime's blit kernel), so that HBM
-    byte counters read under a profiler can be scaled: a copy of B bytes must read and
-    write B bytes."""
-    import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
-    a, b = ctypes.c_void_p(), ctypes.c_void_p()
-    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
-        raise RuntimeError("hipMalloc failed")
-    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
-    for _ in range(reps):
-        hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
-    hip.hipDeviceSynchronize()
-    hip.hipFree(a)
-    hip.hipFree(b)
-    log("copy calibration: %d device-to-device copies of %d bytes" % (reps, nbytes))
-
-
-def cpu_baseline(inputs, p, nthreads):
-    from oracle import oracle as O
-    O.set_threads(nthreads)
-    orc = O.OracleSim(p.ngridpoints, p.grid_blowup, p.conv_domain_size, p.h, p.delta, p.dt, p.n_grains)
-    orc.set_boxes(inputs["bounds"], inputs["exists"])
-    orc.init_distance(inputs["idfield"])
-    orc.set_lists(inputs["offsets"], inputs["ids"])
-    vox = 0
-    for g in range(1, p.n_grains + 1):
-        if orc.state(g) == O.ALIVE:
-            b, _ = orc.bounds(g, 1)
-            vox += int(b[3] - b[0]) ** 3
-    t0 = time.perf_counter()
-    orc.timestep(1)
-    dt = time.perf_counter() - t0
-    orc.close()
-    return vox / dt, dt, vox
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--grains", type=int, default=8000)
-    ap.add_argument("--ppg", type=int, default=20)
-    ap.add_argument("--border", type=int, default=3)
-    ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 path")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--kernel-steps", type=int, default=4, help="eager event-timed steps for the roofline")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="first run device-to-device copies of 1 GiB (scale for profiler byte counters)")
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
-    ap.add_argument("--no-extras", action="store_true",
-                    help="skip the voxel-container and grid-coarsening legs (they follow the timed steps; "
-                         "profiles of the timestep leave them out so per-kernel means cover one grid)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: the per-grain timestep runs every grain box on a single MI355X "
-              "(no sharding); --gpus %d is not supported." % args.gpus, file=sys.stderr)
-        sys.exit(2)
-
-    if args.calibrate_copy:
-        copy_calibration()
-    p = G.derive_params(args.grains, args.ppg, args.border)
-    t = time.perf_counter()
-    idf, bounds, exists = G.generate_voronoi(p, args.seed, True, 0)
-    offs, ids = G.initial_neighbour_lists(p, bounds, exists)
-    log("setup: %d grains, grid %d^3, generator %.2fs" % (args.grains, p.ngridpoints, time.perf_counter() - t))
-    inputs = dict(idfield=idf, bounds=bounds, exists=exists, offsets=offs, ids=ids)
-
-    # ---- FP32 batched, graph replays ------------------------------------------
-    ctx32, wall32, vb32, va32 = run_path(G.FP32_BATCHED, inputs, p, args.steps, args.warmup)
-    vox_step32 = 0.5 * (vb32 + va32)
-    value = vox_step32 * args.steps / wall32
-    log("fp32: %d steps %.3fs -> %.3f ms/step, %.3e voxel-updates/s (boxes %.3e voxels)" %
-        (args.steps, wall32, 1e3 * wall32 / args.steps, value, vox_step32))
-    kbar, knum, kden = kbar_and_voxels(ctx32)
-
-    # ---- FP64 plain, same steps; then the FP32 error against it ------------------
-    fp64 = None
-    err = None
-    if not args.no_fp64:
-        ctx64, wall64, vb64, va64 = run_path(G.FP64_PLAIN, inputs, p, args.steps, args.warmup)
-        vox_step64 = 0.5 * (vb64 + va64)
-        v64 = vox_step64 * args.steps / wall64
-        log("fp64: %d steps %.3fs -> %.3f ms/step, %.3e voxel-updates/s" % (args.steps, wall64,
-                                                                          1e3 * wall64 / args.steps, v64))
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        from fp32_error import phi_volume_errors, trajectory_error
-        err = phi_volume_errors(ctx32, ctx64, p.delta, grains=np.arange(1, p.n_grains + 1, 16))
-        err.update(trajectory_error(ctx32.count_history(), ctx64.count_history()))
-        err["grains_sampled"] = "every 16th grain id"
-        seg64 = kernel_pass(ctx64, max(1, args.kernel_steps // 2))
-        ctx64.close()
-        fp64 = {"value": v64, "ms_per_step": 1e3 * wall64 / args.steps,
-                "segment_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in seg64.items() if v[1]}}
-
-    seg32 = kernel_pass(ctx32, args.kernel_steps)
-    counts32 = ctx32.count_history()
-    container = coarsen = None
-    if not args.no_extras:
-        container = container_leg(ctx32)
-        log("voxel container %d^3: kernel %.3f ms (%.0f GB/s), call with copy-out %.1f ms" % (
-            container["size"], container["kernel_ms"], container["kernel_GBs"], container["call_ms"]))
-        coarsen = coarsening_leg(ctx32, p)
-        log("coarsening %d -> %d: call %.2f ms, resampling %.3f ms (%.0f GB/s)" % (
-            coarsen["grid_from"], coarsen["grid_to"], coarsen["call_ms"], coarsen["resample_ms"],
-            coarsen["resample_GBs"]))
-    ctx32.close()
-
-    # Roofline of the comparison kernel (neighbour max + wall + set comparison), one launch per
-    # box-size class: per voxel it reads K neighbour values and its own value (4 B each) and its
-    # previous IDLocal (4 B), and writes the new value and IDLocal (8 B) -> (4K + 12) B per voxel.
-    # The dominant kernel is the class launch with the largest event-timed duration.
-    cls_rows = []
-    for c, m in enumerate(CLASS_NMAX):
-        ms, nl = seg32["comparison_n%d" % m]
-        avg_s = ms / max(nl, 1) * 1e-3
-        kb = knum[c] / kden[c] if kden[c] else 0.0
-        bts = kden[c] * (4.0 * kb + 12.0)
-        cls_rows.append({"kernel": "k_comparison<float, %d>" % m, "box_voxels": int(kden[c]), "kbar": kb,
-                         "bytes_per_launch": bts, "avg_launch_ms": avg_s * 1e3,
-                         "achieved_GBs": bts / avg_s / 1e9 if avg_s > 0 and bts > 0 else 0.0})
-    dom = max(cls_rows, key=lambda r: r["avg_launch_ms"])
-    achieved = dom["achieved_GBs"]
-    cmp_total_ms = sum(r["avg_launch_ms"] for r in cls_rows)
-    cmp_total_bytes = sum(r["bytes_per_launch"] for r in cls_rows)
-    seg_ms_per_step = {k: round(v[0] / max(v[1], 1), 4) for k, v in seg32.items() if v[1]}
-
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "voxel-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall32 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic",
-        "config": {
-            "workload": "%d grains, seeded Voronoi (seed %d), %d points/grain, border %d, grid %d^3, "
-                        "FP32 batched path, hipGraph-replayed steps" % (args.grains, args.seed, args.ppg,
-                                                                        args.border, p.ngridpoints),
-            "grains": args.grains, "points_per_grain": args.ppg, "border": args.border,
-            "ngridpoints": p.ngridpoints, "box_voxels_per_step": vox_step32, "kbar": kbar,
-        },
-        "roofline": {
-            "kernel": dom["kernel"],
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dom["kernel"]),
-            "traffic_source": "profiles/hbm_traffic_fp32_8000grains.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
-                              "copy-calibrated)",
-            "bytes_per_launch": dom["bytes_per_launch"],
-            "avg_launch_ms": dom["avg_launch_ms"],
-            "kbar": dom["kbar"],
-        },
-        "comparison_by_class": cls_rows,
-        "comparison_all_classes": {"ms_per_step": cmp_total_ms, "bytes_per_step": cmp_total_bytes,
-                                   "GBs": cmp_total_bytes / (cmp_total_ms * 1e-3) / 1e9 if cmp_total_ms else 0.0},
-        "segment_ms_per_step_fp32": seg_ms_per_step,
-        "grain_count_trajectory_fp32": counts32[-3:].tolist(),
-        "voxel_container_fp32": container,
-        "grid_coarsening_fp32": coarsen,
-    }
-
-    if fp64 is not None:
-        out["fp64_plain"] = fp64
-        out["fp32_speedup_vs_fp64"] = value / fp64["value"]
-        out["fp32_error_vs_fp64"] = err
-
-    if not args.no_cpu:
-        try:
-            v_cpu, dt_cpu, vox_cpu = cpu_baseline(inputs, p, args.cpu_threads)
-            out["cpu_baseline"] = {"value": v_cpu, "unit": "voxel-updates/s", "cores": args.cpu_threads,
-                                   "kind": "port",
-                                   "sample": "FP64 CPU oracle (g++ -O3 -fopenmp), one full timestep of the same "
-                                             "%d-grain workload (%.3e box voxels) in %.1fs" % (args.grains, vox_cpu,
-                                                                                              dt_cpu)}
-            log("cpu oracle: %.1fs for one step -> %.3e voxel-updates/s" % (dt_cpu, v_cpu))
-        except Exception as e:  # the checker may be absent on a product install
-            log("cpu baseline skipped: %s" % e)
-            out["cpu_baseline"] = None
-    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":
