@@ -12,8 +12,9 @@ reported beside it.
 value = voxel updates per second = sum over steps of sum_boxes n_b^3 / wall
 time of the timed steps, with every input resident in HBM.
 
-Order of the run: set-up; an event-timed pass of --kernel-steps eager steps
-from the initial state (every kernel segment bracketed by HIP events on the
+Order of the run: set-up; one autotuning timestep (the comparison's tile
+shape per box-size class, gg_autotune); an event-timed pass of --kernel-steps
+eager steps (every kernel segment bracketed by HIP events on the
 library's stream, all launches on that one stream, so each kernel runs alone:
 the per-kernel roofline rows come from it, and a profiler run of
 `--steps 0 --warmup 0 --no-fp64 --no-cpu --no-extras --no-accuracy` sees
@@ -44,6 +45,7 @@ import gragles_3d_amd as G  # noqa: E402
 METRIC = "grain-box voxel updates/sec per timestep, FP32-batched vs FP64-plain; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 CLASS_NMAX = (32, 48, 64, 96)  # box-size classes of the work plan, one launch each per stage
+CMP_SHAPES = ((8, 8), (16, 8), (16, 16), (32, 8))  # comparison tile shapes (layers, gathers in flight), gg_autotune
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_fp32_8000grains.md")
 
 
@@ -334,6 +336,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2, help="whole timesteps of the multi-thread CPU baseline")
     ap.add_argument("--cpu-sample-stride", type=int, default=16,
                     help="the 1-thread CPU baseline times one step over every n-th grain")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default comparison tile shapes instead of autotuning them (one timestep)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the voxel-container and grid-coarsening legs")
     ap.add_argument("--summarize-rocprof", metavar="DIR", help="print markdown tables of a rocprofv3 directory")
@@ -355,8 +359,14 @@ def main():
     log("setup: %d grains, grid %d^3, generator %.2fs" % (args.grains, p.ngridpoints, time.perf_counter() - t))
     inputs = dict(idfield=idf, bounds=bounds, exists=exists, offsets=offs, ids=ids)
 
-    # ---- FP32 batched: event-timed kernel pass from the initial state, then graph replays ----
+    # ---- FP32 batched: autotuned comparison shapes (one timestep), event-timed kernel pass, ----
+    # ---- then graph replays                                                                  ----
     ctx32 = new_context(G.FP32_BATCHED, inputs, p)
+    tune32 = None
+    if not args.no_tune:
+        choice, ms = ctx32.autotune(reps=3)
+        tune32 = {"choice_per_class": choice.tolist(), "ms_per_class_and_shape": np.round(ms, 4).tolist()}
+        log("fp32 comparison shapes per class (n<=32,48,64,96): %s" % choice.tolist())
     seg32, by32, cls_bytes, cls_vox, cls_num = kernel_pass(ctx32, args.kernel_steps, True)
     ks = max(args.kernel_steps, 1)
     rows32 = roofline_rows(seg32, by32, ks)
@@ -366,11 +376,13 @@ def main():
         (args.steps, wall32, 1e3 * wall32 / max(args.steps, 1), value, vox_step32))
 
     cls_rows = []
+    tuning = ctx32.tuning()
     for c, m in enumerate(CLASS_NMAX):
         ms, nl = seg32["comparison_n%d" % m]
         avg_s = ms / max(nl, 1) * 1e-3
         bts = cls_bytes[c] / ks
-        cls_rows.append({"kernel": "k_comparison<float, %d>" % m, "box_voxels": int(cls_vox[c] / ks),
+        zl, qb = CMP_SHAPES[tuning[c]]
+        cls_rows.append({"kernel": "k_cmp_cols<float, %d, %d, %d>" % (m, zl, qb), "box_voxels": int(cls_vox[c] / ks),
                          "kbar": cls_num[c] / cls_vox[c] if cls_vox[c] else 0.0, "bytes_per_launch": bts,
                          "avg_launch_ms": avg_s * 1e3,
                          "achieved_GBs": bts / avg_s / 1e9 if avg_s > 0 and bts > 0 else 0.0})
@@ -385,6 +397,8 @@ def main():
     ctx64 = None
     if not args.no_fp64:
         ctx64 = new_context(G.FP64_PLAIN, inputs, p)
+        if not args.no_tune:
+            ctx64.autotune(reps=2)
         seg64, by64, _, _, _ = kernel_pass(ctx64, max(1, args.kernel_steps // 2), False)
         wall64, vox_step64 = timed_steps(ctx64, args.steps, args.warmup)
         v64 = vox_step64 * args.steps / wall64 if args.steps else 0.0
@@ -476,6 +490,7 @@ def main():
                                          "frac_of_hbm_peak": cmp_red_b / (cmp_red_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
                                          if cmp_red_ms else 0.0, "target_frac": 0.6},
         "comparison_by_class": cls_rows,
+        "comparison_autotune_fp32": tune32,
         "segment_ms_per_step_fp32": seg_ms_per_step,
         "voxel_container_fp32": container,
         "grid_coarsening_fp32": coarsen,

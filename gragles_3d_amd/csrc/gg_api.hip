@@ -58,6 +58,7 @@ struct gg_ctx {
   PlanDev P{};   // plan of the current boxes (input role)
   PlanDev P2{};  // plan of the output boxes, from the level set's first part to the end of the redistancing
   bool p2_valid = false;  // P2 holds the plan of the current output boxes (the redistancing runs over it)
+  int cmp_variant[kClasses] = {kCmpDefault, kCmpDefault, kCmpDefault, kCmpDefault};  // comparison tile shape per class
   int64_t* d_spec_off = nullptr;
   void* d_spec = nullptr;
   int64_t spec_cap = 0;
@@ -282,6 +283,14 @@ void collect_timing(gg_ctx* c) {
   c->evused = 0;
 }
 
+// A captured step graph holds kernel arguments and shapes of the moment.
+void drop_graph(gg_ctx* c) {
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+}
+
 int require_boxes(gg_ctx* c) {
   if (!c->boxes_set) return fail(c, GG_ERR_STATE, "gg_set_boxes has not been called");
   return GG_OK;
@@ -311,7 +320,7 @@ void stage_comparison(gg_ctx* c) {
       SegTimer t(c, seg[cls]);
       // the first class's segment includes building the neighbour table
       if (cls == 0) launch_comparison_table(launch_of(c), c->sp, c->d_state, din(c), c->LD);
-      launch_comparison(launch_of(c), cls, c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur],
+      launch_comparison(launch_of(c), cls, c->cmp_variant[cls], c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur],
                         c->IL[c->il_cur ^ 1], c->d_newb, c->fp32);
     }
   } else {
@@ -323,11 +332,11 @@ void stage_comparison(gg_ctx* c) {
     hipStreamWaitEvent(c->side, c->ev_fork, 0);
     Launch S = launch_of(c);
     S.stream = c->side;
-    launch_comparison(S, 0, c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur], c->IL[c->il_cur ^ 1], c->d_newb,
+    launch_comparison(S, 0, c->cmp_variant[0], c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur], c->IL[c->il_cur ^ 1], c->d_newb,
                       c->fp32);
     hipEventRecord(c->ev_join, c->side);
     for (int cls = 1; cls < kClasses; ++cls)
-      launch_comparison(launch_of(c), cls, c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur],
+      launch_comparison(launch_of(c), cls, c->cmp_variant[cls], c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur],
                         c->IL[c->il_cur ^ 1], c->d_newb, c->fp32);
     hipStreamWaitEvent(c->stream, c->ev_join, 0);
   }
@@ -486,7 +495,7 @@ int gg_create(const gg_params* params, gg_ctx** out) {
   gg_ctx* c = new gg_ctx;
   c->p = *params;
   if (c->p.max_list <= 0) c->p.max_list = 512;
-  if (c->p.max_list > 1024 || c->p.ngridpoints > 32767) {
+  if (c->p.max_list > 1024 || c->p.ngridpoints > kMaxGrid) {
     delete c;
     return GG_ERR_ARG;
   }
@@ -559,6 +568,7 @@ int gg_create(const gg_params* params, gg_ctx** out) {
       gg_destroy(c);
       return rc;
     }
+
   // the second plan shares the (constant) spatial grain order
   c->P2.order = c->P.order;
   if ((rc = dalloc(c, &c->P2.cls_count, kClasses)) || (rc = dalloc(c, &c->P2.cls_start, kClasses)) ||
@@ -857,10 +867,7 @@ int gg_coarsen_grid(gg_ctx* c, int32_t new_size, double new_dt) {
   for (const void* q : {(const void*)old_tb.tw_off, (const void*)old_tb.twc, (const void*)old_tb.tws,
                         (const void*)old_tb.g_off, (const void*)old_tb.gtab, (const void*)old_tb.circ})
     dfree(c, q);
-  if (c->gexec) hipGraphExecDestroy(c->gexec);
-  if (c->graph) hipGraphDestroy(c->graph);
-  c->gexec = nullptr;
-  c->graph = nullptr;
+  drop_graph(c);
   const CoarsenArgs a{gb, h_old / h_new, h_new / h_old, c->p.delta};
   Launch L = launch_of(c);
   // resizeGrid: the output buffer (the last redistancing's result) is
@@ -878,7 +885,7 @@ int gg_coarsen_grid(gg_ctx* c, int32_t new_size, double new_dt) {
   launch_idl_resize(L, c->sp, c->d_state, dout(c), c->IL[c->il_cur ^ 1]);
   launch_comparison_table(L, c->sp, c->d_state, din(c), c->LD, true);
   for (int cls = 0; cls < kClasses; ++cls)
-    launch_comparison(L, cls, c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur], c->IL[c->il_cur ^ 1],
+    launch_comparison(L, cls, c->cmp_variant[cls], c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur], c->IL[c->il_cur ^ 1],
                       c->d_newb, c->fp32);
   c->il_cur ^= 1;
   // extractContour on the resampled input buffer
@@ -970,6 +977,84 @@ int gg_timestep(gg_ctx* c, int32_t nsteps, int32_t use_graph) {
     s += 2;
   }
   for (; s < nsteps; ++s) enqueue_step(c);
+  return GG_OK;
+}
+
+int gg_autotune(gg_ctx* c, int32_t reps, int32_t* choice, float* ms) {
+  if (!c || reps < 1) return GG_ERR_ARG;
+  int rc = require_boxes(c);
+  if (rc) return rc;
+  if (!c->lists_set) return fail(c, GG_ERR_STATE, "gg_set_neighbour_lists has not been called");
+  hipSetDevice(c->p.device);
+  if ((rc = gg_synchronize(c))) return rc;
+  const bool timing = c->timing;
+  c->timing = false;
+  // one timestep up to the comparison's inputs
+  c->sel ^= 1;
+  stage_convolution(c);
+  c->sel ^= 1;
+  stage_lists(c);
+  launch_comparison_table(launch_of(c), c->sp, c->d_state, din(c), c->LD);
+  // The comparison is a function of its inputs (the input role, the previous
+  // IDLocal, the lists; the bounds atomics are minima and maxima), so every
+  // variant can run on the same inputs: time each, per size class.
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  for (int cls = 0; cls < kClasses; ++cls) {
+    float best = 0.f;
+    int bv = c->cmp_variant[cls];
+    for (int v = 0; v < kCmpVariants; ++v) {
+      launch_comparison(launch_of(c), cls, v, c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur],
+                        c->IL[c->il_cur ^ 1], c->d_newb, c->fp32);  // warm
+      hipEventRecord(e0, c->stream);
+      for (int r = 0; r < reps; ++r)
+        launch_comparison(launch_of(c), cls, v, c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur],
+                          c->IL[c->il_cur ^ 1], c->d_newb, c->fp32);
+      hipEventRecord(e1, c->stream);
+      hipError_t e = hipEventSynchronize(e1);
+      float t = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+      if (e != hipSuccess) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        c->timing = timing;
+        return fail(c, GG_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+      }
+      t /= (float)reps;
+      if (ms) ms[cls * kCmpVariants + v] = t;
+      if (v == 0 || t < best) {
+        best = t;
+        bv = v;
+      }
+    }
+    c->cmp_variant[cls] = bv;
+    if (choice) choice[cls] = bv;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  // the rest of the timestep, as gg_timestep runs it
+  c->il_cur ^= 1;
+  c->sel ^= 1;
+  stage_level_set_and_redistancing(c);
+  stage_count(c);
+  c->timing = timing;
+  drop_graph(c);  // a captured step holds the old shapes
+  return gg_synchronize(c);
+}
+
+int gg_get_tuning(gg_ctx* c, int32_t* variants) {
+  if (!c || !variants) return GG_ERR_ARG;
+  for (int k = 0; k < kClasses; ++k) variants[k] = c->cmp_variant[k];
+  return GG_OK;
+}
+
+int gg_set_tuning(gg_ctx* c, const int32_t* variants) {
+  if (!c || !variants) return GG_ERR_ARG;
+  for (int k = 0; k < kClasses; ++k)
+    if (variants[k] < 0 || variants[k] >= kCmpVariants) return fail(c, GG_ERR_ARG, "tuning variant out of range");
+  for (int k = 0; k < kClasses; ++k) c->cmp_variant[k] = variants[k];
+  drop_graph(c);
   return GG_OK;
 }
 
@@ -1114,11 +1199,7 @@ int gg_enable_hull_topology(gg_ctx* c, int32_t enable) {
   }
   if (on != c->topo.on) {
     c->topo.on = on;
-    // a captured step graph holds the old setting
-    if (c->gexec) hipGraphExecDestroy(c->gexec);
-    if (c->graph) hipGraphDestroy(c->graph);
-    c->gexec = nullptr;
-    c->graph = nullptr;
+    drop_graph(c);  // a captured step graph holds the old setting
   }
   return GG_OK;
 }

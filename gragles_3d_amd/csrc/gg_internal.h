@@ -114,6 +114,9 @@ struct PlanDev {
   int32_t* tile_base;   // [kNumQ * kClasses] first entry of each class in tmap[q]
   uint32_t* tmap[kNumQ];  // [kMaxTilesPerGrain * N] tile -> grain << 8 | tile index inside the grain
 };
+constexpr int kCmpVariants = 4;  // comparison tile shapes the autotuner chooses from
+constexpr int kCmpDefault = 1;   // 16 layers, 8 gathers in flight
+constexpr int kMaxGrid = 32767;  // ngridpoints limit
 
 // Tiles of one grain for any quantity: n layers <= 96, ceil(n*n/64) <= 144
 // column blocks, ceil(n*(n/2+1)/32) <= 147 spectral blocks; tile indices fit 8
@@ -296,9 +299,10 @@ void launch_lists(const Launch& L, const SimParams& sp, const int32_t* state, co
 // over neighbours that are not alive.
 void launch_comparison_table(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& din,
                              const ListDev& ld, bool coarsen = false);
-void launch_comparison(const Launch& L, int cls, const SimParams& sp, const PlanDev& P, const DescDev& din,
-                       const DescDev& dout, const ListDev& ld, const IdlDev& il_old, const IdlDev& il_new,
-                       int32_t* newb, bool fp32);
+// variant: tile shape (0..kCmpVariants-1), chosen per size class by the autotuner
+void launch_comparison(const Launch& L, int cls, int variant, const SimParams& sp, const PlanDev& P,
+                       const DescDev& din, const DescDev& dout, const ListDev& ld, const IdlDev& il_old,
+                       const IdlDev& il_new, int32_t* newb, bool fp32);
 // Clears the grains' topology tables; before every hull with topology on.
 void launch_topo_clear(const Launch& L, const SimParams& sp, const TopoDev& topo);
 void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, int32_t* exflag,

@@ -75,6 +75,9 @@ SIGNATURES = {
     "gg_count_grains": (ctypes.c_int, [_vp, _i32p]),
     "gg_timestep": (ctypes.c_int, [_vp, _i32, _i32]),
     "gg_synchronize": (ctypes.c_int, [_vp]),
+    "gg_autotune": (ctypes.c_int, [_vp, _i32, _i32p, _P(ctypes.c_float)]),
+    "gg_get_tuning": (ctypes.c_int, [_vp, _i32p]),
+    "gg_set_tuning": (ctypes.c_int, [_vp, _i32p]),
     "gg_get_count_history": (ctypes.c_int, [_vp, _i32p, _i32, _i32p]),
     "gg_get_grain_stats": (ctypes.c_int, [_vp, _i32p, _f64p, _f64p]),
     "gg_get_bounds": (ctypes.c_int, [_vp, _i32, _i32, _i32p]),
@@ -374,6 +377,23 @@ class LevelSetContext:
 
     def timestep(self, nsteps=1, use_graph=False):
         self._chk(lib().gg_timestep(self.h, int(nsteps), 1 if use_graph else 0))
+
+    def autotune(self, reps=3):
+        """One timestep that also times the comparison's tile shapes per box-size class on its
+        own inputs and keeps the fastest; returns (choice per class, ms[class][shape])."""
+        ch = np.zeros(4, dtype=np.int32)
+        ms = np.zeros((4, 4), dtype=np.float32)
+        self._chk(lib().gg_autotune(self.h, int(reps), _p(ch, ctypes.c_int32), _p(ms, ctypes.c_float)))
+        return ch, ms
+
+    def tuning(self):
+        v = np.zeros(4, dtype=np.int32)
+        self._chk(lib().gg_get_tuning(self.h, _p(v, ctypes.c_int32)))
+        return v
+
+    def set_tuning(self, variants):
+        v = np.ascontiguousarray(variants, dtype=np.int32)
+        self._chk(lib().gg_set_tuning(self.h, _p(v, ctypes.c_int32)))
 
     def synchronize(self):
         self._chk(lib().gg_synchronize(self.h))

@@ -177,6 +177,21 @@ int gg_count_grains(gg_ctx* ctx, int32_t* count);
  * a captured HIP graph of two timesteps (captured on first use). */
 int gg_timestep(gg_ctx* ctx, int32_t nsteps, int32_t use_graph);
 
+/* Autotuner of the comparison kernel (no counterpart in the reference, whose
+ * only tuning is its static grain schedulers, src/IterativeGrainScheduler.cpp,
+ * src/SquaresGrainScheduler.cpp): runs one timestep, and inside it times each
+ * of the comparison's tile shapes per box-size class (n <= 32, 48, 64, 96) on
+ * the step's own inputs (reps launches each), keeps the fastest per class and
+ * drops a captured step graph.  The shapes (z-layers per wavefront, gathers in
+ * flight per neighbour) are 0: (8, 8), 1: (16, 8), 2: (16, 16), 3: (32, 8);
+ * every shape gives the same result.  choice: kClasses entries (may be NULL);
+ * ms: kClasses * 4 mean launch times (may be NULL). */
+int gg_autotune(gg_ctx* ctx, int32_t reps, int32_t* choice, float* ms);
+/* The shape per size class in use (4 entries), and setting it (for instance a
+ * choice saved from an earlier gg_autotune); default 1 for every class. */
+int gg_get_tuning(gg_ctx* ctx, int32_t* variants);
+int gg_set_tuning(gg_ctx* ctx, const int32_t* variants);
+
 /* Wait for the stream and report device-side capacity errors. */
 int gg_synchronize(gg_ctx* ctx);
 
