@@ -338,6 +338,8 @@ def main():
                     help="the 1-thread CPU baseline times one step over every n-th grain")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default comparison tile shapes instead of autotuning them (one timestep)")
+    ap.add_argument("--tuning", metavar="S0,S1,S2,S3",
+                    help="comparison tile shape per size class (0..3), instead of autotuning them")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the voxel-container and grid-coarsening legs")
     ap.add_argument("--summarize-rocprof", metavar="DIR", help="print markdown tables of a rocprofv3 directory")
@@ -361,12 +363,24 @@ def main():
 
     # ---- FP32 batched: autotuned comparison shapes (one timestep), event-timed kernel pass, ----
     # ---- then graph replays                                                                  ----
-    ctx32 = new_context(G.FP32_BATCHED, inputs, p)
+    # The autotuner runs a timestep; it runs on a context of its own, so that the measured
+    # context starts from the initial state (as a profiled run with --tuning does).
     tune32 = None
-    if not args.no_tune:
-        choice, ms = ctx32.autotune(reps=3)
-        tune32 = {"choice_per_class": choice.tolist(), "ms_per_class_and_shape": np.round(ms, 4).tolist()}
-        log("fp32 comparison shapes per class (n<=32,48,64,96): %s" % choice.tolist())
+    choice32 = None
+    if args.tuning:
+        choice32 = [int(v) for v in args.tuning.split(",")]
+        tune32 = {"choice_per_class": choice32, "source": "--tuning"}
+    elif not args.no_tune:
+        tctx = new_context(G.FP32_BATCHED, inputs, p)
+        choice, ms = tctx.autotune(reps=3)
+        tctx.close()
+        choice32 = choice.tolist()
+        tune32 = {"choice_per_class": choice32, "ms_per_class_and_shape": np.round(ms, 4).tolist(),
+                  "source": "autotuner (one timestep on a separate context)"}
+    ctx32 = new_context(G.FP32_BATCHED, inputs, p)
+    if choice32 is not None:
+        ctx32.set_tuning(choice32)
+        log("fp32 comparison shapes per class (n<=32,48,64,96): %s" % choice32)
     seg32, by32, cls_bytes, cls_vox, cls_num = kernel_pass(ctx32, args.kernel_steps, True)
     ks = max(args.kernel_steps, 1)
     rows32 = roofline_rows(seg32, by32, ks)
@@ -396,9 +410,15 @@ def main():
     fp64 = None
     ctx64 = None
     if not args.no_fp64:
+        if not args.no_tune and not args.tuning:
+            tctx = new_context(G.FP64_PLAIN, inputs, p)
+            choice64 = tctx.autotune(reps=2)[0].tolist()
+            tctx.close()
+        else:
+            choice64 = choice32
         ctx64 = new_context(G.FP64_PLAIN, inputs, p)
-        if not args.no_tune:
-            ctx64.autotune(reps=2)
+        if choice64 is not None:
+            ctx64.set_tuning(choice64)
         seg64, by64, _, _, _ = kernel_pass(ctx64, max(1, args.kernel_steps // 2), False)
         wall64, vox_step64 = timed_steps(ctx64, args.steps, args.warmup)
         v64 = vox_step64 * args.steps / wall64 if args.steps else 0.0

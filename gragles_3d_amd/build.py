@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libgg_levelset.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 HIP_SOURCES = ["gg_api.hip", "gg_plan.hip", "gg_conv.hip", "gg_compare.hip", "gg_comparison.hip", "gg_hull.hip",
-               "gg_redist.hip", "gg_coarsen.hip"]
+               "gg_redist.hip", "gg_coarsen.hip", "gg_correct.hip"]
 CXX_SOURCES = ["gg_host.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/gg_grainhdl.h"
 #include "../../include/gg_levelset.h"
 #include "gg_internal.h"
 
@@ -41,6 +42,11 @@ struct gg_ctx {
   uint32_t* d_cont = nullptr;  // voxel container (allocated on first use)
   int64_t cont_cap = 0;
   TopoDev topo{nullptr, nullptr, nullptr, 0, nullptr, nullptr};  // hull topology tables (allocated when first enabled)
+  int topo_user = 0;  // topology asked for through gg_enable_hull_topology (the correction needs it too)
+  // convolution correction (gg_set_physics)
+  bool corr_on = false;
+  double tl_drag = 0.0;
+  CorrDev cd{};
   int32_t* d_exflag = nullptr;
   double* d_volume = nullptr;
   double* d_area = nullptr;
@@ -142,6 +148,7 @@ int check_device_error(gg_ctx* c) {
     if (e & kErrEmptyIO) m += " empty in/out intersection in redistancing;";
     if (e & kErrHistory) m += " count history full;";
     if (e & kErrTopo) m += " hull topology table full;";
+    if (e & kErrTri) m += " correction barycenter pool full;";
     return fail(c, GG_ERR_CAPACITY, m);
   }
   return GG_OK;
@@ -246,7 +253,7 @@ hipEvent_t take_event(gg_ctx* c) {
 enum Seg {
   kSegConvA, kSegConvB, kSegConvC, kSegIdl, kSegLists, kSegCompare, kSegHull, kSegFinalize, kSegPlan,
   kSegRedistXY, kSegRedistZ, kSegPost, kSegCount, kSegCompare48, kSegCompare64, kSegCompare96, kSegResample,
-  kSegContainer, kNumSeg
+  kSegContainer, kSegCorrect, kSegElements, kNumSeg
 };
 static_assert(kNumSeg == GG_NUM_SEGMENTS, "segment table of the C ABI");
 
@@ -283,6 +290,25 @@ void collect_timing(gg_ctx* c) {
   c->evused = 0;
 }
 
+// Launch arguments of the correction kernels: the time step and the triple-line
+// mobility term follow the grid (grid coarsening changes both).
+CorrDev corr_args(gg_ctx* c) {
+  CorrDev d = c->cd;
+  d.dt = c->p.dt;
+  const double ds = 1.0 / (double)c->p.conv_domain_size;  // grainhdl::get_ds()
+  const double dsl = 3 * sqrt(3 * ds * ds);
+  d.tl_inv = 1. / (dsl * c->tl_drag);
+  return d;
+}
+
+// The interfacial elements of the hull just computed (after the level set's
+// last part, or a coarsening's extractContour).
+void stage_elements(gg_ctx* c) {
+  if (!c->corr_on) return;
+  SegTimer t(c, kSegElements);
+  launch_elements(launch_of(c), c->sp, c->d_state, c->topo, corr_args(c), c->d_err);
+}
+
 // A captured step graph holds kernel arguments and shapes of the moment.
 void drop_graph(gg_ctx* c) {
   if (c->gexec) hipGraphExecDestroy(c->gexec);
@@ -306,6 +332,13 @@ void stage_convolution(gg_ctx* c) {
     launch_conv(L, 2, c->sp, c->P, din(c), dout(c), c->TB, c->d_spec_off, c->d_spec, c->fp32);
   }
   { SegTimer t(c, kSegIdl); launch_idl_resize(L, c->sp, c->d_state, dout(c), c->IL[c->il_cur ^ 1]); }
+  if (c->corr_on) {
+    // IL[il_cur] holds the last comparison's IDLocal, read as its resize to
+    // the output bounds leaves it
+    SegTimer t(c, kSegCorrect);
+    launch_correction(L, c->sp, c->P, c->d_state, din(c), dout(c), c->IL[c->il_cur], c->topo, corr_args(c),
+                      c->fp32);
+  }
 }
 
 void stage_lists(gg_ctx* c) {
@@ -398,6 +431,7 @@ void stage_level_set(gg_ctx* c) {
   stage_output_plan(c);
   stage_hull(c);
   stage_finalize_state(c);
+  stage_elements(c);
 }
 
 void redist_parts(gg_ctx* c) {
@@ -441,6 +475,7 @@ void stage_level_set_and_redistancing(gg_ctx* c) {
   for (int part = 0; part < 2; ++part) launch_redist(L, part, c->sp, c->P2, din(c), dout(c), c->d_err, c->fp32);
   hipStreamWaitEvent(c->stream, c->ev_join, 0);
   stage_finalize_state(c);
+  stage_elements(c);
   launch_post_redist(L, c->sp, c->d_state, din(c), dout(c));
   std::swap(c->P, c->P2);
   c->p2_valid = false;
@@ -700,6 +735,7 @@ int gg_set_boxes(gg_ctx* c, const int32_t* bounds, const uint8_t* exists) {
   // IDLocal starts zero (std::vector value-initialisation, src/box.cpp:221)
   HIPCHK(c, hipMemset(c->IL[0].arena, 0, 4 * c->arena_cap));
   HIPCHK(c, hipMemset(c->IL[1].arena, 0, 4 * c->arena_cap));
+  if (c->cd.ready) HIPCHK(c, hipMemset(c->cd.ready, 0, sizeof(int32_t)));  // no hull yet
   HIPCHK(c, hipDeviceSynchronize());
   c->boxes_set = true;
   return GG_OK;
@@ -893,6 +929,7 @@ int gg_coarsen_grid(gg_ctx* c, int32_t new_size, double new_dt) {
   launch_hull(L, c->sp, c->P, din(c), c->d_exflag, c->d_partials, c->IL[c->il_cur], c->topo, c->d_err, c->fp32);
   launch_finalize(L, c->sp, c->P, c->d_state, c->LD, c->d_exflag, c->d_partials, c->d_cnb, din(c), dout(c),
                   c->d_volume, c->d_area, kFinBounds | kFinState | kFinCoarsen);
+  stage_elements(c);
   plan_layout(c, dout(c));
   c->p2_valid = false;
   c->sel ^= 1;
@@ -1180,26 +1217,135 @@ int gg_download_list(gg_ctx* c, int32_t id, int32_t which, int32_t* dst, int32_t
   return GG_OK;
 }
 
+static int alloc_topology(gg_ctx* c) {
+  if (c->topo.key) return GG_OK;
+  const size_t n = (size_t)(c->p.n_grains + 1) * kTopoCap;
+  int rc;
+  const size_t nj = (size_t)(c->p.n_grains + 1) * kJuncCap;
+  if ((rc = dalloc(c, &c->topo.key, n)) || (rc = dalloc(c, &c->topo.ord, n)) || (rc = dalloc(c, &c->topo.jkey, nj)) ||
+      (rc = dalloc(c, &c->topo.jpos, 3 * nj)) || (rc = dalloc(c, &c->topo.jcnt, nj)))
+    return rc;
+  // empty tables: no grain touched yet
+  HIPCHK(c, hipMemset(c->topo.key, 0xFF, n * sizeof(int32_t)));
+  HIPCHK(c, hipMemset(c->topo.jkey, 0, nj * sizeof(unsigned long long)));
+  HIPCHK(c, hipMemset(c->topo.jcnt, 0, nj * sizeof(int32_t)));
+  HIPCHK(c, hipMemset(c->topo.jpos, 0, 3 * nj * sizeof(double)));
+  return GG_OK;
+}
+
 int gg_enable_hull_topology(gg_ctx* c, int32_t enable) {
   if (!c) return GG_ERR_ARG;
   hipSetDevice(c->p.device);
-  const int on = enable ? 1 : 0;
-  if (on && !c->topo.key) {
-    const size_t n = (size_t)(c->p.n_grains + 1) * kTopoCap;
-    int rc;
-    const size_t nj = (size_t)(c->p.n_grains + 1) * kJuncCap;
-    if ((rc = dalloc(c, &c->topo.key, n)) || (rc = dalloc(c, &c->topo.ord, n)) || (rc = dalloc(c, &c->topo.jkey, nj)) ||
-        (rc = dalloc(c, &c->topo.jpos, 3 * nj)) || (rc = dalloc(c, &c->topo.jcnt, nj)))
-      return rc;
-    // empty tables: no grain touched yet
-    HIPCHK(c, hipMemset(c->topo.key, 0xFF, n * sizeof(int32_t)));
-    HIPCHK(c, hipMemset(c->topo.jkey, 0, nj * sizeof(unsigned long long)));
-    HIPCHK(c, hipMemset(c->topo.jcnt, 0, nj * sizeof(int32_t)));
-    HIPCHK(c, hipMemset(c->topo.jpos, 0, 3 * nj * sizeof(double)));
+  c->topo_user = enable ? 1 : 0;
+  const int on = (c->topo_user || c->corr_on) ? 1 : 0;
+  if (on) {
+    int rc = alloc_topology(c);
+    if (rc) return rc;
   }
   if (on != c->topo.on) {
     c->topo.on = on;
     drop_graph(c);  // a captured step graph holds the old setting
+  }
+  return GG_OK;
+}
+
+int gg_set_physics(gg_ctx* c, const gg_physics* ph, const double* quat, const double* see) {
+  if (!c || !ph) return GG_ERR_ARG;
+  int rc = require_boxes(c);
+  if (rc) return rc;
+  hipSetDevice(c->p.device);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->side));
+  drop_graph(c);  // kernel arguments change
+  const int N = c->p.n_grains;
+  if (!ph->correction) {
+    c->corr_on = false;
+    c->topo.corr = 0;
+    c->topo.on = c->topo_user;
+    return GG_OK;
+  }
+  const bool aniso = ph->mobility_model || !ph->unique_energies;
+  if (aniso && !quat) return fail(c, GG_ERR_ARG, "orientations are needed for misorientation-dependent energies or mobilities");
+  if (ph->stored_energy && !see) return fail(c, GG_ERR_ARG, "stored energies are needed with stored_energy on");
+  if (aniso && N > GG_MAX_ANISO_GRAINS)
+    return fail(c, GG_ERR_CAPACITY, "misorientation-dependent energies need n_grains <= GG_MAX_ANISO_GRAINS");
+  if ((rc = alloc_topology(c))) return rc;
+  CorrDev& d = c->cd;
+  const size_t n1 = (size_t)N + 1;
+  if (!d.ready) {
+    const size_t ne = n1 * kElemCap;
+    const int64_t cap = c->arena_cap / 8 + 65536;  // triangles of one hull pass (about 3 per surface voxel)
+    if ((rc = dalloc(c, &c->topo.jord, n1 * kJuncCap)) || (rc = dalloc(c, &c->topo.raw, (size_t)cap)) ||
+        (rc = dalloc(c, &c->topo.raw_n, 1)) || (rc = dalloc(c, &d.ready, 1)) || (rc = dalloc(c, &d.ecount, n1)) ||
+        (rc = dalloc(c, &d.eclass, 4 * n1)) || (rc = dalloc(c, &d.epos, 3 * ne)) || (rc = dalloc(c, &d.emask, ne)) ||
+        (rc = dalloc(c, &d.energy, ne)) || (rc = dalloc(c, &d.mobility, ne)) ||
+        (rc = dalloc(c, &d.efirst, n1 * (kElemCap + 1))) || (rc = dalloc(c, &d.ecur, ne)) ||
+        (rc = dalloc(c, &d.slot_elem, n1 * kJuncCap)) || (rc = dalloc(c, &d.tbase, n1)) ||
+        (rc = dalloc(c, &d.tri, 3 * (size_t)cap)) || (rc = dalloc(c, &d.tri_n, 1)) || (rc = dalloc(c, (double**)&d.see, n1)))
+      return rc;
+    c->topo.raw_cap = cap;
+    d.tri_cap = cap;
+    HIPCHK(c, hipMemset(d.ecount, 0, 4 * n1));
+    HIPCHK(c, hipMemset(d.slot_elem, 0xFF, 4 * n1 * kJuncCap));
+  }
+  if (aniso) {
+    if (!d.gam) {
+      if ((rc = dalloc(c, (double**)&d.gam, n1 * n1)) || (rc = dalloc(c, (double**)&d.mob, n1 * n1))) return rc;
+    }
+    std::vector<double> gam(n1 * n1), mob(n1 * n1);
+    gg_gb_pair_tables((int32_t)n1, quat, ph->mobility_model, ph->unique_energies, gam.data(), mob.data(), 0);
+    HIPCHK(c, hipMemcpy((void*)d.gam, gam.data(), 8 * n1 * n1, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy((void*)d.mob, mob.data(), 8 * n1 * n1, hipMemcpyHostToDevice));
+  }
+  std::vector<double> sv(n1, 0.0);
+  if (ph->stored_energy)
+    for (size_t g = 0; g < n1; ++g) sv[g] = see[g];
+  HIPCHK(c, hipMemcpy((void*)d.see, sv.data(), 8 * n1, hipMemcpyHostToDevice));
+  // the correction starts after the next hull (run_sim's loop != 0)
+  HIPCHK(c, hipMemset(d.ready, 0, sizeof(int32_t)));
+  d.see_on = ph->stored_energy ? 1 : 0;
+  d.aniso = aniso ? 1 : 0;
+  const double time_slope = 1 / 1.21;  // grainhdl::TimeSlope in the Gaussian convolution mode
+  d.tsq = time_slope * time_slope;
+  c->tl_drag = ph->triple_line_drag;
+  c->corr_on = true;
+  c->topo.corr = 1;
+  c->topo.on = 1;
+  return GG_OK;
+}
+
+int gg_download_elements(gg_ctx* c, int32_t id, int32_t* cls, int32_t* nnb, int32_t* nb8, int32_t* nbary,
+                         double* pos, double* energy, double* mobility, int32_t cap, int32_t* n) {
+  if (!c || !n || id < 1 || id > c->p.n_grains) return GG_ERR_ARG;
+  if (!c->cd.ready) return fail(c, GG_ERR_STATE, "the convolution correction was never enabled");
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  int32_t E = 0, ec[4] = {0, 0, 0, 0};
+  HIPCHK(c, hipMemcpy(&E, c->cd.ecount + id, 4, hipMemcpyDeviceToHost));
+  *n = E;
+  if (!cls) return GG_OK;
+  if (E > cap || !nnb || !nb8 || !nbary || !pos || !energy || !mobility)
+    return fail(c, GG_ERR_ARG, "destination too small");
+  if (E == 0) return GG_OK;
+  const int64_t eb = (int64_t)id * kElemCap;
+  std::vector<int32_t> key(kTopoCap), ef(E + 1);
+  std::vector<unsigned long long> em(E);
+  HIPCHK(c, hipMemcpy(ec, c->cd.eclass + 4 * id, 12, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(key.data(), c->topo.key + (int64_t)id * kTopoCap, 4 * kTopoCap, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(em.data(), c->cd.emask + eb, 8 * E, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(ef.data(), c->cd.efirst + (int64_t)id * (kElemCap + 1), 4 * (E + 1), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(pos, c->cd.epos + 3 * eb, 24 * E, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(energy, c->cd.energy + eb, 8 * E, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(mobility, c->cd.mobility + eb, 8 * E, hipMemcpyDeviceToHost));
+  for (int e = 0; e < E; ++e) {
+    cls[e] = e < ec[0] ? 3 : e < ec[0] + ec[1] ? 2 : e < ec[0] + ec[1] + ec[2] ? 1 : 0;
+    std::vector<int64_t> ids;
+    for (int b = 0; b < kTopoCap; ++b)
+      if ((em[e] >> b) & 1ull) ids.push_back((int64_t)(uint32_t)key[b]);
+    std::sort(ids.begin(), ids.end());
+    nnb[e] = (int32_t)ids.size();
+    for (int k = 0; k < 8; ++k) nb8[8 * e + k] = k < (int)ids.size() ? (int32_t)ids[k] : -1;
+    nbary[e] = ef[e + 1] - ef[e];
   }
   return GG_OK;
 }

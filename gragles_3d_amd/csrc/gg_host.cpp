@@ -66,9 +66,123 @@ int resolve_threads(int nthreads) {
   return n > 0 ? n : 1;
 }
 
+// ---- grain-boundary model of the convolution correction --------------------
+struct Quat {
+  double q0, q1, q2, q3;
+};
+
+// myQuaternion::operator* (src/myQuaternion.cpp:59-71), terms in its order.
+Quat qmul(const Quat& a, const Quat& b) {
+  Quat r;
+  r.q0 = +a.q0 * b.q0 - a.q1 * b.q1 - a.q2 * b.q2 - a.q3 * b.q3;
+  r.q1 = +a.q1 * b.q0 + a.q0 * b.q1 - a.q3 * b.q2 + a.q2 * b.q3;
+  r.q2 = +a.q2 * b.q0 + a.q3 * b.q1 + a.q0 * b.q2 - a.q1 * b.q3;
+  r.q3 = +a.q3 * b.q0 - a.q2 * b.q1 + a.q1 * b.q2 + a.q0 * b.q3;
+  return r;
+}
+
+// misorientationCubicQxQ (src/myQuaternion.cpp:309-367): the rotation a^-1 b
+// taken to the cubic fundamental zone by the largest absolute component over
+// its six symmetry-equivalent component sets (24 operators), angle 2 acos.
+// computeMisorientation (src/box.cpp:2704-2719) floors it at 1 degree.
+double cubic_misorientation(const Quat& a, const Quat& b) {
+  const Quat r = qmul(Quat{a.q0, -a.q1, -a.q2, -a.q3}, b);
+  const double w = r.q0, x = r.q1, y = r.q2, z = r.q3;
+  const double f = 1.0 / sqrt(2.0);
+  const double sets[6][4] = {{(w + x) * f, (w - x) * f, (y + z) * f, (y - z) * f},
+                             {(w + y) * f, (w - y) * f, (x + z) * f, (x - z) * f},
+                             {(w + z) * f, (w - z) * f, (x + y) * f, (x - y) * f},
+                             {(w + x + y + z) * 0.5, (w + x - y - z) * 0.5, (w - x + y - z) * 0.5, (w - x - y + z) * 0.5},
+                             {(w + x + y - z) * 0.5, (w + x - y + z) * 0.5, (w - x + y + z) * 0.5, (w - x - y - z) * 0.5},
+                             {w, x, y, z}};
+  double big = 0.0;
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j < 4; j++)
+      if (std::fabs(sets[i][j]) > big) big = std::fabs(sets[i][j]);
+  if (big > 1.0) big = (double)(int)big;
+  const double angle = 2 * acos(big);
+  return angle > 1 * kPi / 180.0 ? angle : 1 * kPi / 180.0;
+}
+
+// InterfacialElement::computeMobilityMisori (src/InterfacialElement.cpp:23-35)
+double gb_mobility(double theta, bool model) {
+  if (!model) return 1.0;
+  const double m = 1.0 - (0.99 * exp(-5. * (pow(theta / (15 * kPi / 180), 9.))));
+  return m < 0.1 ? 0.1 : m;
+}
+
+// InterfacialElement::computeReadShockleyEnergy (src/InterfacialElement.cpp:37-50)
+double gb_energy(double theta, bool unique) {
+  if (unique) return 1.0;
+  const double ref = 15 * kPi / 180;
+  if (theta > ref) return 1.0;
+  return 1.0 * (theta / ref) * (1.0 - log(theta / ref));
+}
+
+double normal_deviate(SplitMix64& r) {  // Box-Muller, u1 in (0, 1]
+  const double u1 = 1.0 - r.uniform(), u2 = r.uniform();
+  return sqrt(-2.0 * log(u1)) * cos(2 * kPi * u2);
+}
+
 }  // namespace
 
 extern "C" {
+
+double gg_misorientation(const double* qa, const double* qb) {
+  return cubic_misorientation(Quat{qa[0], qa[1], qa[2], qa[3]}, Quat{qb[0], qb[1], qb[2], qb[3]});
+}
+
+int gg_gb_pair_tables(int32_t n_ids, const double* quat, int32_t mobility_model, int32_t unique_energies,
+                      double* gam, double* mob, int32_t nthreads) {
+  if (n_ids < 1 || !quat || !gam || !mob) return GG_ERR_ARG;
+  const int64_t n = n_ids;
+  parallel_for(n_ids, nthreads, [&](int a, int) {
+    const Quat qa{quat[4 * a], quat[4 * a + 1], quat[4 * a + 2], quat[4 * a + 3]};
+    for (int64_t b = 0; b < n; ++b) {
+      const double th = cubic_misorientation(qa, Quat{quat[4 * b], quat[4 * b + 1], quat[4 * b + 2], quat[4 * b + 3]});
+      gam[a * n + b] = gb_energy(th, unique_energies != 0);
+      mob[a * n + b] = gb_mobility(th, mobility_model != 0);
+    }
+  });
+  return GG_OK;
+}
+
+void gg_euler_to_quaternion(const double* euler, double* q) {  // euler2Quaternion, src/myQuaternion.cpp:168-191
+  const double p1 = euler[0], t = euler[1], p2 = euler[2];
+  const double c = cos(t / 2), s = sin(t / 2);
+  q[0] = c * cos((p1 + p2) / 2);
+  q[1] = s * cos((p1 - p2) / 2);
+  q[2] = s * sin((p1 - p2) / 2);
+  q[3] = c * sin((p1 + p2) / 2);
+}
+
+int gg_random_orientations(int32_t n_grains, uint64_t seed, int32_t reference_abs, double* quat) {
+  if (n_grains < 1 || !quat) return GG_ERR_ARG;
+  const double wall[3] = {0.0, 35 * (kPi / 180), 0.0};
+  gg_euler_to_quaternion(wall, quat);
+  SplitMix64 r{seed ^ 0x5851F42D4C957F2Dull};
+  for (int g = 1; g <= n_grains; ++g) {
+    const double s = r.uniform(), t1 = 2 * kPi * r.uniform(), t2 = 2 * kPi * r.uniform();
+    const double s1 = sqrt(1 - s), s2 = sqrt(s);
+    double q[4] = {s1 * sin(t1), s1 * cos(t1), s2 * sin(t2), s2 * cos(t2)};
+    double nrm = 0;
+    for (int i = 0; i < 4; ++i) {
+      if (reference_abs) q[i] = std::fabs(q[i]);
+      nrm += q[i] * q[i];
+    }
+    nrm = sqrt(nrm);
+    for (int i = 0; i < 4; ++i) quat[4 * g + i] = q[i] / nrm;
+  }
+  return GG_OK;
+}
+
+int gg_lognormal_energies(int32_t n_grains, uint64_t seed, double median, double sigma, double* see) {
+  if (n_grains < 1 || !see || !(median > 0) || !(sigma >= 0)) return GG_ERR_ARG;
+  see[0] = 0.0;
+  SplitMix64 r{seed ^ 0xD1B54A32D192ED03ull};
+  for (int g = 1; g <= n_grains; ++g) see[g] = median * exp(sigma * normal_deviate(r));
+  return GG_OK;
+}
 
 int gg_derive_params(int32_t n_grains, int32_t points_per_grain, int32_t border, double time_step_size,
                      gg_params* out) {

@@ -381,6 +381,76 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
             }
           }
           __syncthreads();
+          // 4. with the convolution correction on: per cut cell the grain-table
+          //    slot of its list and that list's first appearance; then every
+          //    triangle's barycenter as a record (grain, slot), appended to the
+          //    pool (one reservation per workgroup and pass)
+          if (topo.corr) {
+            int rslot[kCPT];
+            int cnt = 0;
+#pragma unroll
+            for (int r = 0; r < kCPT; ++r) {
+              rslot[r] = -1;
+              const unsigned long long m = cmask[r];
+              if (m == 0ull) continue;
+              uint32_t hsl = (uint32_t)((m * 0x9E3779B97F4A7C15ull) >> (64 - kJuncCapBits));
+              for (int pr = 0; pr < kJuncCap; ++pr) {
+                const unsigned long long k = __hip_atomic_load(&gj[hsl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (k == m) {
+                  rslot[r] = (int)hsl;
+                  break;
+                }
+                if (k == 0ull) break;  // not in the table (it was full: reported above)
+                hsl = (hsl + 1) & (kJuncCap - 1);
+              }
+              if (rslot[r] < 0) continue;
+              const int cell = (kl * m1 + (int)((code[r] >> 8) & 0xFF)) * m1 + (int)(code[r] >> 16);
+              atomicMin(&topo.jord[(int64_t)g * kJuncCap + rslot[r]], cell);
+              cnt += s_ntri[code[r] & 0xFFu];
+            }
+            int sc = cnt;  // inclusive scan over the wavefront
+            for (int off = 1; off < 64; off <<= 1) {
+              const int y = __shfl_up(sc, off);
+              if (lane >= off) sc += y;
+            }
+            if (lane == 63) s_wave[wv] = sc;
+            __syncthreads();
+            int pos = sc - cnt, tot = 0;
+            for (int w = 0; w < kHT / 64; ++w) {
+              if (w < wv) pos += s_wave[w];
+              tot += s_wave[w];
+            }
+            __shared__ unsigned long long s_rbase;
+            if (tid == 0) s_rbase = tot ? atomicAdd(topo.raw_n, (unsigned long long)tot) : 0ull;
+            __syncthreads();
+            const unsigned long long rb = s_rbase;
+            if (tot && rb + (unsigned long long)tot > (unsigned long long)topo.raw_cap) {
+              if (tid == 0) atomicOr(err, kErrTri);
+            } else if (tot) {
+#pragma unroll
+              for (int r = 0; r < kCPT; ++r) {
+                if (rslot[r] < 0) continue;
+                const unsigned ci = code[r] & 0xFFu;
+                const int cyl = (code[r] >> 8) & 0xFF, cxl = (code[r] >> 16) & 0xFF;
+                const int nt = s_ntri[ci];
+                uint64_t row = s_tri[ci];
+                for (int j = 0; j < nt; ++j, row >>= 12) {
+                  double p0x, p0y, p0z, p1x, p1y, p1z, p2x, p2y, p2z;
+                  edge_vertex<T, NMAX>((int)(row & 15u), L, cxl, cyl, n, x0 + cxl, y0 + cyl, z, p0x, p0y, p0z);
+                  edge_vertex<T, NMAX>((int)((row >> 4) & 15u), L, cxl, cyl, n, x0 + cxl, y0 + cyl, z, p1x, p1y, p1z);
+                  edge_vertex<T, NMAX>((int)((row >> 8) & 15u), L, cxl, cyl, n, x0 + cxl, y0 + cyl, z, p2x, p2y, p2z);
+                  TriRec t;
+                  t.x = (p0x + p1x + p2x) / 3;
+                  t.y = (p0y + p1y + p2y) / 3;
+                  t.z = (p0z + p1z + p2z) / 3;
+                  t.g = g;
+                  t.slot = rslot[r];
+                  topo.raw[rb + pos++] = t;
+                }
+              }
+            }
+            __syncthreads();  // s_wave and s_rbase are reused
+          }
           // 3. interfacial elements (src/grainHull.cpp:227-270) of the class
           //    without LDS for them: each cut cell adds its sums to its list's
           //    slot in the grain's junction table
@@ -619,6 +689,8 @@ __global__ __launch_bounds__(256) void k_topo_clear(SimParams sp, TopoDev topo) 
     }
     topo.jkey[i] = 0ull;
     topo.jcnt[i] = 0;
+    if (topo.corr) topo.jord[i] = INT_MAX;
+    if (topo.corr && i == 0) *topo.raw_n = 0ull;
     topo.jpos[3 * i] = 0.0;
     topo.jpos[3 * i + 1] = 0.0;
     topo.jpos[3 * i + 2] = 0.0;

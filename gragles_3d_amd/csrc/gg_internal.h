@@ -45,7 +45,8 @@ enum {
   kErrSpec = 8,       // spectral scratch capacity exceeded
   kErrEmptyIO = 16,   // redistancing with an empty in/out intersection
   kErrHistory = 32,   // count history full
-  kErrTopo = 64       // a grain's hull touches more grains than its topology table holds
+  kErrTopo = 64,      // a grain's hull touches more grains than its topology table holds
+  kErrTri = 128       // the correction's triangle pool is full
 };
 
 // Hull topology (optional output of the level set): per grain, the grains its
@@ -59,6 +60,14 @@ enum {
 // (GrainHull::computeGrainBoundaryElements' classes).
 constexpr int kTopoCap = 64;
 constexpr int kJuncCap = 512;  // ~160 lists at most per hull at 8000 grains
+// With the convolution correction on, the hull also records every triangle's
+// barycenter with the grain and the junction slot of its cell's list (an
+// unsorted pool, appended to per tile), and each list's first appearance (the
+// smallest cell index in the (z, y, x) loop whose list it is).
+struct TriRec {
+  double x, y, z;  // barycenter, grid coordinates
+  int32_t g, slot;
+};
 struct TopoDev {
   int32_t* key;                // [(N+1) * kTopoCap]
   int32_t* ord;                // [(N+1) * kTopoCap]
@@ -66,6 +75,43 @@ struct TopoDev {
   int on;                      // 0: not computed
   double* jpos;                // [(N+1) * kJuncCap * 3] barycenter sums of each list's triangles
   int32_t* jcnt;               // [(N+1) * kJuncCap] triangles of each list
+  int corr;                    // also the triangle records and first appearances below
+  int32_t* jord;               // [(N+1) * kJuncCap] first appearance of each list
+  TriRec* raw;                 // [raw_cap] triangle records of the last hull
+  unsigned long long* raw_n;   // records written (may exceed raw_cap: then the topology error bit is set)
+  int64_t raw_cap;
+};
+
+// Interfacial elements of every grain's last hull and the convolution
+// correction's inputs (src/box.cpp:428-536, src/grainHull.cpp:187-274,
+// 483-531, 570-660).  Elements are kept per grain in the order
+// projectPointToGrainBoundary visits them: high-order junctions, quadruple
+// junctions, triple lines, boundaries.  An element's neighbour set is a mask
+// over the grain's hull-id table slots (TopoDev::key).
+constexpr int kElemCap = kJuncCap;
+struct CorrDev {
+  int32_t* ready;              // set by the element build: a hull with elements exists (run_sim's loop != 0)
+  int see_on;                  // Settings::UseStoredElasticEnergy
+  int aniso;                   // energies / mobilities from the pair tables (else 1)
+  double tl_inv;               // 1 / (3 sqrt(3 ds^2) * TripleLineDrag), TripleLine::computeMobility
+  double dt;
+  double tsq;                  // TimeSlope^2
+  const double* gam;           // [(N+1)^2] Read-Shockley energy of the ordered pair (a, b)
+  const double* mob;           // [(N+1)^2] mobility of the ordered pair
+  const double* see;           // [N+1] stored elastic energy (0: the wall, pinned 0)
+  int32_t* ecount;             // [N+1] elements of the grain
+  int32_t* eclass;             // [(N+1) * 4] high-order junctions, quadruple junctions, triple lines (rest: boundaries)
+  double* epos;                // [(N+1) * kElemCap * 3] junction positions (0 for lines and boundaries)
+  unsigned long long* emask;   // [(N+1) * kElemCap] neighbour set
+  double* energy;              // [(N+1) * kElemCap] m_energy
+  double* mobility;            // [(N+1) * kElemCap] m_mobility
+  int32_t* efirst;             // [(N+1) * (kElemCap + 1)] first barycenter of each element, relative to tbase
+  int32_t* ecur;               // [(N+1) * kElemCap] fill cursors
+  int32_t* slot_elem;          // [(N+1) * kJuncCap] element of each junction slot
+  int64_t* tbase;              // [N+1] the grain's first barycenter in tri
+  double* tri;                 // [3 * tri_cap] barycenters grouped by element
+  unsigned long long* tri_n;   // barycenters placed
+  int64_t tri_cap;
 };
 
 struct DescDev {
@@ -338,4 +384,12 @@ void launch_post_redist(const Launch& L, const SimParams& sp, const int32_t* sta
 void launch_count(const Launch& L, const SimParams& sp, const int32_t* state, int32_t* step,
                   int32_t* history, int32_t history_cap, int32_t* err);
 void upload_mc_tables();
+// Convolution correction (gg_correct.hip): the elements of every alive grain
+// from its last hull (after launch_hull with topo.corr), their barycenters
+// grouped by element, and the correction of the convolved output buffer.
+void launch_elements(const Launch& L, const SimParams& sp, const int32_t* state, const TopoDev& topo,
+                     const CorrDev& cd, int32_t* err);
+void launch_correction(const Launch& L, const SimParams& sp, const PlanDev& P, const int32_t* state,
+                       const DescDev& din, const DescDev& dout, const IdlDev& il_stale, const TopoDev& topo,
+                       const CorrDev& cd, bool fp32);
 }  // namespace gg

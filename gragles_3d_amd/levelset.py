@@ -19,7 +19,7 @@ FP64_PLAIN, FP32_BATCHED = 0, 1
 DELETED, ALIVE, VANISHED = 0, 1, 2
 SEGMENTS = ("conv_xy", "conv_z", "conv_xy_inv", "idl_resize", "lists", "comparison_n32", "hull", "finalize", "plan",
             "redist_xy", "redist_z", "post_redist", "count", "comparison_n48", "comparison_n64", "comparison_n96",
-            "resample", "container")
+            "resample", "container", "correction", "elements")
 
 
 class GGParams(ctypes.Structure):
@@ -71,6 +71,9 @@ SIGNATURES = {
     "gg_coarsen_grid": (ctypes.c_int, [_vp, _i32, ctypes.c_double]),
     "gg_voxel_container": (ctypes.c_int, [_vp, _u32p, ctypes.c_int64, _i32p]),
     "gg_download_hull_elements": (ctypes.c_int, [_vp, _i32, _i32p, _i32p, _i32p, _f64p, _i32, _i32p]),
+    "gg_set_physics": (ctypes.c_int, [_vp, ctypes.c_void_p, _f64p, _f64p]),
+    "gg_download_elements": (ctypes.c_int, [_vp, _i32, _i32p, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p, _i32,
+                                            _i32p]),
     "gg_redistancing": (ctypes.c_int, [_vp]),
     "gg_count_grains": (ctypes.c_int, [_vp, _i32p]),
     "gg_timestep": (ctypes.c_int, [_vp, _i32, _i32]),
@@ -100,6 +103,11 @@ SIGNATURES = {
     "gg_resize_to_cube": (None, [_i32p, _i32]),
     "gg_generate_voronoi": (ctypes.c_int, [_P(GGParams), ctypes.c_uint64, _i32p, _i32p, _u8p, _i32]),
     "gg_voronoi_seeds": (ctypes.c_int, [_i32, ctypes.c_uint64, _f64p]),
+    "gg_misorientation": (ctypes.c_double, [_f64p, _f64p]),
+    "gg_gb_pair_tables": (ctypes.c_int, [_i32, _f64p, _i32, _i32, _f64p, _f64p, _i32]),
+    "gg_euler_to_quaternion": (None, [_f64p, _f64p]),
+    "gg_random_orientations": (ctypes.c_int, [_i32, ctypes.c_uint64, _i32, _f64p]),
+    "gg_lognormal_energies": (ctypes.c_int, [_i32, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, _f64p]),
     "gg_initial_neighbour_lists": (ctypes.c_int, [_P(GGParams), _i32p, _u8p, _i32p, _i32p, ctypes.c_int64,
                                                   _i64p]),
 }
@@ -199,6 +207,59 @@ def voronoi_seeds(n_grains, seed):
     if rc != GG_OK:
         raise GGError("gg_voronoi_seeds failed (%d)" % rc)
     return xyz
+
+
+class GGPhysics(ctypes.Structure):
+    """gg_physics: settings of the convolution correction."""
+    _fields_ = [("correction", ctypes.c_int32), ("mobility_model", ctypes.c_int32),
+                ("unique_energies", ctypes.c_int32), ("stored_energy", ctypes.c_int32),
+                ("triple_line_drag", ctypes.c_double)]
+
+
+def misorientation(qa, qb):
+    """Cubic misorientation of two unit quaternions, floored at 1 degree (radians)."""
+    a = np.ascontiguousarray(qa, dtype=np.float64)
+    b = np.ascontiguousarray(qb, dtype=np.float64)
+    return float(lib().gg_misorientation(_p(a, ctypes.c_double), _p(b, ctypes.c_double)))
+
+
+def gb_pair_tables(quat, mobility_model=True, unique_energies=False, nthreads=0):
+    """(energy, mobility) of every ordered id pair, each (n, n) for quat of shape (n, 4)."""
+    q = np.ascontiguousarray(quat, dtype=np.float64)
+    n = q.shape[0]
+    gam, mob = np.zeros((n, n)), np.zeros((n, n))
+    rc = lib().gg_gb_pair_tables(int(n), _p(q, ctypes.c_double), int(mobility_model), int(unique_energies),
+                                 _p(gam, ctypes.c_double), _p(mob, ctypes.c_double), int(nthreads))
+    if rc != GG_OK:
+        raise GGError("gg_gb_pair_tables failed (%d)" % rc)
+    return gam, mob
+
+
+def euler_to_quaternion(euler):
+    e = np.ascontiguousarray(euler, dtype=np.float64)
+    q = np.zeros(4)
+    lib().gg_euler_to_quaternion(_p(e, ctypes.c_double), _p(q, ctypes.c_double))
+    return q
+
+
+def random_orientations(n_grains, seed, reference_abs=False):
+    """(n_grains+1, 4) unit quaternions: row 0 the wall (Euler 0, 35 deg, 0), then Shoemake's."""
+    q = np.zeros((n_grains + 1, 4))
+    rc = lib().gg_random_orientations(int(n_grains), ctypes.c_uint64(seed), int(reference_abs),
+                                      _p(q, ctypes.c_double))
+    if rc != GG_OK:
+        raise GGError("gg_random_orientations failed (%d)" % rc)
+    return q
+
+
+def lognormal_energies(n_grains, seed, median=0.1, sigma=1.0):
+    """n_grains+1 stored energies (entry 0, the wall, is 0)."""
+    v = np.zeros(n_grains + 1)
+    rc = lib().gg_lognormal_energies(int(n_grains), ctypes.c_uint64(seed), float(median), float(sigma),
+                                     _p(v, ctypes.c_double))
+    if rc != GG_OK:
+        raise GGError("gg_lognormal_energies failed (%d)" % rc)
+    return v
 
 
 def initial_neighbour_lists(params, bounds, exists):
@@ -314,6 +375,36 @@ class LevelSetContext:
                                                   ctypes.byref(n)))
         return [(tuple(ids8[8 * r:8 * r + min(8, nids[r])].tolist()), int(ntri[r]), tuple(pos[3 * r:3 * r + 3]))
                 for r in range(n.value)]
+
+    def set_physics(self, correction=True, mobility_model=False, unique_energies=True, stored_energy=False,
+                    triple_line_drag=0.0, quat=None, see=None):
+        """gg_set_physics: the convolution correction for the following steps."""
+        ph = GGPhysics(int(correction), int(mobility_model), int(unique_energies), int(stored_energy),
+                       float(triple_line_drag))
+        q = np.ascontiguousarray(quat, dtype=np.float64) if quat is not None else None
+        e = np.ascontiguousarray(see, dtype=np.float64) if see is not None else None
+        self._keep_phys = (q, e)
+        self._chk(lib().gg_set_physics(self.h, ctypes.byref(ph), _p(q, ctypes.c_double) if q is not None else None,
+                                       _p(e, ctypes.c_double) if e is not None else None))
+
+    def correction_elements(self, gid):
+        """The interfacial elements of grain gid's last hull in projection order: dict of arrays
+        cls (3 high-order junction, 2 quadruple, 1 triple line, 0 boundary), nb (tuples of the
+        bordering ids, ascending), nbary, pos (n, 3), energy, mobility."""
+        n = ctypes.c_int32(0)
+        self._chk(lib().gg_download_elements(self.h, int(gid), None, None, None, None, None, None, None, 0,
+                                             ctypes.byref(n)))
+        k = max(n.value, 1)
+        cls, nnb, nb8, nbary = (np.zeros(k, np.int32), np.zeros(k, np.int32), np.zeros(8 * k, np.int32),
+                                np.zeros(k, np.int32))
+        pos, en, mo = np.zeros(3 * k), np.zeros(k), np.zeros(k)
+        self._chk(lib().gg_download_elements(self.h, int(gid), _p(cls, ctypes.c_int32), _p(nnb, ctypes.c_int32),
+                                             _p(nb8, ctypes.c_int32), _p(nbary, ctypes.c_int32),
+                                             _p(pos, ctypes.c_double), _p(en, ctypes.c_double),
+                                             _p(mo, ctypes.c_double), k, ctypes.byref(n)))
+        m = n.value
+        return {"cls": cls[:m], "nb": [tuple(nb8[8 * e:8 * e + min(8, nnb[e])].tolist()) for e in range(m)],
+                "nbary": nbary[:m], "pos": pos[:3 * m].reshape(m, 3), "energy": en[:m], "mobility": mo[:m]}
 
     def voxel_container(self):
         """saveNetworkAsVoxelContainer's id container (Container_size_*.raw

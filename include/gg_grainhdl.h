@@ -58,6 +58,43 @@ int gg_voronoi_seeds(int32_t n_grains, uint64_t seed, double* xyz);
 int gg_initial_neighbour_lists(const gg_params* params, const int32_t* bounds, const uint8_t* exists,
                                int32_t* offsets, int32_t* ids, int64_t cap, int64_t* n);
 
+/* ---- grain-boundary model of the convolution correction (gg_set_physics) ---- */
+
+/* Misorientation of two cubic orientations given as unit quaternions
+ * (q0, q1, q2, q3): myQuaternion::misorientationCubicQxQ
+ * (src/myQuaternion.cpp:309-367) of qa^-1 qb, floored at 1 degree as
+ * LSbox::computeMisorientation does (src/box.cpp:2704-2719); radians. */
+double gg_misorientation(const double* qa, const double* qb);
+
+/* Grain-boundary energy and mobility of every ordered pair (a, b) of ids
+ * 0..n_ids-1, at gam[a*n_ids+b] and mob[a*n_ids+b]: Read-Shockley with
+ * theta_ref = 15 degrees (1 if unique_energies) and
+ * 1 - 0.99 exp(-5 (theta/15deg)^9) floored at 0.1 (1 without the mobility
+ * model), of the misorientation above (InterfacialElement::
+ * computeReadShockleyEnergy / computeMobilityMisori,
+ * src/InterfacialElement.cpp:23-50).  quat: 4 doubles per id. */
+int gg_gb_pair_tables(int32_t n_ids, const double* quat, int32_t mobility_model, int32_t unique_energies,
+                      double* gam, double* mob, int32_t nthreads);
+
+/* Bunge (phi1, PHI, phi2) Euler angles to a quaternion
+ * (myQuaternion::euler2Quaternion, src/myQuaternion.cpp:168-191). */
+void gg_euler_to_quaternion(const double* euler, double* q);
+
+/* Seeded orientations, 4 doubles per id in quat[0 .. 4*(n_grains+1)): id 0 (the
+ * domain wall) gets the reference's boundary grain orientation, Euler
+ * (0, 35 deg, 0) (src/grainhdl.cpp:86); ids 1..n_grains Shoemake's uniform
+ * random unit quaternions from splitmix64(seed).  reference_abs != 0 takes
+ * the absolute value of every component before normalising, as the
+ * reference's randomOriShoemakeQuat does (src/myQuaternion.cpp:291-307), which
+ * confines the orientations to one orthant (not uniform). */
+int gg_random_orientations(int32_t n_grains, uint64_t seed, int32_t reference_abs, double* quat);
+
+/* Seeded lognormal stored elastic energies in normalised units (the value
+ * LSbox's constructor stores after scaling by DislocEnPerM / HAGB_Energy *
+ * Physical_Domain_Size, src/box.cpp:232-233): see[g] = median * exp(sigma *
+ * N(0,1)) for g = 1..n_grains, see[0] = 0 (the wall). */
+int gg_lognormal_energies(int32_t n_grains, uint64_t seed, double median, double sigma, double* see);
+
 #ifdef __cplusplus
 }
 #endif

@@ -169,6 +169,60 @@ int gg_voxel_container(gg_ctx* ctx, uint32_t* dst, int64_t cap, int32_t* size);
 int gg_download_hull_elements(gg_ctx* ctx, int32_t id, int32_t* nids, int32_t* ids8, int32_t* ntri, double* pos,
                               int32_t cap, int32_t* n);
 
+/* ---- convolution correction (configurations with
+ * Settings::DisableConvolutionCorrection = 0) ---- */
+
+/* Settings of the correction branch of LSbox::executeConvolution
+ * (src/box.cpp:428-536; the magnetic term is not part of this path). */
+typedef struct gg_physics {
+  int32_t correction;       /* !Settings::DisableConvolutionCorrection */
+  int32_t mobility_model;   /* Settings::UseMobilityModel: sigmoidal GB mobility in the misorientation */
+  int32_t unique_energies;  /* Settings::UniqueGBEnergies: 1 = unit GB energy, 0 = Read-Shockley */
+  int32_t stored_energy;    /* Settings::UseStoredElasticEnergy */
+  double triple_line_drag;  /* Settings::TripleLineDrag (0 gives triple lines zero mobility) */
+} gg_physics;
+
+/* Grain counts up to which misorientation-dependent energies or mobilities are
+ * accepted: the ordered-pair tables take 16 (n_grains+1)^2 bytes of device
+ * memory. */
+#define GG_MAX_ANISO_GRAINS 16384
+
+/* Turns the convolution correction on (or off) for the following steps.
+ * After every hull (gg_level_set, gg_timestep, gg_coarsen_grid) the grain's
+ * interfacial elements are built as GrainHull::computeGrainBoundaryElements,
+ * subDivideTrianglesToInterfacialElements and computeJunctionPosition with
+ * mergeJunction build them (src/grainHull.cpp:187-274, 483-531), with the
+ * energies and mobilities of src/InterfacialElement.cpp:52-404 (grain
+ * boundaries: Read-Shockley energy and misorientation mobility of the pair;
+ * triple lines: sigma = g(me,a) - g(a,b) + g(me,b) clamped to [0.01, 1],
+ * mobility 1/(1/(3 sqrt(3 ds^2) TripleLineDrag) + mean pair mobility), ds =
+ * 1/conv_domain_size; quadruple junctions: means over their three triple
+ * lines; high-order junctions: 1, 1); after every later convolution each
+ * voxel with |phi_in| < delta is corrected with the element nearest to it
+ * among those bordering its IDLocal grain (projectPointToGrainBoundary,
+ * src/grainHull.cpp:570-660): phi_out = phi_in + (phi_out - phi_in) * m * g,
+ * then, with stored energies, minus (see[self] - see[id]) * g * dt /
+ * TimeSlope^2 (TimeSlope = 1/1.21) for an id that is the wall or a grain not
+ * deleted -- where (m, g) are the element's mobility and energy (GBInfo's
+ * swapped fields, as get_GBInfo passes them).  The first convolution after the
+ * call is not corrected (run_sim's loop 0, before any hull).
+ * quat: 4 doubles per id 0..n_grains (id 0: the domain wall's pseudo grain),
+ * needed when energies or mobilities depend on the misorientation; see:
+ * n_grains+1 doubles (see[0] for the wall), needed with stored_energy.  The
+ * pair tables are computed on the host (gg_gb_pair_tables).  Drops a
+ * captured step graph; enables the hull topology. */
+int gg_set_physics(gg_ctx* ctx, const gg_physics* phys, const double* quat, const double* see);
+
+/* The interfacial elements of grain id's last hull, in the order the
+ * projection visits them (high-order junctions, quadruple junctions, triple
+ * lines, grain boundaries): per element its class (3, 2, 1, 0), the grains it
+ * borders besides the grain itself (nnb[e], ascending in nb8[8e..8e+7], -1
+ * padded), its triangles (nbary), position (junctions: the mean barycenter,
+ * summed in no fixed order; else 0), energy and mobility.  *n is set first;
+ * cls may be NULL to query it. */
+int gg_download_elements(gg_ctx* ctx, int32_t id, int32_t* cls, int32_t* nnb, int32_t* nb8, int32_t* nbary,
+                         double* pos, double* energy, double* mobility, int32_t cap, int32_t* n);
+
 /* countGrains: synchronises and returns the number of grains not deleted. */
 int gg_count_grains(gg_ctx* ctx, int32_t* count);
 
@@ -279,9 +333,10 @@ int gg_box_voxels(gg_ctx* ctx, int64_t* voxels);
  * 10 redistancing z sweeps + clamp   11 input-buffer resize   12 grain count
  * 13 comparison, 32 < n <= 48   14 comparison, 48 < n <= 64
  * 15 comparison, 64 < n <= 96   16 grid-coarsening resampling
- * 17 voxel container (clear + scatter) */
+ * 17 voxel container (clear + scatter)
+ * 18 convolution correction   19 interfacial elements of the hull */
 int gg_enable_stage_timing(gg_ctx* ctx, int32_t enable);
-#define GG_NUM_SEGMENTS 18
+#define GG_NUM_SEGMENTS 20
 /* Accumulated milliseconds and number of timed passes per segment since the
  * last reset; fills up to n entries (launches may be NULL). */
 int gg_get_stage_times(gg_ctx* ctx, double* ms, int64_t* launches, int32_t n, int32_t reset);

@@ -577,6 +577,333 @@ void hull_elements(const Buf<double>& B, const Buf<uint32_t>& idl, unsigned self
 }
 
 // ---------------------------------------------------------------------------
+// Orientation and grain-boundary model of the convolution correction
+// (src/box.cpp:428-536; configurations with DisableConvolutionCorrection = 0).
+
+struct Quat {
+  double q0, q1, q2, q3;
+};
+
+Quat quat_inverse(const Quat& q) { return Quat{q.q0, -q.q1, -q.q2, -q.q3}; }  // src/myQuaternion.cpp:54-57
+
+Quat quat_mul(const Quat& a, const Quat& b) {  // myQuaternion::operator*, src/myQuaternion.cpp:59-71
+  Quat r;
+  r.q0 = +a.q0 * b.q0 - a.q1 * b.q1 - a.q2 * b.q2 - a.q3 * b.q3;
+  r.q1 = +a.q1 * b.q0 + a.q0 * b.q1 - a.q3 * b.q2 + a.q2 * b.q3;
+  r.q2 = +a.q2 * b.q0 + a.q3 * b.q1 + a.q0 * b.q2 - a.q1 * b.q3;
+  r.q3 = +a.q3 * b.q0 - a.q2 * b.q1 + a.q1 * b.q2 + a.q0 * b.q3;
+  return r;
+}
+
+// myQuaternion::misorientationCubicQxQ (src/myQuaternion.cpp:309-367): the
+// smallest rotation angle of me^-1 * p over the 24 cubic symmetry operators
+// (the largest |component| of the six fundamental quaternion sets).
+double misorientation_cubic(const Quat& me, const Quat& p) {
+  const Quat r = quat_mul(quat_inverse(me), p);
+  const double a = r.q0, b = r.q1, c = r.q2, d = r.q3;
+  const double fac = 1.0 / sqrt(2.0);
+  const double r0[6][4] = {{(a + b) * fac, (a - b) * fac, (c + d) * fac, (c - d) * fac},
+                           {(a + c) * fac, (a - c) * fac, (b + d) * fac, (b - d) * fac},
+                           {(a + d) * fac, (a - d) * fac, (b + c) * fac, (b - c) * fac},
+                           {(a + b + c + d) * 0.5, (a + b - c - d) * 0.5, (a - b + c - d) * 0.5, (a - b - c + d) * 0.5},
+                           {(a + b + c - d) * 0.5, (a + b - c + d) * 0.5, (a - b + c + d) * 0.5, (a - b - c - d) * 0.5},
+                           {a, b, c, d}};
+  double omega = 0.0;
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j < 4; j++)
+      if (std::fabs(r0[i][j]) > omega) omega = std::fabs(r0[i][j]);
+  if (omega > 1.0) omega = (double)(int)omega;
+  omega = 2 * acos(omega);
+  return omega;
+}
+
+// LSbox::computeMisorientation (src/box.cpp:2704-2719): at least 1 degree.
+double grain_misorientation(const Quat& me, const Quat& other) {
+  const double result = misorientation_cubic(me, other);
+  if (result > 1 * kPi / 180.0) return result;
+  return 1 * kPi / 180.0;
+}
+
+// InterfacialElement::computeMobilityMisori (src/InterfacialElement.cpp:23-35).
+double mobility_misori(double misori, bool model) {
+  if (!model) return 1.0;
+  const double mob = 1.0 - (0.99 * exp(-5. * (pow(misori / (15 * kPi / 180), 9.))));
+  return mob < 0.1 ? 0.1 : mob;
+}
+
+// InterfacialElement::computeReadShockleyEnergy (src/InterfacialElement.cpp:37-50).
+double read_shockley(double misori, bool unique) {
+  if (unique) return 1.0;
+  const double gamma_hagb = 1.0;
+  const double theta_ref = 15 * kPi / 180;
+  if (misori > theta_ref) return gamma_hagb;
+  return gamma_hagb * (misori / theta_ref) * (1.0 - log(misori / theta_ref));
+}
+
+struct Physics {
+  bool on = false;               // !Settings::DisableConvolutionCorrection
+  bool mobility_model = false;   // Settings::UseMobilityModel
+  bool unique_energies = true;   // Settings::UniqueGBEnergies
+  bool stored_energy = false;    // Settings::UseStoredElasticEnergy
+  double tl_drag = 0.0;          // Settings::TripleLineDrag
+  std::vector<Quat> ori;         // m_orientationQuat per grain id (0: the domain-wall pseudo grain)
+  std::vector<double> see;       // m_StoredElasticEnergy per grain id (0: the wall, pinned 0)
+};
+
+// One interfacial element (src/InterfacialElement.h:52-235): the grains it
+// borders besides its own (m_neighborIDs), its triangles' barycenters in the
+// order they were added (m_barycenterTriangles), its GB energy and mobility
+// and, for junctions, its position (mean barycenter).
+struct Elem {
+  std::vector<unsigned> nb;
+  std::vector<V3> bary;
+  double energy = 1.0, mobility = 1.0;
+  V3 pos{0, 0, 0};
+};
+
+// The elements of a grain's last hull, per class in the order GrainHull keeps
+// them (m_HighOrderJunctions, m_QuadruplePoints, m_TripleLines, m_Grainboundary).
+struct Elements {
+  std::vector<Elem> hj, qj, tl, gb;
+  void clear() { hj.clear(); qj.clear(); tl.clear(); gb.clear(); }
+};
+
+// A hull as MarchingCubesAlgorithm::generateHull leaves it
+// (src/marchingCubes.cpp:364-478, 501-617): the distinct cell id lists in
+// first-appearance order (ascending ids, 0xFFFFFFFF entries dropped; two
+// lists that differ only in them are one list), and every triangle's
+// barycenter (Triangle::computeBarycenter, src/triangle.h:12-18) with the
+// index of its cell's list, in cell then table order.
+struct HullTris {
+  std::vector<std::vector<unsigned>> lists;
+  std::vector<V3> bary;
+  std::vector<int> key;
+};
+
+void hull_triangles(const Buf<double>& B, const Buf<uint32_t>& idl, unsigned self, HullTris& H) {
+  H.lists.clear();
+  H.bary.clear();
+  H.key.clear();
+  V3 vert[12];
+  for (int z = B.z0; z < B.z1 - 1; ++z)
+    for (int y = B.y0; y < B.y1 - 1; ++y)
+      for (int x = B.x0; x < B.x1 - 1; ++x) {
+        double val[8];
+        V3 pos[8];
+        unsigned c = 0, ids[8];
+        for (int q = 0; q < 8; ++q) {
+          int cx = x + kCorner[q][0], cy = y + kCorner[q][1], cz = z + kCorner[q][2];
+          val[q] = B.at(cy, cx, cz);
+          pos[q] = V3{(double)cx, (double)cy, (double)cz};
+          const bool inside = val[q] >= 0;
+          if (inside) c |= 1u << q;
+          if (inside) ids[q] = self;
+          else if (cx >= idl.x0 && cx < idl.x1 && cy >= idl.y0 && cy < idl.y1 && cz >= idl.z0 && cz < idl.z1)
+            ids[q] = idl.at(cy, cx, cz);
+          else
+            ids[q] = 0xFFFFFFFFu;
+        }
+        const int mask = g_edge_mask[c];
+        if (mask == 0) continue;
+        std::vector<unsigned> list;
+        for (int q = 0; q < 8; ++q)
+          if (ids[q] != 0xFFFFFFFFu && std::find(list.begin(), list.end(), ids[q]) == list.end()) list.push_back(ids[q]);
+        std::sort(list.begin(), list.end());
+        int key = (int)(std::find(H.lists.begin(), H.lists.end(), list) - H.lists.begin());
+        if (key == (int)H.lists.size()) H.lists.push_back(list);
+        for (int e = 0; e < 12; ++e)
+          if (mask & (1 << e)) {
+            int a = kEdge[e][0], b = kEdge[e][1];
+            vert[e] = vertex_interp(pos[a], pos[b], val[a], val[b]);
+          }
+        for (int t = 0; g_tri[c][t] != -1; t += 3) {
+          const V3 p0 = vert[g_tri[c][t]], p1 = vert[g_tri[c][t + 1]], p2 = vert[g_tri[c][t + 2]];
+          H.bary.push_back(V3{(p0.x + p1.x + p2.x) / 3, (p0.y + p1.y + p2.y) / 3, (p0.z + p1.z + p2.z) / 3});
+          H.key.push_back(key);
+        }
+      }
+}
+
+// Grain-boundary model of one element kind, for grain `me` and its
+// neighbours (LSbox::computeMisorientation between the grains' orientations;
+// getGrainByID(0) is the domain wall's pseudo grain).
+struct GbModel {
+  const Physics& ph;
+  double ds;  // grainhdl::get_ds() = 1 / realDomainSize as the run holds it
+  double mis(unsigned a, unsigned b) const { return grain_misorientation(ph.ori.at(a), ph.ori.at(b)); }
+  double mob(unsigned a, unsigned b) const { return mobility_misori(mis(a, b), ph.mobility_model); }
+  double gam(unsigned a, unsigned b) const { return read_shockley(mis(a, b), ph.unique_energies); }
+  // TripleLine::computeMobility / computeEnergy (src/InterfacialElement.cpp:118-164)
+  void triple(unsigned me, unsigned n0, unsigned n1, double& energy, double& mobility) const {
+    double avg = 0;
+    avg += mob(me, n0);
+    avg += mob(n0, n1);
+    avg += mob(me, n1);
+    avg /= 3;
+    const double dsl = 3 * sqrt(3 * ds * ds);
+    mobility = 1. / ((1. / (dsl * ph.tl_drag)) + 1. / avg);
+    double gamma[3];
+    gamma[0] = gam(me, n0);
+    gamma[1] = gam(n0, n1);
+    gamma[2] = gam(me, n1);
+    double sigma = gamma[0] - gamma[1] + gamma[2];
+    if (sigma < 0.01) sigma = 0.01;
+    if (sigma > 1.0) sigma = 1.0;
+    energy = sigma;
+  }
+};
+
+V3 mean_position(const std::vector<V3>& b) {  // computePosition, src/InterfacialElement.cpp:406-418, 457-469
+  V3 p{0, 0, 0};
+  for (const V3& v : b) {
+    p.x += v.x;
+    p.y += v.y;
+    p.z += v.z;
+  }
+  const double n = (double)b.size();
+  p.x /= n;
+  p.y /= n;
+  p.z /= n;
+  return p;
+}
+
+double dist3(const V3& a, const V3& b) {  // (a - b).norm()
+  const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+// GrainHull::computeGrainBoundaryElements + subDivideTrianglesToInterfacialElements
+// + computeJunctionPosition with mergeJunction (src/grainHull.cpp:187-274,
+// 483-531) and the element constructors (src/InterfacialElement.cpp:52-404).
+void build_elements(const GbModel& M, unsigned me, const HullTris& H, Elements& E) {
+  E.clear();
+  const int L = (int)H.lists.size();
+  std::vector<int> cat(L), idx(L);  // class (0 GB, 1 TL, 2 QJ, 3 HJ) and index in it, per list key
+  for (int k = 0; k < L; ++k) {
+    const std::vector<unsigned>& l = H.lists[k];
+    Elem e;
+    for (unsigned id : l)
+      if (id != me) e.nb.push_back(id);
+    switch ((int)l.size()) {
+      case 2:  // GrainBoundary: the other id (src/InterfacialElement.cpp:52-79)
+        e.nb.assign(1, me != l[0] ? l[0] : l[1]);
+        e.mobility = M.mob(me, e.nb[0]);
+        e.energy = M.gam(me, e.nb[0]);
+        cat[k] = 0; idx[k] = (int)E.gb.size(); E.gb.push_back(e);
+        break;
+      case 3:  // TripleLine
+        M.triple(me, e.nb.at(0), e.nb.at(1), e.energy, e.mobility);
+        cat[k] = 1; idx[k] = (int)E.tl.size(); E.tl.push_back(e);
+        break;
+      case 4: {  // QuadrupleJunction: means over its three triple lines (src/InterfacialElement.cpp:371-404)
+        double e1, m1, e2, m2, e3, m3;
+        M.triple(me, e.nb.at(0), e.nb.at(1), e1, m1);
+        M.triple(me, e.nb.at(1), e.nb.at(2), e2, m2);
+        M.triple(me, e.nb.at(0), e.nb.at(2), e3, m3);
+        e.mobility = (m1 + m2 + m3) / 3.0;
+        e.energy = (e1 + e2 + e3) / 3.0;
+        cat[k] = 2; idx[k] = (int)E.qj.size(); E.qj.push_back(e);
+        break;
+      }
+      default:  // HighOrderJunction: energy and mobility 1 (src/InterfacialElement.cpp:313-340)
+        e.energy = e.mobility = 1.0;
+        cat[k] = 3; idx[k] = (int)E.hj.size(); E.hj.push_back(e);
+    }
+  }
+  for (size_t t = 0; t < H.bary.size(); ++t) {
+    const int k = H.key[t];
+    std::vector<Elem>& v = cat[k] == 0 ? E.gb : cat[k] == 1 ? E.tl : cat[k] == 2 ? E.qj : E.hj;
+    v[idx[k]].bary.push_back(H.bary[t]);
+  }
+  for (Elem& q : E.qj) q.pos = mean_position(q.bary);
+  for (Elem& q : E.hj) q.pos = mean_position(q.bary);
+  // mergeJunction (src/grainHull.cpp:483-531)
+  for (int i = 0; i < (int)E.qj.size(); i++) {
+    for (int j = i + 1; j < (int)E.qj.size(); j++) {
+      if (dist3(E.qj[i].pos, E.qj[j].pos) < 2) {
+        // HighOrderJunction(A, B) (src/InterfacialElement.cpp:263-311): A's three
+        // ids, then those of B's ids that are already among them; A's then B's
+        // barycenters
+        const Elem& A = E.qj[i];
+        const Elem& Bq = E.qj[j];
+        Elem h;
+        h.nb.assign(A.nb.begin(), A.nb.begin() + 3);
+        for (int b = 0; b < 3; ++b) {
+          bool found = false;
+          for (int a = 0; a < 3; ++a) found |= h.nb[a] == Bq.nb.at(b);
+          if (found) h.nb.push_back(Bq.nb[b]);
+        }
+        h.bary = A.bary;
+        h.bary.insert(h.bary.end(), Bq.bary.begin(), Bq.bary.end());
+        h.pos = mean_position(h.bary);
+        h.energy = h.mobility = 1.0;
+        E.qj.erase(E.qj.begin() + i);
+        E.qj.erase(E.qj.begin() + j - 1);
+        i--;
+        j--;
+        E.hj.push_back(h);
+        break;
+      }
+    }
+  }
+  // mergeWith (src/InterfacialElement.cpp:342-364): every id of B is appended
+  auto merge_with = [](Elem& h, const Elem& b) {
+    h.bary.insert(h.bary.end(), b.bary.begin(), b.bary.end());
+    for (unsigned id : b.nb) h.nb.push_back(id);
+    h.pos = mean_position(h.bary);
+    h.energy = h.mobility = 1.0;
+  };
+  for (int k = 0; k < (int)E.hj.size(); k++)
+    for (int i = 0; i < (int)E.qj.size(); i++)
+      if (dist3(E.qj[i].pos, E.hj[k].pos) < 2) {
+        merge_with(E.hj[k], E.qj[i]);
+        E.qj.erase(E.qj.begin() + i);
+        i--;
+      }
+  for (int i = 0; i < (int)E.hj.size(); i++)
+    for (int k = i + 1; k < (int)E.hj.size(); k++)
+      if (dist3(E.hj[i].pos, E.hj[k].pos) < 2) {
+        const Elem other = E.hj[k];
+        merge_with(E.hj[i], other);
+        E.hj.erase(E.hj.begin() + k);
+        k--;
+      }
+}
+
+// GrainHull::projectPointToGrainBoundary (src/grainHull.cpp:570-660): the
+// energy and mobility of the element, among those bordering grain `id`, with
+// the triangle barycenter nearest to `point` (squared distance; the first
+// element in the order high-order junctions, quadruple junctions, triple
+// lines, boundaries wins ties).  The weight is returned as GBInfo holds it:
+// get_GBInfo passes (m_energy, m_mobility) to GBInfo(mobility, energy)
+// (src/InterfacialElement.h:22-31, 70-72), so w_energy is the element's
+// mobility and w_mobility its energy.  Default GBInfo(1, 1).
+void project_point(const Elements& E, const V3& point, unsigned id, double& w_energy, double& w_mobility) {
+  double minimal = 10000000.0;
+  w_energy = 1.;
+  w_mobility = 1.;
+  auto scan = [&](const Elem& e) {
+    for (const V3& b : e.bary) {
+      const double dx = point.x - b.x, dy = point.y - b.y, dz = point.z - b.z;
+      const double d = dx * dx + dy * dy + dz * dz;
+      if (d < minimal) {
+        minimal = d;
+        w_energy = e.mobility;
+        w_mobility = e.energy;
+      }
+    }
+  };
+  for (const Elem& e : E.hj)
+    if (std::find(e.nb.begin(), e.nb.end(), id) != e.nb.end()) scan(e);
+  for (const Elem& e : E.qj)
+    if (e.nb.at(0) == id || e.nb.at(1) == id || e.nb.at(2) == id) scan(e);
+  for (const Elem& e : E.tl)
+    if (e.nb.at(0) == id || e.nb.at(1) == id) scan(e);
+  for (const Elem& e : E.gb)
+    if (e.nb.at(0) == id) scan(e);
+}
+
+// ---------------------------------------------------------------------------
 enum { kDeleted = 0, kAlive = 1, kVanished = 2 };
 
 struct Grain {
@@ -592,6 +919,7 @@ struct Grain {
   std::vector<unsigned> hull_nb;  // grains the last hull touched, in first-appearance order
   int hull_junc[4] = {0, 0, 0, 0};  // distinct cell id lists of 2, 3, 4 and other sizes
   std::vector<JRec> hull_je;        // the distinct lists with their triangles, ids ascending
+  Elements elem;                    // interfacial elements of the last hull (convolution correction on)
 };
 
 }  // namespace
@@ -610,6 +938,8 @@ struct orc_params {
 struct orc_sim {
   orc_params p;
   std::vector<Grain> g;  // index = grain id; 0 is the domain wall pseudo grain
+  Physics phys;          // convolution-correction settings, orientations and stored energies
+  bool hull_ready = false;  // a level set has built the hulls the correction reads (run_sim's loop != 0)
   std::map<int, ConvTables> tables;
   std::string err;
 
@@ -773,15 +1103,23 @@ void set_comparison(orc_sim& S, Grain& G) {  // executeSetComparison, src/box.cp
   G.nb[3] = nxmax; G.nb[4] = nymax; G.nb[5] = nzmax;
 }
 
+// Timing sample (CPU baseline only): when above 1, every stage visits only the
+// grains whose id is a multiple of it, so a single-thread run of a timestep
+// costs a bounded time; the other grains keep stale state, so a sampled run's
+// results are not checked against anything.
+int g_sample_stride = 1;
+
 template <class F>
 void for_grains(orc_sim& S, F f) {
   // static round-robin grain lists, src/IterativeGrainScheduler.cpp:13-26
   const int N = (int)S.g.size() - 1;
+  const int stride = g_sample_stride;
   std::string err;
 #pragma omp parallel
   {
     int nt = omp_get_num_threads(), t = omp_get_thread_num();
     for (int id = 1 + t; id <= N; id += nt) {
+      if (stride > 1 && id % stride != 0) continue;
       try {
         f(S.g[id]);
       } catch (std::exception& e) {
@@ -791,6 +1129,54 @@ void for_grains(orc_sim& S, F f) {
     }
   }
   if (!err.empty()) throw std::runtime_error(err);
+}
+
+// The convolution correction of one grain (src/box.cpp:428-536; the magnetic
+// term is not part of this path): for every voxel of the output box (IDLocal
+// was just resized to it, so the IDLocal/output intersection is the box) with
+// |phi_in| < delta, the convolved value is pulled back towards phi_in by the
+// energy times mobility of the interfacial element nearest to the voxel among
+// those bordering the IDLocal grain there, and with stored energies on, the
+// driving-force shift (SEE_self - SEE_neighbour) * mobility * dt / TimeSlope^2
+// is subtracted (for a neighbour getGrainByID returns: the wall, or a grain not
+// deleted).  The point is Vector3d(i, j, k) = (row y, column x, depth z),
+// compared with barycenters in (x, y, z): the reference's order, kept.
+void correction(const orc_sim& S, Grain& G) {
+  const Buf<double>& in = *G.in;
+  Buf<double>& out = *G.out;
+  const double delta = S.p.delta, dt = S.p.dt;
+  const double time_slope = 1 / 1.21;  // grainhdl::TimeSlope, Gaussian mode (src/grainhdl.cpp:1432)
+  const double radiuscorrection = 1.0;
+  for (int k = out.z0; k < out.z1; k++)
+    for (int i = out.y0; i < out.y1; i++)
+      for (int j = out.x0; j < out.x1; j++) {
+        const double val = in.at(i, j, k);
+        if (std::abs(val) < delta) {
+          const V3 point{(double)i, (double)j, (double)k};
+          const unsigned id = G.idl.at(i, j, k);
+          double w_energy, w_mobility;
+          project_point(G.elem, point, id, w_energy, w_mobility);
+          out.at(i, j, k) = val + (out.at(i, j, k) - val) * w_energy * w_mobility * radiuscorrection;
+          if (S.phys.stored_energy) {
+            const bool neighbour = id == 0 || (id < S.g.size() && S.g[id].state != kDeleted);
+            double f_stored = 0;
+            if (neighbour)
+              f_stored = (S.phys.see[G.id] - S.phys.see[id]) * w_mobility * dt / (time_slope * time_slope);
+            out.at(i, j, k) = (out.at(i, j, k) - f_stored);
+          }
+        }
+      }
+}
+
+// The interfacial elements of a grain's new hull (extractContour ->
+// computeSurfaceElements, src/box.cpp:1144-1180), with the convolution
+// correction on.
+void rebuild_elements(const orc_sim& S, Grain& G) {
+  if (!S.phys.on) return;
+  HullTris H;
+  hull_triangles(*G.in, G.idl, G.id, H);
+  const GbModel M{S.phys, 1.0 / (double)S.p.conv_domain_size};
+  build_elements(M, G.id, H, G.elem);
 }
 
 // LSbox::resizeGrid (src/box.cpp:2595-2697) of one grain.  The new bounds are
@@ -857,6 +1243,9 @@ orc_sim* orc_create(const orc_params* p) {
 void orc_destroy(orc_sim* S) { delete S; }
 
 void orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+void orc_set_sample_stride(int stride) { g_sample_stride = stride > 1 ? stride : 1; }
+
+
 
 #define ORC_TRY(S, ...)                                       \
   try {                                                       \
@@ -886,6 +1275,110 @@ int orc_set_boxes(orc_sim* S, const int32_t* bounds, const uint8_t* exists) {
     }
   })
 }
+
+// Convolution-correction settings (Settings::DisableConvolutionCorrection,
+// UseMobilityModel, UniqueGBEnergies, UseStoredElasticEnergy, TripleLineDrag),
+// orientations quat[4*id .. 4*id+3] = (q0, q1, q2, q3) and stored energies
+// see[id] for ids 0..n_grains (id 0: the domain wall's pseudo grain).
+int orc_set_physics(orc_sim* S, int correction_on, int mobility_model, int unique_energies, int stored_energy,
+                    double tl_drag, const double* quat, const double* see) {
+  ORC_TRY(S, {
+    Physics& P = S->phys;
+    P.on = correction_on != 0;
+    P.mobility_model = mobility_model != 0;
+    P.unique_energies = unique_energies != 0;
+    P.stored_energy = stored_energy != 0;
+    P.tl_drag = tl_drag;
+    const int N = S->p.n_grains;
+    P.ori.assign((size_t)N + 1, Quat{1, 0, 0, 0});
+    P.see.assign((size_t)N + 1, 0.0);
+    for (int id = 0; id <= N; ++id) {
+      if (quat) P.ori[id] = Quat{quat[4 * id], quat[4 * id + 1], quat[4 * id + 2], quat[4 * id + 3]};
+      if (see) P.see[id] = see[id];
+    }
+  })
+}
+
+// The interfacial elements of grain id's last hull in projection order
+// (high-order junctions, quadruple junctions, triple lines, boundaries): per
+// element its class (3, 2, 1, 0), the number of ids and the first 8 of them,
+// the number of barycenters, the position (junctions; else 0), energy and
+// mobility.  Returns the element count (or -1 if cap is too small); cls may
+// be NULL to query.
+long orc_get_elements(orc_sim* S, int id, int32_t* cls, int32_t* nnb, int32_t* nb8, int32_t* nbary, double* pos,
+                      double* energy, double* mobility, long cap) {
+  const Elements& E = S->g.at(id).elem;
+  const std::vector<Elem>* v[4] = {&E.hj, &E.qj, &E.tl, &E.gb};
+  long n = 0;
+  for (int c = 0; c < 4; ++c) n += (long)v[c]->size();
+  if (!cls) return n;
+  if (n > cap) return -1;
+  long r = 0;
+  for (int c = 0; c < 4; ++c)
+    for (const Elem& e : *v[c]) {
+      cls[r] = 3 - c;
+      nnb[r] = (int32_t)e.nb.size();
+      for (int q = 0; q < 8; ++q) nb8[8 * r + q] = q < (int)e.nb.size() ? (int32_t)e.nb[q] : -1;
+      nbary[r] = (int32_t)e.bary.size();
+      pos[3 * r] = e.pos.x;
+      pos[3 * r + 1] = e.pos.y;
+      pos[3 * r + 2] = e.pos.z;
+      energy[r] = e.energy;
+      mobility[r] = e.mobility;
+      ++r;
+    }
+  return n;
+}
+
+// The barycenters of element e (projection order, as orc_get_elements lists
+// them) of grain id's last hull, xyz per triangle.  Returns their number (or -1
+// if e is out of range or cap too small); xyz may be NULL to query.
+long orc_get_element_barycenters(orc_sim* S, int id, long e, double* xyz, long cap) {
+  const Elements& E = S->g.at(id).elem;
+  const std::vector<Elem>* v[4] = {&E.hj, &E.qj, &E.tl, &E.gb};
+  for (int c = 0; c < 4; ++c) {
+    if (e < (long)v[c]->size()) {
+      const Elem& el = (*v[c])[e];
+      const long n = (long)el.bary.size();
+      if (!xyz) return n;
+      if (n > cap) return -1;
+      for (long t = 0; t < n; ++t) {
+        xyz[3 * t] = el.bary[t].x;
+        xyz[3 * t + 1] = el.bary[t].y;
+        xyz[3 * t + 2] = el.bary[t].z;
+      }
+      return n;
+    }
+    e -= (long)v[c]->size();
+  }
+  return -1;
+}
+
+// The neighbour ids of element e (m_neighborIDs, with the repeats merging
+// leaves); returns their number (or -1), ids may be NULL to query.
+long orc_get_element_ids(orc_sim* S, int id, long e, int32_t* ids, long cap) {
+  const Elements& E = S->g.at(id).elem;
+  const std::vector<Elem>* v[4] = {&E.hj, &E.qj, &E.tl, &E.gb};
+  for (int c = 0; c < 4; ++c) {
+    if (e < (long)v[c]->size()) {
+      const Elem& el = (*v[c])[e];
+      const long n = (long)el.nb.size();
+      if (!ids) return n;
+      if (n > cap) return -1;
+      for (long t = 0; t < n; ++t) ids[t] = (int32_t)el.nb[t];
+      return n;
+    }
+    e -= (long)v[c]->size();
+  }
+  return -1;
+}
+
+// Scalar parts of the grain-boundary model, for tests.
+double orc_misorientation(const double* a, const double* b) {
+  return grain_misorientation(Quat{a[0], a[1], a[2], a[3]}, Quat{b[0], b[1], b[2], b[3]});
+}
+double orc_read_shockley(double misori, int unique) { return read_shockley(misori, unique != 0); }
+double orc_mobility(double misori, int model) { return mobility_misori(misori, model != 0); }
 
 // distanceInitialisation (src/grainhdl.cpp:580-597) -> calculateDistanceFunction
 // (src/box.cpp:374-397).  idfield is ngridpoints^3, x fastest.
@@ -948,6 +1441,7 @@ int orc_convolution(orc_sim* S) {  // executeConvolution, src/box.cpp:414-539
       if (G.in->v.size() < (size_t)n * n * n) throw std::runtime_error("input buffer too small");
       convolve_box(S->tables.at(n), G.in->v.data(), G.out->v.data());
       G.idl.resize(G.out->x0, G.out->y0, G.out->z0, G.out->x1, G.out->y1, G.out->z1);
+      if (S->phys.on && S->hull_ready) correction(*S, G);
     });
   })
 }
@@ -989,13 +1483,16 @@ int orc_level_set(orc_sim* S) {  // level_set, src/grainhdl.cpp:669-684 + extrac
       hull_elements(*G.in, G.idl, G.id, G.hull_je);
       if (!ex) {
         G.state = kVanished;
+        G.elem.clear();
         return;
       }
+      rebuild_elements(*S, G);
       G.out->resize(G.nb[0], G.nb[1], G.nb[2], G.nb[3], G.nb[4], G.nb[5]);
       resize_to_cube(*G.out, L);
       G.volume = std::abs(vr) * h * h * h;
       G.area = ar * h * h;
     });
+    S->hull_ready = true;
   })
 }
 
@@ -1051,8 +1548,10 @@ int orc_coarsen(orc_sim* S, int new_size, double new_dt) {
       hull_elements(*G.in, G.idl, G.id, G.hull_je);
       if (!ex) {
         G.state = kVanished;
+        G.elem.clear();
         return;
       }
+      rebuild_elements(*S, G);
       G.out->resize(G.nb[0], G.nb[1], G.nb[2], G.nb[3], G.nb[4], G.nb[5]);
       resize_to_cube(*G.out, L);
       G.volume = std::abs(vr) * h * h * h;
@@ -1062,6 +1561,7 @@ int orc_coarsen(orc_sim* S, int new_size, double new_dt) {
       Grain& G = S->g[id];
       if (G.state != kDeleted) std::swap(G.in, G.out);
     }
+    S->hull_ready = true;
   })
 }
 

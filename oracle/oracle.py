@@ -51,6 +51,16 @@ def lib():
             "orc_create": (vp, [P(OrcParams)]),
             "orc_destroy": (None, [vp]),
             "orc_set_threads": (None, [ctypes.c_int]),
+            "orc_set_sample_stride": (None, [ctypes.c_int]),
+            "orc_set_physics": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_double, f64p, f64p]),
+            "orc_get_elements": (ctypes.c_long, [vp, ctypes.c_int, i32p, i32p, i32p, i32p, f64p, f64p, f64p,
+                                                 ctypes.c_long]),
+            "orc_get_element_barycenters": (ctypes.c_long, [vp, ctypes.c_int, ctypes.c_long, f64p, ctypes.c_long]),
+            "orc_get_element_ids": (ctypes.c_long, [vp, ctypes.c_int, ctypes.c_long, i32p, ctypes.c_long]),
+            "orc_misorientation": (ctypes.c_double, [f64p, f64p]),
+            "orc_read_shockley": (ctypes.c_double, [ctypes.c_double, ctypes.c_int]),
+            "orc_mobility": (ctypes.c_double, [ctypes.c_double, ctypes.c_int]),
             "orc_set_boxes": (ctypes.c_int, [vp, i32p, u8p]),
             "orc_init_distance": (ctypes.c_int, [vp, i32p]),
             "orc_set_lists": (ctypes.c_int, [vp, i32p, i32p]),
@@ -106,6 +116,26 @@ def _p(a, ct):
 
 def set_threads(n):
     lib().orc_set_threads(int(n))
+
+
+def misorientation(q1, q2):
+    """LSbox::computeMisorientation of two unit quaternions (q0, q1, q2, q3), radians, at least 1 degree."""
+    a = np.ascontiguousarray(q1, dtype=np.float64)
+    b = np.ascontiguousarray(q2, dtype=np.float64)
+    return float(lib().orc_misorientation(_p(a, ctypes.c_double), _p(b, ctypes.c_double)))
+
+
+def read_shockley(misori, unique=False):
+    return float(lib().orc_read_shockley(float(misori), 1 if unique else 0))
+
+
+def mobility(misori, model=True):
+    return float(lib().orc_mobility(float(misori), 1 if model else 0))
+
+
+def set_sample_stride(stride):
+    """Timing only: stages visit just the grains whose id is a multiple of stride."""
+    lib().orc_set_sample_stride(int(stride))
 
 
 # ---- standalone helpers ---------------------------------------------------
@@ -305,6 +335,49 @@ class OracleSim:
         """gridCoarsement's coarsening branch; a coarsening step is coarsen()
         followed by timestep()."""
         self._chk(lib().orc_coarsen(self.h, int(new_size), float(new_dt)))
+
+    def set_physics(self, correction=True, mobility_model=False, unique_energies=True, stored_energy=False,
+                    tl_drag=0.0, quat=None, see=None):
+        """Convolution-correction settings, orientations (rows 0..n_grains of (q0, q1, q2, q3); row 0
+        the domain wall's pseudo grain) and stored energies (0..n_grains)."""
+        q = None if quat is None else np.ascontiguousarray(quat, dtype=np.float64)
+        e = None if see is None else np.ascontiguousarray(see, dtype=np.float64)
+        self._chk(lib().orc_set_physics(self.h, int(bool(correction)), int(bool(mobility_model)),
+                                        int(bool(unique_energies)), int(bool(stored_energy)), float(tl_drag),
+                                        None if q is None else _p(q, ctypes.c_double),
+                                        None if e is None else _p(e, ctypes.c_double)))
+
+    def correction_elements(self, gid):
+        """The interfacial elements of the grain's last hull in projection order: [(class 3 HJ /
+        2 QJ / 1 TL / 0 GB, ids tuple, barycenter count, position, energy, mobility)]."""
+        n = lib().orc_get_elements(self.h, int(gid), None, None, None, None, None, None, None, 0)
+        k = max(n, 1)
+        cls, nnb, nbary = np.zeros(k, np.int32), np.zeros(k, np.int32), np.zeros(k, np.int32)
+        nb8 = np.zeros(8 * k, np.int32)
+        pos, en, mob = np.zeros(3 * k), np.zeros(k), np.zeros(k)
+        lib().orc_get_elements(self.h, int(gid), _p(cls, ctypes.c_int32), _p(nnb, ctypes.c_int32),
+                               _p(nb8, ctypes.c_int32), _p(nbary, ctypes.c_int32), _p(pos, ctypes.c_double),
+                               _p(en, ctypes.c_double), _p(mob, ctypes.c_double), k)
+        return [(int(cls[r]), tuple(nb8[8 * r:8 * r + min(8, nnb[r])].tolist()), int(nbary[r]),
+                 tuple(pos[3 * r:3 * r + 3]), float(en[r]), float(mob[r])) for r in range(n)]
+
+    def element_barycenters(self, gid, e):
+        """(n, 3) barycenters (x, y, z) of element e of correction_elements(gid)."""
+        n = lib().orc_get_element_barycenters(self.h, int(gid), int(e), None, 0)
+        if n < 0:
+            raise IndexError("no element %d" % e)
+        xyz = np.zeros(3 * max(n, 1))
+        lib().orc_get_element_barycenters(self.h, int(gid), int(e), _p(xyz, ctypes.c_double), n)
+        return xyz[:3 * n].reshape(n, 3)
+
+    def element_ids(self, gid, e):
+        """All neighbour ids of element e of correction_elements(gid), repeats included."""
+        n = lib().orc_get_element_ids(self.h, int(gid), int(e), None, 0)
+        if n < 0:
+            raise IndexError("no element %d" % e)
+        ids = np.zeros(max(n, 1), np.int32)
+        lib().orc_get_element_ids(self.h, int(gid), int(e), _p(ids, ctypes.c_int32), n)
+        return ids[:n].copy()
 
     def hull_elements(self, gid):
         """The grain's interfacial elements sorted by ids: [(ids tuple, ntri,

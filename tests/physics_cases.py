@@ -1,0 +1,138 @@
+"""Inputs of the convolution-correction tests (CPU oracle and GPU parity).
+
+* sphere_in_matrix(): grain 1 a ball inside grain 2, which fills the rest of
+  the domain interior, so grain 1's hull borders only grain 2 and grain 2's
+  only grain 1 and the domain wall: every element is a grain boundary and the
+  correction of each band voxel follows from the pair's energy and mobility.
+* voronoi_case(): a seeded Voronoi microstructure with random orientations and
+  lognormal stored energies, whose hulls have triple lines, quadruple and
+  high-order junctions.
+* cubic_group(): the 24 rotations of the cube as unit quaternions, for a
+  brute-force misorientation.
+"""
+import itertools
+
+import numpy as np
+
+import gragles_3d_amd as G
+
+
+def quat_mul(a, b):
+    """Hamilton product of (..., 4) arrays (w, x, y, z)."""
+    aw, ax, ay, az = np.moveaxis(a, -1, 0)
+    bw, bx, by, bz = np.moveaxis(b, -1, 0)
+    return np.stack([aw * bw - ax * bx - ay * by - az * bz,
+                     aw * bx + ax * bw + ay * bz - az * by,
+                     aw * by - ax * bz + ay * bw + az * bx,
+                     aw * bz + ax * by - ay * bx + az * bw], axis=-1)
+
+
+def axis_angle(axis, angle):
+    axis = np.asarray(axis, dtype=np.float64)
+    axis = axis / np.linalg.norm(axis)
+    return np.concatenate([[np.cos(angle / 2)], np.sin(angle / 2) * axis])
+
+
+def cubic_group():
+    """The 24 proper rotations of the cube: identity; 90/180/270 degrees about the
+    three <100> axes; 180 about the six <110>; 120/240 about the four <111>."""
+    ops = [np.array([1.0, 0, 0, 0])]
+    for ax in np.eye(3):
+        for k in (1, 2, 3):
+            ops.append(axis_angle(ax, k * np.pi / 2))
+    for ax in ([1, 1, 0], [1, -1, 0], [1, 0, 1], [1, 0, -1], [0, 1, 1], [0, 1, -1]):
+        ops.append(axis_angle(ax, np.pi))
+    for sx, sy in itertools.product((1, -1), repeat=2):
+        for k in (1, 2):
+            ops.append(axis_angle([1, sx, sy], k * 2 * np.pi / 3))
+    return np.array(ops)
+
+
+def misorientation_brute_force(qa, qb):
+    """Smallest rotation angle of qa^-1 qb s over the 24 cubic operators s,
+    floored at 1 degree (radians)."""
+    inv = np.asarray(qa, dtype=np.float64) * np.array([1, -1, -1, -1])
+    r = quat_mul(inv, np.asarray(qb, dtype=np.float64))
+    w = np.abs(quat_mul(np.broadcast_to(r, (24, 4)), cubic_group())[:, 0]).max()
+    ang = 2 * np.arccos(min(w, 1.0))
+    return max(ang, np.pi / 180)
+
+
+def read_shockley(theta, unique=False):
+    if unique:
+        return 1.0
+    ref = 15 * np.pi / 180
+    return 1.0 if theta > ref else (theta / ref) * (1 - np.log(theta / ref))
+
+
+def sigmoid_mobility(theta, model=True):
+    if not model:
+        return 1.0
+    return max(0.1, 1 - 0.99 * np.exp(-5 * (theta / (15 * np.pi / 180)) ** 9))
+
+
+def sphere_in_matrix(rds=30, border=3, radius=6.5):
+    """(params, idfield, bounds, exists, offsets, ids): grain 1 is the ball of
+    `radius` voxels around the interior centre, grain 2 the rest of the
+    interior; the border shell is the wall (id 0)."""
+    ngp = rds + 2 * border
+    p = G.GGParams()
+    p.ngridpoints, p.grid_blowup, p.conv_domain_size, p.boundary_grain_tube = ngp, border, rds - 1, 0
+    p.h, p.delta, p.dt = 1.0 / rds, border / rds, 0.2 / rds ** 2
+    p.precision, p.n_grains = G.FP64_PLAIN, 2
+    idf = np.zeros((ngp, ngp, ngp), dtype=np.int32)  # [z][y][x]
+    lo, hi = border, ngp - 1 - border  # interior: lo <= i, j, k < hi
+    z, y, x = np.mgrid[0:ngp, 0:ngp, 0:ngp]
+    c = (lo + hi - 1) / 2.0
+    inside = (x >= lo) & (x < hi) & (y >= lo) & (y < hi) & (z >= lo) & (z < hi)
+    ball = inside & ((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2 <= radius ** 2)
+    idf[inside] = 2
+    idf[ball] = 1
+    bounds = np.zeros((3, 6), dtype=np.int32)
+    exists = np.zeros(3, dtype=np.uint8)
+    for g in (1, 2):
+        zz, yy, xx = np.nonzero(idf == g)
+        bounds[g] = [max(xx.min() - border, 0), max(yy.min() - border, 0), max(zz.min() - border, 0),
+                     min(xx.max() + 1 + border, ngp), min(yy.max() + 1 + border, ngp),
+                     min(zz.max() + 1 + border, ngp)]
+        exists[g] = 1
+    offsets = np.array([0, 0, 1, 2], dtype=np.int32)
+    ids = np.array([2, 1], dtype=np.int32)
+    return p, idf, bounds, exists, offsets, ids
+
+
+def voronoi_case(n_grains=27, ppg=10, border=3, seed=5):
+    p = G.derive_params(n_grains, ppg, border)
+    idf, bounds, exists = G.generate_voronoi(p, seed)
+    offs, ids = G.initial_neighbour_lists(p, bounds, exists)
+    return p, idf, bounds, exists, offs, ids
+
+
+def oracle_sim(O, case):
+    p, idf, bounds, exists, offs, ids = case
+    orc = O.OracleSim(p.ngridpoints, p.grid_blowup, p.conv_domain_size, p.h, p.delta, p.dt, p.n_grains)
+    orc.set_boxes(bounds, exists)
+    orc.init_distance(idf)
+    orc.set_lists(offs, ids)
+    return orc
+
+
+def gpu_context(case, precision=G.FP64_PLAIN):
+    p, idf, bounds, exists, offs, ids = case
+    ctx = G.LevelSetContext(p, precision=precision)
+    ctx.set_boxes(bounds, exists)
+    ctx.init_distance(idf)
+    ctx.set_neighbour_lists(offs, ids)
+    return ctx
+
+
+def low_angle_orientations(n_grains, seed, spread_deg=12.0):
+    """Orientations scattered by up to spread_deg about one reference (so most
+    pairs are low-angle boundaries); row 0 the wall as the library gives it."""
+    q = G.random_orientations(n_grains, seed)
+    rng = np.random.default_rng(seed)
+    ref = axis_angle([1, 2, 3], 0.4)
+    for g in range(1, n_grains + 1):
+        d = axis_angle(rng.normal(size=3), np.deg2rad(spread_deg) * rng.random())
+        q[g] = quat_mul(ref, d)
+    return q

@@ -17,7 +17,7 @@ def header_symbols():
     for f in os.listdir(os.path.join(ROOT, "include")):
         if f.endswith(".h"):
             txt = open(os.path.join(ROOT, "include", f)).read()
-            syms |= set(re.findall(r"^\s*(?:int|void|const char\*)\s+(gg_\w+)\s*\(", txt, re.M))
+            syms |= set(re.findall(r"^\s*(?:int|void|double|const char\*)\s+(gg_\w+)\s*\(", txt, re.M))
     return syms
 
 
