@@ -45,7 +45,7 @@ import gragles_3d_amd as G  # noqa: E402
 METRIC = "grain-box voxel updates/sec per timestep, FP32-batched vs FP64-plain; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 CLASS_NMAX = (32, 48, 64, 96)  # box-size classes of the work plan, one launch each per stage
-CMP_SHAPES = ((8, 8), (16, 8), (16, 16), (32, 8))  # comparison tile shapes (layers, gathers in flight), gg_autotune
+CMP_SHAPES = G.levelset.CMP_SHAPES  # comparison tile shapes of gg_autotune (layers, gathers in flight, run-major)
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_fp32_8000grains.md")
 
 
@@ -340,6 +340,10 @@ def main():
                     help="keep the default comparison tile shapes instead of autotuning them (one timestep)")
     ap.add_argument("--tuning", metavar="S0,S1,S2,S3",
                     help="comparison tile shape per size class (0..3), instead of autotuning them")
+    ap.add_argument("--no-physics", action="store_true",
+                    help="skip the configurations with the convolution correction (anisotropic, stored energy)")
+    ap.add_argument("--physics-steps", type=int, default=10)
+    ap.add_argument("--physics-accuracy-steps", type=int, default=50)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the voxel-container and grid-coarsening legs")
     ap.add_argument("--summarize-rocprof", metavar="DIR", help="print markdown tables of a rocprofv3 directory")
@@ -395,8 +399,8 @@ def main():
         ms, nl = seg32["comparison_n%d" % m]
         avg_s = ms / max(nl, 1) * 1e-3
         bts = cls_bytes[c] / ks
-        zl, qb = CMP_SHAPES[tuning[c]]
-        cls_rows.append({"kernel": "k_cmp_cols<float, %d, %d, %d>" % (m, zl, qb), "box_voxels": int(cls_vox[c] / ks),
+        zl, qb, zmaj = CMP_SHAPES[tuning[c]]
+        cls_rows.append({"kernel": "k_cmp_cols<float, %d, %d, %d, %s>" % (m, zl, qb, "true" if zmaj else "false"), "box_voxels": int(cls_vox[c] / ks),
                          "kbar": cls_num[c] / cls_vox[c] if cls_vox[c] else 0.0, "bytes_per_launch": bts,
                          "avg_launch_ms": avg_s * 1e3,
                          "achieved_GBs": bts / avg_s / 1e9 if avg_s > 0 and bts > 0 else 0.0})
@@ -440,6 +444,18 @@ def main():
             coarsen["grid_from"], coarsen["grid_to"], coarsen["call_ms"], coarsen["resample_ms"],
             coarsen["resample_GBs"]))
     ctx32.close()
+
+    # ---- configurations with the convolution correction ------------------------------------------
+    physics = None
+    if not args.no_physics:
+        physics = {}
+        for kind, grains in (("anisotropic", args.grains), ("stored_energy", 20000)):
+            leg = physics_leg(kind, grains, args.ppg, args.border, args.seed, args.physics_steps, 2, choice32,
+                              args.physics_accuracy_steps)
+            physics["%s_%d_grains" % (kind, grains)] = leg
+            log("%s, %d grains: %.3f ms/step, %.3e voxel-updates/s, correction %.3f ms + elements %.3f ms per step"
+                % (kind, grains, leg["ms_per_step"], leg["value"], leg["segment_ms_per_step"].get("correction", 0),
+                   leg["segment_ms_per_step"].get("elements", 0)))
 
     # ---- accuracy: FP32 beside FP64 (bit-identical to the oracle) over 200 steps ---------------
     accuracy = None
@@ -520,6 +536,8 @@ def main():
         out["fp32_speedup_vs_fp64"] = value / fp64["value"] if fp64["value"] else None
     if accuracy is not None:
         out["fp32_accuracy_vs_fp64"] = accuracy
+    if physics is not None:
+        out["configurations_with_correction_fp32"] = physics
 
     if not args.no_cpu:
         try:
@@ -546,6 +564,73 @@ def main():
             log("cpu baseline skipped: %s" % e)
             out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
+
+
+def physics_leg(kind, grains, ppg, border, seed, steps, warmup, tuning, accuracy_steps):
+    """A configuration with the convolution correction on (src/box.cpp:428-536), FP32 batched:
+    'anisotropic' -- Read-Shockley energies and sigmoidal mobilities from uniform random
+    orientations; 'stored_energy' -- unit energies and mobilities, lognormal stored energies
+    (median 0.1, sigma 1.0) as a driving-force shift.  TripleLineDrag 1e6.  Graph-replayed steps
+    for the rate, one eager event-timed step for the stage split, and FP32 against the FP64 path
+    (equal to the FP64 oracle bit for bit) over accuracy_steps."""
+    p = G.derive_params(grains, ppg, border)
+    t = time.perf_counter()
+    idf, bounds, exists = G.generate_voronoi(p, seed, True, 0)
+    offs, ids = G.initial_neighbour_lists(p, bounds, exists)
+    inputs = dict(idfield=idf, bounds=bounds, exists=exists, offsets=offs, ids=ids)
+    info = {"grains": grains, "points_per_grain": ppg, "border": border, "seed": seed,
+            "ngridpoints": p.ngridpoints}
+    if kind == "anisotropic":
+        q = G.random_orientations(grains, seed)
+        phys = dict(correction=True, mobility_model=True, unique_energies=False, stored_energy=False,
+                    triple_line_drag=1e6, quat=q)
+        th = np.array([G.misorientation(q[a], q[b]) for a in range(1, grains + 1)
+                       for b in ids[offs[a]:offs[a + 1]] if b > a])
+        info["orientations"] = "uniform random unit quaternions (Shoemake), seed %d" % seed
+        info["low_angle_fraction_initial_neighbour_pairs"] = float((th < 15 * np.pi / 180).mean())
+    else:
+        see = G.lognormal_energies(grains, seed, 0.1, 1.0)
+        phys = dict(correction=True, mobility_model=False, unique_energies=True, stored_energy=True,
+                    triple_line_drag=1e6, see=see)
+        info["stored_energy"] = "lognormal, median 0.1, sigma 1.0 (normalised units), seed %d" % seed
+    info["setup_s"] = time.perf_counter() - t
+
+    def context(precision):
+        c = new_context(precision, inputs, p)
+        if tuning is not None:
+            c.set_tuning(tuning)
+        c.set_physics(**phys)
+        return c
+
+    ctx = context(G.FP32_BATCHED)
+    ctx.timestep(1, use_graph=False)  # the first convolution has no hull to correct with
+    ctx.synchronize()
+    ctx.enable_stage_timing(True)
+    ctx.stage_times(reset=True)
+    ctx.timestep(2, use_graph=False)
+    ctx.synchronize()
+    seg = ctx.stage_times(reset=True)
+    ctx.enable_stage_timing(False)
+    wall, vox = timed_steps(ctx, steps, warmup)
+    info.update({"value": vox * steps / wall if steps else 0.0, "unit": "voxel-updates/s",
+                 "ms_per_step": 1e3 * wall / max(steps, 1), "box_voxels_per_step": vox, "steps": steps,
+                 "segment_ms_per_step": {k: round(v[0] / 2, 4) for k, v in seg.items() if v[1]}})
+    ctx.close()
+    if accuracy_steps:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from fp32_error import phi_volume_errors, trajectory_error
+        a32, a64 = context(G.FP32_BATCHED), context(G.FP64_PLAIN)
+        a32.timestep(accuracy_steps)
+        a64.timestep(accuracy_steps)
+        e = phi_volume_errors(a32, a64, p.delta, grains=np.arange(1, grains + 1, 16))
+        e.update(trajectory_error(a32.count_history(), a64.count_history()))
+        e["steps"] = accuracy_steps
+        e["reference"] = "FP64 GPU path with the correction (bit-identical to the FP64 oracle)"
+        e["grains_sampled_for_phi"] = "every 16th grain id"
+        info["fp32_accuracy_vs_fp64"] = e
+        a32.close()
+        a64.close()
+    return info
 
 
 def container_leg(ctx):

@@ -256,6 +256,7 @@ enum Seg {
   kSegContainer, kSegCorrect, kSegElements, kNumSeg
 };
 static_assert(kNumSeg == GG_NUM_SEGMENTS, "segment table of the C ABI");
+static_assert(kCmpVariants == GG_CMP_VARIANTS, "comparison variants of the C ABI");
 
 struct SegTimer {
   gg_ctx* c;

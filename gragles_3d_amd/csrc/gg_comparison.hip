@@ -82,7 +82,7 @@ __device__ inline bool deep_inside(int j, int i, int k, int gb, int m) {
 // reference's order).  The per-entry set-up is paid once per ZL layers, there
 // is no LDS and no barrier.  Then, per voxel, the wall distance, IDLocal (new
 // id or the stale one), the set-comparison and the bounding box, as above.
-template <class T, int NMAX, int ZL, int QB>
+template <class T, int NMAX, int ZL, int QB, bool ZMAJ>
 __global__ __launch_bounds__(256) void k_cmp_cols(SimParams sp, PlanDev P, int cls, DescDev din, DescDev dout,
                                                   ListDev ld, IdlDev ilo, IdlDev iln, int32_t* newb) {
   constexpr int ZCH = (NMAX + ZL - 1) / ZL;  // layer runs per column block
@@ -100,12 +100,22 @@ __global__ __launch_bounds__(256) void k_cmp_cols(SimParams sp, PlanDev P, int c
   for (int bt = tr.first; bt < tr.end; bt += tr.step) {
     const int wt = bt * 4 + wv;
     if (wt >= total) continue;
-    const int ct = wt / ZCH, zc = wt - ct * ZCH;
+    const int ct = wt / ZCH;
+    int zc = wt - ct * ZCH;
     int g, cb;
     plan_tile(P, cls, kQCols, ct, g, cb);
     const int32_t* b = din.bounds + 6 * g;
     const int x0 = b[0], y0 = b[1], z0 = b[2], x1 = b[3], y1 = b[4];
     const int n = x1 - x0, nn2 = n * n;
+    if constexpr (ZMAJ) {
+      // the grain's wavefront tiles in run-major order: consecutive tiles (the
+      // wavefronts of one workgroup, then the next workgroups) take consecutive
+      // column blocks of one run of layers, so they read the same neighbour rows
+      const int nbk = (nn2 + kColBlock - 1) / kColBlock;
+      const int lt = wt - (ct - cb) * ZCH;  // tile index inside the grain
+      zc = lt / nbk;
+      cb = lt - zc * nbk;
+    }
     const int qlo = zc * ZL;
     if (qlo >= n) continue;
     const int qn = min(ZL, n - qlo);  // layers of this run
@@ -289,16 +299,19 @@ __global__ __launch_bounds__(64) void k_cmp_table(SimParams sp, const int32_t* _
 
 constexpr int kColGridPerCU = 64;  // workgroups per CU of a comparison launch (4 wavefront tiles each)
 
-// Shapes of the comparison tile (layers per wavefront, gathers in flight per
-// entry) that the autotuner chooses from per size class.
-constexpr int kCmpShapes[kCmpVariants][2] = {{8, 8}, {16, 8}, {16, 16}, {32, 8}};
+// Shapes of the comparison tile that the autotuner chooses from per size class:
+// layers per wavefront, gathers in flight per entry, and the order of a
+// grain's tiles (0: column-block-major, 1: run-major).
+constexpr int kCmpShapes[kCmpVariants][3] = {{8, 8, 0},  {16, 8, 0}, {16, 16, 0}, {32, 8, 0},
+                                             {8, 8, 1},  {16, 8, 1}, {16, 16, 1}, {32, 8, 1}};
 
 template <class T, int NMAX, int V>
 void cmp_launch(const Launch& L, int cls, const SimParams& sp, const PlanDev& P, const DescDev& din,
                 const DescDev& dout, const ListDev& ld, const IdlDev& ilo, const IdlDev& iln, int32_t* newb) {
   constexpr int ZL = kCmpShapes[V][0], QB = kCmpShapes[V][1];
-  hipLaunchKernelGGL((k_cmp_cols<T, NMAX, ZL, QB>), dim3(tile_grid(L, kColGridPerCU, cls)), dim3(256), 0, L.stream,
-                     sp, P, cls, din, dout, ld, ilo, iln, newb);
+  constexpr bool ZMAJ = kCmpShapes[V][2] != 0;
+  hipLaunchKernelGGL((k_cmp_cols<T, NMAX, ZL, QB, ZMAJ>), dim3(tile_grid(L, kColGridPerCU, cls)), dim3(256), 0,
+                     L.stream, sp, P, cls, din, dout, ld, ilo, iln, newb);
 }
 
 template <class T, int NMAX>
@@ -308,7 +321,11 @@ void cmp_variant(const Launch& L, int v, int cls, const SimParams& sp, const Pla
     case 0: cmp_launch<T, NMAX, 0>(L, cls, sp, P, din, dout, ld, ilo, iln, newb); break;
     case 1: cmp_launch<T, NMAX, 1>(L, cls, sp, P, din, dout, ld, ilo, iln, newb); break;
     case 2: cmp_launch<T, NMAX, 2>(L, cls, sp, P, din, dout, ld, ilo, iln, newb); break;
-    default: cmp_launch<T, NMAX, 3>(L, cls, sp, P, din, dout, ld, ilo, iln, newb);
+    case 3: cmp_launch<T, NMAX, 3>(L, cls, sp, P, din, dout, ld, ilo, iln, newb); break;
+    case 4: cmp_launch<T, NMAX, 4>(L, cls, sp, P, din, dout, ld, ilo, iln, newb); break;
+    case 5: cmp_launch<T, NMAX, 5>(L, cls, sp, P, din, dout, ld, ilo, iln, newb); break;
+    case 6: cmp_launch<T, NMAX, 6>(L, cls, sp, P, din, dout, ld, ilo, iln, newb); break;
+    default: cmp_launch<T, NMAX, 7>(L, cls, sp, P, din, dout, ld, ilo, iln, newb);
   }
 }
 

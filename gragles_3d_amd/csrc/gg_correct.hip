@@ -252,6 +252,7 @@ __global__ __launch_bounds__(kEW) void k_elements(SimParams sp, const int32_t* _
       for (int t = 0; t < nq; ++t) s_ord[E++] = s_qj[t];
       for (int r = 0; r < nl; ++r)
         if (s_cls[r] == kClsTL) s_ord[E++] = (short)r;
+      const int ntl = E - nh - nq;
       for (int r = 0; r < nl; ++r)
         if (s_cls[r] == kClsGB) s_ord[E++] = (short)r;
       int64_t acc = 0;
@@ -268,7 +269,7 @@ __global__ __launch_bounds__(kEW) void k_elements(SimParams sp, const int32_t* _
       cd.ecount[g] = E;
       cd.eclass[4 * g] = nh;
       cd.eclass[4 * g + 1] = nq;
-      cd.eclass[4 * g + 2] = E - nh - nq;
+      cd.eclass[4 * g + 2] = ntl;
       s_E = E;
     }
     __syncthreads();

@@ -160,7 +160,7 @@ struct PlanDev {
   int32_t* tile_base;   // [kNumQ * kClasses] first entry of each class in tmap[q]
   uint32_t* tmap[kNumQ];  // [kMaxTilesPerGrain * N] tile -> grain << 8 | tile index inside the grain
 };
-constexpr int kCmpVariants = 4;  // comparison tile shapes the autotuner chooses from
+constexpr int kCmpVariants = 8;  // comparison tile shapes the autotuner chooses from
 constexpr int kCmpDefault = 1;   // 16 layers, 8 gathers in flight
 constexpr int kMaxGrid = 32767;  // ngridpoints limit
 

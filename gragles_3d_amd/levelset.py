@@ -17,6 +17,9 @@ LIB_PATH = os.path.join(HERE, "libgg_levelset.so")
 GG_OK, GG_ERR_ARG, GG_ERR_HIP, GG_ERR_CAPACITY, GG_ERR_STATE, GG_ERR_DEVICE = range(6)
 FP64_PLAIN, FP32_BATCHED = 0, 1
 DELETED, ALIVE, VANISHED = 0, 1, 2
+# comparison tile shapes of gg_autotune: (layers per wavefront, gathers in flight, run-major order)
+CMP_SHAPES = ((8, 8, 0), (16, 8, 0), (16, 16, 0), (32, 8, 0), (8, 8, 1), (16, 8, 1), (16, 16, 1), (32, 8, 1))
+CMP_VARIANTS = len(CMP_SHAPES)
 SEGMENTS = ("conv_xy", "conv_z", "conv_xy_inv", "idl_resize", "lists", "comparison_n32", "hull", "finalize", "plan",
             "redist_xy", "redist_z", "post_redist", "count", "comparison_n48", "comparison_n64", "comparison_n96",
             "resample", "container", "correction", "elements")
@@ -473,7 +476,7 @@ class LevelSetContext:
         """One timestep that also times the comparison's tile shapes per box-size class on its
         own inputs and keeps the fastest; returns (choice per class, ms[class][shape])."""
         ch = np.zeros(4, dtype=np.int32)
-        ms = np.zeros((4, 4), dtype=np.float32)
+        ms = np.zeros((4, CMP_VARIANTS), dtype=np.float32)
         self._chk(lib().gg_autotune(self.h, int(reps), _p(ch, ctypes.c_int32), _p(ms, ctypes.c_float)))
         return ch, ms
 

@@ -237,9 +237,12 @@ int gg_timestep(gg_ctx* ctx, int32_t nsteps, int32_t use_graph);
  * of the comparison's tile shapes per box-size class (n <= 32, 48, 64, 96) on
  * the step's own inputs (reps launches each), keeps the fastest per class and
  * drops a captured step graph.  The shapes (z-layers per wavefront, gathers in
- * flight per neighbour) are 0: (8, 8), 1: (16, 8), 2: (16, 16), 3: (32, 8);
- * every shape gives the same result.  choice: kClasses entries (may be NULL);
- * ms: kClasses * 4 mean launch times (may be NULL). */
+ * flight per neighbour) are 0: (8, 8), 1: (16, 8), 2: (16, 16), 3: (32, 8)
+ * with a grain's tiles column-block-major, and 4..7 the same four run-major
+ * (consecutive wavefronts take consecutive column blocks of one run of
+ * layers); every shape gives the same result.  choice: 4 entries (may be
+ * NULL); ms: 4 * GG_CMP_VARIANTS mean launch times (may be NULL). */
+#define GG_CMP_VARIANTS 8
 int gg_autotune(gg_ctx* ctx, int32_t reps, int32_t* choice, float* ms);
 /* The shape per size class in use (4 entries), and setting it (for instance a
  * choice saved from an earlier gg_autotune); default 1 for every class. */

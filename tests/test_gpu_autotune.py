@@ -13,7 +13,7 @@ from parity import compare_all, make_pair
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = range(4)
+SHAPES = range(G.levelset.CMP_VARIANTS)
 CASE = (27, 40, 3, 11)  # box sides 39-82: classes n <= 48, 64 and 96
 
 
@@ -53,7 +53,7 @@ def test_autotuned_step_equals_a_timestep():
     ctx, orc, p = make_pair(*CASE)
     choice, ms = ctx.autotune(reps=2)
     assert list(ctx.tuning()) == list(choice)
-    assert all(0 <= v < 4 for v in choice)
+    assert all(0 <= v < G.levelset.CMP_VARIANTS for v in choice)
     assert (ms > 0).all()
     orc.timestep(1)
     compare_all(ctx, orc, "after the autotuned step")
@@ -62,5 +62,5 @@ def test_autotuned_step_equals_a_timestep():
     compare_all(ctx, orc, "two steps later")
     np.testing.assert_array_equal(ctx.count_history(), [orc.count()] * 3)
     with pytest.raises(G.GGError, match="out of range"):
-        ctx.set_tuning([0, 1, 2, 4])
+        ctx.set_tuning([0, 1, 2, G.levelset.CMP_VARIANTS])
     ctx.close()
