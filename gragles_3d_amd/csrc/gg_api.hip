@@ -197,13 +197,23 @@ int build_tables(gg_ctx* c) {
       circ.push_back((float)(acc / n));
     }
   }
-  int32_t *d_two, *d_go;
+  // FFT radices per n: 4 while 4 divides, then 2, then the odd prime factors ascending
+  std::vector<int32_t> fplan((kMaxN + 1) * 8, 0);
+  for (int n = 1; n <= kMaxN; ++n) {
+    int m = n, s = 0;
+    while (m % 4 == 0) { fplan[n * 8 + s++] = 4; m /= 4; }
+    if (m % 2 == 0) { fplan[n * 8 + s++] = 2; m /= 2; }
+    for (int q = 3; m > 1; q += 2)
+      while (m % q == 0) { fplan[n * 8 + s++] = q; m /= q; }
+  }
+  int32_t *d_two, *d_go, *d_fp;
   double *d_twc, *d_tws, *d_gt;
   float* d_circ;
   int rc;
   if ((rc = dalloc(c, &d_two, tw_off.size())) || (rc = dalloc(c, &d_go, g_off.size())) ||
       (rc = dalloc(c, &d_twc, twc.size())) || (rc = dalloc(c, &d_tws, tws.size())) ||
-      (rc = dalloc(c, &d_gt, gt.size())) || (rc = dalloc(c, &d_circ, circ.size())))
+      (rc = dalloc(c, &d_gt, gt.size())) || (rc = dalloc(c, &d_circ, circ.size())) ||
+      (rc = dalloc(c, &d_fp, fplan.size())))
     return rc;
   HIPCHK(c, hipMemcpy(d_two, tw_off.data(), tw_off.size() * 4, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(d_go, g_off.data(), g_off.size() * 4, hipMemcpyHostToDevice));
@@ -211,7 +221,8 @@ int build_tables(gg_ctx* c) {
   HIPCHK(c, hipMemcpy(d_tws, tws.data(), tws.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(d_gt, gt.data(), gt.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(d_circ, circ.data(), circ.size() * 4, hipMemcpyHostToDevice));
-  c->TB = TablesDev{d_two, d_twc, d_tws, d_go, d_gt, d_circ};
+  HIPCHK(c, hipMemcpy(d_fp, fplan.data(), fplan.size() * 4, hipMemcpyHostToDevice));
+  c->TB = TablesDev{d_two, d_twc, d_tws, d_go, d_gt, d_circ, d_fp};
   return GG_OK;
 }
 
@@ -902,7 +913,8 @@ int gg_coarsen_grid(gg_ctx* c, int32_t new_size, double new_dt) {
   const TablesDev old_tb = c->TB;
   if ((rc = build_tables(c))) return rc;
   for (const void* q : {(const void*)old_tb.tw_off, (const void*)old_tb.twc, (const void*)old_tb.tws,
-                        (const void*)old_tb.g_off, (const void*)old_tb.gtab, (const void*)old_tb.circ})
+                        (const void*)old_tb.g_off, (const void*)old_tb.gtab, (const void*)old_tb.circ,
+                        (const void*)old_tb.fplan})
     dfree(c, q);
   drop_graph(c);
   const CoarsenArgs a{gb, h_old / h_new, h_new / h_old, c->p.delta};

@@ -1,18 +1,24 @@
 // Heat-kernel convolution of every grain box (LSbox::executeConvolution,
 // src/box.cpp:414-539, Gaussian branch of convolutionGeneratorMKL :751-767).
 //
-// FP64 plain form: explicit 3-D DFT on the n^3 box, half spectrum along x,
-// spectral factor G, inverse DFT.  Every sum runs over its index from 0
-// upward starting at 0.0 with the products grouped as below, and the library
-// is built with -ffp-contract=off, so each value is the same IEEE double the
-// FP64 CPU restatement produces:
-//   forward x (real in):  re = re + v*c ; im = im - v*s
-//   forward y, z:         re = re + (a*c + b*s) ; im = im + (b*c - a*s)
-//   spectral factor:      re = re*G ; im = im*G
-//   inverse z, y:         re = re + (a*c - b*s) ; im = im + (a*s + b*c)
-//   inverse x (real out): acc = X0.re ; acc = acc + 2.0*(a*c - b*s), kx=1..(n-1)/2 ;
-//                         n even: acc = acc + (x odd ? -Xn/2.re : Xn/2.re)
-// with c = cos(2 pi t/n), s = sin(2 pi t/n) host tables and t = (index*k) mod n.
+// FP64 plain form: mixed-radix FFT (radices 4, 2, 3, 5 specialised, any other
+// prime factor as a direct sum) on the n^3 box, half spectrum along x (two
+// real rows per complex FFT), spectral factor G, inverse FFT.  The library is
+// built with -ffp-contract=off and every operation follows this fixed
+// definition, so each value is the same IEEE double the FP64 CPU
+// restatement produces:
+//   Stockham pass of radix p (T = n/p, Ns = product of the earlier radices;
+//   radices 4 while 4 divides, then 2, then odd primes ascending): for
+//   i < T, j = i mod Ns: v_r = x[i + r*T]; if j > 0, v_r = twiddle(v_r,
+//   r*j*n/(Ns*p)) for r >= 1, forward (a*c + b*s, b*c - a*s), inverse
+//   (a*c - b*s, a*s + b*c), c/s = cos/sin(2 pi t/n) host tables;
+//   y = butterfly_p(v); x'[(i-j)*p + j + k*Ns] = y_k.  Butterflies as in
+//   butterfly() below; other (odd prime) radices as conjugate-pair sums
+//   (fft_pass_any).
+//   x forward: rows 2q, 2q+1 as a + ib; S[2q][k] = (Z_k + conj Z_(n-k))/2,
+//   S[2q+1][k] = (Z_k - conj Z_(n-k))/2i, written as in k_fft_xy_fwd.
+//   G: re*G, im*G.  x inverse: Z_k = A_k + i B_k with the Hermitian
+//   extension for k > n/2; row 2q = Re IFFT(Z), row 2q+1 = Im.
 // Three launches per size class: (x,y) forward per z-layer with the layer in
 // LDS, z forward + G + z inverse per block of 32 spectral columns, (y,x)
 // inverse per z-layer.
@@ -31,11 +37,231 @@ constexpr int kGridPerCU = 128;  // workgroups per CU of a convolution launch
 
 constexpr int kT = 256;
 
+// ---- FP64 mixed-radix FFT, staged in LDS ------------------------------------
+// A batch of nl lines of length n sits in LDS (element e of line l at
+// buf[l*ls + e*es]).  Each Stockham pass (radix p, T = n/p butterflies per
+// line) reads every butterfly's p inputs and keeps its p outputs in registers,
+// and after a barrier writes them to their sorted places, so one buffer
+// serves all passes.  Consecutive lanes take the same butterfly of
+// consecutive lines: columns (ls = 1) are then read and written in contiguous
+// runs, and rows are laid out at an odd stride (in complex values), so the
+// lanes of a 16-byte access fall on distinct bank groups.
+constexpr double kC1 = 0.309016994374947424102293417182819059;
+constexpr double kC2 = -0.809016994374947424102293417182819059;
+constexpr double kS1 = 0.951056516295153572116439333379382143;
+constexpr double kS2 = 0.587785252292473129168705954639072769;
+constexpr double kS3 = 0.866025403784438646763723170752936183;
+constexpr int kPlanLen = 8;  // radices per length in the plan table (0-terminated)
+
+__host__ __device__ constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// a / d for 0 <= a < 2^20, 1 <= d <= 2^10, inv = 1.0f / d
+__device__ inline int sdiv(int a, int d, float inv) {
+  int q = (int)((float)a * inv);
+  const int r = a - q * d;
+  q += (r >= d) - (r < 0);
+  return q;
+}
+
+template <bool INV>
+__device__ inline double2 twid(double2 v, double c, double s) {
+  if (INV) return make_double2(v.x * c - v.y * s, v.x * s + v.y * c);
+  return make_double2(v.x * c + v.y * s, v.y * c - v.x * s);
+}
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+template <int P, bool INV>
+__device__ inline void butterfly(const double2 (&v)[P], double2 (&y)[P]) {
+  if constexpr (P == 2) {
+    y[0] = cadd(v[0], v[1]);
+    y[1] = csub(v[0], v[1]);
+  } else if constexpr (P == 4) {
+    const double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]), a2 = cadd(v[1], v[3]), a3 = csub(v[1], v[3]);
+    y[0] = cadd(a0, a2);
+    y[2] = csub(a0, a2);
+    const double2 m1 = make_double2(a1.x + a3.y, a1.y - a3.x), p1 = make_double2(a1.x - a3.y, a1.y + a3.x);
+    y[1] = INV ? p1 : m1;
+    y[3] = INV ? m1 : p1;
+  } else if constexpr (P == 3) {
+    const double2 t1 = cadd(v[1], v[2]);
+    const double2 t2 = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
+    const double2 t3 = make_double2(kS3 * (v[1].x - v[2].x), kS3 * (v[1].y - v[2].y));
+    y[0] = cadd(v[0], t1);
+    const double2 m1 = make_double2(t2.x + t3.y, t2.y - t3.x), p1 = make_double2(t2.x - t3.y, t2.y + t3.x);
+    y[1] = INV ? p1 : m1;
+    y[2] = INV ? m1 : p1;
+  } else {
+    static_assert(P == 5, "specialised radices: 2, 3, 4, 5");
+    const double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]), t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    y[0] = make_double2((v[0].x + t1.x) + t2.x, (v[0].y + t1.y) + t2.y);
+    const double2 b1 = make_double2((v[0].x + kC1 * t1.x) + kC2 * t2.x, (v[0].y + kC1 * t1.y) + kC2 * t2.y);
+    const double2 b2 = make_double2((v[0].x + kC2 * t1.x) + kC1 * t2.x, (v[0].y + kC2 * t1.y) + kC1 * t2.y);
+    const double2 d1 = make_double2(kS1 * t3.x + kS2 * t4.x, kS1 * t3.y + kS2 * t4.y);
+    const double2 d2 = make_double2(kS2 * t3.x - kS1 * t4.x, kS2 * t3.y - kS1 * t4.y);
+    const double2 m1 = make_double2(b1.x + d1.y, b1.y - d1.x), p1 = make_double2(b1.x - d1.y, b1.y + d1.x);
+    const double2 m2 = make_double2(b2.x + d2.y, b2.y - d2.x), p2 = make_double2(b2.x - d2.y, b2.y + d2.x);
+    y[1] = INV ? p1 : m1;
+    y[4] = INV ? m1 : p1;
+    y[2] = INV ? p2 : m2;
+    y[3] = INV ? m2 : p2;
+  }
+}
+
+// One pass of a specialised radix P over nl lines (MAXOUT >= nl*n).
+template <int P, bool INV, int MAXOUT>
+__device__ inline void fft_pass(double2* buf, int n, int nl, int es, int ls, int Ns, const double* tc,
+                                const double* ts, int tid) {
+  constexpr int U = cdiv(cdiv(MAXOUT, P), kT);
+  const int T = n / P, units = nl * T, step = n / (Ns * P);
+  const float inv_nl = 1.0f / (float)nl, inv_Ns = 1.0f / (float)Ns;
+  double2 y[U][P];
+  int ob[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int w = tid + u * kT;
+    ob[u] = -1;
+    if (w < units) {
+      const int i = sdiv(w, nl, inv_nl), line = w - i * nl;
+      const int j = i - sdiv(i, Ns, inv_Ns) * Ns;
+      const double2* src = buf + line * ls;
+      double2 v[P];
+#pragma unroll
+      for (int r = 0; r < P; ++r) v[r] = src[(i + r * T) * es];
+      if (j > 0) {
+#pragma unroll
+        for (int r = 1; r < P; ++r) {
+          const int t = r * j * step;
+          v[r] = twid<INV>(v[r], tc[t], ts[t]);
+        }
+      }
+      butterfly<P, INV>(v, y[u]);
+      ob[u] = line * ls + ((i - j) * P + j) * es;
+    }
+  }
+  __syncthreads();
+  const int os = Ns * es;
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (ob[u] >= 0) {
+#pragma unroll
+      for (int k = 0; k < P; ++k) buf[ob[u] + k * os] = y[u][k];
+    }
+  __syncthreads();
+}
+
+// One pass of an odd prime radix p > 5, h = (p-1)/2: work item k = 0..h of a
+// butterfly makes y_0 (k = 0) or the conjugate pair y_k, y_(p-k) from the sums
+// over r = 1..h of a_r = v_r + v_(p-r) (times cos) and b_r = v_r - v_(p-r)
+// (times sin); consecutive lanes take the same item of consecutive lines.
+template <bool INV, int MAXOUT>
+__device__ inline void fft_pass_any(double2* buf, int n, int nl, int es, int ls, int Ns, int p, const double* tc,
+                                    const double* ts, int tid) {
+  constexpr int U = cdiv(cdiv(MAXOUT * 4, 7), kT);  // (h+1)/p <= 4/7 for p >= 7
+  const int T = n / p, step = n / (Ns * p), h = (p - 1) / 2, K = h + 1;
+  const int items = nl * T * K;
+  const float inv_K = 1.0f / (float)K, inv_nl = 1.0f / (float)nl, inv_Ns = 1.0f / (float)Ns;
+  double2 ya[U], yb[U];
+  int oa[U], ob[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int w = tid + u * kT;
+    oa[u] = -1;
+    ob[u] = -1;
+    if (w < items) {
+      const int rest = sdiv(w, nl, inv_nl), line = w - rest * nl;
+      const int i = sdiv(rest, K, inv_K), k = rest - i * K;
+      const int j = i - sdiv(i, Ns, inv_Ns) * Ns;
+      const double2* src = buf + line * ls;
+      const double2 v0 = src[i * es];
+      auto input = [&](int r) {
+        double2 v = src[(i + r * T) * es];
+        if (j > 0) {
+          const int t = r * j * step;
+          v = twid<INV>(v, tc[t], ts[t]);
+        }
+        return v;
+      };
+      const int base = line * ls + (i - j) * p * es + j * es;
+      if (k == 0) {
+        double2 y0 = v0;
+        for (int r = 1; r <= h; ++r) {
+          const double2 a = input(r), b = input(p - r);
+          y0.x = y0.x + (a.x + b.x);
+          y0.y = y0.y + (a.y + b.y);
+        }
+        ya[u] = y0;
+        oa[u] = base;
+      } else {
+        double Pr = 0.0, Pi = 0.0, Qr = 0.0, Qi = 0.0;
+        int rk = k;  // r*k mod p
+        for (int r = 1; r <= h; ++r) {
+          const double2 vr = input(r), vq = input(p - r);
+          const double2 a = cadd(vr, vq), b = csub(vr, vq);
+          const int t = rk * T;
+          const double c = tc[t], s = ts[t];
+          Pr = Pr + a.x * c;
+          Pi = Pi + a.y * c;
+          Qr = Qr + b.x * s;
+          Qi = Qi + b.y * s;
+          rk += k;
+          if (rk >= p) rk -= p;
+        }
+        const double mr = v0.x + Pr, mi = v0.y + Pi;
+        const double2 lo = make_double2(mr + Qi, mi - Qr), hi = make_double2(mr - Qi, mi + Qr);
+        ya[u] = INV ? hi : lo;
+        yb[u] = INV ? lo : hi;
+        oa[u] = base + k * Ns * es;
+        ob[u] = base + (p - k) * Ns * es;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    if (oa[u] >= 0) buf[oa[u]] = ya[u];
+    if (ob[u] >= 0) buf[ob[u]] = yb[u];
+  }
+  __syncthreads();
+}
+
+// The whole FFT of nl lines: the passes of n's plan in order.
+template <bool INV, int MAXOUT>
+__device__ inline void fft_lines(double2* buf, int n, int nl, int es, int ls, const int (&radix)[kPlanLen],
+                                 const double* tc, const double* ts, int tid) {
+  int Ns = 1;
+  for (int s = 0; s < kPlanLen; ++s) {
+    const int p = radix[s];
+    if (p == 0) break;
+    if (p == 4) fft_pass<4, INV, MAXOUT>(buf, n, nl, es, ls, Ns, tc, ts, tid);
+    else if (p == 2) fft_pass<2, INV, MAXOUT>(buf, n, nl, es, ls, Ns, tc, ts, tid);
+    else if (p == 3) fft_pass<3, INV, MAXOUT>(buf, n, nl, es, ls, Ns, tc, ts, tid);
+    else if (p == 5) fft_pass<5, INV, MAXOUT>(buf, n, nl, es, ls, Ns, tc, ts, tid);
+    else fft_pass_any<INV, MAXOUT>(buf, n, nl, es, ls, Ns, p, tc, ts, tid);
+    Ns *= p;
+  }
+}
+
+__device__ inline void load_fft_tables(const TablesDev& tb, int n, double* tc, double* ts, int (&radix)[kPlanLen],
+                                       int tid) {
+  const int to = tb.tw_off[n];
+  for (int i = tid; i < n; i += kT) {
+    tc[i] = tb.twc[to + i];
+    ts[i] = tb.tws[to + i];
+  }
+#pragma unroll
+  for (int s = 0; s < kPlanLen; ++s) radix[s] = tb.fplan[n * kPlanLen + s];
+}
+
+// (x, y) forward per z-layer: two real rows per complex x-FFT, the half
+// spectrum separated from it, then the y-FFT of the n/2+1 spectral columns.
 template <int NMAX>
-__global__ __launch_bounds__(kT) void k_dft_xy_fwd(PlanDev P, int cls, DescDev din, TablesDev tb,
+__global__ __launch_bounds__(kT) void k_fft_xy_fwd(PlanDev P, int cls, DescDev din, TablesDev tb,
                                                    const int64_t* __restrict__ spec_off, double2* spec) {
-  constexpr int R = (NMAX * (NMAX / 2 + 1) + kT - 1) / kT;
-  __shared__ double L[NMAX * (NMAX + 2)];
+  constexpr int NR = (NMAX + 1) / 2, N2 = NMAX / 2 + 1;
+  constexpr int R2 = cdiv(NMAX * N2, kT);
+  static_assert(NR * (NMAX | 1) <= NMAX * N2, "padded rows fit the layer buffer");
+  __shared__ double2 L[NMAX * N2];
   __shared__ double tc[NMAX], ts[NMAX];
   const int total = P.totals[kQLayers * kClasses + cls];
   const double* arena = (const double*)din.arena;
@@ -43,65 +269,50 @@ __global__ __launch_bounds__(kT) void k_dft_xy_fwd(PlanDev P, int cls, DescDev d
   for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
     int g, z;
     plan_tile(P, cls, kQLayers, tile, g, z);
-    const int n = box_n(din.bounds, g), n2 = n / 2 + 1, nc = n * n2;
-    const int to = tb.tw_off[n];
-    for (int i = tid; i < n; i += kT) { tc[i] = tb.twc[to + i]; ts[i] = tb.tws[to + i]; }
+    const int n = box_n(din.bounds, g), n2 = n / 2 + 1, nr = (n + 1) / 2, nc = n * n2;
+    int radix[kPlanLen];
+    load_fft_tables(tb, n, tc, ts, radix, tid);
     const double* src = arena + din.off[g] + (int64_t)z * n * n;
-    stage_to_lds(L, src, n * n, tid, kT);
+    const int rs = n | 1;  // row stride of the x-FFT lines (odd: lanes on consecutive rows hit distinct banks)
+    for (int o = tid; o < nr * n; o += kT) {
+      const int q = o / n, x = o - q * n;
+      const double a = src[2 * q * n + x];
+      const double b = 2 * q + 1 < n ? src[(2 * q + 1) * n + x] : 0.0;
+      L[q * rs + x] = make_double2(a, b);
+    }
     __syncthreads();
-    double re[R], im[R];
+    fft_lines<false, NR * NMAX>(L, n, nr, 1, rs, radix, tc, ts, tid);
+    // separate the two rows' half spectra: S[y][k], y = 2q (+1)
+    double2 sv[R2];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int o = tid + r * kT;
-      double a = 0.0, b = 0.0;
+    for (int u = 0; u < R2; ++u) {
+      const int o = tid + u * kT;
       if (o < nc) {
-        const int y = o / n2, kx = o - y * n2;
-        const double* v = L + y * n;
-        int t = 0;
-        for (int x = 0; x < n; ++x) {
-          a = a + v[x] * tc[t];
-          b = b - v[x] * ts[t];
-          t += kx;
-          if (t >= n) t -= n;
-        }
+        const int y = o / n2, k = o - y * n2, q = y >> 1, m = k == 0 ? 0 : n - k;
+        const double2 zk = L[q * rs + k], zm = L[q * rs + m];
+        sv[u] = (y & 1) ? make_double2(0.5 * (zk.y + zm.y), 0.5 * (zm.x - zk.x))
+                        : make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
       }
-      re[r] = a;
-      im[r] = b;
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int o = tid + r * kT;
-      if (o < nc) { L[2 * o] = re[r]; L[2 * o + 1] = im[r]; }
+    for (int u = 0; u < R2; ++u) {
+      const int o = tid + u * kT;
+      if (o < nc) L[o] = sv[u];
     }
     __syncthreads();
+    fft_lines<false, N2 * NMAX>(L, n, n2, n2, 1, radix, tc, ts, tid);
     double2* dst = spec + spec_off[g] + (int64_t)z * nc;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int o = tid + r * kT;
-      if (o < nc) {
-        const int ky = o / n2, kx = o - ky * n2;
-        double a = 0.0, b = 0.0;
-        int t = 0;
-        for (int y = 0; y < n; ++y) {
-          const double ar = L[2 * (y * n2 + kx)], ai = L[2 * (y * n2 + kx) + 1];
-          a = a + (ar * tc[t] + ai * ts[t]);
-          b = b + (ai * tc[t] - ar * ts[t]);
-          t += ky;
-          if (t >= n) t -= n;
-        }
-        dst[o] = make_double2(a, b);
-      }
-    }
+    for (int o = tid; o < nc; o += kT) dst[o] = L[o];
     __syncthreads();
   }
 }
 
+// z forward, spectral factor, z inverse per block of kSpecBlock spectral columns.
 template <int NMAX>
-__global__ __launch_bounds__(kT) void k_dft_z(PlanDev P, int cls, DescDev dref, TablesDev tb,
+__global__ __launch_bounds__(kT) void k_fft_z(PlanDev P, int cls, DescDev dref, TablesDev tb,
                                               const int64_t* __restrict__ spec_off, double2* spec) {
   constexpr int CB = kSpecBlock;
-  constexpr int R = (NMAX * CB + kT - 1) / kT;
   __shared__ double2 B[NMAX * CB];
   __shared__ double tc[NMAX], ts[NMAX];
   const int total = P.totals[kQSpecCols * kClasses + cls];
@@ -110,71 +321,42 @@ __global__ __launch_bounds__(kT) void k_dft_z(PlanDev P, int cls, DescDev dref, 
     int g, blk;
     plan_tile(P, cls, kQSpecCols, tile, g, blk);
     const int n = box_n(dref.bounds, g), n2 = n / 2 + 1, nc = n * n2;
-    const int c0 = blk * CB;
-    const int to = tb.tw_off[n];
+    const int c0 = blk * CB, width = min(CB, nc - c0);
     const double* G = tb.gtab + tb.g_off[n];
-    for (int i = tid; i < n; i += kT) { tc[i] = tb.twc[to + i]; ts[i] = tb.tws[to + i]; }
+    int radix[kPlanLen];
+    load_fft_tables(tb, n, tc, ts, radix, tid);
     double2* base = spec + spec_off[g];
-    stage_rows_to_lds(B, CB, base + c0, nc, n, min(CB, nc - c0), tid, kT);  // columns past nc are never read
+    stage_rows_to_lds(B, CB, base + c0, nc, n, width, tid, kT);
     __syncthreads();
-    double fr[R], fi[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int o = tid + r * kT;
+    fft_lines<false, CB * NMAX>(B, n, width, CB, 1, radix, tc, ts, tid);
+    for (int o = tid; o < n * CB; o += kT) {
       const int kz = o / CB, cb = o - kz * CB, c = c0 + cb;
-      double a = 0.0, b = 0.0;
-      if (kz < n && c < nc) {
-        int t = 0;
-        for (int z = 0; z < n; ++z) {
-          const double2 v = B[z * CB + cb];
-          a = a + (v.x * tc[t] + v.y * ts[t]);
-          b = b + (v.y * tc[t] - v.x * ts[t]);
-          t += kz;
-          if (t >= n) t -= n;
-        }
+      if (cb < width) {
         const int ky = c / n2, kx = c - ky * n2;
         const int kz2 = min(kz, n - kz), ky2 = min(ky, n - ky), kx2 = min(kx, n - kx);
         const double gf = G[kz2 * kz2 + ky2 * ky2 + kx2 * kx2];
-        a = a * gf;
-        b = b * gf;
+        const double2 v = B[o];
+        B[o] = make_double2(v.x * gf, v.y * gf);
       }
-      fr[r] = a;
-      fi[r] = b;
     }
     __syncthreads();
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int o = tid + r * kT;
-      if (o < n * CB) B[o] = make_double2(fr[r], fi[r]);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int o = tid + r * kT;
-      const int z = o / CB, cb = o - z * CB, c = c0 + cb;
-      if (z < n && c < nc) {
-        double a = 0.0, b = 0.0;
-        int t = 0;
-        for (int kz = 0; kz < n; ++kz) {
-          const double2 v = B[kz * CB + cb];
-          a = a + (v.x * tc[t] - v.y * ts[t]);
-          b = b + (v.x * ts[t] + v.y * tc[t]);
-          t += z;
-          if (t >= n) t -= n;
-        }
-        base[(int64_t)z * nc + c] = make_double2(a, b);
-      }
+    fft_lines<true, CB * NMAX>(B, n, width, CB, 1, radix, tc, ts, tid);
+    for (int o = tid; o < n * width; o += kT) {
+      const int z = sdiv(o, width, 1.0f / (float)width), cb = o - z * width;
+      base[(int64_t)z * nc + c0 + cb] = B[z * CB + cb];
     }
     __syncthreads();
   }
 }
 
+// (y, x) inverse per z-layer: y-IFFT of the spectral columns, two rows' half
+// spectra packed into one full spectrum, x-IFFT, real and imaginary parts out.
 template <int NMAX>
-__global__ __launch_bounds__(kT) void k_dft_xy_inv(PlanDev P, int cls, DescDev dout, TablesDev tb,
+__global__ __launch_bounds__(kT) void k_fft_xy_inv(PlanDev P, int cls, DescDev dout, TablesDev tb,
                                                    const int64_t* __restrict__ spec_off, const double2* spec) {
-  constexpr int R = (NMAX * (NMAX / 2 + 1) + kT - 1) / kT;
-  constexpr int R3 = (NMAX * NMAX + kT - 1) / kT;
-  __shared__ __align__(16) double L[NMAX * (NMAX + 2)];
+  constexpr int NR = (NMAX + 1) / 2, N2 = NMAX / 2 + 1;
+  constexpr int R1 = cdiv(NR * NMAX, kT);
+  __shared__ double2 L[NMAX * N2];
   __shared__ double tc[NMAX], ts[NMAX];
   const int total = P.totals[kQLayers * kClasses + cls];
   double* arena = (double*)dout.arena;
@@ -182,59 +364,44 @@ __global__ __launch_bounds__(kT) void k_dft_xy_inv(PlanDev P, int cls, DescDev d
   for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
     int g, z;
     plan_tile(P, cls, kQLayers, tile, g, z);
-    const int n = box_n(dout.bounds, g), n2 = n / 2 + 1, nc = n * n2;
-    const int to = tb.tw_off[n];
-    for (int i = tid; i < n; i += kT) { tc[i] = tb.twc[to + i]; ts[i] = tb.tws[to + i]; }
+    const int n = box_n(dout.bounds, g), n2 = n / 2 + 1, nr = (n + 1) / 2, nc = n * n2;
+    int radix[kPlanLen];
+    load_fft_tables(tb, n, tc, ts, radix, tid);
     const double2* src = spec + spec_off[g] + (int64_t)z * nc;
-    stage_to_lds((double2*)L, src, nc, tid, kT);
+    stage_to_lds(L, src, nc, tid, kT);
     __syncthreads();
-    double re[R], im[R];
+    fft_lines<true, N2 * NMAX>(L, n, n2, n2, 1, radix, tc, ts, tid);
+    double2 zv[R1];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int o = tid + r * kT;
-      double a = 0.0, b = 0.0;
-      if (o < nc) {
-        const int y = o / n2, kx = o - y * n2;
-        int t = 0;
-        for (int ky = 0; ky < n; ++ky) {
-          const double ar = L[2 * (ky * n2 + kx)], ai = L[2 * (ky * n2 + kx) + 1];
-          a = a + (ar * tc[t] - ai * ts[t]);
-          b = b + (ar * ts[t] + ai * tc[t]);
-          t += y;
-          if (t >= n) t -= n;
-        }
+    for (int u = 0; u < R1; ++u) {
+      const int o = tid + u * kT;
+      if (o < nr * n) {
+        const int q = o / n, k = o - q * n;
+        const bool lo = k < n2, two = 2 * q + 1 < n;
+        const int m = lo ? k : n - k;
+        const double2 A = L[2 * q * n2 + m];
+        const double2 Bv = two ? L[(2 * q + 1) * n2 + m] : make_double2(0.0, 0.0);
+        zv[u] = lo ? make_double2(A.x - Bv.y, A.y + Bv.x) : make_double2(A.x + Bv.y, Bv.x - A.y);
       }
-      re[r] = a;
-      im[r] = b;
     }
     __syncthreads();
+    const int rs = n | 1;  // row stride of the x-FFT lines, as in the forward kernel
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int o = tid + r * kT;
-      if (o < nc) { L[2 * o] = re[r]; L[2 * o + 1] = im[r]; }
+    for (int u = 0; u < R1; ++u) {
+      const int o = tid + u * kT;
+      if (o < nr * n) {
+        const int q = o / n;
+        L[o + q * (rs - n)] = zv[u];
+      }
     }
     __syncthreads();
+    fft_lines<true, NR * NMAX>(L, n, nr, 1, rs, radix, tc, ts, tid);
     double* dst = arena + dout.off[g] + (int64_t)z * n * n;
-    const int half = (n - 1) / 2;
-#pragma unroll
-    for (int r = 0; r < R3; ++r) {
-      const int o = tid + r * kT;
-      if (o < n * n) {
-        const int y = o / n, x = o - y * n;
-        const double* row = L + 2 * (y * n2);
-        double acc = row[0];
-        int t = x;
-        for (int kx = 1; kx <= half; ++kx) {
-          acc = acc + 2.0 * (row[2 * kx] * tc[t] - row[2 * kx + 1] * ts[t]);
-          t += x;
-          if (t >= n) t -= n;
-        }
-        if ((n & 1) == 0) {
-          const double a = row[2 * (n / 2)];
-          acc = acc + ((x & 1) ? -a : a);
-        }
-        dst[o] = acc;
-      }
+    for (int o = tid; o < nr * n; o += kT) {
+      const int q = o / n, x = o - q * n;
+      const double2 v = L[q * rs + x];
+      dst[2 * q * n + x] = v.x;
+      if (2 * q + 1 < n) dst[(2 * q + 1) * n + x] = v.y;
     }
     __syncthreads();
   }
@@ -384,13 +551,13 @@ void conv_class(const Launch& L, int part, const PlanDev& P, int cls, const Desc
     if (part == 1) hipLaunchKernelGGL(k_circ_z<NMAX>, dim3(tile_grid(L, kGridPerCU, cls)), dim3(kT), 0, L.stream, P, cls, dout, tb);
   } else {
     if (part == 0)
-      hipLaunchKernelGGL(k_dft_xy_fwd<NMAX>, dim3(tile_grid(L, kGridPerCU, cls)), dim3(kT), 0, L.stream, P, cls, din, tb, spec_off,
+      hipLaunchKernelGGL(k_fft_xy_fwd<NMAX>, dim3(tile_grid(L, kGridPerCU, cls)), dim3(kT), 0, L.stream, P, cls, din, tb, spec_off,
                          (double2*)spec);
     if (part == 1)
-      hipLaunchKernelGGL(k_dft_z<NMAX>, dim3(tile_grid(L, kGridPerCU, cls)), dim3(kT), 0, L.stream, P, cls, dout, tb, spec_off,
+      hipLaunchKernelGGL(k_fft_z<NMAX>, dim3(tile_grid(L, kGridPerCU, cls)), dim3(kT), 0, L.stream, P, cls, dout, tb, spec_off,
                          (double2*)spec);
     if (part == 2)
-      hipLaunchKernelGGL(k_dft_xy_inv<NMAX>, dim3(tile_grid(L, kGridPerCU, cls)), dim3(kT), 0, L.stream, P, cls, dout, tb, spec_off,
+      hipLaunchKernelGGL(k_fft_xy_inv<NMAX>, dim3(tile_grid(L, kGridPerCU, cls)), dim3(kT), 0, L.stream, P, cls, dout, tb, spec_off,
                          (const double2*)spec);
   }
 }

@@ -187,6 +187,7 @@ struct TablesDev {
   const int32_t* g_off;   // [kMaxN+1] offset into gtab
   const double* gtab;     // spectral factor G(s), s = kx2^2+ky2^2+kz2^2
   const float* circ;      // FP32 circulant heat-kernel row (one axis) per n
+  const int32_t* fplan;   // [(kMaxN+1) * 8] FFT radices per n in pass order, 0-terminated
 };
 
 // ---- helpers --------------------------------------------------------------

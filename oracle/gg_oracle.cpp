@@ -23,9 +23,10 @@
 //   voxel container        src/grainhdl.cpp:1074-1122, src/box.cpp:3008-3032
 //
 // The FFT libraries the reference calls (Intel MKL DFTI / FFTW3, versions not
-// pinned by the reference) are replaced by an explicit DFT whose every sum runs
-// in a fixed order; its definition is in convolve_box() below.  It is checked
-// against numpy's pocketfft (tests/golden) and against closed forms.
+// pinned by the reference) are replaced by a mixed-radix FFT whose every
+// operation runs in a fixed order; its definition is above fft_line() below.
+// It is checked against numpy's pocketfft (tests/golden) and against closed
+// forms.
 //
 // Build: g++ -O3 -fopenmp -ffp-contract=off (oracle/Makefile).  No fast-math.
 
@@ -200,24 +201,172 @@ bool hull_reduce(const Buf<double>& B, double* vol_raw, double* area_raw) {
 }
 
 // ---------------------------------------------------------------------------
-// Explicit DFT heat-kernel convolution of one n^3 box (stands in for
-// convolutionGeneratorMKL, src/box.cpp:685-795, Gaussian branch :751-767).
+// FFT heat-kernel convolution of one n^3 box (stands in for
+// convolutionGeneratorMKL, src/box.cpp:685-795, Gaussian branch :751-767, and
+// the FFTW plans of :575-601).  The reference's libraries are replaced by a
+// mixed-radix complex FFT whose every operation is fixed here, so that another
+// implementation that follows the same steps in IEEE double without FMA
+// contraction produces the same bits.
 //
-// Spectrum layout as the reference's: S[kx + n2*(ky + n*kz)], n2 = n/2+1.
-// Every sum runs over its index from 0 upward, starting at 0.0, with the
-// products grouped as written; tw_c[t] = cos(2*pi*t/n), tw_s[t] = sin(2*pi*t/n).
-//   forward x (real->half spectrum): re += v*c ; im -= v*s
-//   forward y, z (complex):          re += (a*c + b*s) ; im += (b*c - a*s)
-//   spectral factor:                  re *= G ; im *= G
-//   inverse z, y (complex):          re += (a*c - b*s) ; im += (a*s + b*c)
-//   inverse x (half spectrum->real): acc = X0.re ; acc += 2*(a*c - b*s) for
-//                                    kx=1..(n-1)/2 ; n even: acc += (-1)^x Xn/2.re
-// G(s), s = kx2^2+ky2^2+kz2^2 with k2 = min(k, n-k), is the reference's
-// expression evaluated left to right (src/box.cpp:760-761).
+// Complex FFT of length n, forward (e^{-2 pi i jk/n}) or inverse (e^{+...},
+// unnormalised), Stockham form.  Radices, in this order: 4 while 4 divides
+// what is left, then 2 if 2 divides it, then every odd prime factor ascending
+// (with multiplicity).  With Ns = product of the radices before this pass and
+// T = n/p, for i = 0..T-1, j = i mod Ns:
+//   v_r = x[i + r*T], r = 0..p-1;
+//   if j > 0, for r >= 1: v_r = twiddle(v_r, t), t = r*j*(n/(Ns*p))
+//     forward: (a*c + b*s, b*c - a*s); inverse: (a*c - b*s, a*s + b*c)
+//     with (a, b) = v_r, c = cos(2 pi t/n), s = sin(2 pi t/n) (tables);
+//   y = butterfly_p(v); x'[(i - j)*p + j + k*Ns] = y_k.
+// Butterflies (component-wise adds; "-i*z" = (z.im, -z.re)):
+//   p = 2: y0 = v0 + v1, y1 = v0 - v1.
+//   p = 4: a0 = v0+v2, a1 = v0-v2, a2 = v1+v3, a3 = v1-v3; y0 = a0+a2,
+//          y2 = a0-a2, y1 = (a1.re + a3.im, a1.im - a3.re),
+//          y3 = (a1.re - a3.im, a1.im + a3.re); inverse swaps y1 and y3.
+//   p = 3: t1 = v1+v2, t2 = v0 - 0.5*t1, t3 = S3*(v1-v2); y0 = v0 + t1,
+//          y1 = (t2.re + t3.im, t2.im - t3.re), y2 = (t2.re - t3.im,
+//          t2.im + t3.re); inverse swaps y1 and y2.
+//   p = 5: t1 = v1+v4, t2 = v2+v3, t3 = v1-v4, t4 = v2-v3; y0 = (v0+t1)+t2,
+//          b1 = (v0 + C1*t1) + C2*t2, b2 = (v0 + C2*t1) + C1*t2,
+//          d1 = S1*t3 + S2*t4, d2 = S2*t3 - S1*t4; y1 = b1 - i*d1,
+//          y4 = b1 + i*d1, y2 = b2 - i*d2, y3 = b2 + i*d2; inverse swaps
+//          y1/y4 and y2/y3.
+//   other (odd prime) p, h = (p-1)/2, a_r = v_r + v_(p-r), b_r = v_r - v_(p-r):
+//          y0 = v0 + a_1 + ... + a_h (left to right); for k = 1..h, with
+//          c, s = cos, sin table at (r*k mod p)*(n/p) and sums from 0 over
+//          r = 1..h: Pr += a_r.re*c, Pi += a_r.im*c, Qr += b_r.re*s,
+//          Qi += b_r.im*s; m = (v0.re + Pr, v0.im + Pi); forward
+//          y_k = (m.re + Qi, m.im - Qr), y_(p-k) = (m.re - Qi, m.im + Qr);
+//          inverse swaps y_k and y_(p-k).
+//   C1 = cos(2pi/5), C2 = cos(4pi/5), S1 = sin(2pi/5), S2 = sin(4pi/5),
+//   S3 = sin(2pi/3), as the decimal literals below.
+//
+// Real 3-D convolution, box [z][y][x], spectrum S[kz][ky][kx], kx < n2 = n/2+1:
+//   x forward: rows y = 2q and 2q+1 (0 when 2q+1 = n) form z = a + i*b; Z =
+//     FFT(z); for k < n2, m = (n - k) mod n: S[2q][k] = (0.5*(Z_k.re + Z_m.re),
+//     0.5*(Z_k.im - Z_m.im)), S[2q+1][k] = (0.5*(Z_k.im + Z_m.im),
+//     0.5*(Z_m.re - Z_k.re));
+//   y forward, z forward, re*G and im*G with G[kz2^2+ky2^2+kx2^2]
+//     (k2 = min(k, n-k)), z inverse, y inverse: complex FFTs along the axis;
+//   x inverse: with A = S[2q][.], B = S[2q+1][.] (0 when 2q+1 = n):
+//     Z_k = (A_k.re - B_k.im, A_k.im + B_k.re) for k < n2 and
+//     Z_k = (A_m.re + B_m.im, B_m.re - A_m.im), m = n - k, for k >= n2;
+//     z = IFFT(Z); row 2q = z.re, row 2q+1 = z.im.
+// G already holds 1/n^3.  G(s) is the reference's expression evaluated left to
+// right (src/box.cpp:760-761).
 struct ConvTables {
   int n = 0;
   std::vector<double> c, s, g;
+  std::vector<int> radix;
 };
+
+const double kC1 = 0.309016994374947424102293417182819059;
+const double kC2 = -0.809016994374947424102293417182819059;
+const double kS1 = 0.951056516295153572116439333379382143;
+const double kS2 = 0.587785252292473129168705954639072769;
+const double kS3 = 0.866025403784438646763723170752936183;
+
+std::vector<int> fft_radices(int n) {
+  std::vector<int> r;
+  int m = n;
+  while (m % 4 == 0) { r.push_back(4); m /= 4; }
+  if (m % 2 == 0) { r.push_back(2); m /= 2; }
+  for (int p = 3; m > 1; p += 2)
+    while (m % p == 0) { r.push_back(p); m /= p; }
+  return r;
+}
+
+struct Cx {
+  double re, im;
+};
+
+inline Cx twiddle(Cx v, double c, double s, bool inv) {
+  if (inv) return Cx{v.re * c - v.im * s, v.re * s + v.im * c};
+  return Cx{v.re * c + v.im * s, v.im * c - v.re * s};
+}
+
+// One FFT of the contiguous line x (length n); w is scratch of the same length.
+void fft_line(const ConvTables& T, Cx* x, Cx* w, bool inv) {
+  const int n = T.n;
+  const double* C = T.c.data();
+  const double* S = T.s.data();
+  int Ns = 1;
+  Cx v[128], y[128];
+  for (int p : T.radix) {
+    const int Tn = n / p, step = n / (Ns * p);
+    for (int i = 0; i < Tn; ++i) {
+      const int j = i % Ns;
+      for (int r = 0; r < p; ++r) v[r] = x[i + r * Tn];
+      if (j > 0)
+        for (int r = 1; r < p; ++r) {
+          const int t = r * j * step;
+          v[r] = twiddle(v[r], C[t], S[t], inv);
+        }
+      if (p == 2) {
+        y[0] = Cx{v[0].re + v[1].re, v[0].im + v[1].im};
+        y[1] = Cx{v[0].re - v[1].re, v[0].im - v[1].im};
+      } else if (p == 4) {
+        const Cx a0{v[0].re + v[2].re, v[0].im + v[2].im}, a1{v[0].re - v[2].re, v[0].im - v[2].im};
+        const Cx a2{v[1].re + v[3].re, v[1].im + v[3].im}, a3{v[1].re - v[3].re, v[1].im - v[3].im};
+        y[0] = Cx{a0.re + a2.re, a0.im + a2.im};
+        y[2] = Cx{a0.re - a2.re, a0.im - a2.im};
+        const Cx m1{a1.re + a3.im, a1.im - a3.re}, p1{a1.re - a3.im, a1.im + a3.re};
+        y[1] = inv ? p1 : m1;
+        y[3] = inv ? m1 : p1;
+      } else if (p == 3) {
+        const Cx t1{v[1].re + v[2].re, v[1].im + v[2].im};
+        const Cx t2{v[0].re - 0.5 * t1.re, v[0].im - 0.5 * t1.im};
+        const Cx t3{kS3 * (v[1].re - v[2].re), kS3 * (v[1].im - v[2].im)};
+        y[0] = Cx{v[0].re + t1.re, v[0].im + t1.im};
+        const Cx m1{t2.re + t3.im, t2.im - t3.re}, p1{t2.re - t3.im, t2.im + t3.re};
+        y[1] = inv ? p1 : m1;
+        y[2] = inv ? m1 : p1;
+      } else if (p == 5) {
+        const Cx t1{v[1].re + v[4].re, v[1].im + v[4].im}, t2{v[2].re + v[3].re, v[2].im + v[3].im};
+        const Cx t3{v[1].re - v[4].re, v[1].im - v[4].im}, t4{v[2].re - v[3].re, v[2].im - v[3].im};
+        y[0] = Cx{(v[0].re + t1.re) + t2.re, (v[0].im + t1.im) + t2.im};
+        const Cx b1{(v[0].re + kC1 * t1.re) + kC2 * t2.re, (v[0].im + kC1 * t1.im) + kC2 * t2.im};
+        const Cx b2{(v[0].re + kC2 * t1.re) + kC1 * t2.re, (v[0].im + kC2 * t1.im) + kC1 * t2.im};
+        const Cx d1{kS1 * t3.re + kS2 * t4.re, kS1 * t3.im + kS2 * t4.im};
+        const Cx d2{kS2 * t3.re - kS1 * t4.re, kS2 * t3.im - kS1 * t4.im};
+        const Cx m1{b1.re + d1.im, b1.im - d1.re}, p1{b1.re - d1.im, b1.im + d1.re};
+        const Cx m2{b2.re + d2.im, b2.im - d2.re}, p2{b2.re - d2.im, b2.im + d2.re};
+        y[1] = inv ? p1 : m1;
+        y[4] = inv ? m1 : p1;
+        y[2] = inv ? p2 : m2;
+        y[3] = inv ? m2 : p2;
+      } else {
+        const int h = (p - 1) / 2;
+        Cx y0 = v[0];
+        for (int r = 1; r <= h; ++r) {
+          y0.re = y0.re + (v[r].re + v[p - r].re);
+          y0.im = y0.im + (v[r].im + v[p - r].im);
+        }
+        y[0] = y0;
+        for (int k = 1; k <= h; ++k) {
+          double Pr = 0.0, Pi = 0.0, Qr = 0.0, Qi = 0.0;
+          for (int r = 1; r <= h; ++r) {
+            const Cx a{v[r].re + v[p - r].re, v[r].im + v[p - r].im};
+            const Cx b{v[r].re - v[p - r].re, v[r].im - v[p - r].im};
+            const int t = (r * k % p) * Tn;
+            Pr = Pr + a.re * C[t];
+            Pi = Pi + a.im * C[t];
+            Qr = Qr + b.re * S[t];
+            Qi = Qi + b.im * S[t];
+          }
+          const Cx m{v[0].re + Pr, v[0].im + Pi};
+          const Cx lo{m.re + Qi, m.im - Qr}, hi{m.re - Qi, m.im + Qr};
+          y[k] = inv ? hi : lo;
+          y[p - k] = inv ? lo : hi;
+        }
+      }
+      const int base = (i - j) * p + j;
+      for (int k = 0; k < p; ++k) w[base + k * Ns] = y[k];
+    }
+    std::copy(w, w + n, x);
+    Ns *= p;
+  }
+}
 
 // The tables are computed by the C library's exp/cos/sin called through
 // volatile pointers, so no compiler folds, fuses (sincos) or vectorises the
@@ -245,106 +394,76 @@ ConvTables make_tables(int n, int nn, double dt) {
   int h2 = n / 2;
   T.g.resize(3 * h2 * h2 + 1);
   for (int q = 0; q <= 3 * h2 * h2; ++q) T.g[q] = gauss_factor(q, n, nn, dt);
+  T.radix = fft_radices(n);
   return T;
 }
 
 void convolve_box(const ConvTables& T, const double* in, double* out) {
-  const int n = T.n, n2 = n / 2 + 1;
-  const double* C = T.c.data();
-  const double* S = T.s.data();
-  std::vector<double> ar((size_t)n * n * n2), ai((size_t)n * n * n2);
-  std::vector<double> br((size_t)n * n * n2), bi((size_t)n * n * n2);
-  auto sidx = [&](int kz, int ky, int kx) { return (size_t)kx + (size_t)n2 * (ky + (size_t)n * kz); };
-  // forward x: a[z][y][kx]
+  const int n = T.n, n2 = n / 2 + 1, nr = (n + 1) / 2;
+  // spectrum S[kz][ky][kx]
+  std::vector<Cx> Sp((size_t)n * n * n2);
+  auto sidx = [&](int z, int y, int kx) { return (size_t)kx + (size_t)n2 * (y + (size_t)n * z); };
+  std::vector<Cx> line(n), w(n);
+  // x forward, two real rows per complex FFT
   for (int z = 0; z < n; ++z)
-    for (int y = 0; y < n; ++y) {
-      const double* v = in + ((size_t)z * n + y) * n;
-      for (int kx = 0; kx < n2; ++kx) {
-        double re = 0.0, im = 0.0;
-        for (int x = 0; x < n; ++x) {
-          int t = (x * kx) % n;
-          re = re + v[x] * C[t];
-          im = im - v[x] * S[t];
-        }
-        ar[sidx(z, y, kx)] = re;
-        ai[sidx(z, y, kx)] = im;
+    for (int q = 0; q < nr; ++q) {
+      const double* a = in + ((size_t)z * n + 2 * q) * n;
+      const bool two = 2 * q + 1 < n;
+      for (int x = 0; x < n; ++x) line[x] = Cx{a[x], two ? a[n + x] : 0.0};
+      fft_line(T, line.data(), w.data(), false);
+      for (int k = 0; k < n2; ++k) {
+        const int m = (n - k) % n;
+        const Cx zk = line[k], zm = line[m];
+        Sp[sidx(z, 2 * q, k)] = Cx{0.5 * (zk.re + zm.re), 0.5 * (zk.im - zm.im)};
+        if (two) Sp[sidx(z, 2 * q + 1, k)] = Cx{0.5 * (zk.im + zm.im), 0.5 * (zm.re - zk.re)};
       }
     }
-  // forward y: b[z][ky][kx]
+  // y forward
   for (int z = 0; z < n; ++z)
-    for (int kx = 0; kx < n2; ++kx)
-      for (int ky = 0; ky < n; ++ky) {
-        double re = 0.0, im = 0.0;
-        for (int y = 0; y < n; ++y) {
-          int t = (y * ky) % n;
-          double a = ar[sidx(z, y, kx)], b = ai[sidx(z, y, kx)];
-          re = re + (a * C[t] + b * S[t]);
-          im = im + (b * C[t] - a * S[t]);
-        }
-        br[sidx(z, ky, kx)] = re;
-        bi[sidx(z, ky, kx)] = im;
-      }
-  // forward z, spectral factor, inverse z: a[z][ky][kx]
-  std::vector<double> fr(n), fi(n);
+    for (int kx = 0; kx < n2; ++kx) {
+      for (int y = 0; y < n; ++y) line[y] = Sp[sidx(z, y, kx)];
+      fft_line(T, line.data(), w.data(), false);
+      for (int y = 0; y < n; ++y) Sp[sidx(z, y, kx)] = line[y];
+    }
+  // z forward, spectral factor, z inverse
   for (int ky = 0; ky < n; ++ky)
     for (int kx = 0; kx < n2; ++kx) {
-      int ky2 = std::min(ky, n - ky), kx2 = std::min(kx, n - kx);
+      const int ky2 = std::min(ky, n - ky), kx2 = std::min(kx, n - kx);
+      for (int z = 0; z < n; ++z) line[z] = Sp[sidx(z, ky, kx)];
+      fft_line(T, line.data(), w.data(), false);
       for (int kz = 0; kz < n; ++kz) {
-        double re = 0.0, im = 0.0;
-        for (int z = 0; z < n; ++z) {
-          int t = (z * kz) % n;
-          double a = br[sidx(z, ky, kx)], b = bi[sidx(z, ky, kx)];
-          re = re + (a * C[t] + b * S[t]);
-          im = im + (b * C[t] - a * S[t]);
-        }
-        int kz2 = std::min(kz, n - kz);
-        double G = T.g[kz2 * kz2 + ky2 * ky2 + kx2 * kx2];
-        fr[kz] = re * G;
-        fi[kz] = im * G;
+        const int kz2 = std::min(kz, n - kz);
+        const double G = T.g[kz2 * kz2 + ky2 * ky2 + kx2 * kx2];
+        line[kz] = Cx{line[kz].re * G, line[kz].im * G};
       }
-      for (int z = 0; z < n; ++z) {
-        double re = 0.0, im = 0.0;
-        for (int kz = 0; kz < n; ++kz) {
-          int t = (z * kz) % n;
-          double a = fr[kz], b = fi[kz];
-          re = re + (a * C[t] - b * S[t]);
-          im = im + (a * S[t] + b * C[t]);
-        }
-        ar[sidx(z, ky, kx)] = re;
-        ai[sidx(z, ky, kx)] = im;
+      fft_line(T, line.data(), w.data(), true);
+      for (int z = 0; z < n; ++z) Sp[sidx(z, ky, kx)] = line[z];
+    }
+  // y inverse
+  for (int z = 0; z < n; ++z)
+    for (int kx = 0; kx < n2; ++kx) {
+      for (int y = 0; y < n; ++y) line[y] = Sp[sidx(z, y, kx)];
+      fft_line(T, line.data(), w.data(), true);
+      for (int y = 0; y < n; ++y) Sp[sidx(z, y, kx)] = line[y];
+    }
+  // x inverse, two real rows per complex FFT
+  for (int z = 0; z < n; ++z)
+    for (int q = 0; q < nr; ++q) {
+      const bool two = 2 * q + 1 < n;
+      for (int k = 0; k < n; ++k) {
+        const bool lo = k < n2;
+        const int m = lo ? k : n - k;
+        const Cx A = Sp[sidx(z, 2 * q, m)];
+        const Cx B = two ? Sp[sidx(z, 2 * q + 1, m)] : Cx{0.0, 0.0};
+        line[k] = lo ? Cx{A.re - B.im, A.im + B.re} : Cx{A.re + B.im, B.re - A.im};
+      }
+      fft_line(T, line.data(), w.data(), true);
+      double* o = out + ((size_t)z * n + 2 * q) * n;
+      for (int x = 0; x < n; ++x) {
+        o[x] = line[x].re;
+        if (two) o[n + x] = line[x].im;
       }
     }
-  // inverse y: b[z][y][kx]
-  for (int z = 0; z < n; ++z)
-    for (int kx = 0; kx < n2; ++kx)
-      for (int y = 0; y < n; ++y) {
-        double re = 0.0, im = 0.0;
-        for (int ky = 0; ky < n; ++ky) {
-          int t = (y * ky) % n;
-          double a = ar[sidx(z, ky, kx)], b = ai[sidx(z, ky, kx)];
-          re = re + (a * C[t] - b * S[t]);
-          im = im + (a * S[t] + b * C[t]);
-        }
-        br[sidx(z, y, kx)] = re;
-        bi[sidx(z, y, kx)] = im;
-      }
-  // inverse x (half spectrum -> real)
-  const int half = (n - 1) / 2;
-  for (int z = 0; z < n; ++z)
-    for (int y = 0; y < n; ++y)
-      for (int x = 0; x < n; ++x) {
-        double acc = br[sidx(z, y, 0)];
-        for (int kx = 1; kx <= half; ++kx) {
-          int t = (x * kx) % n;
-          double a = br[sidx(z, y, kx)], b = bi[sidx(z, y, kx)];
-          acc = acc + 2.0 * (a * C[t] - b * S[t]);
-        }
-        if ((n & 1) == 0) {
-          double a = br[sidx(z, y, n / 2)];
-          acc = acc + ((x & 1) ? -a : a);
-        }
-        out[((size_t)z * n + y) * n + x] = acc;
-      }
 }
 
 // ---------------------------------------------------------------------------
