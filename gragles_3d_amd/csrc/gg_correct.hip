@@ -23,12 +23,15 @@
 // After each convolution:
 //   k_correct      one workgroup per (grain, z-layer) of the output box: the
 //                  band voxels (|phi_in| < delta) are queued with the slot of
-//                  their IDLocal id in the grain's hull-id table; each lane
-//                  takes one, walks the grain's elements in projection order
-//                  (wave-uniform loop; the barycenters of an element that some
-//                  lane needs are read with scalar loads) and keeps the first
-//                  nearest barycenter (squared distance, strict '<') among the
-//                  elements that border its id; then phi_out = phi_in +
+//                  their IDLocal id in the grain's hull-id table and grouped by
+//                  slot (counting sort in LDS), so a wavefront's 64 voxels
+//                  mostly border the same grain; each lane takes one, walks
+//                  the grain's elements in projection order (wave-uniform
+//                  loop that skips the elements no lane needs; the
+//                  barycenters are read with wave-uniform loads, four per
+//                  round) and keeps the first nearest barycenter (squared
+//                  distance, strict '<') among the elements that border its
+//                  id; then phi_out = phi_in +
 //                  (phi_out - phi_in) * w_energy * w_mobility and, with stored
 //                  energies on, minus (SEE_self - SEE_id) * w_mobility * dt /
 //                  TimeSlope^2.  The weights are GBInfo's as get_GBInfo hands
@@ -320,18 +323,30 @@ __global__ __launch_bounds__(256) void k_tri_scatter(TopoDev topo, CorrDev cd) {
   }
 }
 
+// Queue entry of a band voxel: its layer cell, and its IDLocal id's slot in the
+// grain's hull-id table + 1 (0: not there).  Bucket of the slot grouping: the
+// slot, or kTopoCap for an id the hull does not name.
+__device__ inline int bucket_of(uint32_t qv) {
+  const int f = (int)(qv >> kCellBits);
+  return f == 0 ? kTopoCap : f - 1;
+}
+
 template <class T, int NMAX>
 __global__ __launch_bounds__(256) void k_correct(SimParams sp, PlanDev P, int cls, const int32_t* __restrict__ state,
                                                  DescDev din, DescDev dout, IdlDev il, TopoDev topo, CorrDev cd) {
   if (*cd.ready == 0) return;  // no hull yet (run_sim's first step)
+  constexpr int kBuckets = kTopoCap + 1;  // a slot, or the last bucket: an id the hull does not name
+  static_assert(kTopoCap == 64, "one wavefront scans the slot buckets");
   __shared__ int s_key[kTopoCap];
   __shared__ unsigned long long s_em[kElemCap];
   __shared__ double s_en[kElemCap], s_mo[kElemCap];
   __shared__ int s_ef[kElemCap + 1];
-  __shared__ uint32_t s_q[NMAX * NMAX];
+  __shared__ uint32_t s_raw[NMAX * NMAX];  // band voxels in layer order
+  __shared__ uint32_t s_q[NMAX * NMAX];    // the same, grouped by slot
+  __shared__ int s_cnt[kBuckets], s_at[kBuckets];
   __shared__ int s_nq;
   const int total = P.totals[kQLayers * kClasses + cls];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const T* __restrict__ ain = (const T*)din.arena;
   T* __restrict__ aout = (T*)dout.arena;
   const double delta = sp.delta;
@@ -352,6 +367,7 @@ __global__ __launch_bounds__(256) void k_correct(SimParams sp, PlanDev P, int cl
       s_mo[e] = cd.mobility[(int64_t)g * kElemCap + e];
     }
     for (int e = tid; e <= E; e += 256) s_ef[e] = cd.efirst[(int64_t)g * (kElemCap + 1) + e];
+    if (tid < kBuckets) s_cnt[tid] = 0;
     if (tid == 0) s_nq = 0;
     __syncthreads();
     // IDLocal as the resize to the output bounds leaves it: the old content at
@@ -378,15 +394,36 @@ __global__ __launch_bounds__(256) void k_correct(SimParams sp, PlanDev P, int cl
           if (k == -1) break;
           hsl = (hsl + 1) & (kTopoCap - 1);
         }
-        s_q[atomicAdd(&s_nq, 1)] = (uint32_t)c | (uint32_t)(slot + 1) << kCellBits;
+        const uint32_t qv = (uint32_t)c | (uint32_t)(slot + 1) << kCellBits;
+        atomicAdd(&s_cnt[bucket_of(qv)], 1);
+        s_raw[atomicAdd(&s_nq, 1)] = qv;
       }
     }
     __syncthreads();
+    // group the queue by slot (counting sort), so that a wavefront's voxels
+    // mostly border one grain and it walks only that grain's elements
+    if (wv == 0) {
+      const int v = s_cnt[lane];
+      int incl = v;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += u;
+      }
+      s_at[lane] = incl - v;
+      if (lane == 63) s_at[kTopoCap] = incl;
+    }
+    __syncthreads();
     const int nq = s_nq;
+    for (int w = tid; w < nq; w += 256) {
+      const uint32_t qv = s_raw[w];
+      s_q[atomicAdd(&s_at[bucket_of(qv)], 1)] = qv;
+    }
+    __syncthreads();
     const double* __restrict__ tri = cd.tri + 3 * tb;
     const double see_g = cd.see_on ? cd.see[g] : 0.0;
-    for (int w0 = 0; w0 < nq; w0 += 256) {
-      const int w = w0 + tid;
+    for (int w0 = wv * 64; w0 < nq; w0 += 256) {
+      const int w = w0 + lane;  // a wavefront takes 64 consecutive queue entries
       const bool act = w < nq;
       const uint32_t qv = act ? s_q[w] : 0u;
       const int c = (int)(qv & ((1u << kCellBits) - 1u));
@@ -395,14 +432,31 @@ __global__ __launch_bounds__(256) void k_correct(SimParams sp, PlanDev P, int cl
       // Vector3d(i, j, k): row, column, depth, against barycenters (x, y, z)
       const double px = (double)(y0 + yl), py = (double)(x0 + xl), pz = (double)(z0 + kl);
       double minimal = 10000000.0, w_energy = 1., w_mobility = 1.;
-      const unsigned long long bit = slot >= 0 ? 1ull << slot : 0ull;
+      const unsigned long long bit = (act && slot >= 0) ? 1ull << slot : 0ull;
       for (int e = 0; e < E; ++e) {
-        const bool need = act && (s_em[e] & bit) != 0ull;
+        const bool need = (s_em[e] & bit) != 0ull;
         if (__ballot(need) == 0ull) continue;
-        const int t0 = __builtin_amdgcn_readfirstlane(s_ef[e]);
         const int t1 = __builtin_amdgcn_readfirstlane(s_ef[e + 1]);
+        int t = __builtin_amdgcn_readfirstlane(s_ef[e]);
         const double en = s_en[e], mo = s_mo[e];
-        for (int t = t0; t < t1; ++t) {
+        // four barycenters per round: their loads issue together
+        for (; t + 4 <= t1; t += 4) {
+          const double* q = tri + 3 * t;
+          double d[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const double dx = px - q[3 * u], dy = py - q[3 * u + 1], dz = pz - q[3 * u + 2];
+            d[u] = dx * dx + dy * dy + dz * dz;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (need && d[u] < minimal) {
+              minimal = d[u];
+              w_energy = mo;
+              w_mobility = en;
+            }
+        }
+        for (; t < t1; ++t) {
           const double dx = px - tri[3 * t], dy = py - tri[3 * t + 1], dz = pz - tri[3 * t + 2];
           const double d = dx * dx + dy * dy + dz * dz;
           if (need && d < minimal) {
@@ -427,7 +481,7 @@ __global__ __launch_bounds__(256) void k_correct(SimParams sp, PlanDev P, int cl
         out[li] = (T)o;
       }
     }
-    __syncthreads();  // s_q, s_nq and the element tables are refilled for the next tile
+    __syncthreads();  // the queues and the element tables are refilled for the next tile
   }
 }
 
