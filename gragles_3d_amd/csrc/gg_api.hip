@@ -39,6 +39,7 @@ struct gg_ctx {
   int32_t* d_state = nullptr;
   int32_t* d_newb = nullptr;
   int32_t* d_cnb = nullptr;  // resizeGrid's bounds of the last grid coarsening (allocated on first use)
+  int32_t* d_vorder = nullptr;  // volume-sorted work order of the plans (allocated on first use)
   uint32_t* d_cont = nullptr;  // voxel container (allocated on first use)
   int64_t cont_cap = 0;
   TopoDev topo{nullptr, nullptr, nullptr, 0, nullptr, nullptr};  // hull topology tables (allocated when first enabled)
@@ -1096,6 +1097,29 @@ int gg_autotune(gg_ctx* c, int32_t reps, int32_t* choice, float* ms) {
 int gg_get_tuning(gg_ctx* c, int32_t* variants) {
   if (!c || !variants) return GG_ERR_ARG;
   for (int k = 0; k < kClasses; ++k) variants[k] = c->cmp_variant[k];
+  return GG_OK;
+}
+
+int gg_set_work_order(gg_ctx* c, int32_t order) {
+  if (!c) return GG_ERR_ARG;
+  if (order != GG_ORDER_SPATIAL && order != GG_ORDER_VOLUME) return fail(c, GG_ERR_ARG, "unknown work order");
+  int32_t* v = nullptr;
+  if (order == GG_ORDER_VOLUME) {
+    if (!c->d_vorder) {
+      int rc = dalloc(c, &c->d_vorder, (size_t)c->p.n_grains);
+      if (rc) return rc;
+    }
+    v = c->d_vorder;
+  }
+  c->P.vorder = v;
+  c->P2.vorder = v;
+  drop_graph(c);
+  return GG_OK;
+}
+
+int gg_get_work_order(gg_ctx* c, int32_t* order) {
+  if (!c || !order) return GG_ERR_ARG;
+  *order = c->P.vorder ? GG_ORDER_VOLUME : GG_ORDER_SPATIAL;
   return GG_OK;
 }
 

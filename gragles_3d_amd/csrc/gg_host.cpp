@@ -312,6 +312,216 @@ int gg_generate_voronoi(const gg_params* p, uint64_t seed, int32_t* idfield, int
   return GG_OK;
 }
 
+int gg_bimodal_seeds(int32_t n_grains, uint64_t seed, double frac_large, double size_ratio, double sigma,
+                     double* xyz, double* radius) {
+  if (n_grains < 1 || !xyz || !radius || !(frac_large >= 0 && frac_large <= 1) || !(size_ratio > 0) ||
+      !(sigma >= 0))
+    return GG_ERR_ARG;
+  const int N = n_grains;
+  SplitMix64 rng{seed};
+  // radii: the mode (1 or size_ratio) times a lognormal factor exp(sigma z),
+  // z standard normal by Box-Muller from two uniforms
+  std::vector<double> r(N);
+  double vol = 0;
+  for (int i = 0; i < N; ++i) {
+    const bool large = rng.uniform() < frac_large;
+    const double u1 = 1.0 - rng.uniform(), u2 = rng.uniform();
+    const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * kPi * u2);
+    r[i] = (large ? size_ratio : 1.0) * std::exp(sigma * z);
+    vol += r[i] * r[i] * r[i];
+  }
+  // scale: the balls' volumes add up to the unit cube's
+  const double s = std::cbrt(1.0 / (4.0 / 3.0 * kPi * vol));
+  double rmax = 0;
+  for (int i = 0; i < N; ++i) {
+    r[i] *= s;
+    rmax = std::max(rmax, r[i]);
+  }
+  // random sequential addition, largest first: a seed is kept if its ball of
+  // radius alpha * r_i misses the ball every earlier seed kept (alpha shrinks
+  // for a grain that finds no room in kTries attempts)
+  std::vector<int> ord(N);
+  for (int i = 0; i < N; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return r[a] > r[b]; });
+  constexpr double kAlpha = 0.85;
+  constexpr int kTries = 200;
+  const double cs = kAlpha * 2.0 * rmax;  // a bucket spans the largest exclusion distance
+  const int B = std::max(1, std::min(256, (int)(1.0 / cs)));
+  std::vector<std::vector<int>> bucket((size_t)B * B * B);
+  auto bi = [&](double v) { int q = (int)(v * B); return q < 0 ? 0 : (q >= B ? B - 1 : q); };
+  for (int i : ord) {
+    double alpha = kAlpha, px = 0, py = 0, pz = 0;
+    for (int attempt = 0;; ++attempt) {
+      if (attempt > 0 && attempt % kTries == 0) alpha *= 0.9;
+      px = rng.uniform();
+      py = rng.uniform();
+      pz = rng.uniform();
+      const int bx = bi(px), by = bi(py), bz = bi(pz);
+      bool ok = true;
+      for (int cz = std::max(0, bz - 1); ok && cz <= std::min(B - 1, bz + 1); ++cz)
+        for (int cy = std::max(0, by - 1); ok && cy <= std::min(B - 1, by + 1); ++cy)
+          for (int cx = std::max(0, bx - 1); ok && cx <= std::min(B - 1, bx + 1); ++cx)
+            for (int j : bucket[((size_t)cz * B + cy) * B + cx]) {
+              const double dx = px - xyz[3 * j], dy = py - xyz[3 * j + 1], dz = pz - xyz[3 * j + 2];
+              const double m = alpha * r[i] + radius[j];  // the ball j keeps
+              if (dx * dx + dy * dy + dz * dz < m * m) {
+                ok = false;
+                break;
+              }
+            }
+      if (ok) break;
+    }
+    xyz[3 * i] = px;
+    xyz[3 * i + 1] = py;
+    xyz[3 * i + 2] = pz;
+    bucket[((size_t)bi(pz) * B + bi(py)) * B + bi(px)].push_back(i);
+    radius[i] = alpha * r[i];
+  }
+  return GG_OK;
+}
+
+int gg_generate_power(const gg_params* p, const double* xyz, const double* radius, int32_t* idfield,
+                      int32_t* bounds, uint8_t* exists, int32_t nthreads) {
+  if (!p || !xyz || !radius || !bounds || !exists) return GG_ERR_ARG;
+  const int N = p->n_grains, L = p->ngridpoints, gb = p->grid_blowup;
+  const double h = p->h;
+  std::vector<double> w(N);
+  double wmax = 0;
+  for (int i = 0; i < N; ++i) {
+    w[i] = radius[i] * radius[i];
+    wmax = std::max(wmax, w[i]);
+  }
+  // seed buckets over [0,1)^3 with each bucket's largest weight
+  const int B = std::max(1, (int)std::cbrt((double)N));
+  const double cs = 1.0 / B;
+  auto bidx = [&](double v) { int q = (int)(v * B); return q < 0 ? 0 : (q >= B ? B - 1 : q); };
+  std::vector<int> cstart((size_t)B * B * B + 1, 0), cell(N);
+  for (int i = 0; i < N; ++i) {
+    cell[i] = (bidx(xyz[3 * i + 2]) * B + bidx(xyz[3 * i + 1])) * B + bidx(xyz[3 * i]);
+    cstart[cell[i] + 1]++;
+  }
+  for (size_t c = 0; c < (size_t)B * B * B; ++c) cstart[c + 1] += cstart[c];
+  std::vector<int> items(N), fillp(cstart.begin(), cstart.end() - 1);
+  for (int i = 0; i < N; ++i) items[fillp[cell[i]]++] = i;
+  std::vector<double> bw((size_t)B * B * B, 0.0);
+  for (int i = 0; i < N; ++i) bw[cell[i]] = std::max(bw[cell[i]], w[i]);
+  // voxel blocks of 8^3 over the interior: the seeds that can own a voxel of
+  // the block are those whose smallest power distance over the block does not
+  // exceed the smallest largest-power-distance U of any seed (exact: another
+  // seed is then strictly better everywhere in the block)
+  constexpr int kBlk = 8;
+  const int lo = gb, hi = L - 1 - gb;  // interior voxels: lo <= i, j, k < hi
+  const int nb = hi > lo ? (hi - lo + kBlk - 1) / kBlk : 0;
+  const int T = resolve_threads(nthreads);
+  std::vector<int32_t> blo((size_t)T * (N + 1) * 3, INT32_MAX), bhi((size_t)T * (N + 1) * 3, INT32_MIN);
+  if (idfield) std::memset(idfield, 0, sizeof(int32_t) * (size_t)L * L * L);
+  parallel_for(nb * nb * nb, T, [&](int blk, int t) {
+    const int kb = blk / (nb * nb), ib = (blk / nb) % nb, jb = blk % nb;
+    const int k0 = lo + kb * kBlk, i0 = lo + ib * kBlk, j0 = lo + jb * kBlk;
+    const int k1 = std::min(hi, k0 + kBlk), i1 = std::min(hi, i0 + kBlk), j1 = std::min(hi, j0 + kBlk);
+    // the block's box in domain coordinates (x from j, y from i, z from k)
+    const double ax = (j0 - gb) * h, bx_ = (j1 - 1 - gb) * h, ay = (i0 - gb) * h, by_ = (i1 - 1 - gb) * h,
+                 az = (k0 - gb) * h, bz_ = (k1 - 1 - gb) * h;
+    auto lbound = [&](double px, double py, double pz) {  // squared distance, box to point
+      const double dx = px < ax ? ax - px : (px > bx_ ? px - bx_ : 0.0);
+      const double dy = py < ay ? ay - py : (py > by_ ? py - by_ : 0.0);
+      const double dz = pz < az ? az - pz : (pz > bz_ ? pz - bz_ : 0.0);
+      return dx * dx + dy * dy + dz * dz;
+    };
+    auto ubound = [&](double px, double py, double pz) {  // largest squared distance over the box
+      const double dx = std::max(std::fabs(px - ax), std::fabs(px - bx_));
+      const double dy = std::max(std::fabs(py - ay), std::fabs(py - by_));
+      const double dz = std::max(std::fabs(pz - az), std::fabs(pz - bz_));
+      return dx * dx + dy * dy + dz * dz;
+    };
+    const int cx0 = bidx(ax), cx1 = bidx(bx_), cy0 = bidx(ay), cy1 = bidx(by_), cz0 = bidx(az), cz1 = bidx(bz_);
+    double U = 1e300;
+    std::vector<int> cand;
+    std::vector<double> candlb;
+    for (int r = 0; r <= B; ++r) {
+      // buckets at Chebyshev distance r from the block's bucket range
+      for (int cz = cz0 - r; cz <= cz1 + r; ++cz) {
+        if (cz < 0 || cz >= B) continue;
+        for (int cy = cy0 - r; cy <= cy1 + r; ++cy) {
+          if (cy < 0 || cy >= B) continue;
+          for (int cx = cx0 - r; cx <= cx1 + r; ++cx) {
+            if (cx < 0 || cx >= B) continue;
+            const int ring = std::max(std::max(std::max(cz0 - cz, cz - cz1), std::max(cy0 - cy, cy - cy1)),
+                                      std::max(cx0 - cx, cx - cx1));
+            if (ring != r) continue;
+            const int c = (cz * B + cy) * B + cx;
+            for (int q = cstart[c]; q < cstart[c + 1]; ++q) {
+              const int s = items[q];
+              const double sx = xyz[3 * s], sy = xyz[3 * s + 1], sz = xyz[3 * s + 2];
+              const double lb = lbound(sx, sy, sz) - w[s];
+              if (lb > U) continue;
+              U = std::min(U, ubound(sx, sy, sz) - w[s]);
+              cand.push_back(s);
+              candlb.push_back(lb);
+            }
+          }
+        }
+      }
+      // a seed in a bucket at ring > r lies more than r*cs from the block
+      const double dmin = r * cs;
+      if (U < 1e300 && dmin * dmin - wmax > U) break;
+    }
+    // keep the candidates that can win, in id order (ties go to the lowest id)
+    std::vector<int> keep;
+    const double slack = 1e-12 * (std::fabs(U) + 1.0);  // rounding of the two bounds
+    for (size_t q = 0; q < cand.size(); ++q)
+      if (candlb[q] <= U + slack) keep.push_back(cand[q]);
+    std::sort(keep.begin(), keep.end());
+    int32_t* mylo = &blo[(size_t)t * (N + 1) * 3];
+    int32_t* myhi = &bhi[(size_t)t * (N + 1) * 3];
+    for (int k = k0; k < k1; ++k)
+      for (int i = i0; i < i1; ++i)
+        for (int j = j0; j < j1; ++j) {
+          const double x = double((j - gb) * h), y = double((i - gb) * h), z = double((k - gb) * h);
+          double best = 1e300;
+          int bid = -1;
+          for (int s : keep) {
+            const double dx = x - xyz[3 * s], dy = y - xyz[3 * s + 1], dz = z - xyz[3 * s + 2];
+            const double d2 = dx * dx + dy * dy + dz * dz - w[s];
+            if (d2 < best) {
+              best = d2;
+              bid = s;
+            }
+          }
+          const int32_t id = bid + 1;
+          int32_t* l = mylo + 3 * id;
+          int32_t* u = myhi + 3 * id;
+          if (j < l[0]) l[0] = j;
+          if (j > u[0]) u[0] = j;
+          if (i < l[1]) l[1] = i;
+          if (i > u[1]) u[1] = i;
+          if (k < l[2]) l[2] = k;
+          if (k > u[2]) u[2] = k;
+          if (idfield) idfield[((size_t)k * L + i) * L + j] = id;
+        }
+  });
+  std::memset(bounds, 0, sizeof(int32_t) * 6 * (N + 1));
+  exists[0] = 0;
+  for (int g = 1; g <= N; ++g) {
+    int32_t l[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, u[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    for (int t = 0; t < T; ++t)
+      for (int a = 0; a < 3; ++a) {
+        l[a] = std::min(l[a], blo[((size_t)t * (N + 1) + g) * 3 + a]);
+        u[a] = std::max(u[a], bhi[((size_t)t * (N + 1) + g) * 3 + a]);
+      }
+    if (l[0] == INT32_MAX) {
+      exists[g] = 0;
+      continue;
+    }
+    exists[g] = 1;
+    for (int a = 0; a < 3; ++a) {
+      bounds[6 * g + a] = std::max(0, l[a] - gb);
+      bounds[6 * g + 3 + a] = std::min(L, u[a] + 1 + gb);
+    }
+  }
+  return GG_OK;
+}
+
 int gg_initial_neighbour_lists(const gg_params* p, const int32_t* bounds, const uint8_t* exists, int32_t* offsets,
                                int32_t* ids, int64_t cap, int64_t* n) {
   if (!p || !bounds || !exists || !offsets || !n) return GG_ERR_ARG;

@@ -153,6 +153,7 @@ struct PlanDev {
   int32_t* grains;      // [N] alive grain ids grouped by class, in `order` inside a class
   int32_t* grain_pos;   // [N+1] index of a grain in grains[] (-1 if not alive)
   int32_t* order;       // [N] permutation of 1..N: the spatial (Morton) order of the initial box centres
+  int32_t* vorder;      // [N] work order of the step when it is volume-sorted (nullptr: the spatial order)
   int32_t* ncls;        // [N] per position in `order`: box side << 8 | size class, or -1 (not planned)
   int32_t* pre[kNumQ];  // [N + kClasses] per-class exclusive prefixes (segment c at cls_start[c]+c)
   int32_t* totals;      // [kNumQ * kClasses]

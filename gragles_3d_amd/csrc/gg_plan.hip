@@ -59,6 +59,73 @@ __global__ __launch_bounds__(256) void k_plan_classes(SimParams sp, const int32_
   }
 }
 
+// The volume-sorted work queue: the grains of the spatial order sorted by box
+// side, largest first (not-alive grains last), equal sides kept in the spatial
+// order.  One workgroup: a side histogram, its descending exclusive scan, then
+// rounds of kPlanThreads positions in which each lane's rank among equal sides
+// comes from ballots inside its wavefront and the counts of the wavefronts
+// before it, so the sort is stable and deterministic.
+__global__ __launch_bounds__(kPlanThreads) void k_plan_vorder(SimParams sp, const int32_t* __restrict__ state,
+                                                              const int32_t* __restrict__ bounds, PlanDev P) {
+  constexpr int kBins = kMaxN + 1;  // side 0: not planned
+  constexpr int kWaves = kPlanThreads / 64;
+  __shared__ int s_start[kBins];
+  __shared__ int s_wcnt[kWaves][kBins];
+  const int N = sp.N, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  auto key_at = [&](int idx) {
+    const int g = P.order[idx];
+    if (state[g] != kAlive) return 0;
+    const int n = bounds[6 * g + 3] - bounds[6 * g];
+    return n < 1 ? 0 : (n > kMaxN ? kMaxN : n);
+  };
+  for (int i = t; i < kBins; i += kPlanThreads) s_start[i] = 0;
+  __syncthreads();
+  for (int idx = t; idx < N; idx += kPlanThreads) atomicAdd(&s_start[key_at(idx)], 1);
+  __syncthreads();
+  if (t == 0) {
+    int acc = 0;
+    for (int k = kMaxN; k >= 0; --k) {
+      const int c = s_start[k];
+      s_start[k] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+  for (int base = 0; base < N; base += kPlanThreads) {
+    const int idx = base + t;
+    const int k = idx < N ? key_at(idx) : -1;
+    for (int i = t; i < kWaves * kBins; i += kPlanThreads) (&s_wcnt[0][0])[i] = 0;
+    __syncthreads();
+    int rank = 0;
+    bool done = k < 0;
+    for (;;) {  // one round per distinct side in the wavefront
+      const unsigned long long pend = __ballot(!done);
+      if (pend == 0ull) break;
+      const int k0 = __shfl(k, __ffsll((long long)pend) - 1);
+      const bool mine = !done && k == k0;
+      const unsigned long long m = __ballot(mine);
+      if (mine) {
+        rank = __popcll(m & ((1ull << lane) - 1ull));
+        done = true;
+      }
+      if (lane == 0) s_wcnt[wv][k0] = __popcll(m);
+    }
+    __syncthreads();
+    if (k >= 0) {
+      int pos = s_start[k] + rank;
+      for (int w = 0; w < wv; ++w) pos += s_wcnt[w][k];
+      P.vorder[pos] = P.order[idx];
+    }
+    __syncthreads();
+    for (int i = t; i < kBins; i += kPlanThreads) {
+      int add = 0;
+      for (int w = 0; w < kWaves; ++w) add += s_wcnt[w][i];
+      s_start[i] += add;
+    }
+    __syncthreads();
+  }
+}
+
 __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32_t* __restrict__ state,
                                                        const int32_t* __restrict__ bounds, int64_t* off_out,
                                                        int64_t* spec_off, PlanDev P, int64_t arena_cap,
@@ -247,13 +314,19 @@ __global__ __launch_bounds__(1024) void k_count(SimParams sp, const int32_t* __r
 void launch_plan(const Launch& L, const SimParams& sp, const int32_t* state, const int32_t* bounds,
                  int64_t* off_out, int64_t* spec_off, const PlanDev& P, int64_t arena_cap, int64_t spec_cap,
                  int32_t* err, int64_t* voxels) {
+  // with a volume-sorted queue the plan visits the grains in that order
+  PlanDev Q = P;
+  if (P.vorder) {
+    hipLaunchKernelGGL(k_plan_vorder, dim3(1), dim3(kPlanThreads), 0, L.stream, sp, state, bounds, P);
+    Q.order = P.vorder;
+  }
   const int cblocks = (sp.N + 255) / 256 < 4096 ? (sp.N + 255) / 256 : 4096;
-  hipLaunchKernelGGL(k_plan_classes, dim3(cblocks > 0 ? cblocks : 1), dim3(256), 0, L.stream, sp, state, bounds, P,
+  hipLaunchKernelGGL(k_plan_classes, dim3(cblocks > 0 ? cblocks : 1), dim3(256), 0, L.stream, sp, state, bounds, Q,
                      err);
-  hipLaunchKernelGGL(k_plan, dim3(1), dim3(kPlanThreads), 0, L.stream, sp, state, bounds, off_out, spec_off, P,
+  hipLaunchKernelGGL(k_plan, dim3(1), dim3(kPlanThreads), 0, L.stream, sp, state, bounds, off_out, spec_off, Q,
                      arena_cap, spec_cap, err, voxels);
   const int blocks = (sp.N + 255) / 256 < 1024 ? (sp.N + 255) / 256 : 1024;
-  hipLaunchKernelGGL(k_tilemap, dim3(blocks), dim3(256), 0, L.stream, sp, P);
+  hipLaunchKernelGGL(k_tilemap, dim3(blocks), dim3(256), 0, L.stream, sp, Q);
 }
 
 void launch_init_phi(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din,

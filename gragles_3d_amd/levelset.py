@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libgg_levelset.so")
 
 GG_OK, GG_ERR_ARG, GG_ERR_HIP, GG_ERR_CAPACITY, GG_ERR_STATE, GG_ERR_DEVICE = range(6)
 FP64_PLAIN, FP32_BATCHED = 0, 1
+ORDER_SPATIAL, ORDER_VOLUME = 0, 1
 DELETED, ALIVE, VANISHED = 0, 1, 2
 # comparison tile shapes of gg_autotune: (layers per wavefront, gathers in flight, run-major order)
 CMP_SHAPES = ((8, 8, 0), (16, 8, 0), (16, 16, 0), (32, 8, 0), (8, 8, 1), (16, 8, 1), (16, 16, 1), (32, 8, 1))
@@ -84,6 +85,8 @@ SIGNATURES = {
     "gg_autotune": (ctypes.c_int, [_vp, _i32, _i32p, _P(ctypes.c_float)]),
     "gg_get_tuning": (ctypes.c_int, [_vp, _i32p]),
     "gg_set_tuning": (ctypes.c_int, [_vp, _i32p]),
+    "gg_set_work_order": (ctypes.c_int, [_vp, _i32]),
+    "gg_get_work_order": (ctypes.c_int, [_vp, _i32p]),
     "gg_get_count_history": (ctypes.c_int, [_vp, _i32p, _i32, _i32p]),
     "gg_get_grain_stats": (ctypes.c_int, [_vp, _i32p, _f64p, _f64p]),
     "gg_get_bounds": (ctypes.c_int, [_vp, _i32, _i32, _i32p]),
@@ -106,6 +109,9 @@ SIGNATURES = {
     "gg_resize_to_cube": (None, [_i32p, _i32]),
     "gg_generate_voronoi": (ctypes.c_int, [_P(GGParams), ctypes.c_uint64, _i32p, _i32p, _u8p, _i32]),
     "gg_voronoi_seeds": (ctypes.c_int, [_i32, ctypes.c_uint64, _f64p]),
+    "gg_bimodal_seeds": (ctypes.c_int, [_i32, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        _f64p, _f64p]),
+    "gg_generate_power": (ctypes.c_int, [_P(GGParams), _f64p, _f64p, _i32p, _i32p, _u8p, _i32]),
     "gg_misorientation": (ctypes.c_double, [_f64p, _f64p]),
     "gg_gb_pair_tables": (ctypes.c_int, [_i32, _f64p, _i32, _i32, _f64p, _f64p, _i32]),
     "gg_euler_to_quaternion": (None, [_f64p, _f64p]),
@@ -201,6 +207,39 @@ def generate_voronoi(params, seed, with_idfield=True, nthreads=0):
                                    _p(bounds, ctypes.c_int32), _p(exists, ctypes.c_uint8), int(nthreads))
     if rc != GG_OK:
         raise GGError("gg_generate_voronoi failed (%d)" % rc)
+    return idf, bounds, exists
+
+
+def bimodal_seeds(n_grains, seed, frac_large=0.1, size_ratio=4.0, sigma=0.2):
+    """Seeds (n, 3) and radii (n,) of an uneven grain population: a fraction
+    frac_large of the grains size_ratio times larger, each radius lognormal
+    (sigma) about its mode, placed largest first by random sequential addition
+    (gg_bimodal_seeds)."""
+    xyz = np.zeros((n_grains, 3))
+    r = np.zeros(n_grains)
+    rc = lib().gg_bimodal_seeds(int(n_grains), ctypes.c_uint64(seed), float(frac_large), float(size_ratio),
+                                float(sigma), _p(xyz, ctypes.c_double), _p(r, ctypes.c_double))
+    if rc != GG_OK:
+        raise GGError("gg_bimodal_seeds failed (%d)" % rc)
+    return xyz, r
+
+
+def generate_power(params, xyz, radius, with_idfield=True, nthreads=0):
+    """Power-diagram microstructure of the given seeds and radii:
+    (idfield[L,L,L] or None, bounds[N+1,6], exists[N+1]), conventions as generate_voronoi."""
+    N, L = params.n_grains, params.ngridpoints
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    radius = np.ascontiguousarray(radius, dtype=np.float64)
+    if xyz.shape != (N, 3) or radius.shape != (N,):
+        raise GGError("need %d seeds and radii" % N)
+    idf = np.empty((L, L, L), dtype=np.int32) if with_idfield else None
+    bounds = np.zeros((N + 1, 6), dtype=np.int32)
+    exists = np.zeros(N + 1, dtype=np.uint8)
+    rc = lib().gg_generate_power(ctypes.byref(params), _p(xyz, ctypes.c_double), _p(radius, ctypes.c_double),
+                                 _p(idf, ctypes.c_int32) if idf is not None else None,
+                                 _p(bounds, ctypes.c_int32), _p(exists, ctypes.c_uint8), int(nthreads))
+    if rc != GG_OK:
+        raise GGError("gg_generate_power failed (%d)" % rc)
     return idf, bounds, exists
 
 
@@ -488,6 +527,16 @@ class LevelSetContext:
     def set_tuning(self, variants):
         v = np.ascontiguousarray(variants, dtype=np.int32)
         self._chk(lib().gg_set_tuning(self.h, _p(v, ctypes.c_int32)))
+
+    def set_work_order(self, order):
+        """ORDER_SPATIAL (Morton order of the initial boxes) or ORDER_VOLUME (a
+        device-side queue sorted by box side, largest first, rebuilt every plan)."""
+        self._chk(lib().gg_set_work_order(self.h, int(order)))
+
+    def work_order(self):
+        v = ctypes.c_int32(0)
+        self._chk(lib().gg_get_work_order(self.h, ctypes.byref(v)))
+        return v.value
 
     def synchronize(self):
         self._chk(lib().gg_synchronize(self.h))

@@ -48,6 +48,29 @@ int gg_generate_voronoi(const gg_params* params, uint64_t seed, int32_t* idfield
  * xyz[3*i .. 3*i+2] for grain id i+1. */
 int gg_voronoi_seeds(int32_t n_grains, uint64_t seed, double* xyz);
 
+/* Seeds and radii of an uneven (bimodal) grain population, for the power
+ * diagram below; no counterpart in the reference, whose VOROMicrostructure
+ * places uniform seeds for voro++ (src/grainhdl.cpp:226-396).  Per grain
+ * (id i+1, splitmix64(seed)): large with probability frac_large, radius =
+ * (size_ratio if large else 1) * exp(sigma * z), z standard normal (Box-Muller);
+ * radii scaled so the balls' volumes add up to the unit cube's.  Seeds are
+ * placed largest first by random sequential addition: a seed is kept at
+ * distance >= a_i r_i + a_j r_j from every earlier seed j, a_i = 0.85 (shrunk
+ * by 0.9 after each 200 failed draws for that grain).  xyz: 3 per grain;
+ * radius: a_i r_i, so the balls (xyz, radius) are disjoint and each lies
+ * inside its own cell of the power diagram of these weights (no cell is empty). */
+int gg_bimodal_seeds(int32_t n_grains, uint64_t seed, double frac_large, double size_ratio, double sigma,
+                     double* xyz, double* radius);
+
+/* Power (Laguerre) diagram microstructure: each interior voxel belongs to the
+ * grain minimising |x - s_i|^2 - radius_i^2 (ties to the lowest id); border
+ * shell, voxel coordinates, ID field and bounds exactly as
+ * gg_generate_voronoi, to which it reduces (bit for bit) when every radius is
+ * 0.  Exact labels by blocks of 8^3 voxels: only the seeds whose smallest
+ * power distance over the block can beat every seed's largest are tested. */
+int gg_generate_power(const gg_params* params, const double* xyz, const double* radius, int32_t* idfield,
+                      int32_t* bounds, uint8_t* exists, int32_t nthreads);
+
 /* Initial neighbour lists, as grainhdl::find_neighbors builds them with an
  * R-tree over the box bounds (src/grainhdl.cpp:1029-1054,
  * src/box.cpp:2871-2887): for every existing grain, the other existing grains

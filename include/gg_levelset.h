@@ -247,6 +247,19 @@ int gg_autotune(gg_ctx* ctx, int32_t reps, int32_t* choice, float* ms);
 /* The shape per size class in use (4 entries), and setting it (for instance a
  * choice saved from an earlier gg_autotune); default 1 for every class. */
 int gg_get_tuning(gg_ctx* ctx, int32_t* variants);
+/* Order in which every stage's launches take the grains (no counterpart in
+ * the reference, whose grainhdl hands grains to OpenMP threads by static
+ * schedulers, src/IterativeGrainScheduler.cpp:13-26,
+ * src/SquaresGrainScheduler.cpp:26-105).  GG_ORDER_SPATIAL (default): the
+ * Morton order of the initial box centres, so tiles that run together read
+ * neighbouring boxes; GG_ORDER_VOLUME: a device-side queue rebuilt at every
+ * plan, grains sorted by box side, largest first (equal sides in the spatial
+ * order), so the largest boxes start first and the tail of a launch is made
+ * of small tiles.  Results do not depend on the order. */
+#define GG_ORDER_SPATIAL 0
+#define GG_ORDER_VOLUME 1
+int gg_set_work_order(gg_ctx* ctx, int32_t order);
+int gg_get_work_order(gg_ctx* ctx, int32_t* order);
 int gg_set_tuning(gg_ctx* ctx, const int32_t* variants);
 
 /* Wait for the stream and report device-side capacity errors. */

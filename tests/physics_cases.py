@@ -108,6 +108,16 @@ def voronoi_case(n_grains=27, ppg=10, border=3, seed=5):
     return p, idf, bounds, exists, offs, ids
 
 
+def bimodal_case(n_grains=300, ppg=16, border=3, seed=4):
+    """The uneven configuration's microstructure at a reduced grain count: 10 %
+    of the grains 4x larger, lognormal sizes (sigma 0.2), power diagram."""
+    p = G.derive_params(n_grains, ppg, border)
+    xyz, rho = G.bimodal_seeds(n_grains, seed)
+    idf, bounds, exists = G.generate_power(p, xyz, rho)
+    offs, ids = G.initial_neighbour_lists(p, bounds, exists)
+    return p, idf, bounds, exists, offs, ids
+
+
 def oracle_sim(O, case):
     p, idf, bounds, exists, offs, ids = case
     orc = O.OracleSim(p.ngridpoints, p.grid_blowup, p.conv_domain_size, p.h, p.delta, p.dt, p.n_grains)
