@@ -344,6 +344,10 @@ def main():
                     help="skip the configurations with the convolution correction (anisotropic, stored energy)")
     ap.add_argument("--physics-steps", type=int, default=10)
     ap.add_argument("--physics-accuracy-steps", type=int, default=50)
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the uneven batch (50,000 bimodal grains, volume-sorted queue)")
+    ap.add_argument("--uneven-steps", type=int, default=10)
+    ap.add_argument("--uneven-accuracy-steps", type=int, default=100)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the voxel-container and grid-coarsening legs")
     ap.add_argument("--summarize-rocprof", metavar="DIR", help="print markdown tables of a rocprofv3 directory")
@@ -457,6 +461,14 @@ def main():
                 % (kind, grains, leg["ms_per_step"], leg["value"], leg["segment_ms_per_step"].get("correction", 0),
                    leg["segment_ms_per_step"].get("elements", 0)))
 
+    # ---- the uneven batch: 50,000 bimodal grains -------------------------------------------------
+    uneven = None
+    if not args.no_uneven:
+        uneven = uneven_leg(50000, 16, args.border, 4, args.uneven_steps, 2, args.uneven_accuracy_steps)
+        log("uneven batch, %d grains: %.3f ms/step (%s order), %.3e voxel-updates/s; the other order %.3f ms/step"
+            % (uneven["grains"], uneven["ms_per_step"], uneven["work_order"], uneven["value"],
+               uneven["ms_per_step_other_order"]))
+
     # ---- accuracy: FP32 beside FP64 (bit-identical to the oracle) over 200 steps ---------------
     accuracy = None
     if not args.no_accuracy and not args.no_fp64:
@@ -538,6 +550,8 @@ def main():
         out["fp32_accuracy_vs_fp64"] = accuracy
     if physics is not None:
         out["configurations_with_correction_fp32"] = physics
+    if uneven is not None:
+        out["uneven_batch_fp32"] = uneven
 
     if not args.no_cpu:
         try:
@@ -627,6 +641,71 @@ def physics_leg(kind, grains, ppg, border, seed, steps, warmup, tuning, accuracy
         e["steps"] = accuracy_steps
         e["reference"] = "FP64 GPU path with the correction (bit-identical to the FP64 oracle)"
         e["grains_sampled_for_phi"] = "every 16th grain id"
+        info["fp32_accuracy_vs_fp64"] = e
+        a32.close()
+        a64.close()
+    return info
+
+
+def uneven_leg(grains, ppg, border, seed, steps, warmup, accuracy_steps):
+    """The uneven batch: 10 % of the grains four times larger, lognormal sizes (sigma 0.2)
+    about each mode, power-diagram microstructure (gg_bimodal_seeds, gg_generate_power), FP32
+    batched.  The comparison tile shapes come from the autotuner (one timestep on a separate
+    context); the work order (device-side volume-sorted queue or spatial order) is chosen by
+    timing graph-replayed steps in each; then FP32 against the FP64 path over accuracy_steps."""
+    p = G.derive_params(grains, ppg, border)
+    t = time.perf_counter()
+    xyz, rho = G.bimodal_seeds(grains, seed)
+    idf, bounds, exists = G.generate_power(p, xyz, rho)
+    offs, ids = G.initial_neighbour_lists(p, bounds, exists)
+    inputs = dict(idfield=idf, bounds=bounds, exists=exists, offsets=offs, ids=ids)
+    cb = bounds[1:].copy()
+    for r in cb:
+        G.resize_to_cube(r, p.ngridpoints)
+    side = (cb[:, 3] - cb[:, 0])[exists[1:] == 1]
+    big = rho > np.median(rho) * 2.5
+    info = {"grains": grains, "points_per_mean_diameter": ppg, "border": border, "seed": seed,
+            "ngridpoints": p.ngridpoints, "setup_s": round(time.perf_counter() - t, 2),
+            "microstructure": "bimodal: 10%% of the grains 4x larger, lognormal sigma 0.2, random sequential "
+                              "addition + power diagram (seed %d)" % seed,
+            "large_grain_fraction": float(big.mean()),
+            "box_side_min_median_max": [int(side.min()), float(np.median(side)), int(side.max())],
+            "box_side_median_small_large": [float(np.median(side[~big[exists[1:] == 1]])),
+                                            float(np.median(side[big[exists[1:] == 1]]))],
+            "grains_per_class": np.bincount(np.searchsorted(CLASS_NMAX, side), minlength=4).tolist()}
+    tctx = new_context(G.FP32_BATCHED, inputs, p)
+    choice, ms = tctx.autotune(reps=2)
+    tctx.close()
+    info["comparison_shapes_per_class"] = choice.tolist()
+    timing = {}
+    for order, name in ((G.ORDER_VOLUME, "volume-sorted"), (G.ORDER_SPATIAL, "spatial")):
+        c = new_context(G.FP32_BATCHED, inputs, p)
+        c.set_tuning(choice)
+        c.set_work_order(order)
+        wall, vox = timed_steps(c, steps, warmup)
+        timing[name] = (wall, vox)
+        c.close()
+    best = min(timing, key=lambda k: timing[k][0])
+    other = [k for k in timing if k != best][0]
+    wall, vox = timing[best]
+    info.update({"work_order": best, "value": vox * steps / wall if steps else 0.0, "unit": "voxel-updates/s",
+                 "ms_per_step": 1e3 * wall / max(steps, 1), "box_voxels_per_step": vox, "steps": steps,
+                 "ms_per_step_other_order": 1e3 * timing[other][0] / max(steps, 1)})
+    if accuracy_steps:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from fp32_error import phi_volume_errors, trajectories, trajectory_error
+        a32 = new_context(G.FP32_BATCHED, inputs, p)
+        a64 = new_context(G.FP64_PLAIN, inputs, p)
+        for c in (a32, a64):
+            c.set_tuning(choice)
+            c.set_work_order(G.ORDER_VOLUME if best == "volume-sorted" else G.ORDER_SPATIAL)
+        tr = trajectories(a32, a64, accuracy_steps, every=10)
+        e = phi_volume_errors(a32, a64, p.delta, grains=np.arange(1, grains + 1, 50))
+        e.update(trajectory_error(a32.count_history(), a64.count_history()))
+        e["steps"] = accuracy_steps
+        e["reference"] = "FP64 GPU path (bit-identical to the FP64 oracle)"
+        e["grains_sampled_for_phi"] = "every 50th grain id"
+        e.update(tr)
         info["fp32_accuracy_vs_fp64"] = e
         a32.close()
         a64.close()
