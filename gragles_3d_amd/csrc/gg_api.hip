@@ -30,6 +30,7 @@ struct gg_ctx {
   gg_params p{};
   SimParams sp{};
   bool fp32 = false;
+  bool conv_bf16 = false;  // FP32 path: split-BF16 circulant on the matrix cores
   size_t esz = 8;
   hipStream_t stream = nullptr;
   std::string err;
@@ -335,14 +336,19 @@ int require_boxes(gg_ctx* c) {
   return GG_OK;
 }
 
+int conv_mode(const gg_ctx* c) {
+  if (!c->fp32) return kConvFFT64;
+  return c->conv_bf16 ? kConvSplitBF16 : kConvCirculantF32;
+}
+
 // ---- stages (each enqueues its kernels on the context stream) -------------
 void stage_convolution(gg_ctx* c) {
   Launch L = launch_of(c);
-  { SegTimer t(c, kSegConvA); launch_conv(L, 0, c->sp, c->P, din(c), dout(c), c->TB, c->d_spec_off, c->d_spec, c->fp32); }
-  { SegTimer t(c, kSegConvB); launch_conv(L, 1, c->sp, c->P, din(c), dout(c), c->TB, c->d_spec_off, c->d_spec, c->fp32); }
+  { SegTimer t(c, kSegConvA); launch_conv(L, 0, c->sp, c->P, din(c), dout(c), c->TB, c->d_spec_off, c->d_spec, conv_mode(c)); }
+  { SegTimer t(c, kSegConvB); launch_conv(L, 1, c->sp, c->P, din(c), dout(c), c->TB, c->d_spec_off, c->d_spec, conv_mode(c)); }
   if (!c->fp32) {
     SegTimer t(c, kSegConvC);
-    launch_conv(L, 2, c->sp, c->P, din(c), dout(c), c->TB, c->d_spec_off, c->d_spec, c->fp32);
+    launch_conv(L, 2, c->sp, c->P, din(c), dout(c), c->TB, c->d_spec_off, c->d_spec, conv_mode(c));
   }
   { SegTimer t(c, kSegIdl); launch_idl_resize(L, c->sp, c->d_state, dout(c), c->IL[c->il_cur ^ 1]); }
   if (c->corr_on) {
@@ -1097,6 +1103,16 @@ int gg_autotune(gg_ctx* c, int32_t reps, int32_t* choice, float* ms) {
 int gg_get_tuning(gg_ctx* c, int32_t* variants) {
   if (!c || !variants) return GG_ERR_ARG;
   for (int k = 0; k < kClasses; ++k) variants[k] = c->cmp_variant[k];
+  return GG_OK;
+}
+
+int gg_set_convolution_form(gg_ctx* c, int32_t form) {
+  if (!c) return GG_ERR_ARG;
+  if (form != GG_CONV_DEFAULT && form != GG_CONV_SPLIT_BF16) return fail(c, GG_ERR_ARG, "unknown convolution form");
+  if (form == GG_CONV_SPLIT_BF16 && !c->fp32)
+    return fail(c, GG_ERR_ARG, "the split-BF16 convolution is a form of the FP32 batched path");
+  c->conv_bf16 = form == GG_CONV_SPLIT_BF16;
+  drop_graph(c);
   return GG_OK;
 }
 

@@ -23,12 +23,14 @@
 // LDS, z forward + G + z inverse per block of 32 spectral columns, (y,x)
 // inverse per z-layer.
 //
-// FP32 batched form: the Gaussian factor is separable,
+// FP32 batched forms: the Gaussian factor is separable,
 // G(kx,ky,kz) = f(kx) f(ky) f(kz), so IDFT . diag(G) . DFT on the box equals
 // three 1-D circular convolutions with the real kernel
 // c[m] = (1/n) sum_k f(min(k,n-k)) cos(2 pi k m / n) (host long double, stored
 // FP32).  Two launches per size class: x then y per z-layer in LDS (vector
-// FMAs), then z per block of 64 columns (matrix cores).
+// FMAs), then z per block of 64 columns (matrix cores, FP32); or, as the
+// split-BF16 form, all three passes as GEMMs on the matrix cores with 3-term
+// BF16 products accumulated in FP32.
 #include "gg_internal.h"
 
 namespace gg {
@@ -542,13 +544,170 @@ __global__ __launch_bounds__(kT) void k_circ_z(PlanDev P, int cls, DescDev dout,
   }
 }
 
+// ---- split-BF16 (3-term) circulant on the matrix cores ----------------------
+// The same three circulant products as above, each as a GEMM on
+// v_mfma_f32_16x16x32_bf16: every FP32 operand v is split into hi = bf16(v)
+// (nearest even) and lo = bf16(v - hi), and a 32-deep k block adds
+// a_lo b_hi + a_hi b_lo + a_hi b_hi (smallest terms first) to an FP32
+// accumulator; the dropped a_lo b_lo term and the rounding of lo leave a
+// relative error of about 2^-16 per product.  Operand layout: lane l supplies
+// A[l % 16][k] and B[k][l % 16] for k = 32 kb + 8 (l / 16) + j, j = 0..7
+// (the same k slots in both operands, so the product does not depend on how
+// the hardware orders them), and receives D[4 (l / 16) + v][l % 16].
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+__device__ inline v4f mfma_split3(const float (&a)[8], const float (&b)[8], v4f acc) {
+  v8bf ah, al, bh, bl;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const __bf16 x = (__bf16)a[j], y = (__bf16)b[j];
+    ah[j] = x;
+    al[j] = (__bf16)(a[j] - (float)x);
+    bh[j] = y;
+    bl[j] = (__bf16)(b[j] - (float)y);
+  }
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+  return acc;
+}
+
+// x then y on a z-layer: D[x][y] = sum_k C[x][k] L[y][k], then
+// D[y][x] = sum_k C[y][k] B[k][x]; layer rows at stride n + 1 in LDS.
+template <int NMAX>
+__global__ __launch_bounds__(kT) void k_circ_xy_bf16(PlanDev P, int cls, DescDev din, DescDev dout, TablesDev tb) {
+  constexpr int RS = NMAX + 1;
+  constexpr int KBM = (NMAX + 31) / 32;
+  __shared__ float A[NMAX * RS];
+  __shared__ float B[NMAX * RS];
+  __shared__ float crr[2 * NMAX + kRB];
+  const int total = P.totals[kQLayers * kClasses + cls];
+  const float* ain = (const float*)din.arena;
+  float* aout = (float*)dout.arena;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, lr = lane & 15, lg = lane >> 4;
+  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    int g, z;
+    plan_tile(P, cls, kQLayers, tile, g, z);
+    const int n = box_n(din.bounds, g), nn = n * n, rs = n + 1;
+    const int T16 = (n + 15) >> 4, KB = (n + 31) >> 5;
+    load_circ_row(crr, tb, n, tid, kT);
+    const float* src = ain + din.off[g] + (int64_t)z * nn;
+    for (int i = tid; i < nn; i += kT) {
+      const int y = i / n;
+      A[y * rs + (i - y * n)] = src[i];
+    }
+    __syncthreads();
+    for (int t = wv; t < T16 * T16; t += kT / 64) {
+      const int xt = t / T16, yt = t - xt * T16;
+      const int xa = xt * 16 + lr, yb = yt * 16 + lr;
+      v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < KBM; ++kb) {
+        if (kb >= KB) break;
+        float a[8], b[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int k = kb * 32 + 8 * lg + j;
+          a[j] = (k < n && xa < n) ? crr[xa - k + n] : 0.f;
+          b[j] = (k < n && yb < n) ? A[yb * rs + k] : 0.f;
+        }
+        acc = mfma_split3(a, b, acc);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int x = xt * 16 + 4 * lg + v;
+        if (x < n && yb < n) B[yb * rs + x] = acc[v];
+      }
+    }
+    __syncthreads();
+    float* dst = aout + dout.off[g] + (int64_t)z * nn;
+    for (int t = wv; t < T16 * T16; t += kT / 64) {
+      const int yt = t / T16, xt = t - yt * T16;
+      const int ya = yt * 16 + lr, xb = xt * 16 + lr;
+      v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < KBM; ++kb) {
+        if (kb >= KB) break;
+        float a[8], b[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int k = kb * 32 + 8 * lg + j;
+          a[j] = (k < n && ya < n) ? crr[ya - k + n] : 0.f;
+          b[j] = (k < n && xb < n) ? B[k * rs + xb] : 0.f;
+        }
+        acc = mfma_split3(a, b, acc);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int y = yt * 16 + 4 * lg + v;
+        if (y < n && xb < n) dst[y * n + xb] = acc[v];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// z on blocks of 64 columns: D[z][p] = sum_k C[z][k] Lz[k][p].
+template <int NMAX>
+__global__ __launch_bounds__(kT) void k_circ_z_bf16(PlanDev P, int cls, DescDev dout, TablesDev tb) {
+  constexpr int CB = kColBlock;
+  constexpr int KBM = (NMAX + 31) / 32;
+  constexpr int SZ = CB + 1;
+  __shared__ float Lz[NMAX * SZ];
+  __shared__ float crr[2 * NMAX + kRB];
+  const int total = P.totals[kQCols * kClasses + cls];
+  float* aout = (float*)dout.arena;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, lr = lane & 15, lg = lane >> 4;
+  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    int g, blk;
+    plan_tile(P, cls, kQCols, tile, g, blk);
+    const int n = box_n(dout.bounds, g), nn = n * n;
+    const int c0 = blk * CB, width = min(CB, nn - c0);
+    const int T16 = (n + 15) >> 4, KB = (n + 31) >> 5;
+    load_circ_row(crr, tb, n, tid, kT);
+    float* base = aout + dout.off[g];
+    stage_rows_to_lds(Lz, SZ, base + c0, nn, n, width, tid, kT);
+    __syncthreads();
+    for (int t = wv; t < T16 * (CB / 16); t += kT / 64) {
+      const int zt = t / (CB / 16), pt = t - zt * (CB / 16);
+      const int za = zt * 16 + lr, pb = pt * 16 + lr;
+      const bool col = pb < width;
+      v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < KBM; ++kb) {
+        if (kb >= KB) break;
+        float a[8], b[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int k = kb * 32 + 8 * lg + j;
+          a[j] = (k < n && za < n) ? crr[za - k + n] : 0.f;
+          b[j] = (k < n && col) ? Lz[k * SZ + pb] : 0.f;
+        }
+        acc = mfma_split3(a, b, acc);
+      }
+      if (col) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int zz = zt * 16 + 4 * lg + v;
+          if (zz < n) base[(int64_t)zz * nn + c0 + pb] = acc[v];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // part 0: (x,y) forward / x-then-y circulant; part 1: z; part 2: (y,x) inverse (FP64 only)
 template <int NMAX>
 void conv_class(const Launch& L, int part, const PlanDev& P, int cls, const DescDev& din, const DescDev& dout,
-                const TablesDev& tb, const int64_t* spec_off, void* spec, bool fp32) {
-  if (fp32) {
-    if (part == 0) hipLaunchKernelGGL(k_circ_xy<NMAX>, dim3(tile_grid(L, kGridPerCU, cls)), dim3(kT), 0, L.stream, P, cls, din, dout, tb);
-    if (part == 1) hipLaunchKernelGGL(k_circ_z<NMAX>, dim3(tile_grid(L, kGridPerCU, cls)), dim3(kT), 0, L.stream, P, cls, dout, tb);
+                const TablesDev& tb, const int64_t* spec_off, void* spec, int mode) {
+  const int grid = tile_grid(L, kGridPerCU, cls);
+  if (mode == kConvCirculantF32) {
+    if (part == 0) hipLaunchKernelGGL(k_circ_xy<NMAX>, dim3(grid), dim3(kT), 0, L.stream, P, cls, din, dout, tb);
+    if (part == 1) hipLaunchKernelGGL(k_circ_z<NMAX>, dim3(grid), dim3(kT), 0, L.stream, P, cls, dout, tb);
+  } else if (mode == kConvSplitBF16) {
+    if (part == 0) hipLaunchKernelGGL(k_circ_xy_bf16<NMAX>, dim3(grid), dim3(kT), 0, L.stream, P, cls, din, dout, tb);
+    if (part == 1) hipLaunchKernelGGL(k_circ_z_bf16<NMAX>, dim3(grid), dim3(kT), 0, L.stream, P, cls, dout, tb);
   } else {
     if (part == 0)
       hipLaunchKernelGGL(k_fft_xy_fwd<NMAX>, dim3(tile_grid(L, kGridPerCU, cls)), dim3(kT), 0, L.stream, P, cls, din, tb, spec_off,
@@ -565,12 +724,12 @@ void conv_class(const Launch& L, int part, const PlanDev& P, int cls, const Desc
 }  // namespace
 
 void launch_conv(const Launch& L, int part, const SimParams& sp, const PlanDev& P, const DescDev& din,
-                 const DescDev& dout, const TablesDev& tb, const int64_t* spec_off, void* spec, bool fp32) {
+                 const DescDev& dout, const TablesDev& tb, const int64_t* spec_off, void* spec, int mode) {
   (void)sp;
-  conv_class<32>(L, part, P, 0, din, dout, tb, spec_off, spec, fp32);
-  conv_class<48>(L, part, P, 1, din, dout, tb, spec_off, spec, fp32);
-  conv_class<64>(L, part, P, 2, din, dout, tb, spec_off, spec, fp32);
-  conv_class<96>(L, part, P, 3, din, dout, tb, spec_off, spec, fp32);
+  conv_class<32>(L, part, P, 0, din, dout, tb, spec_off, spec, mode);
+  conv_class<48>(L, part, P, 1, din, dout, tb, spec_off, spec, mode);
+  conv_class<64>(L, part, P, 2, din, dout, tb, spec_off, spec, mode);
+  conv_class<96>(L, part, P, 3, din, dout, tb, spec_off, spec, mode);
 }
 
 }  // namespace gg

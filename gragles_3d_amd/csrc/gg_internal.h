@@ -334,8 +334,10 @@ void launch_plan(const Launch& L, const SimParams& sp, const int32_t* state, con
 void launch_init_phi(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din,
                      const int32_t* idfield, bool fp32, int k_lo, int k_hi);
 // part 0: (x,y) forward DFT | x,y circulant; part 1: z; part 2: (y,x) inverse DFT (FP64 only)
+// convolution forms: the FP64 FFT, the FP32 circulant, the split-BF16 circulant
+enum { kConvFFT64 = 0, kConvCirculantF32 = 1, kConvSplitBF16 = 2 };
 void launch_conv(const Launch& L, int part, const SimParams& sp, const PlanDev& P, const DescDev& din,
-                 const DescDev& dout, const TablesDev& tb, const int64_t* spec_off, void* spec, bool fp32);
+                 const DescDev& dout, const TablesDev& tb, const int64_t* spec_off, void* spec, int mode);
 void launch_idl_resize(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& dout,
                        const IdlDev& il);
 void launch_init_lists(const Launch& L, const SimParams& sp, const int32_t* state, const DescDev& d,

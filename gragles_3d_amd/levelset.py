@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libgg_levelset.so")
 GG_OK, GG_ERR_ARG, GG_ERR_HIP, GG_ERR_CAPACITY, GG_ERR_STATE, GG_ERR_DEVICE = range(6)
 FP64_PLAIN, FP32_BATCHED = 0, 1
 ORDER_SPATIAL, ORDER_VOLUME = 0, 1
+CONV_DEFAULT, CONV_SPLIT_BF16 = 0, 1
 DELETED, ALIVE, VANISHED = 0, 1, 2
 # comparison tile shapes of gg_autotune: (layers per wavefront, gathers in flight, run-major order)
 CMP_SHAPES = ((8, 8, 0), (16, 8, 0), (16, 16, 0), (32, 8, 0), (8, 8, 1), (16, 8, 1), (16, 16, 1), (32, 8, 1))
@@ -86,6 +87,7 @@ SIGNATURES = {
     "gg_get_tuning": (ctypes.c_int, [_vp, _i32p]),
     "gg_set_tuning": (ctypes.c_int, [_vp, _i32p]),
     "gg_set_work_order": (ctypes.c_int, [_vp, _i32]),
+    "gg_set_convolution_form": (ctypes.c_int, [_vp, _i32]),
     "gg_get_work_order": (ctypes.c_int, [_vp, _i32p]),
     "gg_get_count_history": (ctypes.c_int, [_vp, _i32p, _i32, _i32p]),
     "gg_get_grain_stats": (ctypes.c_int, [_vp, _i32p, _f64p, _f64p]),
@@ -527,6 +529,11 @@ class LevelSetContext:
     def set_tuning(self, variants):
         v = np.ascontiguousarray(variants, dtype=np.int32)
         self._chk(lib().gg_set_tuning(self.h, _p(v, ctypes.c_int32)))
+
+    def set_convolution_form(self, form):
+        """CONV_DEFAULT or CONV_SPLIT_BF16 (FP32 batched path only): the circulant
+        convolution as split-BF16 (3-term) matrix products on the matrix cores."""
+        self._chk(lib().gg_set_convolution_form(self.h, int(form)))
 
     def set_work_order(self, order):
         """ORDER_SPATIAL (Morton order of the initial boxes) or ORDER_VOLUME (a

@@ -256,6 +256,17 @@ int gg_get_tuning(gg_ctx* ctx, int32_t* variants);
  * plan, grains sorted by box side, largest first (equal sides in the spatial
  * order), so the largest boxes start first and the tail of a launch is made
  * of small tiles.  Results do not depend on the order. */
+/* Form of the FP32 batched path's heat-kernel convolution (executeConvolution,
+ * src/box.cpp:414-539, the MKL/FFTW transforms of :685-795).
+ * GG_CONV_DEFAULT: the separable circulant in FP32 (x, y on the vector units,
+ * z on the matrix cores; the FP64 path always uses its FFT).
+ * GG_CONV_SPLIT_BF16: the three circulant passes as dense matrix products on
+ * the matrix cores with every FP32 operand split into two BF16 parts and three
+ * of the four partial products (all but lo*lo) accumulated in FP32: a relative
+ * error of about 2^-16 per product.  GG_ERR_ARG on the FP64 path. */
+#define GG_CONV_DEFAULT 0
+#define GG_CONV_SPLIT_BF16 1
+int gg_set_convolution_form(gg_ctx* ctx, int32_t form);
 #define GG_ORDER_SPATIAL 0
 #define GG_ORDER_VOLUME 1
 int gg_set_work_order(gg_ctx* ctx, int32_t order);
