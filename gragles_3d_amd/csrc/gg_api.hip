@@ -1328,16 +1328,18 @@ int gg_set_physics(gg_ctx* c, const gg_physics* ph, const double* quat, const do
   if (!d.ready) {
     const size_t ne = n1 * kElemCap;
     const int64_t cap = c->arena_cap / 8 + 65536;  // triangles of one hull pass (about 3 per surface voxel)
+    const int64_t tcap = (cap + cap / 4) / kChunk * kChunk;  // the same, grouped, with each element's padding
     if ((rc = dalloc(c, &c->topo.jord, n1 * kJuncCap)) || (rc = dalloc(c, &c->topo.raw, (size_t)cap)) ||
         (rc = dalloc(c, &c->topo.raw_n, 1)) || (rc = dalloc(c, &d.ready, 1)) || (rc = dalloc(c, &d.ecount, n1)) ||
         (rc = dalloc(c, &d.eclass, 4 * n1)) || (rc = dalloc(c, &d.epos, 3 * ne)) || (rc = dalloc(c, &d.emask, ne)) ||
         (rc = dalloc(c, &d.energy, ne)) || (rc = dalloc(c, &d.mobility, ne)) ||
         (rc = dalloc(c, &d.efirst, n1 * (kElemCap + 1))) || (rc = dalloc(c, &d.ecur, ne)) ||
         (rc = dalloc(c, &d.slot_elem, n1 * kJuncCap)) || (rc = dalloc(c, &d.tbase, n1)) ||
-        (rc = dalloc(c, &d.tri, 3 * (size_t)cap)) || (rc = dalloc(c, &d.tri_n, 1)) || (rc = dalloc(c, (double**)&d.see, n1)))
+        (rc = dalloc(c, &d.tri, 3 * (size_t)tcap)) || (rc = dalloc(c, &d.cbox, 6 * (size_t)(tcap / kChunk))) ||
+        (rc = dalloc(c, &d.tri_n, 1)) || (rc = dalloc(c, (double**)&d.see, n1)))
       return rc;
     c->topo.raw_cap = cap;
-    d.tri_cap = cap;
+    d.tri_cap = tcap;
     HIPCHK(c, hipMemset(d.ecount, 0, 4 * n1));
     HIPCHK(c, hipMemset(d.slot_elem, 0xFF, 4 * n1 * kJuncCap));
   }
@@ -1381,12 +1383,12 @@ int gg_download_elements(gg_ctx* c, int32_t id, int32_t* cls, int32_t* nnb, int3
     return fail(c, GG_ERR_ARG, "destination too small");
   if (E == 0) return GG_OK;
   const int64_t eb = (int64_t)id * kElemCap;
-  std::vector<int32_t> key(kTopoCap), ef(E + 1);
+  std::vector<int32_t> key(kTopoCap), cnt(E);
   std::vector<unsigned long long> em(E);
   HIPCHK(c, hipMemcpy(ec, c->cd.eclass + 4 * id, 12, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(key.data(), c->topo.key + (int64_t)id * kTopoCap, 4 * kTopoCap, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(em.data(), c->cd.emask + eb, 8 * E, hipMemcpyDeviceToHost));
-  HIPCHK(c, hipMemcpy(ef.data(), c->cd.efirst + (int64_t)id * (kElemCap + 1), 4 * (E + 1), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(cnt.data(), c->cd.ecur + eb, 4 * E, hipMemcpyDeviceToHost));  // barycenters scattered
   HIPCHK(c, hipMemcpy(pos, c->cd.epos + 3 * eb, 24 * E, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(energy, c->cd.energy + eb, 8 * E, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(mobility, c->cd.mobility + eb, 8 * E, hipMemcpyDeviceToHost));
@@ -1398,7 +1400,7 @@ int gg_download_elements(gg_ctx* c, int32_t id, int32_t* cls, int32_t* nnb, int3
     std::sort(ids.begin(), ids.end());
     nnb[e] = (int32_t)ids.size();
     for (int k = 0; k < 8; ++k) nb8[8 * e + k] = k < (int)ids.size() ? (int32_t)ids[k] : -1;
-    nbary[e] = ef[e + 1] - ef[e];
+    nbary[e] = cnt[e];
   }
   return GG_OK;
 }

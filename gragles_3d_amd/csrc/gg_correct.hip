@@ -50,7 +50,7 @@ constexpr int kEW = 256;                           // threads of an element-buil
 constexpr int kRec = kJuncCap + kJuncCap / 2;      // lists + high-order junctions made by merging
 constexpr int kTopoCapBitsC = 6;
 static_assert((1 << kTopoCapBitsC) == kTopoCap, "grain topology table size");
-constexpr int kCorrGridPerCU = 16;                 // workgroups per CU of a correction launch
+constexpr int kCorrGridPerCU = 128;                // workgroups per CU of a correction launch (about one per tile)
 constexpr int kCellBits = 14;                      // a layer cell index (n^2 <= 9216) in a queue entry
 static_assert(kMaxN * kMaxN < (1 << kCellBits), "queue entry layout");
 
@@ -97,6 +97,8 @@ __global__ __launch_bounds__(kEW) void k_elements(SimParams sp, const int32_t* _
   __shared__ short s_elem[kRec];  // element index of a record that is an element
   __shared__ short s_hj[kRec], s_qj[kJuncCap];
   __shared__ short s_ord[kJuncCap];  // element -> record
+  __shared__ int s_efp[kJuncCap + 1];  // first barycenter of each element (whole chunks)
+  __shared__ int64_t s_tb;
   const int N = sp.N;
   const int tid = threadIdx.x;
   if (blockIdx.x == 0 && tid == 0) *cd.ready = 1;
@@ -263,12 +265,15 @@ __global__ __launch_bounds__(kEW) void k_elements(SimParams sp, const int32_t* _
       for (int e = 0; e < E; ++e) {
         s_elem[s_ord[e]] = (short)e;
         ef[e] = (int32_t)acc;
-        acc += s_cnt[s_ord[e]];
+        s_efp[e] = (int32_t)acc;
+        acc += (s_cnt[s_ord[e]] + kChunk - 1) / kChunk * kChunk;  // whole chunks
       }
       ef[E] = (int32_t)acc;
+      s_efp[E] = (int32_t)acc;
       const unsigned long long tb = atomicAdd(cd.tri_n, (unsigned long long)acc);
       if (tb + (unsigned long long)acc > (unsigned long long)cd.tri_cap) atomicOr(err, kErrTri);
       cd.tbase[g] = (int64_t)tb;
+      s_tb = (int64_t)tb;
       cd.ecount[g] = E;
       cd.eclass[4 * g] = nh;
       cd.eclass[4 * g + 1] = nq;
@@ -295,6 +300,15 @@ __global__ __launch_bounds__(kEW) void k_elements(SimParams sp, const int32_t* _
       while (s_red[rec] >= 0) rec = s_red[rec];
       cd.slot_elem[jb + s_ls[s_sorted[r]]] = s_elem[rec];
     }
+    // the padding points that complete each element's last chunk
+    for (int e = tid; e < E; e += kEW) {
+      const int64_t a = s_tb + s_efp[e] + s_cnt[s_ord[e]], b = s_tb + s_efp[e + 1];
+      for (int64_t q = a; q < b && q < cd.tri_cap; ++q) {
+        cd.tri[3 * q] = kFarPoint;
+        cd.tri[3 * q + 1] = kFarPoint;
+        cd.tri[3 * q + 2] = kFarPoint;
+      }
+    }
     __syncthreads();  // the LDS tables are refilled for the next grain
   }
 }
@@ -306,19 +320,77 @@ __global__ void k_clear_counter(unsigned long long* c) {
 }
 
 // Every triangle record of the last hull into its element's barycenter range.
+// The records of one hull tile are consecutive in the pool and in cell order;
+// a wavefront reserves one run per (grain, element) among its 64 records, so
+// an element's range is made of such runs in the hull's order and consecutive
+// barycenters lie close together (which keeps the chunk boxes small).
 __global__ __launch_bounds__(256) void k_tri_scatter(TopoDev topo, CorrDev cd) {
   unsigned long long n = *topo.raw_n;
   if (n > (unsigned long long)topo.raw_cap) n = (unsigned long long)topo.raw_cap;
-  for (unsigned long long r = blockIdx.x * 256ull + threadIdx.x; r < n; r += (unsigned long long)gridDim.x * 256ull) {
-    const TriRec t = topo.raw[r];
-    const int e = cd.slot_elem[(int64_t)t.g * kJuncCap + t.slot];
-    if ((unsigned)e >= (unsigned)cd.ecount[t.g]) continue;  // only after a table overflow (device error set)
-    const int64_t pos = cd.tbase[t.g] + cd.efirst[(int64_t)t.g * (kElemCap + 1) + e] +
-                        atomicAdd(&cd.ecur[(int64_t)t.g * kElemCap + e], 1);
-    if (pos < cd.tri_cap) {
-      cd.tri[3 * pos] = t.x;
-      cd.tri[3 * pos + 1] = t.y;
-      cd.tri[3 * pos + 2] = t.z;
+  const int lane = threadIdx.x & 63;
+  for (unsigned long long base = blockIdx.x * 256ull; base < n; base += (unsigned long long)gridDim.x * 256ull) {
+    const unsigned long long r = base + threadIdx.x;
+    bool valid = r < n;
+    TriRec t{};
+    int e = -1;
+    unsigned key = 0u;
+    if (valid) {
+      t = topo.raw[r];
+      e = cd.slot_elem[(int64_t)t.g * kJuncCap + t.slot];
+      valid = (unsigned)e < (unsigned)cd.ecount[t.g];  // false only after a table overflow (device error set)
+      key = (unsigned)t.g * (unsigned)kElemCap + (unsigned)e;
+    }
+    int64_t pos = 0;
+    bool done = !valid;
+    for (;;) {  // one round per distinct (grain, element) among the wavefront's records
+      const unsigned long long pend = __ballot(!done);
+      if (pend == 0ull) break;
+      const int leader = __ffsll((long long)pend) - 1;
+      const unsigned k0 = __shfl(key, leader);
+      const bool mine = !done && key == k0;
+      const unsigned long long m = __ballot(mine);
+      int b0 = 0;
+      if (lane == leader) b0 = atomicAdd(&cd.ecur[k0], __popcll(m));
+      b0 = __shfl(b0, leader);
+      if (mine) {
+        pos = b0 + __popcll(m & ((1ull << lane) - 1ull));
+        done = true;
+      }
+    }
+    if (valid) {
+      const int64_t q = cd.tbase[t.g] + cd.efirst[(int64_t)t.g * (kElemCap + 1) + e] + pos;
+      if (q < cd.tri_cap) {
+        cd.tri[3 * q] = t.x;
+        cd.tri[3 * q + 1] = t.y;
+        cd.tri[3 * q + 2] = t.z;
+      }
+    }
+  }
+}
+
+// Bounding box of every chunk of kChunk barycenters (padding points left
+// out), widened by 1e-3 voxel so the FP32 box holds its FP64 points.
+__global__ __launch_bounds__(256) void k_chunk_boxes(CorrDev cd) {
+  unsigned long long nt = *cd.tri_n;
+  if (nt > (unsigned long long)cd.tri_cap) nt = (unsigned long long)cd.tri_cap;
+  const int64_t nc = (int64_t)(nt / kChunk);
+  for (int64_t c = blockIdx.x * 256ll + threadIdx.x; c < nc; c += (int64_t)gridDim.x * 256ll) {
+    float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
+    const double* q = cd.tri + 3 * c * kChunk;
+    for (int j = 0; j < kChunk; ++j) {
+      if (q[3 * j] >= 0.5 * kFarPoint) continue;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float v = (float)q[3 * j + a];
+        lo[a] = fminf(lo[a], v - 1e-3f);
+        hi[a] = fmaxf(hi[a], v + 1e-3f);
+      }
+    }
+    float* bx = cd.cbox + 6 * c;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      bx[a] = lo[a];
+      bx[3 + a] = hi[a];
     }
   }
 }
@@ -431,41 +503,66 @@ __global__ __launch_bounds__(256) void k_correct(SimParams sp, PlanDev P, int cl
       const int yl = c / n, xl = c - yl * n;
       // Vector3d(i, j, k): row, column, depth, against barycenters (x, y, z)
       const double px = (double)(y0 + yl), py = (double)(x0 + xl), pz = (double)(z0 + kl);
-      double minimal = 10000000.0, w_energy = 1., w_mobility = 1.;
+      // the first nearest barycenter in the projection order is the smallest
+      // (squared distance, element) pair, found here in any order: elements
+      // from boundaries back to junctions, skipping every chunk whose box no
+      // lane can reach below its bound or its current pair
+      double dmin = 10000000.0;
+      int emin = -1;
       const unsigned long long bit = (act && slot >= 0) ? 1ull << slot : 0ull;
-      for (int e = 0; e < E; ++e) {
+      const float* __restrict__ cbx = cd.cbox + 6 * (tb / kChunk);
+      // first pass: bound = the smallest farthest-corner distance to a box of
+      // a chunk the lane needs (the nearest barycenter is no farther); a chunk
+      // whose box lies beyond it cannot hold the answer.  Box distances are
+      // FP32 and only decide which chunks to skip, so each is taken with a
+      // margin that covers its rounding (about 3 ulp): the skip is
+      // conservative, and every barycenter that can win is compared in FP64.
+      const float fpx = (float)px, fpy = (float)py, fpz = (float)pz;  // integers: exact
+      float bound = 3e38f;
+      for (int e = E - 1; e >= 0; --e) {
         const bool need = (s_em[e] & bit) != 0ull;
         if (__ballot(need) == 0ull) continue;
-        const int t1 = __builtin_amdgcn_readfirstlane(s_ef[e + 1]);
-        int t = __builtin_amdgcn_readfirstlane(s_ef[e]);
-        const double en = s_en[e], mo = s_mo[e];
-        // four barycenters per round: their loads issue together
-        for (; t + 4 <= t1; t += 4) {
-          const double* q = tri + 3 * t;
-          double d[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const double dx = px - q[3 * u], dy = py - q[3 * u + 1], dz = pz - q[3 * u + 2];
-            d[u] = dx * dx + dy * dy + dz * dz;
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-            if (need && d[u] < minimal) {
-              minimal = d[u];
-              w_energy = mo;
-              w_mobility = en;
-            }
+        const int ch1 = __builtin_amdgcn_readfirstlane(s_ef[e + 1]) / kChunk;
+        for (int ch = __builtin_amdgcn_readfirstlane(s_ef[e]) / kChunk; ch < ch1; ++ch) {
+          const float* bx = cbx + 6 * ch;
+          const float fx = fmaxf(fabsf(fpx - bx[0]), fabsf(fpx - bx[3]));
+          const float fy = fmaxf(fabsf(fpy - bx[1]), fabsf(fpy - bx[4]));
+          const float fz = fmaxf(fabsf(fpz - bx[2]), fabsf(fpz - bx[5]));
+          const float ub = (fx * fx + fy * fy + fz * fz) * (1.0f + 1e-6f) + 1e-6f;
+          if (need && bx[0] <= bx[3] && ub < bound) bound = ub;  // (a box with no barycenter is empty)
         }
-        for (; t < t1; ++t) {
-          const double dx = px - tri[3 * t], dy = py - tri[3 * t + 1], dz = pz - tri[3 * t + 2];
-          const double d = dx * dx + dy * dy + dz * dz;
-          if (need && d < minimal) {
-            minimal = d;
-            w_energy = mo;
-            w_mobility = en;
+      }
+      for (int e = E - 1; e >= 0; --e) {
+        const bool need = (s_em[e] & bit) != 0ull;
+        if (__ballot(need) == 0ull) continue;
+        const int ch1 = __builtin_amdgcn_readfirstlane(s_ef[e + 1]) / kChunk;
+        for (int ch = __builtin_amdgcn_readfirstlane(s_ef[e]) / kChunk; ch < ch1; ++ch) {
+          const float* bx = cbx + 6 * ch;
+          const float ex = fmaxf(fmaxf(bx[0] - fpx, fpx - bx[3]), 0.0f);
+          const float ey = fmaxf(fmaxf(bx[1] - fpy, fpy - bx[4]), 0.0f);
+          const float ez = fmaxf(fmaxf(bx[2] - fpz, fpz - bx[5]), 0.0f);
+          const float lb = (ex * ex + ey * ey + ez * ez) * (1.0f - 1e-6f) - 1e-6f;  // below the true one
+          const bool want = need && lb <= bound && (double)lb <= dmin;
+          if (__ballot(want) == 0ull) continue;
+          const double* q = tri + 3 * kChunk * ch;
+#pragma unroll
+          for (int u0 = 0; u0 < kChunk; u0 += 4) {
+            double d[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const double dx = px - q[3 * (u0 + u)], dy = py - q[3 * (u0 + u) + 1], dz = pz - q[3 * (u0 + u) + 2];
+              d[u] = dx * dx + dy * dy + dz * dz;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (want && (d[u] < dmin || (d[u] == dmin && e < emin))) {
+                dmin = d[u];
+                emin = e;
+              }
           }
         }
       }
+      const double w_energy = emin >= 0 ? s_mo[emin] : 1.0, w_mobility = emin >= 0 ? s_en[emin] : 1.0;
       if (act) {
         const int64_t li = lbase + c;
         const double val = (double)in[li];
@@ -506,6 +603,7 @@ void launch_elements(const Launch& L, const SimParams& sp, const int32_t* state,
   const int blocks = sp.N < L.grid ? sp.N : L.grid;
   hipLaunchKernelGGL(k_elements, dim3(blocks > 0 ? blocks : 1), dim3(kEW), 0, L.stream, sp, state, topo, cd, err);
   hipLaunchKernelGGL(k_tri_scatter, dim3(L.grid), dim3(256), 0, L.stream, topo, cd);
+  hipLaunchKernelGGL(k_chunk_boxes, dim3(L.grid), dim3(256), 0, L.stream, cd);
 }
 
 void launch_correction(const Launch& L, const SimParams& sp, const PlanDev& P, const int32_t* state,

@@ -89,6 +89,8 @@ struct TopoDev {
 // junctions, triple lines, boundaries.  An element's neighbour set is a mask
 // over the grain's hull-id table slots (TopoDev::key).
 constexpr int kElemCap = kJuncCap;
+constexpr int kChunk = 16;         // barycenters per bounding box of the nearest-element search
+constexpr double kFarPoint = 1e30;  // coordinate of a chunk's padding points
 struct CorrDev {
   int32_t* ready;              // set by the element build: a hull with elements exists (run_sim's loop != 0)
   int see_on;                  // Settings::UseStoredElasticEnergy
@@ -109,7 +111,9 @@ struct CorrDev {
   int32_t* ecur;               // [(N+1) * kElemCap] fill cursors
   int32_t* slot_elem;          // [(N+1) * kJuncCap] element of each junction slot
   int64_t* tbase;              // [N+1] the grain's first barycenter in tri
-  double* tri;                 // [3 * tri_cap] barycenters grouped by element
+  double* tri;                 // [3 * tri_cap] barycenters grouped by element, each element padded
+                               // to whole chunks of kChunk with far-away points
+  float* cbox;                 // [6 * (tri_cap / kChunk)] bounding box of each chunk (lo xyz, hi xyz)
   unsigned long long* tri_n;   // barycenters placed
   int64_t tri_cap;
 };
