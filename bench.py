@@ -293,6 +293,25 @@ def summarize_rocprof(d):
             per.setdefault((name, cn), []).append(val)
     if not per:
         return
+    sq = sorted({cn for _, cn in per if cn.startswith("SQ_")})
+    if sq:
+        # SQ counters: mean per dispatch; LDS bank-conflict cycles as a share of the cycles LDS
+        # instructions were active (both counted by the SQ in the same units)
+        print("| kernel | dispatches | " + " | ".join(c[3:] for c in sq) +
+              (" | LDS bank-conflict share |" if {"SQ_LDS_BANK_CONFLICT", "SQ_ACTIVE_INST_LDS"} <= set(sq) else " |"))
+        print("|---|---|" + "---|" * len(sq) + ("---|" if "SQ_LDS_BANK_CONFLICT" in sq else ""))
+        names = sorted({n for n, cn in per if cn in sq}, key=lambda n: -np.mean(per.get((n, sq[0]), [0])))
+        for n in names:
+            vals = [np.mean(per.get((n, c), [0.0])) for c in sq]
+            row = "| %s | %d | " % (n, len(per.get((n, sq[0]), []))) + " | ".join("%.3g" % v for v in vals)
+            if "SQ_LDS_BANK_CONFLICT" in sq and "SQ_ACTIVE_INST_LDS" in sq:
+                a = np.mean(per.get((n, "SQ_ACTIVE_INST_LDS"), [0.0]))
+                b = np.mean(per.get((n, "SQ_LDS_BANK_CONFLICT"), [0.0]))
+                row += " | %s" % ("%.3f" % (b / a) if a > 0 else "-")
+            print(row + " |")
+        print()
+        if not any(cn in ("FETCH_SIZE", "WRITE_SIZE") for _, cn in per):
+            return
     cal = {}
     for (name, cn), vals in per.items():
         if "copyBuffer" in name:
@@ -561,8 +580,8 @@ def main():
                                                                          args.cpu_sample_stride)
             out["cpu_baseline"] = {
                 "value": v_cpu, "unit": "voxel-updates/s", "cores": threads, "kind": "port",
-                "sample": "FP64 CPU oracle (g++ -O3 -fopenmp -ffp-contract=off; explicit DFT in place of the "
-                          "reference's MKL/FFTW, hull topology every step as the reference), %d whole timesteps "
+                "sample": "FP64 CPU oracle (g++ -O3 -fopenmp -ffp-contract=off; its own mixed-radix FFT in place "
+                          "of the reference's MKL/FFTW, hull topology every step as the reference), %d whole timesteps "
                           "of the same %d-grain workload from its initial state (%.3e box voxels) in %.1f s with "
                           "%d threads" % (args.cpu_steps, args.grains, vox_full, t_full, threads),
                 "cpu_model": cpu["model"], "physical_cores": cpu["physical_cores"],
@@ -595,12 +614,13 @@ def physics_leg(kind, grains, ppg, border, seed, steps, warmup, tuning, accuracy
     info = {"grains": grains, "points_per_grain": ppg, "border": border, "seed": seed,
             "ngridpoints": p.ngridpoints}
     if kind == "anisotropic":
-        q = G.random_orientations(grains, seed)
+        q = G.textured_orientations(grains, seed)
         phys = dict(correction=True, mobility_model=True, unique_energies=False, stored_energy=False,
                     triple_line_drag=1e6, quat=q)
         th = np.array([G.misorientation(q[a], q[b]) for a in range(1, grains + 1)
                        for b in ids[offs[a]:offs[a + 1]] if b > a])
-        info["orientations"] = "uniform random unit quaternions (Shoemake), seed %d" % seed
+        info["orientations"] = ("uniform random unit quaternions (Shoemake), 45%% of them replaced by a texture "
+                                "component scattered by up to 7.5 degrees (seed %d)" % seed)
         info["low_angle_fraction_initial_neighbour_pairs"] = float((th < 15 * np.pi / 180).mean())
     else:
         see = G.lognormal_energies(grains, seed, 0.1, 1.0)
@@ -609,11 +629,13 @@ def physics_leg(kind, grains, ppg, border, seed, steps, warmup, tuning, accuracy
         info["stored_energy"] = "lognormal, median 0.1, sigma 1.0 (normalised units), seed %d" % seed
     info["setup_s"] = time.perf_counter() - t
 
-    def context(precision):
+    def context(precision, form=G.CONV_DEFAULT):
         c = new_context(precision, inputs, p)
         if tuning is not None:
             c.set_tuning(tuning)
         c.set_physics(**phys)
+        if form != G.CONV_DEFAULT:
+            c.set_convolution_form(form)
         return c
 
     ctx = context(G.FP32_BATCHED)
@@ -630,18 +652,48 @@ def physics_leg(kind, grains, ppg, border, seed, steps, warmup, tuning, accuracy
                  "ms_per_step": 1e3 * wall / max(steps, 1), "box_voxels_per_step": vox, "steps": steps,
                  "segment_ms_per_step": {k: round(v[0] / 2, 4) for k, v in seg.items() if v[1]}})
     ctx.close()
+    if kind == "stored_energy":
+        # the split-BF16 (3-term) convolution on the matrix cores, timed the same way
+        cb = context(G.FP32_BATCHED, G.CONV_SPLIT_BF16)
+        cb.timestep(1, use_graph=False)
+        cb.synchronize()
+        cb.enable_stage_timing(True)
+        cb.stage_times(reset=True)
+        cb.timestep(2, use_graph=False)
+        cb.synchronize()
+        segb = cb.stage_times(reset=True)
+        cb.enable_stage_timing(False)
+        wall_b, vox_b = timed_steps(cb, steps, warmup)
+        info["split_bf16_convolution"] = {
+            "value": vox_b * steps / wall_b if steps else 0.0, "ms_per_step": 1e3 * wall_b / max(steps, 1),
+            "segment_ms_per_step": {k: round(v[0] / 2, 4) for k, v in segb.items() if v[1]}}
+        cb.close()
     if accuracy_steps:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from fp32_error import phi_volume_errors, trajectory_error
         a32, a64 = context(G.FP32_BATCHED), context(G.FP64_PLAIN)
         a32.timestep(accuracy_steps)
         a64.timestep(accuracy_steps)
-        e = phi_volume_errors(a32, a64, p.delta, grains=np.arange(1, grains + 1, 16))
+        sample = np.arange(1, grains + 1, 16)
+        e = phi_volume_errors(a32, a64, p.delta, grains=sample)
         e.update(trajectory_error(a32.count_history(), a64.count_history()))
         e["steps"] = accuracy_steps
         e["reference"] = "FP64 GPU path with the correction (bit-identical to the FP64 oracle)"
         e["grains_sampled_for_phi"] = "every 16th grain id"
         info["fp32_accuracy_vs_fp64"] = e
+        if kind == "stored_energy":
+            ab = context(G.FP32_BATCHED, G.CONV_SPLIT_BF16)
+            ab.timestep(accuracy_steps)
+            eb = phi_volume_errors(ab, a64, p.delta, grains=sample)
+            eb.update(trajectory_error(ab.count_history(), a64.count_history()))
+            eb["steps"] = accuracy_steps
+            eb["reference"] = e["reference"]
+            info["split_bf16_accuracy_vs_fp64"] = eb
+            ef = phi_volume_errors(ab, a32, p.delta, grains=sample)
+            ef.update(trajectory_error(ab.count_history(), a32.count_history()))
+            ef["reference"] = "FP32 batched path with the FP32 circulant convolution"
+            info["split_bf16_accuracy_vs_fp32_circulant"] = ef
+            ab.close()
         a32.close()
         a64.close()
     return info

@@ -7,5 +7,5 @@ from .levelset import (ALIVE, CONV_DEFAULT, CONV_SPLIT_BF16, DELETED, FP32_BATCH
                        GGError, GGParams, GGPhysics, LevelSetContext, ORDER_SPATIAL, ORDER_VOLUME, VANISHED,
                        bimodal_seeds, coarsening_plan, derive_params, euler_to_quaternion, gb_pair_tables,
                        generate_power, generate_voronoi, initial_neighbour_lists, lib, lognormal_energies,
-                       misorientation, random_orientations, resize_to_cube, setup_simulation, uds_text,
-                       voronoi_seeds)
+                       misorientation, random_orientations, resize_to_cube, setup_simulation, textured_orientations,
+                       uds_text, voronoi_seeds)

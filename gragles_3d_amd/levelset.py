@@ -296,6 +296,31 @@ def random_orientations(n_grains, seed, reference_abs=False):
     return q
 
 
+def textured_orientations(n_grains, seed, fraction=0.45, scatter_deg=7.5):
+    """(n_grains+1, 4) unit quaternions: random_orientations(n_grains, seed), then a
+    fraction of the grains (drawn with a generator seeded by `seed`) replaced by
+    rotations of one texture component (the first grain's orientation) by an
+    angle uniform in [0, scatter_deg] about a uniformly random axis, as the
+    reference's UseTexture scatters orientations about a texture component
+    (src/grainhdl.cpp:93-96).  Two such grains are at most 2 * scatter_deg
+    apart, so with 7.5 degrees their boundary is low-angle (below 15 degrees):
+    about fraction^2 of the boundaries."""
+    q = random_orientations(n_grains, seed)
+    rng = np.random.default_rng(seed)
+    ref = q[1].copy()
+    pick = rng.random(n_grains) < fraction
+    for g in np.nonzero(pick)[0] + 1:
+        axis = rng.normal(size=3)
+        axis /= np.linalg.norm(axis)
+        ang = np.deg2rad(scatter_deg) * rng.random()
+        d = np.concatenate([[np.cos(ang / 2)], np.sin(ang / 2) * axis])
+        w1, x1, y1, z1 = ref
+        w2, x2, y2, z2 = d
+        q[g] = [w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2]
+    return q
+
+
 def lognormal_energies(n_grains, seed, median=0.1, sigma=1.0):
     """n_grains+1 stored energies (entry 0, the wall, is 0)."""
     v = np.zeros(n_grains + 1)
