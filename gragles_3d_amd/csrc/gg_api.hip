@@ -1276,13 +1276,15 @@ static int alloc_topology(gg_ctx* c) {
   int rc;
   const size_t nj = (size_t)(c->p.n_grains + 1) * kJuncCap;
   if ((rc = dalloc(c, &c->topo.key, n)) || (rc = dalloc(c, &c->topo.ord, n)) || (rc = dalloc(c, &c->topo.jkey, nj)) ||
-      (rc = dalloc(c, &c->topo.jpos, 3 * nj)) || (rc = dalloc(c, &c->topo.jcnt, nj)))
+      (rc = dalloc(c, &c->topo.jpos, 3 * nj)) || (rc = dalloc(c, &c->topo.jcnt, nj)) ||
+      (rc = dalloc(c, &c->topo.jarea, nj)))
     return rc;
   // empty tables: no grain touched yet
   HIPCHK(c, hipMemset(c->topo.key, 0xFF, n * sizeof(int32_t)));
   HIPCHK(c, hipMemset(c->topo.jkey, 0, nj * sizeof(unsigned long long)));
   HIPCHK(c, hipMemset(c->topo.jcnt, 0, nj * sizeof(int32_t)));
   HIPCHK(c, hipMemset(c->topo.jpos, 0, 3 * nj * sizeof(double)));
+  HIPCHK(c, hipMemset(c->topo.jarea, 0, nj * sizeof(double)));
   return GG_OK;
 }
 
@@ -1457,6 +1459,70 @@ int gg_get_hull_junction_counts(gg_ctx* c, int32_t* counts4) {
     }
   }
   return GG_OK;
+}
+
+int gg_get_element_counts(gg_ctx* c, int32_t* counts4) {
+  if (!c || !counts4) return GG_ERR_ARG;
+  if (!c->cd.ready) return fail(c, GG_ERR_STATE, "the convolution correction was never enabled");
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  const size_t n1 = (size_t)c->p.n_grains + 1;
+  std::vector<int32_t> e(n1), cls(4 * n1);
+  HIPCHK(c, hipMemcpy(e.data(), c->cd.ecount, 4 * n1, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(cls.data(), c->cd.eclass, 16 * n1, hipMemcpyDeviceToHost));
+  for (size_t g = 0; g < n1; ++g) {
+    const int32_t E = e[g];
+    const int32_t nh = E ? cls[4 * g] : 0, nq = E ? cls[4 * g + 1] : 0, nt = E ? cls[4 * g + 2] : 0;
+    counts4[4 * g] = E - nh - nq - nt;  // grain boundaries
+    counts4[4 * g + 1] = nt;
+    counts4[4 * g + 2] = nq;
+    counts4[4 * g + 3] = nh;
+  }
+  return GG_OK;
+}
+
+int gg_get_faces(gg_ctx* c, int32_t* grain_a, int32_t* grain_b, double* area, int64_t cap, int64_t* n) {
+  if (!c || !n) return GG_ERR_ARG;
+  if (!c->topo.key) return fail(c, GG_ERR_STATE, "hull topology was never enabled");
+  int rc = gg_synchronize(c);
+  if (rc) return rc;
+  const int N = c->p.n_grains;
+  const size_t nk = (size_t)(N + 1) * kTopoCap, nj = (size_t)(N + 1) * kJuncCap;
+  std::vector<int32_t> key(nk), st(N + 1);
+  std::vector<unsigned long long> jk(nj);
+  std::vector<double> ja(nj);
+  HIPCHK(c, hipMemcpy(key.data(), c->topo.key, nk * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(jk.data(), c->topo.jkey, nj * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(ja.data(), c->topo.jarea, nj * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(st.data(), c->d_state, (N + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+  const double h2 = c->p.h * c->p.h;
+  int64_t cnt = 0;
+  std::vector<std::pair<int32_t, double>> mine;
+  for (int g = 1; g <= N; ++g) {
+    if (st[g] != kAlive) continue;
+    const int32_t* kg = &key[(size_t)g * kTopoCap];
+    mine.clear();
+    for (int i = 0; i < kJuncCap; ++i) {
+      const unsigned long long m = jk[(size_t)g * kJuncCap + i];
+      if (__builtin_popcountll(m) != 2) continue;  // a grain boundary: the grain and one neighbour
+      const int b0 = __builtin_ctzll(m), b1 = 63 - __builtin_clzll(m);
+      const int32_t a0 = kg[b0], a1 = kg[b1];
+      const int32_t nb = a0 == g ? a1 : a0;
+      if ((a0 != g && a1 != g) || g <= nb) continue;  // get_Faces: only owner id > neighbour id
+      mine.push_back({nb, ja[(size_t)g * kJuncCap + i] * h2});
+    }
+    std::sort(mine.begin(), mine.end());
+    for (const auto& f : mine) {
+      if (grain_a && cnt < cap) {
+        grain_a[cnt] = g;
+        grain_b[cnt] = f.first;
+        area[cnt] = f.second;
+      }
+      ++cnt;
+    }
+  }
+  *n = cnt;
+  return grain_a && cnt > cap ? fail(c, GG_ERR_CAPACITY, "faces: cap too small") : GG_OK;
 }
 
 int gg_download_hull_neighbours(gg_ctx* c, int32_t id, int32_t* dst, int32_t cap, int32_t* n) {

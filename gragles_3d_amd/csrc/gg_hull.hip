@@ -134,6 +134,7 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
   // cell's sums to the grain's table directly.
   constexpr bool kElemLds = TOPO && !(sizeof(T) == 8 && NMAX == 96);
   __shared__ double s_jp[kElemLds ? 3 * kJuncLds : 1];
+  __shared__ double s_ja[kElemLds ? kJuncLds : 1];  // summed triangle areas
   __shared__ int s_jc[kElemLds ? kJuncLds : 1];
   __shared__ int s_gk[kTopoCap];                   // copy of the grain's id table
   const int total = P.totals[kQLayers * kClasses + cls];
@@ -155,6 +156,7 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
       s_jm[i] = 0ull;
       if constexpr (kElemLds) {
         s_jc[i] = 0;
+        s_ja[i] = 0.0;
         s_jp[3 * i] = s_jp[3 * i + 1] = s_jp[3 * i + 2] = 0.0;
       }
     }
@@ -227,12 +229,12 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
           };
           // the barycenter sums (src/triangle.h:12-18) and number of a cut
           // cell's triangles, vertices as the hull's
-          auto cell_elems = [&](uint32_t cd, double& sx, double& sy, double& sz) -> int {
+          auto cell_elems = [&](uint32_t cd, double& sx, double& sy, double& sz, double& sa) -> int {
             const unsigned ci = cd & 0xFFu;
             const int cyl = (cd >> 8) & 0xFF, cxl = (cd >> 16) & 0xFF;
             const int nt = s_ntri[ci];
             uint64_t row = s_tri[ci];
-            sx = sy = sz = 0.0;
+            sx = sy = sz = sa = 0.0;
             for (int j = 0; j < nt; ++j, row >>= 12) {
               double p0x, p0y, p0z, p1x, p1y, p1z, p2x, p2y, p2z;
               edge_vertex<T, NMAX>((int)(row & 15u), L, cxl, cyl, n, x0 + cxl, y0 + cyl, z, p0x, p0y, p0z);
@@ -241,6 +243,11 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
               sx += (p0x + p1x + p2x) / 3;
               sy += (p0y + p1y + p2y) / 3;
               sz += (p0z + p1z + p2z) / 3;
+              // GrainBoundary::computeArea (src/InterfacialElement.cpp:81-92)
+              const double abx = p0x - p1x, aby = p0y - p1y, abz = p0z - p1z;
+              const double acx = p0x - p2x, acy = p0y - p2y, acz = p0z - p2z;
+              const double cx = aby * acz - abz * acy, cy = abz * acx - abx * acz, cz = abx * acy - aby * acx;
+              sa += sqrt(cx * cx + cy * cy + cz * cz) / 2.0;
             }
             return nt;
           };
@@ -343,11 +350,12 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
             }
             if constexpr (kElemLds) {
               // 3. interfacial elements (src/grainHull.cpp:227-270), per tile list
-              double sx, sy, sz;
-              const int nt = cell_elems(code[r], sx, sy, sz);
+              double sx, sy, sz, sa;
+              const int nt = cell_elems(code[r], sx, sy, sz, sa);
               atomicAdd(&s_jp[3 * jsl], sx);
               atomicAdd(&s_jp[3 * jsl + 1], sy);
               atomicAdd(&s_jp[3 * jsl + 2], sz);
+              atomicAdd(&s_ja[jsl], sa);
               atomicAdd(&s_jc[jsl], nt);
             }
           }
@@ -355,6 +363,7 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
           unsigned long long* gj = topo.jkey + (int64_t)g * kJuncCap;
           double* gp = topo.jpos + (int64_t)g * kJuncCap * 3;
           int32_t* gc = topo.jcnt + (int64_t)g * kJuncCap;
+          double* ga = topo.jarea + (int64_t)g * kJuncCap;
           for (int i = tid; i < kJuncLds; i += kHT) {
             const unsigned long long m = s_jm[i];
             if (m != 0ull) {
@@ -371,12 +380,14 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
                 atomicAdd(&gp[3 * gsl], s_jp[3 * i]);
                 atomicAdd(&gp[3 * gsl + 1], s_jp[3 * i + 1]);
                 atomicAdd(&gp[3 * gsl + 2], s_jp[3 * i + 2]);
+                atomicAdd(&ga[gsl], s_ja[i]);
                 atomicAdd(&gc[gsl], s_jc[i]);
               }
             }
             s_jm[i] = 0ull;
             if constexpr (kElemLds) {
               s_jc[i] = 0;
+              s_ja[i] = 0.0;
               s_jp[3 * i] = s_jp[3 * i + 1] = s_jp[3 * i + 2] = 0.0;
             }
           }
@@ -471,11 +482,12 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
               hsl = (hsl + 1) & (kJuncCap - 1);
             }
             if (slot < 0) continue;
-            double sx, sy, sz;
-            const int nt = cell_elems(code[r], sx, sy, sz);
+            double sx, sy, sz, sa;
+            const int nt = cell_elems(code[r], sx, sy, sz, sa);
             atomicAdd(&gp[3 * slot], sx);
             atomicAdd(&gp[3 * slot + 1], sy);
             atomicAdd(&gp[3 * slot + 2], sz);
+            atomicAdd(&ga[slot], sa);
             atomicAdd(&gc[slot], nt);
           }
         }
@@ -694,6 +706,7 @@ __global__ __launch_bounds__(256) void k_topo_clear(SimParams sp, TopoDev topo) 
     topo.jpos[3 * i] = 0.0;
     topo.jpos[3 * i + 1] = 0.0;
     topo.jpos[3 * i + 2] = 0.0;
+    topo.jarea[i] = 0.0;
   }
 }
 

@@ -75,6 +75,7 @@ struct TopoDev {
   int on;                      // 0: not computed
   double* jpos;                // [(N+1) * kJuncCap * 3] barycenter sums of each list's triangles
   int32_t* jcnt;               // [(N+1) * kJuncCap] triangles of each list
+  double* jarea;               // [(N+1) * kJuncCap] summed triangle areas of each list (grid units)
   int corr;                    // also the triangle records and first appearances below
   int32_t* jord;               // [(N+1) * kJuncCap] first appearance of each list
   TriRec* raw;                 // [raw_cap] triangle records of the last hull

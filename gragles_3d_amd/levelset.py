@@ -102,6 +102,8 @@ SIGNATURES = {
     "gg_get_hull_neighbour_counts": (ctypes.c_int, [_vp, _i32p, _u8p]),
     "gg_download_hull_neighbours": (ctypes.c_int, [_vp, _i32, _i32p, _i32, _i32p]),
     "gg_get_hull_junction_counts": (ctypes.c_int, [_vp, _i32p]),
+    "gg_get_faces": (ctypes.c_int, [_vp, _i32p, _i32p, _f64p, ctypes.c_int64, _i64p]),
+    "gg_get_element_counts": (ctypes.c_int, [_vp, _i32p]),
     "gg_download_all_lists": (ctypes.c_int, [_vp, _i32, _i32p, _i32p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     "gg_get_new_bounds": (ctypes.c_int, [_vp, _i32p]),
     "gg_box_voxels": (ctypes.c_int, [_vp, _i64p]),
@@ -637,6 +639,25 @@ class LevelSetContext:
         wall = np.zeros(self.params.n_grains + 1, dtype=np.uint8)
         self._chk(lib().gg_get_hull_neighbour_counts(self.h, _p(cnt, ctypes.c_int32), _p(wall, ctypes.c_uint8)))
         return cnt, wall.astype(bool)
+
+    def element_counts(self):
+        """(n_grains+1, 4): boundaries, triple lines, quadruple and high-order
+        junctions after mergeJunction (correction on)."""
+        out = np.zeros((self.params.n_grains + 1, 4), dtype=np.int32)
+        self._chk(lib().gg_get_element_counts(self.h, _p(out, ctypes.c_int32)))
+        return out
+
+    def faces(self):
+        """get_Faces of every alive grain (what Faces_<loop>.bin holds): arrays
+        grain_a, grain_b (grain_a > grain_b), area (physical units)."""
+        n = ctypes.c_int64(0)
+        self._chk(lib().gg_get_faces(self.h, None, None, None, 0, ctypes.byref(n)))
+        k = max(n.value, 1)
+        a, b, ar = np.zeros(k, np.int32), np.zeros(k, np.int32), np.zeros(k)
+        self._chk(lib().gg_get_faces(self.h, _p(a, ctypes.c_int32), _p(b, ctypes.c_int32), _p(ar, ctypes.c_double),
+                                     k, ctypes.byref(n)))
+        m = n.value
+        return a[:m], b[:m], ar[:m]
 
     def hull_junction_counts(self):
         """Per grain (rows 0..n_grains): distinct cell id lists of its last hull with

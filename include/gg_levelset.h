@@ -343,6 +343,26 @@ int gg_download_hull_neighbours(gg_ctx* ctx, int32_t id, int32_t* dst, int32_t c
  * other size (higher-order junctions); before GrainHull::mergeJunction.  At
  * most 512 lists per hull (more sets the topology device error). */
 int gg_get_hull_junction_counts(gg_ctx* ctx, int32_t* counts4);
+/* GrainHull::get_Faces (src/grainHull.cpp:1330-1338) of every alive grain, as
+ * grainhdl::save_Texture writes them to Faces_<loop>.bin
+ * (src/grainhdl.cpp:762-771, struct Face src/Structs.h:12-19): one face per
+ * grain-boundary list (the grain and one neighbour) of the grain's last hull
+ * whose neighbour id is smaller than the grain's, with the boundary's area
+ * (GrainBoundary::computeArea, src/InterfacialElement.cpp:81-92: the summed
+ * triangle areas times h^2).  Grains ascending, a grain's faces by neighbour
+ * id (the reference lists them in its elements' order).  The areas are summed
+ * with atomics, so their last bits vary from run to run.  Needs hull
+ * topology on.  grain_a/grain_b/area: cap entries each, or NULL to count;
+ * *n receives the number of faces. */
+int gg_get_faces(gg_ctx* ctx, int32_t* grain_a, int32_t* grain_b, double* area, int64_t cap, int64_t* n);
+/* The interfacial elements' census of every grain after mergeJunction, what
+ * LSbox::get_NumberOfTriplelines / get_NumberOfQuadruplePoints /
+ * get_NumberOfHighOrderJunctions report (src/box.h, src/grainHull.cpp:187-274,
+ * 483-531): counts4[4*id .. 4*id+3] = grain boundaries, triple lines, quadruple
+ * junctions, high-order junctions of the grain's last hull (0 for grains not
+ * alive).  Built with the convolution correction on (gg_set_physics);
+ * gg_get_hull_junction_counts gives the lists before merging. */
+int gg_get_element_counts(gg_ctx* ctx, int32_t* counts4);
 
 /* New bounds from executeSetComparison (m_new{X,Y,Z}{Min,Max}), rows of 6. */
 int gg_get_new_bounds(gg_ctx* ctx, int32_t* bounds);

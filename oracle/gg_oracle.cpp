@@ -579,6 +579,7 @@ struct JRec {
   std::vector<unsigned> ids;
   int ntri = 0;
   double sum[3] = {0, 0, 0};
+  double area = 0;  // summed triangle areas, grid units (GrainBoundary::computeArea without h^2)
 };
 
 // Grains the hull touches (MarchingCubesAlgorithm::generateAdditionalInformation
@@ -688,6 +689,11 @@ void hull_elements(const Buf<double>& B, const Buf<uint32_t>& idl, unsigned self
           r.sum[0] += (p0.x + p1.x + p2.x) / 3;
           r.sum[1] += (p0.y + p1.y + p2.y) / 3;
           r.sum[2] += (p0.z + p1.z + p2.z) / 3;
+          // GrainBoundary::computeArea (src/InterfacialElement.cpp:81-92): |AB x BC| / 2
+          const double abx = p0.x - p1.x, aby = p0.y - p1.y, abz = p0.z - p1.z;
+          const double acx = p0.x - p2.x, acy = p0.y - p2.y, acz = p0.z - p2.z;
+          const double cx = aby * acz - abz * acy, cy = abz * acx - abx * acz, cz = abx * acy - aby * acx;
+          r.area += std::sqrt(cx * cx + cy * cy + cz * cz) / 2.0;
           ++r.ntri;
         }
       }
@@ -1801,7 +1807,8 @@ long orc_hull_neighbours_box(int n, int x0, int y0, int z0, const double* phi, c
 // hull_elements of one cube box (as orc_hull_neighbours_box); output as
 // orc_get_hull_elements.
 long orc_hull_elements_box(int n, int x0, int y0, int z0, const double* phi, const int32_t* b6, const uint32_t* idl,
-                           int self, int32_t* nids, int32_t* ids8, int32_t* ntri, double* pos, long cap) {
+                           int self, int32_t* nids, int32_t* ids8, int32_t* ntri, double* pos, double* area,
+                           long cap) {
   Buf<double> B;
   B.resize(x0, y0, z0, x0 + n, y0 + n, z0 + n);
   std::memcpy(B.v.data(), phi, sizeof(double) * B.v.size());
@@ -1816,6 +1823,7 @@ long orc_hull_elements_box(int n, int x0, int y0, int z0, const double* phi, con
     for (int k = 0; k < 8; ++k) ids8[8 * r + k] = k < (int)E[r].ids.size() ? (int32_t)E[r].ids[k] : -1;
     ntri[r] = E[r].ntri;
     for (int k = 0; k < 3; ++k) pos[3 * r + k] = E[r].sum[k] / E[r].ntri;
+    if (area) area[r] = E[r].area;
   }
   return (long)E.size();
 }
@@ -1842,6 +1850,15 @@ long orc_get_hull_elements(orc_sim* S, int id, int32_t* nids, int32_t* ids8, int
     ntri[r] = E[r].ntri;
     for (int k = 0; k < 3; ++k) pos[3 * r + k] = E[r].sum[k] / E[r].ntri;
   }
+  return (long)E.size();
+}
+
+// The summed triangle areas (grid units) of the grain's interfacial elements,
+// in orc_get_hull_elements' order.  Returns the count, or -1 if cap is too small.
+long orc_get_hull_element_areas(orc_sim* S, int id, double* area, long cap) {
+  const std::vector<JRec>& E = S->g.at(id).hull_je;
+  if ((long)E.size() > cap) return -1;
+  for (size_t r = 0; r < E.size(); ++r) area[r] = E[r].area;
   return (long)E.size();
 }
 

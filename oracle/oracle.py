@@ -75,8 +75,9 @@ def lib():
             "orc_coarsen": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double]),
             "orc_voxel_container": (ctypes.c_long, [vp, u32p, ctypes.c_long]),
             "orc_get_hull_elements": (ctypes.c_long, [vp, ctypes.c_int, i32p, i32p, i32p, f64p, ctypes.c_long]),
+            "orc_get_hull_element_areas": (ctypes.c_long, [vp, ctypes.c_int, f64p, ctypes.c_long]),
             "orc_hull_elements_box": (ctypes.c_long, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, f64p,
-                                                      i32p, u32p, ctypes.c_int, i32p, i32p, i32p, f64p,
+                                                      i32p, u32p, ctypes.c_int, i32p, i32p, i32p, f64p, f64p,
                                                       ctypes.c_long]),
             "orc_resize_grid_box": (ctypes.c_long, [i32p, f64p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                                     ctypes.c_double, ctypes.c_double, i32p, i32p, f64p,
@@ -188,9 +189,10 @@ def hull_neighbours_box(phi, origin, idl_bounds, idl, self_id, junctions=False):
     return (out[:k].copy(), j4) if junctions else out[:k].copy()
 
 
-def hull_elements_box(phi, origin, idl_bounds, idl, self_id):
+def hull_elements_box(phi, origin, idl_bounds, idl, self_id, with_area=False):
     """Interfacial elements of a cube box's hull: [(ids, ntri, mean barycenter)]
-    sorted by ids."""
+    sorted by ids (with_area: a fourth field, the summed triangle areas in grid
+    units)."""
     phi = np.ascontiguousarray(phi, dtype=np.float64)
     n = phi.shape[0]
     b6 = np.ascontiguousarray(idl_bounds, dtype=np.int32)
@@ -198,13 +200,17 @@ def hull_elements_box(phi, origin, idl_bounds, idl, self_id):
     cap = 1024
     nids, ids8, ntri = np.zeros(cap, np.int32), np.zeros(8 * cap, np.int32), np.zeros(cap, np.int32)
     pos = np.zeros(3 * cap)
+    area = np.zeros(cap)
     k = lib().orc_hull_elements_box(n, int(origin[0]), int(origin[1]), int(origin[2]), _p(phi, ctypes.c_double),
                                     _p(b6, ctypes.c_int32), _p(idl, ctypes.c_uint32), int(self_id),
                                     _p(nids, ctypes.c_int32), _p(ids8, ctypes.c_int32), _p(ntri, ctypes.c_int32),
-                                    _p(pos, ctypes.c_double), cap)
+                                    _p(pos, ctypes.c_double), _p(area, ctypes.c_double), cap)
     assert k >= 0
-    return [(tuple(ids8[8 * r:8 * r + min(8, nids[r])].tolist()), int(ntri[r]), tuple(pos[3 * r:3 * r + 3]))
-            for r in range(k)]
+    out = [(tuple(ids8[8 * r:8 * r + min(8, nids[r])].tolist()), int(ntri[r]), tuple(pos[3 * r:3 * r + 3]))
+           for r in range(k)]
+    if with_area:
+        out = [e + (float(area[r]),) for r, e in enumerate(out)]
+    return out
 
 
 def redistance_box(in_bounds, phi_in, out_bounds, h, delta):
@@ -393,6 +399,15 @@ class OracleSim:
             raise RuntimeError("too many elements")
         return [(tuple(ids8[8 * r:8 * r + min(8, nids[r])].tolist()), int(ntri[r]), tuple(pos[3 * r:3 * r + 3]))
                 for r in range(n)]
+
+    def hull_element_areas(self, gid):
+        """Summed triangle areas (grid units) of hull_elements(gid), same order."""
+        cap = 4096
+        a = np.zeros(cap)
+        n = lib().orc_get_hull_element_areas(self.h, int(gid), _p(a, ctypes.c_double), cap)
+        if n < 0:
+            raise RuntimeError("too many elements")
+        return a[:n].copy()
 
     def voxel_container(self):
         """saveNetworkAsVoxelContainer's id container, [z][y][x] of side

@@ -41,10 +41,13 @@ def _pair(case, phys, precision=G.FP64_PLAIN):
 def _compare_elements(ctx, orc, where):
     st = orc.states()
     n_el = 0
+    counts = ctx.element_counts()
     for g in np.nonzero(st == O.ALIVE)[0]:
         eg = ctx.correction_elements(g)
         eo = orc.correction_elements(g)
         assert len(eo) == len(eg["cls"]), (where, g, len(eo), len(eg["cls"]))
+        want = [sum(1 for e in eo if e[0] == k) for k in (0, 1, 2, 3)]  # GB, TL, QJ, HJ after merging
+        assert list(counts[g]) == want, (where, g, list(counts[g]), want)
         for e, (c, nb, nbary, pos, en, mo) in enumerate(eo):
             ids = sorted(set(orc.element_ids(g, e).tolist()))
             assert eg["cls"][e] == c, (where, g, e)

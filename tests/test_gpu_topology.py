@@ -1,5 +1,6 @@
 """Hull topology (the grains each marching-cubes hull touches) of the FP64 GPU
-path vs the oracle, grain by grain and in the reference's order.
+path vs the oracle, grain by grain and in the reference's order, and the
+grain-boundary faces (get_Faces: pairs and areas).
 
 Each case prints its name, seed and shape before touching the GPU.
 """
@@ -45,6 +46,22 @@ def _compare(ctx, orc, where):
     return checked
 
 
+def _compare_faces(ctx, orc, h, where):
+    # get_Faces (src/grainHull.cpp:1330-1338): per alive grain its boundary
+    # lists with a smaller neighbour id, area = summed triangle areas * h^2
+    st = orc.states()
+    want = []
+    for g in np.nonzero(st == G.ALIVE)[0]:
+        els, areas = orc.hull_elements(g), orc.hull_element_areas(g)
+        faces = sorted((e[0][0] if e[0][1] == g else e[0][1], a * h * h) for e, a in zip(els, areas)
+                       if len(e[0]) == 2 and g in e[0] and g > min(e[0]))
+        want += [(int(g), int(nb), a) for nb, a in faces]
+    ga, gb, ar = ctx.faces()
+    assert [(int(a), int(b)) for a, b in zip(ga, gb)] == [(w[0], w[1]) for w in want], where
+    np.testing.assert_allclose(ar, [w[2] for w in want], rtol=1e-10, atol=0, err_msg=where)
+    return len(want)
+
+
 @pytest.mark.parametrize("n_grains,ppg,border,seed", [(64, 10, 3, 5), (64, 6, 3, 3)])
 def test_hull_topology_matches_oracle(n_grains, ppg, border, seed):
     _say("hull topology n=%d ppg=%d border=%d seed=%d" % (n_grains, ppg, border, seed))
@@ -55,6 +72,7 @@ def test_hull_topology_matches_oracle(n_grains, ppg, border, seed):
     for s in range(6):
         step_both(ctx, orc)
         assert _compare(ctx, orc, "step %d" % s) > 0
+        assert _compare_faces(ctx, orc, p.h, "faces, step %d" % s) > 0
         walls += int(ctx.hull_neighbour_counts()[1].sum())
         junctions += ctx.hull_junction_counts().sum(axis=0)
     assert walls > 0, "no hull touched the domain wall; the case does not exercise id 0"

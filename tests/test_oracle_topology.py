@@ -109,3 +109,29 @@ def test_elements_match_the_junction_classes():
     sizes = [len(e[0]) for e in el]
     assert [sizes.count(2), sizes.count(3)] == list(junc[:2])
     assert sum(e[1] for e in el) == 2 * (N - 1) ** 2
+
+
+def test_boundary_face_area_closed_forms():
+    # GrainBoundary::computeArea (src/InterfacialElement.cpp:81-92): the
+    # half-space's one boundary element is the flat (N-1) x (N-1) patch at
+    # x = 3.5, area (N-1)^2 in grid units; the oracle's whole-hull surface of
+    # the same box is the same number
+    idl = np.full(N ** 3, 7, dtype=np.uint32)
+    el = O.hull_elements_box(half_space(), ORIGIN, BOX, idl, 3, with_area=True)
+    assert len(el) == 1 and abs(el[0][3] - (N - 1) ** 2) < 1e-9
+    # a tilted plane x + y = const: area grows by sqrt(2) against its x
+    # projection, and the elements of two neighbours split it (their areas add
+    # up to the whole patch)
+    z, y, x = np.mgrid[0:N, 0:N, 0:N]
+    phi = 4.3 - (x + y) / np.sqrt(2.0)
+    idl2 = np.where(z < 4, 5, 6).astype(np.uint32).reshape(-1)
+    el2 = O.hull_elements_box(phi, ORIGIN, BOX, idl2, 3, with_area=True)
+    _, _, area_all = hull_area_box(phi)
+    assert abs(sum(e[3] for e in el2) - area_all) < 1e-9 * area_all
+    gb = {e[0]: e[3] for e in el2 if len(e[0]) == 2}
+    assert set(gb) == {(3, 5), (3, 6)}
+
+
+def hull_area_box(phi):
+    # the oracle's whole-hull reduction on the same box (exists, volume, area; h = 1: grid units)
+    return O.hull_box(phi, ORIGIN, 1.0)
