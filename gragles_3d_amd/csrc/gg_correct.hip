@@ -422,7 +422,10 @@ __global__ __launch_bounds__(256) void k_correct(SimParams sp, PlanDev P, int cl
   const T* __restrict__ ain = (const T*)din.arena;
   T* __restrict__ aout = (T*)dout.arena;
   const double delta = sp.delta;
-  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+  // consecutive tiles (the layers of one grain) on one XCD: the layer two
+  // tiles share, and the grain's tables, come from that XCD's L2
+  const TileRange tr = xcd_tiles(total);
+  for (int tile = tr.first; tile < tr.end; tile += tr.step) {
     int g, kl;
     plan_tile(P, cls, kQLayers, tile, g, kl);
     const int32_t* b = dout.bounds + 6 * g;

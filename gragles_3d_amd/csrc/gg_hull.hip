@@ -161,7 +161,10 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
       }
     }
   }
-  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+  // consecutive tiles (the layers of one grain) on one XCD: the layer two
+  // tiles share, and the grain's tables, come from that XCD's L2
+  const TileRange tr = xcd_tiles(total);
+  for (int tile = tr.first; tile < tr.end; tile += tr.step) {
     int g, kl;
     plan_tile(P, cls, kQLayers, tile, g, kl);
     const int64_t slot = P.layer_base[cls] + tile;
