@@ -258,7 +258,7 @@ __device__ inline void load_fft_tables(const TablesDev& tb, int n, double* tc, d
 // (x, y) forward per z-layer: two real rows per complex x-FFT, the half
 // spectrum separated from it, then the y-FFT of the n/2+1 spectral columns.
 template <int NMAX>
-__global__ __launch_bounds__(kT) void k_fft_xy_fwd(PlanDev P, int cls, DescDev din, TablesDev tb,
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(NMAX <= 32 ? 5 : NMAX <= 48 ? 4 : 1))) void k_fft_xy_fwd(PlanDev P, int cls, DescDev din, TablesDev tb,
                                                    const int64_t* __restrict__ spec_off, double2* spec) {
   constexpr int NR = (NMAX + 1) / 2, N2 = NMAX / 2 + 1;
   constexpr int R2 = cdiv(NMAX * N2, kT);
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(kT) void k_fft_xy_fwd(PlanDev P, int cls, DescDev d
 
 // z forward, spectral factor, z inverse per block of kSpecBlock spectral columns.
 template <int NMAX>
-__global__ __launch_bounds__(kT) void k_fft_z(PlanDev P, int cls, DescDev dref, TablesDev tb,
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(NMAX <= 32 ? 5 : NMAX <= 48 ? 4 : 1))) void k_fft_z(PlanDev P, int cls, DescDev dref, TablesDev tb,
                                               const int64_t* __restrict__ spec_off, double2* spec) {
   constexpr int CB = kSpecBlock;
   __shared__ double2 B[NMAX * CB];
@@ -354,7 +354,7 @@ __global__ __launch_bounds__(kT) void k_fft_z(PlanDev P, int cls, DescDev dref, 
 // (y, x) inverse per z-layer: y-IFFT of the spectral columns, two rows' half
 // spectra packed into one full spectrum, x-IFFT, real and imaginary parts out.
 template <int NMAX>
-__global__ __launch_bounds__(kT) void k_fft_xy_inv(PlanDev P, int cls, DescDev dout, TablesDev tb,
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(NMAX <= 32 ? 5 : NMAX <= 48 ? 4 : 1))) void k_fft_xy_inv(PlanDev P, int cls, DescDev dout, TablesDev tb,
                                                    const int64_t* __restrict__ spec_off, const double2* spec) {
   constexpr int NR = (NMAX + 1) / 2, N2 = NMAX / 2 + 1;
   constexpr int R1 = cdiv(NR * NMAX, kT);
