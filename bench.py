@@ -46,6 +46,7 @@ METRIC = "grain-box voxel updates/sec per timestep, FP32-batched vs FP64-plain; 
 HBM_PEAK_GBS = 8000.0
 CLASS_NMAX = (32, 48, 64, 96)  # box-size classes of the work plan, one launch each per stage
 CMP_SHAPES = G.levelset.CMP_SHAPES  # comparison tile shapes of gg_autotune (layers, gathers in flight, run-major)
+CMP_SPACE_SHAPES = [(8, 8), (16, 8), (16, 16), (32, 8)]  # shapes of the spatially ordered launch
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_fp32_8000grains.md")
 
 
@@ -81,6 +82,30 @@ def new_context(precision, inputs, p):
     ctx.set_neighbour_lists(inputs["offsets"], inputs["ids"])
     ctx.synchronize()
     return ctx
+
+
+def tune_comparison(precision, inputs, p, reps):
+    """gg_autotune in both comparison orders, each on a context of its own taking the same first
+    timestep: the tile shape per size class (grain order) and of the one launch (spatial order),
+    and the faster order.  Returns (shapes per class, order, spatial shape, report)."""
+    tctx = new_context(precision, inputs, p)
+    choice, ms = tctx.autotune(reps=reps)
+    tctx.close()
+    choice = choice.tolist()
+    grain_ms = float(sum(ms[c][choice[c]] for c in range(len(CLASS_NMAX))))
+    tctx = new_context(precision, inputs, p)
+    tctx.set_comparison_order(G.CMP_ORDER_SPACE)
+    schoice, sms = tctx.autotune(reps=reps)
+    tctx.close()
+    shape = int(schoice[0])
+    space_ms = float(sms[0][shape])
+    order = G.CMP_ORDER_SPACE if space_ms < grain_ms else G.CMP_ORDER_GRAIN
+    report = {"choice_per_class": choice, "ms_per_class_and_shape": np.round(ms, 4).tolist(),
+              "grain_order_ms": grain_ms, "spatial_order_ms_per_shape": np.round(sms[0][:4], 4).tolist(),
+              "spatial_order_shape": shape, "spatial_order_ms": space_ms,
+              "order": "spatial" if order == G.CMP_ORDER_SPACE else "grain",
+              "source": "autotuner (one timestep on a separate context per order)"}
+    return choice, order, shape, report
 
 
 def timed_steps(ctx, steps, warmup, use_graph=True):
@@ -357,6 +382,10 @@ def main():
                     help="the 1-thread CPU baseline times one step over every n-th grain")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default comparison tile shapes instead of autotuning them (one timestep)")
+    ap.add_argument("--cmp-order", choices=("auto", "grain", "spatial"), default="auto",
+                    help="comparison order of the FP32 leg (auto: the autotuner's faster one)")
+    ap.add_argument("--space-shape", type=int, choices=range(4), default=None,
+                    help="tile shape of the spatially ordered comparison, instead of autotuning it")
     ap.add_argument("--tuning", metavar="S0,S1,S2,S3",
                     help="comparison tile shape per size class (0..3), instead of autotuning them")
     ap.add_argument("--no-physics", action="store_true",
@@ -394,20 +423,25 @@ def main():
     # context starts from the initial state (as a profiled run with --tuning does).
     tune32 = None
     choice32 = None
+    order32, shape32 = G.CMP_ORDER_GRAIN, 0
     if args.tuning:
         choice32 = [int(v) for v in args.tuning.split(",")]
         tune32 = {"choice_per_class": choice32, "source": "--tuning"}
     elif not args.no_tune:
-        tctx = new_context(G.FP32_BATCHED, inputs, p)
-        choice, ms = tctx.autotune(reps=3)
-        tctx.close()
-        choice32 = choice.tolist()
-        tune32 = {"choice_per_class": choice32, "ms_per_class_and_shape": np.round(ms, 4).tolist(),
-                  "source": "autotuner (one timestep on a separate context)"}
+        choice32, order32, shape32, tune32 = tune_comparison(G.FP32_BATCHED, inputs, p, 3)
+        log("fp32 comparison: grain order %.3f ms (shapes per class %s), spatial order %.3f ms (shape %d) -> %s"
+            % (tune32["grain_order_ms"], choice32, tune32["spatial_order_ms"], shape32, tune32["order"]))
+    if args.space_shape is not None:
+        shape32 = args.space_shape
+    if args.cmp_order == "grain":
+        order32 = G.CMP_ORDER_GRAIN
+    elif args.cmp_order == "spatial":
+        order32 = G.CMP_ORDER_SPACE
     ctx32 = new_context(G.FP32_BATCHED, inputs, p)
     if choice32 is not None:
         ctx32.set_tuning(choice32)
         log("fp32 comparison shapes per class (n<=32,48,64,96): %s" % choice32)
+    ctx32.set_comparison_order(order32, shape32)
     seg32, by32, cls_bytes, cls_vox, cls_num = kernel_pass(ctx32, args.kernel_steps, True)
     ks = max(args.kernel_steps, 1)
     rows32 = roofline_rows(seg32, by32, ks)
@@ -418,15 +452,29 @@ def main():
 
     cls_rows = []
     tuning = ctx32.tuning()
-    for c, m in enumerate(CLASS_NMAX):
-        ms, nl = seg32["comparison_n%d" % m]
+    if order32 == G.CMP_ORDER_SPACE:
+        # one launch for every class; its segment also holds the neighbour table and the
+        # work-list build (about 2 % of it), so `achieved` is a lower bound for the kernel
+        ms, nl = seg32["comparison_n32"]
         avg_s = ms / max(nl, 1) * 1e-3
-        bts = cls_bytes[c] / ks
-        zl, qb, zmaj = CMP_SHAPES[tuning[c]]
-        cls_rows.append({"kernel": "k_cmp_cols<float, %d, %d, %d, %s>" % (m, zl, qb, "true" if zmaj else "false"), "box_voxels": int(cls_vox[c] / ks),
-                         "kbar": cls_num[c] / cls_vox[c] if cls_vox[c] else 0.0, "bytes_per_launch": bts,
+        bts = cls_bytes.sum() / ks
+        zl, qb = CMP_SPACE_SHAPES[shape32]
+        cls_rows.append({"kernel": "k_cmp_space<float, %d, %d>" % (zl, qb), "box_voxels": int(cls_vox.sum() / ks),
+                         "kbar": cls_num.sum() / cls_vox.sum() if cls_vox.sum() else 0.0, "bytes_per_launch": bts,
                          "avg_launch_ms": avg_s * 1e3,
-                         "achieved_GBs": bts / avg_s / 1e9 if avg_s > 0 and bts > 0 else 0.0})
+                         "achieved_GBs": bts / avg_s / 1e9 if avg_s > 0 and bts > 0 else 0.0,
+                         "segment": "neighbour table + work list + comparison"})
+    else:
+        for c, m in enumerate(CLASS_NMAX):
+            ms, nl = seg32["comparison_n%d" % m]
+            avg_s = ms / max(nl, 1) * 1e-3
+            bts = cls_bytes[c] / ks
+            zl, qb, zmaj = CMP_SHAPES[tuning[c]]
+            cls_rows.append({"kernel": "k_cmp_cols<float, %d, %d, %d, %s>" % (m, zl, qb, "true" if zmaj else "false"),
+                             "box_voxels": int(cls_vox[c] / ks),
+                             "kbar": cls_num[c] / cls_vox[c] if cls_vox[c] else 0.0, "bytes_per_launch": bts,
+                             "avg_launch_ms": avg_s * 1e3,
+                             "achieved_GBs": bts / avg_s / 1e9 if avg_s > 0 and bts > 0 else 0.0})
     dom = max(cls_rows, key=lambda r: r["avg_launch_ms"])
     cmp_red_ms = rows32["comparison"]["ms_per_step"] + rows32["redistancing"]["ms_per_step"]
     cmp_red_b = rows32["comparison"]["algorithmic_bytes_per_step"] + rows32["redistancing"][
@@ -437,15 +485,15 @@ def main():
     fp64 = None
     ctx64 = None
     if not args.no_fp64:
+        order64, shape64 = order32, shape32
         if not args.no_tune and not args.tuning:
-            tctx = new_context(G.FP64_PLAIN, inputs, p)
-            choice64 = tctx.autotune(reps=2)[0].tolist()
-            tctx.close()
+            choice64, order64, shape64, _ = tune_comparison(G.FP64_PLAIN, inputs, p, 2)
         else:
             choice64 = choice32
         ctx64 = new_context(G.FP64_PLAIN, inputs, p)
         if choice64 is not None:
             ctx64.set_tuning(choice64)
+        ctx64.set_comparison_order(order64, shape64)
         seg64, by64, _, _, _ = kernel_pass(ctx64, max(1, args.kernel_steps // 2), False)
         wall64, vox_step64 = timed_steps(ctx64, args.steps, args.warmup)
         v64 = vox_step64 * args.steps / wall64 if args.steps else 0.0
@@ -453,6 +501,7 @@ def main():
             args.steps, wall64, 1e3 * wall64 / max(args.steps, 1), v64))
         k64 = max(1, args.kernel_steps // 2)
         fp64 = {"value": v64, "ms_per_step": 1e3 * wall64 / max(args.steps, 1),
+                "comparison_order": "spatial" if order64 == G.CMP_ORDER_SPACE else "grain",
                 "segment_ms_per_step": {k: round(v[0] / k64, 4) for k, v in seg64.items() if v[1]},
                 "stages": roofline_rows(seg64, by64, k64)}
         ctx64.close()

@@ -67,6 +67,9 @@ struct gg_ctx {
   PlanDev P2{};  // plan of the output boxes, from the level set's first part to the end of the redistancing
   bool p2_valid = false;  // P2 holds the plan of the current output boxes (the redistancing runs over it)
   int cmp_variant[kClasses] = {kCmpDefault, kCmpDefault, kCmpDefault, kCmpDefault};  // comparison tile shape per class
+  int cmp_order = GG_CMP_ORDER_GRAIN;  // comparison tiles per class in grain order, or all in spatial order
+  int cmp_space_shape = 0;             // tile shape of the spatially ordered launch
+  CmpSpaceDev W{};                     // its work list (allocated with the arenas)
   int64_t* d_spec_off = nullptr;
   void* d_spec = nullptr;
   int64_t spec_cap = 0;
@@ -151,6 +154,7 @@ int check_device_error(gg_ctx* c) {
     if (e & kErrHistory) m += " count history full;";
     if (e & kErrTopo) m += " hull topology table full;";
     if (e & kErrTri) m += " correction barycenter pool full;";
+    if (e & kErrCmpWork) m += " comparison work list full;";
     return fail(c, GG_ERR_CAPACITY, m);
   }
   return GG_OK;
@@ -367,7 +371,12 @@ void stage_lists(gg_ctx* c) {
 
 void stage_comparison(gg_ctx* c) {
   static const int seg[kClasses] = {kSegCompare, kSegCompare48, kSegCompare64, kSegCompare96};
-  if (c->timing) {
+  if (c->cmp_order == GG_CMP_ORDER_SPACE) {
+    SegTimer t(c, kSegCompare);
+    launch_comparison_table(launch_of(c), c->sp, c->d_state, din(c), c->LD);
+    launch_comparison_space(launch_of(c), c->cmp_space_shape, c->sp, c->P, din(c), dout(c), c->LD,
+                            c->IL[c->il_cur], c->IL[c->il_cur ^ 1], c->d_newb, c->W, c->d_err, c->fp32);
+  } else if (c->timing) {
     for (int cls = 0; cls < kClasses; ++cls) {
       SegTimer t(c, seg[cls]);
       // the first class's segment includes building the neighbour table
@@ -711,6 +720,21 @@ int gg_set_boxes(gg_ctx* c, const int32_t* bounds, const uint8_t* exists) {
   if (!c->fp32 && c->d_spec == nullptr) {
     int rc = dalloc(c, (double2**)&c->d_spec, c->spec_cap);
     if (rc) return rc;
+  }
+  if (c->W.wl == nullptr) {
+    // Wavefront tiles of a grain with side n, runs of >= 8 layers: at most
+    // ceil(n^2/64) * ceil(n/8) <= n^3/128 for n >= 8, one below; the plan
+    // keeps the sum of n^3 within the arena.
+    c->W.bx = (L >> 4) / 8 + 1;  // cell coordinates of positions 0..L
+    c->W.by = (L >> 3) / 8 + 1;
+    c->W.nkeys = c->W.bx * c->W.by * c->W.by * 512;
+    const int64_t cap = c->arena_cap / 128 + N + 64;
+    if (cap > INT32_MAX) return fail(c, GG_ERR_CAPACITY, "comparison work list above 2^31 tiles");
+    c->W.cap = (int)cap;
+    int rc;
+    if ((rc = dalloc(c, &c->W.wl, (size_t)cap)) || (rc = dalloc(c, &c->W.cnt, (size_t)c->W.nkeys + 1)) ||
+        (rc = dalloc(c, &c->W.bsum, (size_t)c->W.nkeys / 4096 + 1)))
+      return rc;
   }
   HIPCHK(c, hipMemcpy(c->d_state, st.data(), 4 * (N + 1), hipMemcpyHostToDevice));
   {
@@ -1058,16 +1082,23 @@ int gg_autotune(gg_ctx* c, int32_t reps, int32_t* choice, float* ms) {
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
-  for (int cls = 0; cls < kClasses; ++cls) {
+  const bool space = c->cmp_order == GG_CMP_ORDER_SPACE;
+  for (int cls = 0; cls < (space ? 1 : kClasses); ++cls) {
     float best = 0.f;
-    int bv = c->cmp_variant[cls];
-    for (int v = 0; v < kCmpVariants; ++v) {
-      launch_comparison(launch_of(c), cls, v, c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur],
-                        c->IL[c->il_cur ^ 1], c->d_newb, c->fp32);  // warm
-      hipEventRecord(e0, c->stream);
-      for (int r = 0; r < reps; ++r)
+    int bv = space ? c->cmp_space_shape : c->cmp_variant[cls];
+    // spatial order: the shapes of its one launch (work list build included)
+    auto run = [&](int v) {
+      if (space)
+        launch_comparison_space(launch_of(c), v, c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur],
+                                c->IL[c->il_cur ^ 1], c->d_newb, c->W, c->d_err, c->fp32);
+      else
         launch_comparison(launch_of(c), cls, v, c->sp, c->P, din(c), dout(c), c->LD, c->IL[c->il_cur],
                           c->IL[c->il_cur ^ 1], c->d_newb, c->fp32);
+    };
+    for (int v = 0; v < (space ? kCmpSpaceShapes : kCmpVariants); ++v) {
+      run(v);  // warm
+      hipEventRecord(e0, c->stream);
+      for (int r = 0; r < reps; ++r) run(v);
       hipEventRecord(e1, c->stream);
       hipError_t e = hipEventSynchronize(e1);
       float t = 0.f;
@@ -1085,8 +1116,14 @@ int gg_autotune(gg_ctx* c, int32_t reps, int32_t* choice, float* ms) {
         bv = v;
       }
     }
-    c->cmp_variant[cls] = bv;
-    if (choice) choice[cls] = bv;
+    if (space) {
+      c->cmp_space_shape = bv;
+      if (choice)
+        for (int k = 0; k < kClasses; ++k) choice[k] = bv;
+    } else {
+      c->cmp_variant[cls] = bv;
+      if (choice) choice[cls] = bv;
+    }
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
@@ -1098,6 +1135,24 @@ int gg_autotune(gg_ctx* c, int32_t reps, int32_t* choice, float* ms) {
   c->timing = timing;
   drop_graph(c);  // a captured step holds the old shapes
   return gg_synchronize(c);
+}
+
+int gg_set_comparison_order(gg_ctx* c, int32_t order, int32_t shape) {
+  if (!c) return GG_ERR_ARG;
+  if (order != GG_CMP_ORDER_GRAIN && order != GG_CMP_ORDER_SPACE)
+    return fail(c, GG_ERR_ARG, "unknown comparison order");
+  if (shape < 0 || shape >= kCmpSpaceShapes) return fail(c, GG_ERR_ARG, "spatial tile shape out of range");
+  c->cmp_order = order;
+  c->cmp_space_shape = shape;
+  drop_graph(c);
+  return GG_OK;
+}
+
+int gg_get_comparison_order(gg_ctx* c, int32_t* order, int32_t* shape) {
+  if (!c || !order) return GG_ERR_ARG;
+  *order = c->cmp_order;
+  if (shape) *shape = c->cmp_space_shape;
+  return GG_OK;
 }
 
 int gg_get_tuning(gg_ctx* c, int32_t* variants) {

@@ -82,20 +82,159 @@ __device__ inline bool deep_inside(int j, int i, int k, int gb, int m) {
 // reference's order).  The per-entry set-up is paid once per ZL layers, there
 // is no LDS and no barrier.  Then, per voxel, the wall distance, IDLocal (new
 // id or the stale one), the set-comparison and the bounding box, as above.
-template <class T, int NMAX, int ZL, int QB, bool ZMAJ>
-__global__ __launch_bounds__(256) void k_cmp_cols(SimParams sp, PlanDev P, int cls, DescDev din, DescDev dout,
-                                                  ListDev ld, IdlDev ilo, IdlDev iln, int32_t* newb) {
-  constexpr int ZCH = (NMAX + ZL - 1) / ZL;  // layer runs per column block
+// One wavefront tile: column block cb (64 consecutive columns (y, x) of grain
+// g's box) over run zc of ZL z-layers.
+template <class T, int ZL, int QB>
+__device__ inline void cmp_tile(const SimParams& sp, int g, int cb, int zc, const DescDev& din, const DescDev& dout,
+                                const ListDev& ld, const IdlDev& ilo, const IdlDev& iln, int32_t* newb) {
   static_assert(ZL % QB == 0, "a run is whole gather batches");
-  // wavefront tiles: the plan's column blocks (grains in spatial order), each
-  // with its ZCH runs next to each other
-  const int total = P.totals[kQCols * kClasses + cls] * ZCH;
   const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: everything per tile is scalar
   const T* __restrict__ ain = (const T*)din.arena;
   T* __restrict__ aout = (T*)dout.arena;
   const T lim = (T)(0.7 * sp.delta);
   const int maxL = ld.max_list;
+  const int32_t* b = din.bounds + 6 * g;
+  const int x0 = b[0], y0 = b[1], z0 = b[2], x1 = b[3], y1 = b[4];
+  const int n = x1 - x0, nn2 = n * n;
+  const int qlo = zc * ZL;
+  if (qlo >= n) return;
+  const int qn = min(ZL, n - qlo);  // layers of this run
+  const int zlo = z0 + qlo;         // its first global layer
+  const int col = cb * 64 + lane;
+  const bool act = col < nn2;
+  const int yl = act ? col / n : 0, xl = act ? col - yl * n : 0;
+  T best[ZL];
+  uint32_t bid[ZL];
+#pragma unroll
+  for (int q = 0; q < ZL; ++q) {
+    best[q] = (T)(-1.0);
+    bid[q] = 0xFFFFFFFFu;  // not covered
+  }
+  const int4* tab = ld.tab + (int64_t)g * maxL;
+  const int32_t* tid_of = ld.tab_id + (int64_t)g * maxL;
+  const int len = ld.tab_len[g];
+  for (int pb = 0; pb < len; pb += 64) {
+    const int idx = pb + lane;
+    bool keep = false;
+    int nb = 0, q2 = 0, q5 = 0;
+    int4 e = make_int4(0, 0, 0, 0);
+    if (idx < len) {
+      e = tab[idx];
+      nb = tid_of[idx];
+      q2 = e.w & 0xFFFF;
+      q5 = (int)((uint32_t)e.w >> 16);
+      keep = q2 < zlo + qn && q5 > zlo;  // the neighbour's box meets the run's layers
+    }
+    uint64_t m = __ballot(keep);
+    while (m) {
+      const int a = (int)__builtin_ctzll(m);  // next entry in list order
+      m &= m - 1;
+      const uint32_t rc = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)e.z, a);
+      const int rx0 = rc & 0xFF, rx1 = (rc >> 8) & 0xFF, ry0 = (rc >> 16) & 0xFF, ry1 = rc >> 24;
+      const bool cov = act & ((uint32_t)(xl - rx0) < (uint32_t)(rx1 - rx0)) & ((uint32_t)(yl - ry0) < (uint32_t)(ry1 - ry0));
+      const uint64_t cmask = __ballot(cov);
+      if (cmask == 0) continue;  // the rectangle misses this wavefront's columns
+      const uint32_t id = (uint32_t)__builtin_amdgcn_readlane(nb, a);
+      const uint32_t elo = (uint32_t)__builtin_amdgcn_readlane(e.x, a);
+      const uint32_t ehi = (uint32_t)__builtin_amdgcn_readlane(e.y, a);
+      const int nn = (int)(ehi >> 16);
+      const int64_t B = (int64_t)((uint64_t)(ehi & 0xFFFFu) << 32 | elo);  // rectangle's first voxel, layer q2
+      const int eq2 = __builtin_amdgcn_readlane(q2, a), eq5 = __builtin_amdgcn_readlane(q5, a);
+      const int qa = max(eq2, zlo) - zlo, qb = min(eq5, zlo + qn) - zlo;  // run layers the entry covers
+      const int ls = nn * nn;
+      // layer zlo + qa of the neighbour; element q of the run is at (q - qa) * ls + off from it
+      const T* __restrict__ src = ain + B + (int64_t)(zlo + qa - eq2) * ls;
+      // lanes off the rectangle read the first covered lane's voxel: a line the
+      // gather fetches anyway (the rectangle's first voxel of every layer would
+      // be one more line per gather)
+      const uint32_t offc = (uint32_t)((yl - ry0) * nn + (xl - rx0));
+      const uint32_t off0 = (uint32_t)__builtin_amdgcn_readlane((int)offc, (int)__builtin_ctzll(cmask));
+      const uint32_t off = cov ? offc : off0;
+#pragma unroll
+      for (int c0 = 0; c0 < ZL; c0 += QB) {
+        if (c0 >= qb || c0 + QB <= qa) continue;
+        T v[QB];
+#pragma unroll
+        for (int u = 0; u < QB; ++u) {
+          const int qc = min(max(c0 + u, qa), qb - 1) - qa;
+          // 32-bit byte offset from a wavefront-uniform base
+          v[u] = *(const T*)((const char*)src + (uint32_t)((qc * ls + (int)off) * (int)sizeof(T)));
+        }
+#pragma unroll
+        for (int u = 0; u < QB; ++u) {
+          const int q = c0 + u;
+          const bool up = cov & (q >= qa) & (q < qb) & (v[u] > best[q]);
+          best[q] = up ? v[u] : best[q];
+          bid[q] = up ? id : bid[q];
+        }
+      }
+    }
+  }
+  // BoundaryIntersection on the output box (equal to the input box here)
+  const int32_t* ob = dout.bounds + 6 * g;
+  const int lo = sp.gb + sp.tube, hi = sp.ngp - sp.gb - sp.tube;
+  const bool wall = !(ob[0] > lo && ob[3] < hi && ob[1] > lo && ob[4] < hi && ob[2] > lo && ob[5] < hi);
+  const int32_t* olb = ilo.bounds + 6 * g;
+  const int64_t oldsize = (int64_t)(olb[3] - olb[0]) * (olb[4] - olb[1]) * (olb[5] - olb[2]);
+  const uint32_t* __restrict__ idl_old = ilo.arena + ilo.off[g];
+  uint32_t* __restrict__ idl_new = iln.arena + iln.off[g];
+  const T* own = ain + din.off[g];
+  T* dst = aout + dout.off[g];
+  const int i = y0 + yl, j = x0 + xl;
+  int kmin = INT_MAX, kmax = -INT_MAX;
+#pragma unroll
+  for (int q = 0; q < ZL; ++q) {
+    if (q >= qn || !act) continue;
+    const int k = zlo + q;
+    if (wall && !deep_inside(j, i, k, sp.gb, sp.ngp)) {
+      const double dist = bc_dist(j, i, k, x0, y0, z0, x1, y1, sp.gb, sp.ngp);
+      if (dist > -sp.gb) {
+        const T dh = (T)(dist * sp.h);
+        if (dh > best[q]) {
+          best[q] = dh;
+          bid[q] = 0;
+        }
+      }
+    }
+    const int64_t li = (int64_t)(qlo + q) * nn2 + col;
+    uint32_t idv = bid[q];
+    if (idv == 0xFFFFFFFFu) idv = li < oldsize ? idl_old[li] : 0u;
+    idl_new[li] = idv;
+    const T a_in = own[li];
+    const T o = (a_in < T(0) ? -a_in : a_in) < lim ? (T)0.5 * (a_in - best[q]) : a_in;
+    dst[li] = o;
+    if (o >= T(0)) {
+      kmin = min(kmin, k);
+      kmax = max(kmax, k);
+    }
+  }
+  // bounding box of the voxels >= 0: the lane's column spans (j, i) x [kmin, kmax]
+  const bool any = kmin != INT_MAX;
+  int vals[6] = {any ? j : INT_MAX, any ? i : INT_MAX, kmin, any ? -j : INT_MAX, any ? -i : INT_MAX, -kmax};
+  for (int q = 0; q < 6; ++q) {
+    int x = vals[q];
+    for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o));
+    vals[q] = x;
+  }
+  if (lane == 0 && vals[0] != INT_MAX) {
+    int32_t* nbx = newb + 6 * g;
+    atomicMin(&nbx[0], vals[0]);
+    atomicMin(&nbx[1], vals[1]);
+    atomicMin(&nbx[2], vals[2]);
+    atomicMax(&nbx[3], -vals[3]);
+    atomicMax(&nbx[4], -vals[4]);
+    atomicMax(&nbx[5], -vals[5]);
+  }
+}
+
+template <class T, int NMAX, int ZL, int QB, bool ZMAJ>
+__global__ __launch_bounds__(256) void k_cmp_cols(SimParams sp, PlanDev P, int cls, DescDev din, DescDev dout,
+                                                  ListDev ld, IdlDev ilo, IdlDev iln, int32_t* newb) {
+  constexpr int ZCH = (NMAX + ZL - 1) / ZL;  // layer runs per column block
+  // wavefront tiles: the plan's column blocks (grains in spatial order), each
+  // with its ZCH runs next to each other
+  const int total = P.totals[kQCols * kClasses + cls] * ZCH;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: everything per tile is scalar
   const TileRange tr = xcd_tiles((total + 3) >> 2);  // workgroups take 4 consecutive wavefront tiles
   for (int bt = tr.first; bt < tr.end; bt += tr.step) {
     const int wt = bt * 4 + wv;
@@ -104,141 +243,18 @@ __global__ __launch_bounds__(256) void k_cmp_cols(SimParams sp, PlanDev P, int c
     int zc = wt - ct * ZCH;
     int g, cb;
     plan_tile(P, cls, kQCols, ct, g, cb);
-    const int32_t* b = din.bounds + 6 * g;
-    const int x0 = b[0], y0 = b[1], z0 = b[2], x1 = b[3], y1 = b[4];
-    const int n = x1 - x0, nn2 = n * n;
     if constexpr (ZMAJ) {
       // the grain's wavefront tiles in run-major order: consecutive tiles (the
       // wavefronts of one workgroup, then the next workgroups) take consecutive
       // column blocks of one run of layers, so they read the same neighbour rows
-      const int nbk = (nn2 + kColBlock - 1) / kColBlock;
+      const int32_t* b = din.bounds + 6 * g;
+      const int n = b[3] - b[0];
+      const int nbk = (n * n + kColBlock - 1) / kColBlock;
       const int lt = wt - (ct - cb) * ZCH;  // tile index inside the grain
       zc = lt / nbk;
       cb = lt - zc * nbk;
     }
-    const int qlo = zc * ZL;
-    if (qlo >= n) continue;
-    const int qn = min(ZL, n - qlo);  // layers of this run
-    const int zlo = z0 + qlo;         // its first global layer
-    const int col = cb * 64 + lane;
-    const bool act = col < nn2;
-    const int yl = act ? col / n : 0, xl = act ? col - yl * n : 0;
-    T best[ZL];
-    uint32_t bid[ZL];
-#pragma unroll
-    for (int q = 0; q < ZL; ++q) {
-      best[q] = (T)(-1.0);
-      bid[q] = 0xFFFFFFFFu;  // not covered
-    }
-    const int4* tab = ld.tab + (int64_t)g * maxL;
-    const int32_t* tid_of = ld.tab_id + (int64_t)g * maxL;
-    const int len = ld.tab_len[g];
-    for (int pb = 0; pb < len; pb += 64) {
-      const int idx = pb + lane;
-      bool keep = false;
-      int nb = 0, q2 = 0, q5 = 0;
-      int4 e = make_int4(0, 0, 0, 0);
-      if (idx < len) {
-        e = tab[idx];
-        nb = tid_of[idx];
-        q2 = e.w & 0xFFFF;
-        q5 = (int)((uint32_t)e.w >> 16);
-        keep = q2 < zlo + qn && q5 > zlo;  // the neighbour's box meets the run's layers
-      }
-      uint64_t m = __ballot(keep);
-      while (m) {
-        const int a = (int)__builtin_ctzll(m);  // next entry in list order
-        m &= m - 1;
-        const uint32_t rc = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)e.z, a);
-        const int rx0 = rc & 0xFF, rx1 = (rc >> 8) & 0xFF, ry0 = (rc >> 16) & 0xFF, ry1 = rc >> 24;
-        const bool cov = act & ((uint32_t)(xl - rx0) < (uint32_t)(rx1 - rx0)) & ((uint32_t)(yl - ry0) < (uint32_t)(ry1 - ry0));
-        if (__ballot(cov) == 0) continue;  // the rectangle misses this wavefront's columns
-        const uint32_t id = (uint32_t)__builtin_amdgcn_readlane(nb, a);
-        const uint32_t elo = (uint32_t)__builtin_amdgcn_readlane(e.x, a);
-        const uint32_t ehi = (uint32_t)__builtin_amdgcn_readlane(e.y, a);
-        const int nn = (int)(ehi >> 16);
-        const int64_t B = (int64_t)((uint64_t)(ehi & 0xFFFFu) << 32 | elo);  // rectangle's first voxel, layer q2
-        const int eq2 = __builtin_amdgcn_readlane(q2, a), eq5 = __builtin_amdgcn_readlane(q5, a);
-        const int qa = max(eq2, zlo) - zlo, qb = min(eq5, zlo + qn) - zlo;  // run layers the entry covers
-        const int ls = nn * nn;
-        // layer zlo + qa of the neighbour; element q of the run is at (q - qa) * ls + off from it
-        const T* __restrict__ src = ain + B + (int64_t)(zlo + qa - eq2) * ls;
-        const uint32_t off = cov ? (uint32_t)((yl - ry0) * nn + (xl - rx0)) : 0u;
-#pragma unroll
-        for (int c0 = 0; c0 < ZL; c0 += QB) {
-          if (c0 >= qb || c0 + QB <= qa) continue;
-          T v[QB];
-#pragma unroll
-          for (int u = 0; u < QB; ++u) {
-            const int qc = min(max(c0 + u, qa), qb - 1) - qa;
-            // 32-bit byte offset from a wavefront-uniform base
-            v[u] = *(const T*)((const char*)src + (uint32_t)((qc * ls + (int)off) * (int)sizeof(T)));
-          }
-#pragma unroll
-          for (int u = 0; u < QB; ++u) {
-            const int q = c0 + u;
-            const bool up = cov & (q >= qa) & (q < qb) & (v[u] > best[q]);
-            best[q] = up ? v[u] : best[q];
-            bid[q] = up ? id : bid[q];
-          }
-        }
-      }
-    }
-    // BoundaryIntersection on the output box (equal to the input box here)
-    const int32_t* ob = dout.bounds + 6 * g;
-    const int lo = sp.gb + sp.tube, hi = sp.ngp - sp.gb - sp.tube;
-    const bool wall = !(ob[0] > lo && ob[3] < hi && ob[1] > lo && ob[4] < hi && ob[2] > lo && ob[5] < hi);
-    const int32_t* olb = ilo.bounds + 6 * g;
-    const int64_t oldsize = (int64_t)(olb[3] - olb[0]) * (olb[4] - olb[1]) * (olb[5] - olb[2]);
-    const uint32_t* __restrict__ idl_old = ilo.arena + ilo.off[g];
-    uint32_t* __restrict__ idl_new = iln.arena + iln.off[g];
-    const T* own = ain + din.off[g];
-    T* dst = aout + dout.off[g];
-    const int i = y0 + yl, j = x0 + xl;
-    int kmin = INT_MAX, kmax = -INT_MAX;
-#pragma unroll
-    for (int q = 0; q < ZL; ++q) {
-      if (q >= qn || !act) continue;
-      const int k = zlo + q;
-      if (wall && !deep_inside(j, i, k, sp.gb, sp.ngp)) {
-        const double dist = bc_dist(j, i, k, x0, y0, z0, x1, y1, sp.gb, sp.ngp);
-        if (dist > -sp.gb) {
-          const T dh = (T)(dist * sp.h);
-          if (dh > best[q]) {
-            best[q] = dh;
-            bid[q] = 0;
-          }
-        }
-      }
-      const int64_t li = (int64_t)(qlo + q) * nn2 + col;
-      uint32_t idv = bid[q];
-      if (idv == 0xFFFFFFFFu) idv = li < oldsize ? idl_old[li] : 0u;
-      idl_new[li] = idv;
-      const T a_in = own[li];
-      const T o = (a_in < T(0) ? -a_in : a_in) < lim ? (T)0.5 * (a_in - best[q]) : a_in;
-      dst[li] = o;
-      if (o >= T(0)) {
-        kmin = min(kmin, k);
-        kmax = max(kmax, k);
-      }
-    }
-    // bounding box of the voxels >= 0: the lane's column spans (j, i) x [kmin, kmax]
-    const bool any = kmin != INT_MAX;
-    int vals[6] = {any ? j : INT_MAX, any ? i : INT_MAX, kmin, any ? -j : INT_MAX, any ? -i : INT_MAX, -kmax};
-    for (int q = 0; q < 6; ++q) {
-      int x = vals[q];
-      for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o));
-      vals[q] = x;
-    }
-    if (lane == 0 && vals[0] != INT_MAX) {
-      int32_t* nbx = newb + 6 * g;
-      atomicMin(&nbx[0], vals[0]);
-      atomicMin(&nbx[1], vals[1]);
-      atomicMin(&nbx[2], vals[2]);
-      atomicMax(&nbx[3], -vals[3]);
-      atomicMax(&nbx[4], -vals[4]);
-      atomicMax(&nbx[5], -vals[5]);
-    }
+    cmp_tile<T, ZL, QB>(sp, g, cb, zc, din, dout, ld, ilo, iln, newb);
   }
 }
 
@@ -294,6 +310,172 @@ __global__ __launch_bounds__(64) void k_cmp_table(SimParams sp, const int32_t* _
       out += __popcll(m);
     }
     if (lane == 0) ld.tab_len[g] = out;
+  }
+}
+
+constexpr int kColGridPerCUSpace = 64;  // workgroups per CU of the spatially ordered launch
+// ---- spatial work list ----------------------------------------------------
+// The wavefront tiles of all grains that read the same neighbour rows are put
+// next to each other: a tile's key is the cell of its centre (x: the middle of
+// the box, whose rows a 64-column block spans; y: its middle row; z: the middle
+// of its run).  The launch then walks the keys in order, each XCD a contiguous
+// range, so the ~7 grains whose boxes cover a voxel read its neighbour values
+// while they are in that XCD's L2, instead of once per grain from HBM.
+__device__ inline int space_key(const CmpSpaceDev& W, int x, int y, int z) {
+  const int cx = x >> 4, cy = y >> 3, cz = z >> 3;
+  const int brick = ((cz >> 3) * W.by + (cy >> 3)) * W.bx + (cx >> 3);
+  int m = 0;
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+    m |= ((cx >> b) & 1) << (3 * b) | ((cy >> b) & 1) << (3 * b + 1) | ((cz >> b) & 1) << (3 * b + 2);
+  return brick * 512 + m;
+}
+
+__global__ __launch_bounds__(256) void k_space_clear(CmpSpaceDev W) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= W.nkeys; i += gridDim.x * blockDim.x) W.cnt[i] = 0;
+}
+
+// One wavefront per planned grain, lanes over its tiles: count per key, or
+// (FILL, after the scan) take a slot in the key's bucket.  Slots inside a
+// bucket come in atomic order; the tiles write disjoint voxels and the bounds
+// atomics are minima and maxima, so the results do not depend on it.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_space_bin(SimParams sp, PlanDev P, DescDev din, CmpSpaceDev W, int zl,
+                                                   int32_t* err) {
+  const int lane = threadIdx.x & 63;
+  const int ng = P.cls_start[kClasses - 1] + P.cls_count[kClasses - 1];
+  const int lim = sp.ngp;
+  for (int i = blockIdx.x * 4 + (threadIdx.x >> 6); i < ng; i += gridDim.x * 4) {
+    const int g = P.grains[i];
+    const int32_t* b = din.bounds + 6 * g;
+    const int x0 = b[0], y0 = b[1], z0 = b[2], n = b[3] - b[0], nn2 = n * n;
+    const int nbk = (nn2 + kColBlock - 1) / kColBlock, nzc = (n + zl - 1) / zl;
+    const int xc = min(max(x0 + (n >> 1), 0), lim);
+    for (int t = lane; t < nbk * nzc; t += 64) {
+      const int zc = t / nbk, cb = t - zc * nbk;
+      const int c0 = cb * kColBlock, c1 = min(nn2, c0 + kColBlock) - 1;
+      const int yc = min(max(y0 + ((c0 / n + c1 / n) >> 1), 0), lim);
+      const int zq = min(zl, n - zc * zl);
+      const int zm = min(max(z0 + zc * zl + (zq >> 1), 0), lim);
+      const int key = space_key(W, xc, yc, zm);
+      if constexpr (!FILL) {
+        atomicAdd(&W.cnt[key], 1);
+      } else {
+        const int pos = atomicAdd(&W.cnt[key], 1);
+        if (pos < W.cap) W.wl[pos] = make_int2(g, cb | zc << 16);
+        else atomicOr(err, kErrCmpWork);
+      }
+    }
+  }
+}
+
+// Exclusive scan in place over a[0..n), a[n] = total: blocks of kScanBlock
+// elements (4 contiguous per thread), then the block sums, then their offsets.
+constexpr int kScanBlock = 4096;
+constexpr int kScanT = kScanBlock / 4;
+
+__device__ inline int block_excl_scan(int v, int* s_w, int& total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int inc = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(inc, o);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) s_w[wv] = inc;
+  __syncthreads();
+  int base = 0;
+  total = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+    if (w < wv) base += s_w[w];
+    total += s_w[w];
+  }
+  __syncthreads();  // s_w is reused by the next call
+  return base + inc - v;
+}
+
+__global__ __launch_bounds__(kScanT) void k_scan_blocks(int32_t* a, int n, int32_t* bsum) {
+  __shared__ int s_w[kScanT / 64];
+  const int i0 = blockIdx.x * kScanBlock + threadIdx.x * 4;
+  int v[4], sum = 0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    v[u] = i0 + u < n ? a[i0 + u] : 0;
+    sum += v[u];
+  }
+  int tot;
+  int run = block_excl_scan(sum, s_w, tot);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    if (i0 + u < n) a[i0 + u] = run;
+    run += v[u];
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanT) void k_scan_sums(int32_t* a, int n, int32_t* bsum, int nb) {
+  __shared__ int s_w[kScanT / 64];
+  int carry = 0;
+  for (int b0 = 0; b0 < nb; b0 += kScanT) {
+    const int b = b0 + (int)threadIdx.x;
+    const int v = b < nb ? bsum[b] : 0;
+    int tot;
+    const int ex = block_excl_scan(v, s_w, tot);
+    if (b < nb) bsum[b] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) a[n] = carry;
+}
+
+__global__ __launch_bounds__(kScanT) void k_scan_add(int32_t* a, int n, const int32_t* __restrict__ bsum) {
+  const int add = bsum[blockIdx.x];
+  const int i0 = blockIdx.x * kScanBlock + threadIdx.x * 4;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (i0 + u < n) a[i0 + u] += add;
+}
+
+template <class T, int ZL, int QB>
+__global__ __launch_bounds__(256) void k_cmp_space(SimParams sp, CmpSpaceDev W, DescDev din, DescDev dout,
+                                                   ListDev ld, IdlDev ilo, IdlDev iln, int32_t* newb) {
+  const int total = min(W.cnt[W.nkeys], W.cap);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const TileRange tr = xcd_tiles((total + 3) >> 2);
+  for (int bt = tr.first; bt < tr.end; bt += tr.step) {
+    const int wt = bt * 4 + wv;
+    if (wt >= total) continue;
+    const int2 e = W.wl[wt];
+    cmp_tile<T, ZL, QB>(sp, e.x, e.y & 0xFFFF, e.y >> 16, din, dout, ld, ilo, iln, newb);
+  }
+}
+
+constexpr int kSpaceShapes[kCmpSpaceShapes][2] = {{8, 8}, {16, 8}, {16, 16}, {32, 8}};
+
+template <class T, int S>
+void space_launch(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, const DescDev& dout,
+                  const ListDev& ld, const IdlDev& ilo, const IdlDev& iln, int32_t* newb, const CmpSpaceDev& W,
+                  int32_t* err) {
+  constexpr int ZL = kSpaceShapes[S][0], QB = kSpaceShapes[S][1];
+  const int gb = L.cus * 4;
+  hipLaunchKernelGGL(k_space_clear, dim3(gb), dim3(256), 0, L.stream, W);
+  hipLaunchKernelGGL(k_space_bin<false>, dim3(gb), dim3(256), 0, L.stream, sp, P, din, W, ZL, err);
+  const int nb = (W.nkeys + kScanBlock - 1) / kScanBlock;
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanT), 0, L.stream, W.cnt, W.nkeys, W.bsum);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanT), 0, L.stream, W.cnt, W.nkeys, W.bsum, nb);
+  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanT), 0, L.stream, W.cnt, W.nkeys, W.bsum);
+  hipLaunchKernelGGL(k_space_bin<true>, dim3(gb), dim3(256), 0, L.stream, sp, P, din, W, ZL, err);
+  hipLaunchKernelGGL((k_cmp_space<T, ZL, QB>), dim3(L.cus * kColGridPerCUSpace), dim3(256), 0, L.stream, sp, W,
+                     din, dout, ld, ilo, iln, newb);
+}
+
+template <class T>
+void space_shape(const Launch& L, int shape, const SimParams& sp, const PlanDev& P, const DescDev& din,
+                 const DescDev& dout, const ListDev& ld, const IdlDev& ilo, const IdlDev& iln, int32_t* newb,
+                 const CmpSpaceDev& W, int32_t* err) {
+  switch (shape) {
+    case 0: space_launch<T, 0>(L, sp, P, din, dout, ld, ilo, iln, newb, W, err); break;
+    case 1: space_launch<T, 1>(L, sp, P, din, dout, ld, ilo, iln, newb, W, err); break;
+    case 2: space_launch<T, 2>(L, sp, P, din, dout, ld, ilo, iln, newb, W, err); break;
+    default: space_launch<T, 3>(L, sp, P, din, dout, ld, ilo, iln, newb, W, err);
   }
 }
 
@@ -354,6 +536,13 @@ void launch_comparison(const Launch& L, int cls, int variant, const SimParams& s
                        const IdlDev& il_new, int32_t* newb, bool fp32) {
   if (fp32) comparison_class<float>(L, cls, variant, sp, P, din, dout, ld, il_old, il_new, newb);
   else comparison_class<double>(L, cls, variant, sp, P, din, dout, ld, il_old, il_new, newb);
+}
+
+void launch_comparison_space(const Launch& L, int shape, const SimParams& sp, const PlanDev& P, const DescDev& din,
+                             const DescDev& dout, const ListDev& ld, const IdlDev& il_old, const IdlDev& il_new,
+                             int32_t* newb, const CmpSpaceDev& W, int32_t* err, bool fp32) {
+  if (fp32) space_shape<float>(L, shape, sp, P, din, dout, ld, il_old, il_new, newb, W, err);
+  else space_shape<double>(L, shape, sp, P, din, dout, ld, il_old, il_new, newb, W, err);
 }
 
 }  // namespace gg

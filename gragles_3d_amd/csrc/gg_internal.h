@@ -46,7 +46,8 @@ enum {
   kErrEmptyIO = 16,   // redistancing with an empty in/out intersection
   kErrHistory = 32,   // count history full
   kErrTopo = 64,      // a grain's hull touches more grains than its topology table holds
-  kErrTri = 128       // the correction's triangle pool is full
+  kErrTri = 128,      // the correction's triangle pool is full
+  kErrCmpWork = 256   // the comparison's spatial work list is full
 };
 
 // Hull topology (optional output of the level set): per grain, the grains its
@@ -166,6 +167,18 @@ struct PlanDev {
   int32_t* tile_base;   // [kNumQ * kClasses] first entry of each class in tmap[q]
   uint32_t* tmap[kNumQ];  // [kMaxTilesPerGrain * N] tile -> grain << 8 | tile index inside the grain
 };
+// Spatial work list of the comparison: every wavefront tile of every planned
+// grain (column block, run of layers), bucketed by the cell of its centre.
+// Cells are 16 x 8 x 8 voxels (x, y, z); a key is the raster index of the
+// cell's brick of 8 x 8 x 8 cells times 512 plus the cell's Morton index
+// inside the brick, so consecutive keys are neighbouring cells.
+struct CmpSpaceDev {
+  int2* wl;      // [cap] (grain, column block | run << 16), in key order
+  int32_t* cnt;   // [nkeys + 1] counts, then bucket starts (scan), then ends (fill); [nkeys] = tiles
+  int32_t* bsum;  // [nkeys / 4096 + 1] block sums of the scan
+  int nkeys, cap, bx, by;  // bricks per axis: bx (x), by (y and z)
+};
+constexpr int kCmpSpaceShapes = 4;  // (layers per run, gathers in flight): (8,8) (16,8) (16,16) (32,8)
 constexpr int kCmpVariants = 8;  // comparison tile shapes the autotuner chooses from
 constexpr int kCmpDefault = 1;   // 16 layers, 8 gathers in flight
 constexpr int kMaxGrid = 32767;  // ngridpoints limit
@@ -358,6 +371,11 @@ void launch_comparison_table(const Launch& L, const SimParams& sp, const int32_t
 void launch_comparison(const Launch& L, int cls, int variant, const SimParams& sp, const PlanDev& P,
                        const DescDev& din, const DescDev& dout, const ListDev& ld, const IdlDev& il_old,
                        const IdlDev& il_new, int32_t* newb, bool fp32);
+// All classes in one launch, wavefront tiles in spatial order (the work list W
+// is rebuilt from the plan first); shape 0..kCmpSpaceShapes-1.
+void launch_comparison_space(const Launch& L, int shape, const SimParams& sp, const PlanDev& P, const DescDev& din,
+                             const DescDev& dout, const ListDev& ld, const IdlDev& il_old, const IdlDev& il_new,
+                             int32_t* newb, const CmpSpaceDev& W, int32_t* err, bool fp32);
 // Clears the grains' topology tables; before every hull with topology on.
 void launch_topo_clear(const Launch& L, const SimParams& sp, const TopoDev& topo);
 void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, int32_t* exflag,

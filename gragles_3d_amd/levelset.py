@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libgg_levelset.so")
 GG_OK, GG_ERR_ARG, GG_ERR_HIP, GG_ERR_CAPACITY, GG_ERR_STATE, GG_ERR_DEVICE = range(6)
 FP64_PLAIN, FP32_BATCHED = 0, 1
 ORDER_SPATIAL, ORDER_VOLUME = 0, 1
+CMP_ORDER_GRAIN, CMP_ORDER_SPACE = 0, 1
 CONV_DEFAULT, CONV_SPLIT_BF16 = 0, 1
 DELETED, ALIVE, VANISHED = 0, 1, 2
 # comparison tile shapes of gg_autotune: (layers per wavefront, gathers in flight, run-major order)
@@ -89,6 +90,8 @@ SIGNATURES = {
     "gg_set_work_order": (ctypes.c_int, [_vp, _i32]),
     "gg_set_convolution_form": (ctypes.c_int, [_vp, _i32]),
     "gg_get_work_order": (ctypes.c_int, [_vp, _i32p]),
+    "gg_set_comparison_order": (ctypes.c_int, [_vp, _i32, _i32]),
+    "gg_get_comparison_order": (ctypes.c_int, [_vp, _i32p, _i32p]),
     "gg_get_count_history": (ctypes.c_int, [_vp, _i32p, _i32, _i32p]),
     "gg_get_grain_stats": (ctypes.c_int, [_vp, _i32p, _f64p, _f64p]),
     "gg_get_bounds": (ctypes.c_int, [_vp, _i32, _i32, _i32p]),
@@ -571,6 +574,21 @@ class LevelSetContext:
         v = ctypes.c_int32(0)
         self._chk(lib().gg_get_work_order(self.h, ctypes.byref(v)))
         return v.value
+
+    def set_comparison_order(self, order, shape=None):
+        """CMP_ORDER_GRAIN (one launch per size class, a grain's tiles together)
+        or CMP_ORDER_SPACE (one launch, tiles sorted by the cell of their
+        centre, so grains that share neighbour voxels read them together);
+        shape 0..3 of the spatial launch (None: keep the current one)."""
+        if shape is None:
+            shape = self.comparison_order()[1]
+        self._chk(lib().gg_set_comparison_order(self.h, int(order), int(shape)))
+
+    def comparison_order(self):
+        """(order, shape of the spatial order's launch)."""
+        v, sh = ctypes.c_int32(0), ctypes.c_int32(0)
+        self._chk(lib().gg_get_comparison_order(self.h, ctypes.byref(v), ctypes.byref(sh)))
+        return v.value, sh.value
 
     def synchronize(self):
         self._chk(lib().gg_synchronize(self.h))

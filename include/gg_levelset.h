@@ -241,9 +241,28 @@ int gg_timestep(gg_ctx* ctx, int32_t nsteps, int32_t use_graph);
  * with a grain's tiles column-block-major, and 4..7 the same four run-major
  * (consecutive wavefronts take consecutive column blocks of one run of
  * layers); every shape gives the same result.  choice: 4 entries (may be
- * NULL); ms: 4 * GG_CMP_VARIANTS mean launch times (may be NULL). */
+ * NULL); ms: 4 * GG_CMP_VARIANTS mean launch times (may be NULL).  With the
+ * comparison in spatial order (gg_set_comparison_order) it times the four
+ * shapes 0..3 of that order's single launch instead (work-list build
+ * included) and keeps the fastest: every choice entry receives it, ms[0..3]
+ * the four times. */
 #define GG_CMP_VARIANTS 8
 int gg_autotune(gg_ctx* ctx, int32_t reps, int32_t* choice, float* ms);
+/* Order of the comparison's wavefront tiles (no counterpart in the reference,
+ * whose comparison_box runs grain by grain, src/grainhdl.cpp:654-667).
+ * GG_CMP_ORDER_GRAIN (default): one launch per size class, a grain's tiles
+ * together (grains in the work order).  GG_CMP_ORDER_SPACE: one launch for
+ * all classes whose tiles are sorted by the cell of their centre (16 x 8 x 8
+ * voxels, cells in brick-raster then Morton order; the list is rebuilt on the
+ * device at every comparison), so the tiles of the grains whose boxes cover a
+ * voxel run together and read its neighbour values from one L2; tile shape
+ * 0..3 as for gg_autotune (kept, but unused, in grain order).  Results do
+ * not depend on the order or the shape.  gg_get_comparison_order also
+ * returns the spatial order's shape (may be NULL). */
+#define GG_CMP_ORDER_GRAIN 0
+#define GG_CMP_ORDER_SPACE 1
+int gg_set_comparison_order(gg_ctx* ctx, int32_t order, int32_t shape);
+int gg_get_comparison_order(gg_ctx* ctx, int32_t* order, int32_t* shape);
 /* The shape per size class in use (4 entries), and setting it (for instance a
  * choice saved from an earlier gg_autotune); default 1 for every class. */
 int gg_get_tuning(gg_ctx* ctx, int32_t* variants);
