@@ -320,6 +320,94 @@ __device__ inline void fwd_step(float& cur, float& nxt, float a, float b, bool a
 // Boxes up to 32 put two layers (two tiles) in one wavefront, 32 lanes each,
 // so no lane idles for want of a row; what was uniform for a tile is then a
 // per-lane value of its half.
+//
+// The sweeps are unrolled over NS <= NMAX positions, NS the smallest of NMAX -
+// 12, - 8, - 4, NMAX that holds the wavefront's largest box side: positions
+// past the side are inactive in every lane (the loop ranges end at no - 1), so
+// a tile chooses its length once, by a uniform branch, and each length stays
+// straight-line code.
+struct XYTile {
+  const int32_t* ib;
+  const void* src;
+  void* dst;
+  int ni, ox0, oy0, no, iymin, iymax, ixmin, ixmax;
+  bool do_xy, tv;
+};
+
+template <class T, int NMAX, int NS, int LW>
+__device__ __forceinline__ void xy_sweeps(T* B, const XYTile& t, int xlo, int xim, int ylo, int yim, int nol, int ln,
+                                          T h) {
+  constexpr int S = NMAX + 1;
+  const T* src = (const T*)t.src;
+  const int no = t.no, ni = t.ni;
+  T o[NS], in[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    o[q] = T(-1);
+    __asm__ volatile("" : "+v"(o[q]));  // opaque: no control flow specialised on the initial value
+  }
+  // (the barriers sit outside the per-tile conditions: the two tiles of a
+  // wavefront may differ in them)
+  if (t.do_xy)
+    stage_rows_to_lds(B + ylo * S + xlo, S, src + (int64_t)(t.iymin - t.ib[1]) * ni + (t.ixmin - t.ib[0]), ni,
+                      t.iymax - t.iymin, t.ixmax - t.ixmin, ln, LW);
+  __syncthreads();
+  // x-forward then x-backward along row oy0 + ln, for rows in [iymin, oy1)
+  const int r = ln;
+  if (t.do_xy && r >= ylo && r < no) {
+    const bool inrow = r <= yim;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) in[q] = B[r * S + q];
+#pragma unroll
+    for (int q = 0; q < NS - 1; ++q)
+      fwd_step(o[q], o[q + 1], in[q], in[q + 1], (q >= xlo) & (q < nol), (q < xim) & inrow, h);
+#pragma unroll
+    for (int q = NS - 1; q >= 1; --q) {
+      const T cand = o[q] + ((T)sgnv(((q > xlo) & inrow) ? in[q - 1] : o[q - 1]) * h);
+      o[q - 1] = ((q <= xim) & (absv(cand) < absv(o[q - 1]))) ? cand : o[q - 1];
+    }
+  }
+  __syncthreads();
+  if (t.do_xy && r < no) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q) B[r * S + q] = o[q];
+  }
+  __syncthreads();
+  if (t.do_xy) {
+    // y-forward then y-backward along column ox0 + ln, for columns in [ixmin, ox1)
+    const int c = ln;
+    if (c < no) {
+#pragma unroll
+      for (int q = 0; q < NS; ++q) o[q] = B[q * S + c];
+    }
+    if (c >= xlo && c < no) {
+      const bool incol = c <= xim;
+      if (incol) {
+        const T* scol = src + (t.ox0 + c - t.ib[0]) + (int64_t)(t.oy0 - t.ib[1]) * ni;  // scol[q * ni] = in(oy0 + q, j)
+#pragma unroll
+        for (int q = 0; q < NS; ++q)
+          if (q >= ylo && q <= yim) in[q] = scol[(int64_t)q * ni];
+      }
+#pragma unroll
+      for (int q = 0; q < NS - 1; ++q)
+        fwd_step(o[q], o[q + 1], in[q], in[q + 1], (q >= ylo) & (q < nol), incol & (q < yim), h);
+#pragma unroll
+      for (int q = NS - 1; q >= 1; --q) {
+        const T cand = o[q] + ((T)sgnv((incol & (q > ylo)) ? in[q - 1] : o[q - 1]) * h);
+        o[q - 1] = ((q <= yim) & (absv(cand) < absv(o[q - 1]))) ? cand : o[q - 1];
+      }
+    }
+  }
+  // lane c writes column ox0 + c of the layer (coalesced across lanes)
+  T* dst = (T*)t.dst;
+  if (t.tv && ln < no) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+      if (q < no) dst[q * no + ln] = o[q];
+  }
+  __syncthreads();
+}
+
 template <class T, int NMAX>
 __global__ __launch_bounds__(64) void k_redist_xy_reg(SimParams sp, PlanDev P, int cls, DescDev din, DescDev dout,
                                                       int32_t* err) {
@@ -350,78 +438,21 @@ __global__ __launch_bounds__(64) void k_redist_xy_reg(SimParams sp, PlanDev P, i
     const bool empty = ixmin >= ixmax || iymin >= iymax || izmin >= izmax;
     if (tv && empty && ln == 0 && kl == 0) atomicOr(err, kErrEmptyIO);
     const bool do_xy = tv && !empty && k >= izmin && k < izmax - 1;
-    const T* src = ain + din.off[g] + (int64_t)(k - ib[2]) * ni * ni;
+    const XYTile t{ib, ain + din.off[g] + (int64_t)(k - ib[2]) * ni * ni, aout + dout.off[g] + (int64_t)kl * no * no,
+                   ni, ox0, oy0, no, iymin, iymax, ixmin, ixmax, do_xy, tv};
     // local (box) coordinates of the loop ranges
     // loop-range bounds as per-lane values: the predicates below then compare in
     // vector registers instead of building a scalar lane mask for every step
     int xlo = ixmin - ox0, xim = ixmax - 1 - ox0, ylo = iymin - oy0, yim = iymax - 1 - oy0, nol = no - 1;
     __asm__ volatile("" : "+v"(xlo), "+v"(xim), "+v"(ylo), "+v"(yim), "+v"(nol));
-    T o[NMAX], in[NMAX];
-#pragma unroll
-    for (int q = 0; q < NMAX; ++q) {
-      o[q] = T(-1);
-      __asm__ volatile("" : "+v"(o[q]));  // opaque: no control flow specialised on the initial value
-    }
-    // (the barriers sit outside the per-tile conditions: the two tiles of a
-    // wavefront may differ in them)
-    if (do_xy)
-      stage_rows_to_lds(B + ylo * S + xlo, S, src + (int64_t)(iymin - ib[1]) * ni + (ixmin - ib[0]), ni,
-                        iymax - iymin, ixmax - ixmin, ln, LW);
-    __syncthreads();
-    // x-forward then x-backward along row oy0 + ln, for rows in [iymin, oy1)
-    const int r = ln;
-    if (do_xy && r >= ylo && r < no) {
-      const bool inrow = r <= yim;
-#pragma unroll
-      for (int q = 0; q < NMAX; ++q) in[q] = B[r * S + q];
-#pragma unroll
-      for (int q = 0; q < NMAX - 1; ++q)
-        fwd_step(o[q], o[q + 1], in[q], in[q + 1], (q >= xlo) & (q < nol), (q < xim) & inrow, h);
-#pragma unroll
-      for (int q = NMAX - 1; q >= 1; --q) {
-        const T cand = o[q] + ((T)sgnv(((q > xlo) & inrow) ? in[q - 1] : o[q - 1]) * h);
-        o[q - 1] = ((q <= xim) & (absv(cand) < absv(o[q - 1]))) ? cand : o[q - 1];
-      }
-    }
-    __syncthreads();
-    if (do_xy && r < no) {
-#pragma unroll
-      for (int q = 0; q < NMAX; ++q) B[r * S + q] = o[q];
-    }
-    __syncthreads();
-    if (do_xy) {
-      // y-forward then y-backward along column ox0 + ln, for columns in [ixmin, ox1)
-      const int c = ln;
-      if (c < no) {
-#pragma unroll
-        for (int q = 0; q < NMAX; ++q) o[q] = B[q * S + c];
-      }
-      if (c >= xlo && c < no) {
-        const bool incol = c <= xim;
-        if (incol) {
-          const T* scol = src + (ox0 + c - ib[0]) + (int64_t)(oy0 - ib[1]) * ni;  // scol[q * ni] = in(oy0 + q, j)
-#pragma unroll
-          for (int q = 0; q < NMAX; ++q)
-            if (q >= ylo && q <= yim) in[q] = scol[(int64_t)q * ni];
-        }
-#pragma unroll
-        for (int q = 0; q < NMAX - 1; ++q)
-          fwd_step(o[q], o[q + 1], in[q], in[q + 1], (q >= ylo) & (q < nol), incol & (q < yim), h);
-#pragma unroll
-        for (int q = NMAX - 1; q >= 1; --q) {
-          const T cand = o[q] + ((T)sgnv((incol & (q > ylo)) ? in[q - 1] : o[q - 1]) * h);
-          o[q - 1] = ((q <= yim) & (absv(cand) < absv(o[q - 1]))) ? cand : o[q - 1];
-        }
-      }
-    }
-    // lane c writes column ox0 + c of the layer (coalesced across lanes)
-    T* dst = aout + dout.off[g] + (int64_t)kl * no * no;
-    if (tv && ln < no) {
-#pragma unroll
-      for (int q = 0; q < NMAX; ++q)
-        if (q < no) dst[q * no + ln] = o[q];
-    }
-    __syncthreads();
+    // the wavefront's largest box side picks the sweep length
+    int nw = tv ? no : 0;
+    if constexpr (PACK == 2) nw = max(nw, __shfl_xor(nw, 32));
+    nw = __builtin_amdgcn_readfirstlane(nw);
+    if (nw <= NMAX - 12) xy_sweeps<T, NMAX, NMAX - 12, LW>(B, t, xlo, xim, ylo, yim, nol, ln, h);
+    else if (nw <= NMAX - 8) xy_sweeps<T, NMAX, NMAX - 8, LW>(B, t, xlo, xim, ylo, yim, nol, ln, h);
+    else if (nw <= NMAX - 4) xy_sweeps<T, NMAX, NMAX - 4, LW>(B, t, xlo, xim, ylo, yim, nol, ln, h);
+    else xy_sweeps<T, NMAX, NMAX, LW>(B, t, xlo, xim, ylo, yim, nol, ln, h);
   }
 }
 
