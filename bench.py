@@ -182,9 +182,11 @@ def measured_traffic(kernel):
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch %s:" % kernel):
-                    return float(line.rsplit(":", 1)[1])
-    except OSError:
+                # a row of the summary table: | kernel | dispatches | ... | traffic GB/dispatch |
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) == 7 and cells[0] == kernel:
+                    return float(cells[6]) * 1e9
+    except (OSError, ValueError):
         pass
     return None
 
