@@ -1338,7 +1338,7 @@ static int alloc_topology(gg_ctx* c) {
   HIPCHK(c, hipMemset(c->topo.key, 0xFF, n * sizeof(int32_t)));
   HIPCHK(c, hipMemset(c->topo.jkey, 0, nj * sizeof(unsigned long long)));
   HIPCHK(c, hipMemset(c->topo.jcnt, 0, nj * sizeof(int32_t)));
-  HIPCHK(c, hipMemset(c->topo.jpos, 0, 3 * nj * sizeof(double)));
+  HIPCHK(c, hipMemset(c->topo.jpos, 0, 3 * nj * sizeof(unsigned long long)));
   HIPCHK(c, hipMemset(c->topo.jarea, 0, nj * sizeof(double)));
   return GG_OK;
 }
@@ -1630,7 +1630,7 @@ int gg_download_hull_elements(gg_ctx* c, int32_t id, int32_t* nids, int32_t* ids
   if (rc) return rc;
   std::vector<int32_t> key(kTopoCap), cnt(kJuncCap);
   std::vector<unsigned long long> jk(kJuncCap);
-  std::vector<double> jp(3 * kJuncCap);
+  std::vector<unsigned long long> jp(3 * kJuncCap);
   HIPCHK(c, hipMemcpy(key.data(), c->topo.key + (int64_t)id * kTopoCap, 4 * kTopoCap, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(jk.data(), c->topo.jkey + (int64_t)id * kJuncCap, 8 * kJuncCap, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(cnt.data(), c->topo.jcnt + (int64_t)id * kJuncCap, 4 * kJuncCap, hipMemcpyDeviceToHost));
@@ -1648,7 +1648,7 @@ int gg_download_hull_elements(gg_ctx* c, int32_t id, int32_t* nids, int32_t* ids
       if ((jk[s] >> b) & 1ull) r.ids.push_back((int64_t)(uint32_t)key[b]);
     std::sort(r.ids.begin(), r.ids.end());
     r.nt = cnt[s];
-    for (int k = 0; k < 3; ++k) r.p[k] = jp[3 * s + k] / cnt[s];
+    for (int k = 0; k < 3; ++k) r.p[k] = (double)(long long)jp[3 * s + k] / kPosScale / cnt[s];
     recs.push_back(r);
   }
   std::sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) { return a.ids < b.ids; });

@@ -39,8 +39,8 @@
 //                  its energy (src/InterfacialElement.h:22-31, 70-72).
 //
 // FP64 arithmetic throughout, in the reference's operation order; the junction
-// positions (used only for the merge test) are the hull's atomically summed
-// barycenters over their count.
+// positions (used only for the merge test) are the hull's barycenter sums
+// (fixed point, summed in any order to the same value) over their count.
 #include "gg_internal.h"
 
 namespace gg {
@@ -178,9 +178,9 @@ __global__ __launch_bounds__(kEW) void k_elements(SimParams sp, const int32_t* _
       s_en[r] = en;
       s_mo[r] = mo;
       s_cnt[r] = topo.jcnt[jb + slot];
-      s_sx[r] = topo.jpos[3 * (jb + slot)];
-      s_sy[r] = topo.jpos[3 * (jb + slot) + 1];
-      s_sz[r] = topo.jpos[3 * (jb + slot) + 2];
+      s_sx[r] = (double)(long long)topo.jpos[3 * (jb + slot)] / kPosScale;
+      s_sy[r] = (double)(long long)topo.jpos[3 * (jb + slot) + 1] / kPosScale;
+      s_sz[r] = (double)(long long)topo.jpos[3 * (jb + slot) + 2] / kPosScale;
       s_red[r] = -1;
     }
     __syncthreads();

@@ -116,6 +116,13 @@ __device__ inline void edge_vertex(int e, const T* L, int xl, int yl, int n, int
 // corners, the IDLocal id at the others), a per-tile LDS table keeps the first
 // appearance of each id, and the tile adds it to the grain's table.  A
 // separate instantiation, so the registers it needs cost nothing when off.
+// A cell's barycenter sum (non-negative grid coordinates) in fixed point:
+// integer adds make a list's position sum independent of the order in which
+// tiles and lanes add to it (exact to 2^-33 grid units per cell).
+__device__ inline unsigned long long fixpos(double v) {
+  return (unsigned long long)__double2ll_rn(v * kPosScale);
+}
+
 template <class T, int NMAX, bool TOPO>
 __global__ __launch_bounds__(kHT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1)))
 void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, double* partials, IdlDev il,
@@ -133,7 +140,7 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
   // and list; the FP64 n <= 96 class has no LDS left for them and adds each
   // cell's sums to the grain's table directly.
   constexpr bool kElemLds = TOPO && !(sizeof(T) == 8 && NMAX == 96);
-  __shared__ double s_jp[kElemLds ? 3 * kJuncLds : 1];
+  __shared__ unsigned long long s_jp[kElemLds ? 3 * kJuncLds : 1];  // fixed point, kPosScale per grid unit
   __shared__ double s_ja[kElemLds ? kJuncLds : 1];  // summed triangle areas
   __shared__ int s_jc[kElemLds ? kJuncLds : 1];
   __shared__ int s_gk[kTopoCap];                   // copy of the grain's id table
@@ -157,7 +164,7 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
       if constexpr (kElemLds) {
         s_jc[i] = 0;
         s_ja[i] = 0.0;
-        s_jp[3 * i] = s_jp[3 * i + 1] = s_jp[3 * i + 2] = 0.0;
+        s_jp[3 * i] = s_jp[3 * i + 1] = s_jp[3 * i + 2] = 0ull;
       }
     }
   }
@@ -355,16 +362,16 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
               // 3. interfacial elements (src/grainHull.cpp:227-270), per tile list
               double sx, sy, sz, sa;
               const int nt = cell_elems(code[r], sx, sy, sz, sa);
-              atomicAdd(&s_jp[3 * jsl], sx);
-              atomicAdd(&s_jp[3 * jsl + 1], sy);
-              atomicAdd(&s_jp[3 * jsl + 2], sz);
+              atomicAdd(&s_jp[3 * jsl], fixpos(sx));
+              atomicAdd(&s_jp[3 * jsl + 1], fixpos(sy));
+              atomicAdd(&s_jp[3 * jsl + 2], fixpos(sz));
               atomicAdd(&s_ja[jsl], sa);
               atomicAdd(&s_jc[jsl], nt);
             }
           }
           __syncthreads();
           unsigned long long* gj = topo.jkey + (int64_t)g * kJuncCap;
-          double* gp = topo.jpos + (int64_t)g * kJuncCap * 3;
+          unsigned long long* gp = topo.jpos + (int64_t)g * kJuncCap * 3;
           int32_t* gc = topo.jcnt + (int64_t)g * kJuncCap;
           double* ga = topo.jarea + (int64_t)g * kJuncCap;
           for (int i = tid; i < kJuncLds; i += kHT) {
@@ -391,7 +398,7 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
             if constexpr (kElemLds) {
               s_jc[i] = 0;
               s_ja[i] = 0.0;
-              s_jp[3 * i] = s_jp[3 * i + 1] = s_jp[3 * i + 2] = 0.0;
+              s_jp[3 * i] = s_jp[3 * i + 1] = s_jp[3 * i + 2] = 0ull;
             }
           }
           __syncthreads();
@@ -487,9 +494,9 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
             if (slot < 0) continue;
             double sx, sy, sz, sa;
             const int nt = cell_elems(code[r], sx, sy, sz, sa);
-            atomicAdd(&gp[3 * slot], sx);
-            atomicAdd(&gp[3 * slot + 1], sy);
-            atomicAdd(&gp[3 * slot + 2], sz);
+            atomicAdd(&gp[3 * slot], fixpos(sx));
+            atomicAdd(&gp[3 * slot + 1], fixpos(sy));
+            atomicAdd(&gp[3 * slot + 2], fixpos(sz));
             atomicAdd(&ga[slot], sa);
             atomicAdd(&gc[slot], nt);
           }
@@ -706,9 +713,9 @@ __global__ __launch_bounds__(256) void k_topo_clear(SimParams sp, TopoDev topo) 
     topo.jcnt[i] = 0;
     if (topo.corr) topo.jord[i] = INT_MAX;
     if (topo.corr && i == 0) *topo.raw_n = 0ull;
-    topo.jpos[3 * i] = 0.0;
-    topo.jpos[3 * i + 1] = 0.0;
-    topo.jpos[3 * i + 2] = 0.0;
+    topo.jpos[3 * i] = 0ull;
+    topo.jpos[3 * i + 1] = 0ull;
+    topo.jpos[3 * i + 2] = 0ull;
     topo.jarea[i] = 0.0;
   }
 }

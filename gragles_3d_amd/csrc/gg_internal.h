@@ -69,12 +69,14 @@ struct TriRec {
   double x, y, z;  // barycenter, grid coordinates
   int32_t g, slot;
 };
+constexpr double kPosScale = 4294967296.0;  // 2^32: fixed-point units of a junction position sum per grid unit
 struct TopoDev {
   int32_t* key;                // [(N+1) * kTopoCap]
   int32_t* ord;                // [(N+1) * kTopoCap]
   unsigned long long* jkey;    // [(N+1) * kJuncCap]
   int on;                      // 0: not computed
-  double* jpos;                // [(N+1) * kJuncCap * 3] barycenter sums of each list's triangles
+  unsigned long long* jpos;    // [(N+1) * kJuncCap * 3] barycenter sums of each list's triangles, fixed point
+                               // (kPosScale units per grid unit), so the sums do not depend on the order of the adds
   int32_t* jcnt;               // [(N+1) * kJuncCap] triangles of each list
   double* jarea;               // [(N+1) * kJuncCap] summed triangle areas of each list (grid units)
   int corr;                    // also the triangle records and first appearances below

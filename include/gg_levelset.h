@@ -163,9 +163,10 @@ int gg_voxel_container(gg_ctx* ctx, uint32_t* dst, int64_t cap, int32_t* size);
  * src/InterfacialElement.cpp:406-469): one record per distinct cell id list,
  * sorted by its ids: nids[r] ids ascending in ids8[8r..8r+7] (-1 padded, the
  * first 8 of longer lists), ntri[r] triangles, pos[3r..3r+2] the mean of their
- * barycenters in grid coordinates (summed in no fixed order: equal to the
- * serial sum to rounding).  Junctions are not merged (mergeJunction).  *n is
- * set first; nids may be NULL to query it. */
+ * barycenters in grid coordinates (each cell's barycenter sum rounded to 2^-32
+ * grid units and summed as integers, so the result does not depend on the
+ * order of the sums and is within 2^-33 of the serial sum).  Junctions are not
+ * merged (mergeJunction).  *n is set first; nids may be NULL to query it. */
 int gg_download_hull_elements(gg_ctx* ctx, int32_t id, int32_t* nids, int32_t* ids8, int32_t* ntri, double* pos,
                               int32_t cap, int32_t* n);
 

@@ -2,7 +2,8 @@
 oracle: the plain FP64 path bit for bit (distance buffers, IDLocal, lists,
 states after every stage or step), the interfacial elements grain by grain
 (class, bordering grains, triangle count, energy and mobility bit for bit;
-junction positions to rounding, as their barycenter sums are atomic), the
+junction positions to 1e-9 grid units, as their barycenter sums are fixed
+point), the
 graph-replayed step equal to the eager one, and the FP32 path within its
 tolerance.
 

@@ -87,7 +87,20 @@ def test_hull_topology_in_graph_steps():
     ctx.timestep(4, use_graph=True)
     orc.timestep(4)
     assert _compare(ctx, orc, "after 4 graph steps") > 0
+    # the element positions are summed in fixed point: eager steps give the
+    # same bits as graph steps, whatever order the tiles added in
+    ctx2, _, _ = make_pair(64, 10, 3, 7)
+    ctx2.enable_hull_topology(True)
+    ctx2.timestep(4, use_graph=False)
+    st, _, _ = ctx.stats()
+    for g in range(1, len(st)):
+        if st[g] == G.ALIVE:
+            a, b = ctx.hull_elements(g), ctx2.hull_elements(g)
+            assert len(a) == len(b)
+            for ea, eb in zip(a, b):
+                assert ea[0] == eb[0] and ea[1] == eb[1] and np.array_equal(ea[2], eb[2]), (g, ea, eb)
     ctx.close()
+    ctx2.close()
 
 
 def test_hull_topology_of_a_grid_coarsening():
