@@ -514,27 +514,13 @@ __global__ __launch_bounds__(256) void k_correct(SimParams sp, PlanDev P, int cl
       int emin = -1;
       const unsigned long long bit = (act && slot >= 0) ? 1ull << slot : 0ull;
       const float* __restrict__ cbx = cd.cbox + 6 * (tb / kChunk);
-      // first pass: bound = the smallest farthest-corner distance to a box of
-      // a chunk the lane needs (the nearest barycenter is no farther); a chunk
-      // whose box lies beyond it cannot hold the answer.  Box distances are
-      // FP32 and only decide which chunks to skip, so each is taken with a
-      // margin that covers its rounding (about 3 ulp): the skip is
+      // A chunk whose FP32 box lies beyond the lane's current pair cannot hold
+      // the answer.  Box distances only decide which chunks to skip, so each is
+      // taken with a margin that covers its rounding (about 3 ulp): the skip is
       // conservative, and every barycenter that can win is compared in FP64.
+      // (A first pass that bounded the search by the nearest box's farthest
+      // corner cost more than it saved.)
       const float fpx = (float)px, fpy = (float)py, fpz = (float)pz;  // integers: exact
-      float bound = 3e38f;
-      for (int e = E - 1; e >= 0; --e) {
-        const bool need = (s_em[e] & bit) != 0ull;
-        if (__ballot(need) == 0ull) continue;
-        const int ch1 = __builtin_amdgcn_readfirstlane(s_ef[e + 1]) / kChunk;
-        for (int ch = __builtin_amdgcn_readfirstlane(s_ef[e]) / kChunk; ch < ch1; ++ch) {
-          const float* bx = cbx + 6 * ch;
-          const float fx = fmaxf(fabsf(fpx - bx[0]), fabsf(fpx - bx[3]));
-          const float fy = fmaxf(fabsf(fpy - bx[1]), fabsf(fpy - bx[4]));
-          const float fz = fmaxf(fabsf(fpz - bx[2]), fabsf(fpz - bx[5]));
-          const float ub = (fx * fx + fy * fy + fz * fz) * (1.0f + 1e-6f) + 1e-6f;
-          if (need && bx[0] <= bx[3] && ub < bound) bound = ub;  // (a box with no barycenter is empty)
-        }
-      }
       for (int e = E - 1; e >= 0; --e) {
         const bool need = (s_em[e] & bit) != 0ull;
         if (__ballot(need) == 0ull) continue;
@@ -545,7 +531,7 @@ __global__ __launch_bounds__(256) void k_correct(SimParams sp, PlanDev P, int cl
           const float ey = fmaxf(fmaxf(bx[1] - fpy, fpy - bx[4]), 0.0f);
           const float ez = fmaxf(fmaxf(bx[2] - fpz, fpz - bx[5]), 0.0f);
           const float lb = (ex * ex + ey * ey + ez * ez) * (1.0f - 1e-6f) - 1e-6f;  // below the true one
-          const bool want = need && lb <= bound && (double)lb <= dmin;
+          const bool want = need && (double)lb <= dmin;
           if (__ballot(want) == 0ull) continue;
           const double* q = tri + 3 * kChunk * ch;
 #pragma unroll
