@@ -136,6 +136,7 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
   __shared__ int s_any[kHT / 64];
   __shared__ int s_tk[kTopoLds], s_to[kTopoLds];  // per-tile topology table (id, first appearance)
   __shared__ unsigned long long s_jm[kJuncLds];    // per-tile junction lists (slot masks)
+  __shared__ int s_jg[TOPO ? kJuncLds : 1];        // the grain-table slot each per-tile list went to
   // Element sums per tile list in LDS, added to the grain's table once per tile
   // and list; the FP64 n <= 96 class has no LDS left for them and adds each
   // cell's sums to the grain's table directly.
@@ -323,9 +324,11 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
           //    taken, so equal lists give equal masks); distinct masks per tile in
           //    LDS, then into the grain's junction table
           uint64_t cmask[kCPT];
+          int cjsl[kCPT];  // the per-tile list of each cut cell
 #pragma unroll
           for (int r = 0; r < kCPT; ++r) {
             cmask[r] = 0;
+            cjsl[r] = -1;
             if (code[r] == 0u) continue;
             uint32_t ids[8];
             bool first[8];
@@ -358,6 +361,7 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
               atomicOr(err, kErrTopo);
               continue;
             }
+            cjsl[r] = jsl;
             if constexpr (kElemLds) {
               // 3. interfacial elements (src/grainHull.cpp:227-270), per tile list
               double sx, sy, sz, sa;
@@ -384,6 +388,7 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
                 if (old == 0ull || old == m) gsl = (int)hsl;
                 hsl = (hsl + 1) & (kJuncCap - 1);
               }
+              s_jg[i] = gsl;
               if (gsl < 0) {
                 atomicOr(err, kErrTopo);
               } else if constexpr (kElemLds) {
@@ -393,6 +398,9 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
                 atomicAdd(&ga[gsl], s_ja[i]);
                 atomicAdd(&gc[gsl], s_jc[i]);
               }
+            }
+            else {
+              s_jg[i] = -1;
             }
             s_jm[i] = 0ull;
             if constexpr (kElemLds) {
@@ -411,19 +419,8 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
             int cnt = 0;
 #pragma unroll
             for (int r = 0; r < kCPT; ++r) {
-              rslot[r] = -1;
-              const unsigned long long m = cmask[r];
-              if (m == 0ull) continue;
-              uint32_t hsl = (uint32_t)((m * 0x9E3779B97F4A7C15ull) >> (64 - kJuncCapBits));
-              for (int pr = 0; pr < kJuncCap; ++pr) {
-                const unsigned long long k = __hip_atomic_load(&gj[hsl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (k == m) {
-                  rslot[r] = (int)hsl;
-                  break;
-                }
-                if (k == 0ull) break;  // not in the table (it was full: reported above)
-                hsl = (hsl + 1) & (kJuncCap - 1);
-              }
+              // the grain-table slot of the cell's list, as the merge above left it
+              rslot[r] = cjsl[r] >= 0 ? s_jg[cjsl[r]] : -1;
               if (rslot[r] < 0) continue;
               const int cell = (kl * m1 + (int)((code[r] >> 8) & 0xFF)) * m1 + (int)(code[r] >> 16);
               atomicMin(&topo.jord[(int64_t)g * kJuncCap + rslot[r]], cell);
@@ -478,19 +475,7 @@ void k_hull(SimParams sp, PlanDev P, int cls, DescDev din, int32_t* exflag, doub
 #pragma unroll
           for (int r = 0; r < kCPT; ++r) {
             if constexpr (kElemLds) break;
-            const unsigned long long m = cmask[r];
-            if (m == 0ull) continue;
-            uint32_t hsl = (uint32_t)((m * 0x9E3779B97F4A7C15ull) >> (64 - kJuncCapBits));
-            int slot = -1;
-            for (int pr = 0; pr < kJuncCap; ++pr) {
-              const unsigned long long k = __hip_atomic_load(&gj[hsl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              if (k == m) {
-                slot = (int)hsl;
-                break;
-              }
-              if (k == 0ull) break;  // not in the table (it was full: reported above)
-              hsl = (hsl + 1) & (kJuncCap - 1);
-            }
+            const int slot = cjsl[r] >= 0 ? s_jg[cjsl[r]] : -1;
             if (slot < 0) continue;
             double sx, sy, sz, sa;
             const int nt = cell_elems(code[r], sx, sy, sz, sa);
