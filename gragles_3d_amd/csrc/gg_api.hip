@@ -705,8 +705,14 @@ int gg_set_boxes(gg_ctx* c, const int32_t* bounds, const uint8_t* exists) {
     st[g] = kAlive;
     total += (int64_t)n * n * n;
   }
-  c->arena_cap = c->p.arena_voxels > 0 ? c->p.arena_voxels : 2 * total + 65536;
-  c->spec_cap = c->arena_cap;
+  if (c->arena[0] == nullptr) {
+    c->arena_cap = c->p.arena_voxels > 0 ? c->p.arena_voxels : 2 * total + 65536;
+    c->spec_cap = c->arena_cap;
+  } else if (total > c->arena_cap) {
+    // the arenas, the spectral scratch and the work lists keep the size of the
+    // first call
+    return fail(c, GG_ERR_CAPACITY, "boxes exceed the arena sized by the first gg_set_boxes");
+  }
   for (int r = 0; r < 2; ++r) {
     if (c->arena[r] == nullptr) {
       int rc;

@@ -83,7 +83,10 @@ const char* gg_last_error(const gg_ctx* ctx);
 /* Box bounds of every grain as the LSbox constructor computes them before
  * resizeToCube (src/box.cpp:185-221): rows 0..n_grains of 6 int32 (row 0
  * ignored), exists[id] != 0 for grains that have a box.  The library applies
- * resizeToCube(ngridpoints) to both buffers and sizes IDLocal to them. */
+ * resizeToCube(ngridpoints) to both buffers and sizes IDLocal to them.  The
+ * first call sizes the device arenas (params.arena_voxels, or twice the boxes'
+ * voxels plus 65536); a later call whose boxes need more voxels than that
+ * returns GG_ERR_CAPACITY. */
 int gg_set_boxes(gg_ctx* ctx, const int32_t* bounds, const uint8_t* exists);
 
 /* calculateDistanceFunction + executeRedistancing of every grain from the

@@ -179,3 +179,22 @@ def test_upload_phi_round_trip(precision):
             assert np.array_equal(ctx.phi(g, 1)[1], ctx2.phi(g, 1)[1]), g
     ctx.close()
     ctx2.close()
+
+
+def test_second_set_boxes_stays_within_the_first_arena():
+    # the first gg_set_boxes sizes the arenas; larger boxes later are refused
+    _say("set_boxes twice n=27 ppg=12 border=3 seed=11")
+    p = G.derive_params(27, 12, 3)
+    idf, bounds, exists = G.generate_voronoi(p, 11)
+    ctx = G.LevelSetContext(p, precision=G.FP64_PLAIN)
+    b = np.array(bounds, dtype=np.int32).reshape(-1, 6).copy()
+    small = b.copy()
+    small[1:, 3:] = small[1:, :3] + 4  # boxes of side 4
+    ctx.set_boxes(small.ravel(), exists)
+    ctx.set_boxes(small.ravel(), exists)  # the same again fits
+    big = b.copy()
+    big[1:, :3] = 0
+    big[1:, 3:] = min(p.ngridpoints, 96)  # every box the whole grid (sides up to 96)
+    with pytest.raises(G.GGError, match="exceed the arena"):
+        ctx.set_boxes(big.ravel(), exists)
+    ctx.close()
