@@ -753,8 +753,8 @@ def physics_leg(kind, grains, ppg, border, seed, steps, warmup, tuning, accuracy
 def uneven_leg(grains, ppg, border, seed, steps, warmup, accuracy_steps):
     """The uneven batch: 10 % of the grains four times larger, lognormal sizes (sigma 0.2)
     about each mode, power-diagram microstructure (gg_bimodal_seeds, gg_generate_power), FP32
-    batched.  The comparison tile shapes come from the autotuner (one timestep on a separate
-    context); the work order (device-side volume-sorted queue or spatial order) is chosen by
+    batched.  The comparison order and tile shapes come from the autotuner (one timestep on a
+    separate context per order); the work order (device-side volume-sorted queue or spatial order) is chosen by
     timing graph-replayed steps in each; then FP32 against the FP64 path over accuracy_steps."""
     p = G.derive_params(grains, ppg, border)
     t = time.perf_counter()
@@ -776,14 +776,15 @@ def uneven_leg(grains, ppg, border, seed, steps, warmup, accuracy_steps):
             "box_side_median_small_large": [float(np.median(side[~big[exists[1:] == 1]])),
                                             float(np.median(side[big[exists[1:] == 1]]))],
             "grains_per_class": np.bincount(np.searchsorted(CLASS_NMAX, side), minlength=4).tolist()}
-    tctx = new_context(G.FP32_BATCHED, inputs, p)
-    choice, ms = tctx.autotune(reps=2)
-    tctx.close()
-    info["comparison_shapes_per_class"] = choice.tolist()
+    choice, corder, cshape, crep = tune_comparison(G.FP32_BATCHED, inputs, p, 2)
+    info["comparison_shapes_per_class"] = choice
+    info["comparison_order"] = crep["order"]
+    info["comparison_ms_grain_spatial"] = [crep["grain_order_ms"], crep["spatial_order_ms"]]
     timing = {}
     for order, name in ((G.ORDER_VOLUME, "volume-sorted"), (G.ORDER_SPATIAL, "spatial")):
         c = new_context(G.FP32_BATCHED, inputs, p)
         c.set_tuning(choice)
+        c.set_comparison_order(corder, cshape)
         c.set_work_order(order)
         wall, vox = timed_steps(c, steps, warmup)
         timing[name] = (wall, vox)
@@ -801,6 +802,7 @@ def uneven_leg(grains, ppg, border, seed, steps, warmup, accuracy_steps):
         a64 = new_context(G.FP64_PLAIN, inputs, p)
         for c in (a32, a64):
             c.set_tuning(choice)
+            c.set_comparison_order(corder, cshape)
             c.set_work_order(G.ORDER_VOLUME if best == "volume-sorted" else G.ORDER_SPATIAL)
         tr = trajectories(a32, a64, accuracy_steps, every=10)
         e = phi_volume_errors(a32, a64, p.delta, grains=np.arange(1, grains + 1, 50))
