@@ -461,7 +461,8 @@ def main():
         avg_s = ms / max(nl, 1) * 1e-3
         bts = cls_bytes.sum() / ks
         zl, qb = CMP_SPACE_SHAPES[shape32]
-        cls_rows.append({"kernel": "k_cmp_space<float, %d, %d>" % (zl, qb), "box_voxels": int(cls_vox.sum() / ks),
+        b16 = "true" if args.grains < 0xFFFF else "false"  # winner ids packed as 16-bit halves below 0xFFFF grains
+        cls_rows.append({"kernel": "k_cmp_space<float, %d, %d, %s>" % (zl, qb, b16), "box_voxels": int(cls_vox.sum() / ks),
                          "kbar": cls_num.sum() / cls_vox.sum() if cls_vox.sum() else 0.0, "bytes_per_launch": bts,
                          "avg_launch_ms": avg_s * 1e3,
                          "achieved_GBs": bts / avg_s / 1e9 if avg_s > 0 and bts > 0 else 0.0,
