@@ -3,7 +3,8 @@ all size classes whose wavefront tiles are sorted by the cell of their centre,
 with the work list rebuilt on the device at every comparison.  It computes the
 same voxels as the per-class launches, so FP64 stays bit for bit equal to the
 oracle (every shape, stage by stage, graph replays, vanishing grains) and FP32
-bit for bit equal to the grain order.
+bit for bit equal to the grain order, with winner ids packed (fewer than 0xFFFF
+grains) and unpacked.
 
 Each case prints its name, seed and shape before touching the GPU.
 """
@@ -53,7 +54,9 @@ def test_many_steps_with_vanishing_fp64():
     ctx.close()
 
 
-@pytest.mark.parametrize("case", [(64, 10, 3, 5), CASE, (1000, 12, 3, 2), (8000, 20, 3, 2)])
+# 70000 grains: ids above 0xFFFF, so the spatial kernel keeps its winner ids
+# unpacked (below that it packs them as 16-bit halves)
+@pytest.mark.parametrize("case", [(64, 10, 3, 5), CASE, (1000, 12, 3, 2), (8000, 20, 3, 2), (70000, 6, 3, 2)])
 def test_fp32_equals_grain_order(case):
     print("spatial vs grain order FP32 n=%d ppg=%d border=%d seed=%d, 3 steps" % case, flush=True)
     res = []
