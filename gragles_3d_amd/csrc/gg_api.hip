@@ -69,6 +69,7 @@ struct gg_ctx {
   int cmp_variant[kClasses] = {kCmpDefault, kCmpDefault, kCmpDefault, kCmpDefault};  // comparison tile shape per class
   int cmp_order = GG_CMP_ORDER_GRAIN;  // comparison tiles per class in grain order, or all in spatial order
   int cmp_space_shape = 0;             // tile shape of the spatially ordered launch
+  bool cmp_prebuilt = false;           // neighbour table and spatial work list already enqueued (side stream)
   CmpSpaceDev W{};                     // its work list (allocated with the arenas)
   int64_t* d_spec_off = nullptr;
   void* d_spec = nullptr;
@@ -373,9 +374,11 @@ void stage_comparison(gg_ctx* c) {
   static const int seg[kClasses] = {kSegCompare, kSegCompare48, kSegCompare64, kSegCompare96};
   if (c->cmp_order == GG_CMP_ORDER_SPACE) {
     SegTimer t(c, kSegCompare);
-    launch_comparison_table(launch_of(c), c->sp, c->d_state, din(c), c->LD);
+    if (!c->cmp_prebuilt) launch_comparison_table(launch_of(c), c->sp, c->d_state, din(c), c->LD);
     launch_comparison_space(launch_of(c), c->cmp_space_shape, c->sp, c->P, din(c), dout(c), c->LD,
-                            c->IL[c->il_cur], c->IL[c->il_cur ^ 1], c->d_newb, c->W, c->d_err, c->fp32);
+                            c->IL[c->il_cur], c->IL[c->il_cur ^ 1], c->d_newb, c->W, c->d_err, c->fp32,
+                            c->cmp_prebuilt ? kSpaceRun : kSpaceAll);
+    c->cmp_prebuilt = false;
   } else if (c->timing) {
     for (int cls = 0; cls < kClasses; ++cls) {
       SegTimer t(c, seg[cls]);
@@ -526,11 +529,20 @@ void enqueue_step(gg_ctx* c) {
     // writes the other list buffer and the set-comparison start bounds; the
     // convolution reads and writes distance values and IDLocal bounds only. So
     // the lists go on the side stream, forked from and joined back to the main
-    // stream (inside a captured graph too), beside the convolution.
+    // stream (inside a captured graph too), beside the convolution.  In spatial
+    // order the comparison's neighbour table (from the new lists) and its work
+    // list (from the plan and the input bounds, which the convolution leaves
+    // alone) follow them there.
     hipEventRecord(c->ev_fork, c->stream);
     hipStreamWaitEvent(c->side, c->ev_fork, 0);
-    launch_lists(Launch{c->side, c->grid, c->grid / 8}, c->sp, c->d_state, dout(c), din(c), c->LD, c->d_newb,
-                 c->d_err);  // the roles after the switch below
+    const Launch S{c->side, c->grid, c->grid / 8};
+    launch_lists(S, c->sp, c->d_state, dout(c), din(c), c->LD, c->d_newb, c->d_err);  // the roles after the switch below
+    if (c->cmp_order == GG_CMP_ORDER_SPACE) {
+      launch_comparison_table(S, c->sp, c->d_state, dout(c), c->LD);
+      launch_comparison_space(S, c->cmp_space_shape, c->sp, c->P, dout(c), din(c), c->LD, c->IL[c->il_cur],
+                              c->IL[c->il_cur ^ 1], c->d_newb, c->W, c->d_err, c->fp32, kSpaceBuild);
+      c->cmp_prebuilt = true;
+    }
     hipEventRecord(c->ev_join, c->side);
     stage_convolution(c);
     c->sel ^= 1;

@@ -474,16 +474,19 @@ constexpr int kSpaceShapes[kCmpSpaceShapes][2] = {{8, 8}, {16, 8}, {16, 16}, {32
 template <class T, int S>
 void space_launch(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, const DescDev& dout,
                   const ListDev& ld, const IdlDev& ilo, const IdlDev& iln, int32_t* newb, const CmpSpaceDev& W,
-                  int32_t* err) {
+                  int32_t* err, int part) {
   constexpr int ZL = kSpaceShapes[S][0], QB = kSpaceShapes[S][1];
   const int gb = L.cus * 4;
-  hipLaunchKernelGGL(k_space_clear, dim3(gb), dim3(256), 0, L.stream, W);
-  hipLaunchKernelGGL(k_space_bin<false>, dim3(gb), dim3(256), 0, L.stream, sp, P, din, W, ZL, err);
-  const int nb = (W.nkeys + kScanBlock - 1) / kScanBlock;
-  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanT), 0, L.stream, W.cnt, W.nkeys, W.bsum);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanT), 0, L.stream, W.cnt, W.nkeys, W.bsum, nb);
-  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanT), 0, L.stream, W.cnt, W.nkeys, W.bsum);
-  hipLaunchKernelGGL(k_space_bin<true>, dim3(gb), dim3(256), 0, L.stream, sp, P, din, W, ZL, err);
+  if (part != kSpaceRun) {
+    hipLaunchKernelGGL(k_space_clear, dim3(gb), dim3(256), 0, L.stream, W);
+    hipLaunchKernelGGL(k_space_bin<false>, dim3(gb), dim3(256), 0, L.stream, sp, P, din, W, ZL, err);
+    const int nb = (W.nkeys + kScanBlock - 1) / kScanBlock;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanT), 0, L.stream, W.cnt, W.nkeys, W.bsum);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanT), 0, L.stream, W.cnt, W.nkeys, W.bsum, nb);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanT), 0, L.stream, W.cnt, W.nkeys, W.bsum);
+    hipLaunchKernelGGL(k_space_bin<true>, dim3(gb), dim3(256), 0, L.stream, sp, P, din, W, ZL, err);
+  }
+  if (part == kSpaceBuild) return;
   if (sp.N < 0xFFFF)
     hipLaunchKernelGGL((k_cmp_space<T, ZL, QB, true>), dim3(L.cus * kColGridPerCUSpace), dim3(256), 0, L.stream, sp,
                        W, din, dout, ld, ilo, iln, newb);
@@ -495,12 +498,12 @@ void space_launch(const Launch& L, const SimParams& sp, const PlanDev& P, const 
 template <class T>
 void space_shape(const Launch& L, int shape, const SimParams& sp, const PlanDev& P, const DescDev& din,
                  const DescDev& dout, const ListDev& ld, const IdlDev& ilo, const IdlDev& iln, int32_t* newb,
-                 const CmpSpaceDev& W, int32_t* err) {
+                 const CmpSpaceDev& W, int32_t* err, int part) {
   switch (shape) {
-    case 0: space_launch<T, 0>(L, sp, P, din, dout, ld, ilo, iln, newb, W, err); break;
-    case 1: space_launch<T, 1>(L, sp, P, din, dout, ld, ilo, iln, newb, W, err); break;
-    case 2: space_launch<T, 2>(L, sp, P, din, dout, ld, ilo, iln, newb, W, err); break;
-    default: space_launch<T, 3>(L, sp, P, din, dout, ld, ilo, iln, newb, W, err);
+    case 0: space_launch<T, 0>(L, sp, P, din, dout, ld, ilo, iln, newb, W, err, part); break;
+    case 1: space_launch<T, 1>(L, sp, P, din, dout, ld, ilo, iln, newb, W, err, part); break;
+    case 2: space_launch<T, 2>(L, sp, P, din, dout, ld, ilo, iln, newb, W, err, part); break;
+    default: space_launch<T, 3>(L, sp, P, din, dout, ld, ilo, iln, newb, W, err, part);
   }
 }
 
@@ -565,9 +568,9 @@ void launch_comparison(const Launch& L, int cls, int variant, const SimParams& s
 
 void launch_comparison_space(const Launch& L, int shape, const SimParams& sp, const PlanDev& P, const DescDev& din,
                              const DescDev& dout, const ListDev& ld, const IdlDev& il_old, const IdlDev& il_new,
-                             int32_t* newb, const CmpSpaceDev& W, int32_t* err, bool fp32) {
-  if (fp32) space_shape<float>(L, shape, sp, P, din, dout, ld, il_old, il_new, newb, W, err);
-  else space_shape<double>(L, shape, sp, P, din, dout, ld, il_old, il_new, newb, W, err);
+                             int32_t* newb, const CmpSpaceDev& W, int32_t* err, bool fp32, int part) {
+  if (fp32) space_shape<float>(L, shape, sp, P, din, dout, ld, il_old, il_new, newb, W, err, part);
+  else space_shape<double>(L, shape, sp, P, din, dout, ld, il_old, il_new, newb, W, err, part);
 }
 
 }  // namespace gg

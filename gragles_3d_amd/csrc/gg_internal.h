@@ -374,10 +374,13 @@ void launch_comparison(const Launch& L, int cls, int variant, const SimParams& s
                        const DescDev& din, const DescDev& dout, const ListDev& ld, const IdlDev& il_old,
                        const IdlDev& il_new, int32_t* newb, bool fp32);
 // All classes in one launch, wavefront tiles in spatial order (the work list W
-// is rebuilt from the plan first); shape 0..kCmpSpaceShapes-1.
+// is rebuilt from the plan first); shape 0..kCmpSpaceShapes-1.  part: both
+// steps, only the work-list build (it reads the plan and din's bounds), or
+// only the comparison over a work list built for the same shape.
+enum { kSpaceAll = 0, kSpaceBuild = 1, kSpaceRun = 2 };
 void launch_comparison_space(const Launch& L, int shape, const SimParams& sp, const PlanDev& P, const DescDev& din,
                              const DescDev& dout, const ListDev& ld, const IdlDev& il_old, const IdlDev& il_new,
-                             int32_t* newb, const CmpSpaceDev& W, int32_t* err, bool fp32);
+                             int32_t* newb, const CmpSpaceDev& W, int32_t* err, bool fp32, int part = kSpaceAll);
 // Clears the grains' topology tables; before every hull with topology on.
 void launch_topo_clear(const Launch& L, const SimParams& sp, const TopoDev& topo);
 void launch_hull(const Launch& L, const SimParams& sp, const PlanDev& P, const DescDev& din, int32_t* exflag,
