@@ -126,6 +126,36 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_vorder(SimParams sp, cons
   }
 }
 
+// Exclusive scan of K values per thread over the workgroup, in place, all K
+// in one pass (one barrier).  sw[w][k] keeps wave w's sum of value k, so the
+// totals can be read from it afterwards (plan_total).
+template <int K>
+__device__ inline void block_scan_excl_multi(int (&x)[K], int (*sw)[K]) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    int s = x[k];
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(s, off);
+      if (lane >= off) s += y;
+    }
+    if (lane == 63) sw[wv][k] = s;
+    x[k] = s - x[k];
+  }
+  __syncthreads();
+  for (int w = 0; w < wv; ++w) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) x[k] += sw[w][k];
+  }
+}
+
+template <int K>
+__device__ inline int plan_total(int (*sw)[K], int k) {
+  int t = 0;
+  for (int w = 0; w < kPlanThreads / 64; ++w) t += sw[w][k];
+  return t;
+}
+
 __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32_t* __restrict__ state,
                                                        const int32_t* __restrict__ bounds, int64_t* off_out,
                                                        int64_t* spec_off, PlanDev P, int64_t arena_cap,
@@ -136,79 +166,115 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(SimParams sp, const int32
   const int gb = 1 + t * chunk;
   const int ge = min(N + 1, gb + chunk);
 
-  // quantities per thread: [0..3] counts, [4..4+4*kNumQ) tile sums (q*4+c), then voxels and spectrum
-  constexpr int kVox = 4 + kNumQ * kClasses, kSpec = kVox + 1;  // loc/ex/tot slots of voxels, spectrum
-  constexpr int NQ = kSpec + 1;
+  // per thread: [0..4) grains per class, [4 + q * 4 + c] tiles of quantity q in
+  // class c (32-bit, as the plan stores them), then voxels and spectrum (64-bit).
+  // A grain's class selects its slots by unrolled compares, never as an index,
+  // so the counters stay in registers.
+  constexpr int K = 4 + kNumQ * kClasses;
   __shared__ int64_t sw[kPlanThreads / 64];
-  int64_t loc[NQ];
-  for (int q = 0; q < NQ; ++q) loc[q] = 0;
+  __shared__ int swk[kPlanThreads / 64][K];
+  __shared__ int s_tot[K];
+  int loc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) loc[k] = 0;
+  int64_t lvox = 0, lspec = 0;
   // grains are visited in the spatial order, so class lists, tiles and the
   // arena layout keep neighbouring boxes close together
   for (int idx = gb; idx < ge; ++idx) {
     const int v = P.ncls[idx - 1];
     if (v < 0) continue;
     const int n = v >> 8, c = v & 0xFF;
-    loc[c] += 1;
-    loc[4 + kQLayers * kClasses + c] += n;
-    loc[4 + kQCols * kClasses + c] += ceil_div((int64_t)n * n, kColBlock);
-    loc[4 + kQSpecCols * kClasses + c] += ceil_div((int64_t)n * (n / 2 + 1), kSpecBlock);
-    loc[kVox] += (int64_t)n * n * n;
-    loc[kSpec] += (int64_t)n * n * (n / 2 + 1);
+    const int tc = (int)ceil_div((int64_t)n * n, kColBlock), ts = (int)ceil_div((int64_t)n * (n / 2 + 1), kSpecBlock);
+#pragma unroll
+    for (int cc = 0; cc < kClasses; ++cc) {
+      const bool h = c == cc;
+      loc[cc] += h ? 1 : 0;
+      loc[4 + kQLayers * kClasses + cc] += h ? n : 0;
+      loc[4 + kQCols * kClasses + cc] += h ? tc : 0;
+      loc[4 + kQSpecCols * kClasses + cc] += h ? ts : 0;
+    }
+    lvox += (int64_t)n * n * n;
+    lspec += (int64_t)n * n * (n / 2 + 1);
   }
-  int64_t ex[NQ], tot[NQ];
-  for (int q = 0; q < NQ; ++q) ex[q] = block_scan_excl(loc[q], sw, &tot[q]);
+  int* const ex = loc;  // becomes the exclusive prefixes
+  block_scan_excl_multi<K>(loc, swk);
+  if (t < K) s_tot[t] = plan_total<K>(swk, t);  // published by the barriers of the scans below
+  int64_t tvox, tspec;
+  int64_t exv = block_scan_excl(lvox, sw, &tvox);
+  int64_t exs = block_scan_excl(lspec, sw, &tspec);
 
-  int64_t cls_start[kClasses];
+  int cls_start[kClasses];
   {
-    int64_t acc = 0;
-    for (int c = 0; c < kClasses; ++c) { cls_start[c] = acc; acc += tot[c]; }
+    int acc = 0;
+#pragma unroll
+    for (int c = 0; c < kClasses; ++c) {
+      cls_start[c] = acc;
+      acc += s_tot[c];
+    }
   }
   // A layout that does not fit its arena leaves an empty plan, so no kernel of
   // this step touches memory beyond the arenas; the host reports the error bit.
-  const bool overflow = (off_out && tot[kVox] > arena_cap) || (spec_off && tot[kSpec] > spec_cap);
+  const bool overflow = (off_out && tvox > arena_cap) || (spec_off && tspec > spec_cap);
   if (t == 0) {
+    const int* tot = s_tot;
     int64_t lb = 0;
+#pragma unroll
     for (int c = 0; c < kClasses; ++c) {
-      P.cls_count[c] = overflow ? 0 : (int32_t)tot[c];
-      P.cls_start[c] = (int32_t)cls_start[c];
-      for (int q = 0; q < kNumQ; ++q)
-        P.totals[q * kClasses + c] = overflow ? 0 : (int32_t)tot[4 + q * kClasses + c];
+      P.cls_count[c] = overflow ? 0 : tot[c];
+      P.cls_start[c] = cls_start[c];
+#pragma unroll
+      for (int q = 0; q < kNumQ; ++q) P.totals[q * kClasses + c] = overflow ? 0 : tot[4 + q * kClasses + c];
       P.layer_base[c] = lb;
       lb += tot[4 + kQLayers * kClasses + c];
-      for (int q = 0; q < kNumQ; ++q)
-        P.pre[q][cls_start[c] + c + tot[c]] = (int32_t)tot[4 + q * kClasses + c];
+#pragma unroll
+      for (int q = 0; q < kNumQ; ++q) P.pre[q][cls_start[c] + c + tot[c]] = tot[4 + q * kClasses + c];
     }
+#pragma unroll
     for (int q = 0; q < kNumQ; ++q) {
-      int64_t tb = 0;
+      int tb = 0;
+#pragma unroll
       for (int c = 0; c < kClasses; ++c) {
-        P.tile_base[q * kClasses + c] = (int32_t)tb;
+        P.tile_base[q * kClasses + c] = tb;
         tb += overflow ? 0 : tot[4 + q * kClasses + c];
       }
     }
-    if (voxels) *voxels = tot[kVox];
-    if (off_out && tot[kVox] > arena_cap) atomicOr(err, kErrArena);
-    if (spec_off && tot[kSpec] > spec_cap) atomicOr(err, kErrSpec);
+    if (voxels) *voxels = tvox;
+    if (off_out && tvox > arena_cap) atomicOr(err, kErrArena);
+    if (spec_off && tspec > spec_cap) atomicOr(err, kErrSpec);
   }
   for (int idx = gb; idx < ge; ++idx) {
     const int v = P.ncls[idx - 1];
     if (v < 0) continue;
     const int n = v >> 8, c = v & 0xFF;
     const int g = P.order[idx - 1];
-    int64_t pos = cls_start[c] + ex[c];
+    const int tc = (int)ceil_div((int64_t)n * n, kColBlock), ts = (int)ceil_div((int64_t)n * (n / 2 + 1), kSpecBlock);
+    int pos = 0, el = 0, ec = 0, es = 0;
+#pragma unroll
+    for (int cc = 0; cc < kClasses; ++cc) {
+      const bool h = c == cc;
+      pos = h ? cls_start[cc] + ex[cc] : pos;
+      el = h ? ex[4 + kQLayers * kClasses + cc] : el;
+      ec = h ? ex[4 + kQCols * kClasses + cc] : ec;
+      es = h ? ex[4 + kQSpecCols * kClasses + cc] : es;
+    }
     P.grains[pos] = g;
-    P.grain_pos[g] = (int32_t)pos;
-    int64_t seg = cls_start[c] + c + ex[c];
-    P.pre[kQLayers][seg] = (int32_t)ex[4 + kQLayers * kClasses + c];
-    P.pre[kQCols][seg] = (int32_t)ex[4 + kQCols * kClasses + c];
-    P.pre[kQSpecCols][seg] = (int32_t)ex[4 + kQSpecCols * kClasses + c];
-    ex[c] += 1;
-    ex[4 + kQLayers * kClasses + c] += n;
-    ex[4 + kQCols * kClasses + c] += ceil_div((int64_t)n * n, kColBlock);
-    ex[4 + kQSpecCols * kClasses + c] += ceil_div((int64_t)n * (n / 2 + 1), kSpecBlock);
-    if (off_out) off_out[g] = ex[kVox];
-    if (spec_off) spec_off[g] = ex[kSpec];
-    ex[kVox] += (int64_t)n * n * n;
-    ex[kSpec] += (int64_t)n * n * (n / 2 + 1);
+    P.grain_pos[g] = pos;
+    const int seg = pos + c;  // class c's segment of the tile prefixes has one extra slot per lower class
+    P.pre[kQLayers][seg] = el;
+    P.pre[kQCols][seg] = ec;
+    P.pre[kQSpecCols][seg] = es;
+#pragma unroll
+    for (int cc = 0; cc < kClasses; ++cc) {
+      const bool h = c == cc;
+      ex[cc] += h ? 1 : 0;
+      ex[4 + kQLayers * kClasses + cc] += h ? n : 0;
+      ex[4 + kQCols * kClasses + cc] += h ? tc : 0;
+      ex[4 + kQSpecCols * kClasses + cc] += h ? ts : 0;
+    }
+    if (off_out) off_out[g] = exv;
+    if (spec_off) spec_off[g] = exs;
+    exv += (int64_t)n * n * n;
+    exs += (int64_t)n * n * (n / 2 + 1);
   }
 }
 
