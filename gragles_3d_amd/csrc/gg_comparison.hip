@@ -336,7 +336,11 @@ __global__ __launch_bounds__(64) void k_cmp_table(SimParams sp, const int32_t* _
   }
 }
 
-constexpr int kColGridPerCUSpace = 64;  // workgroups per CU of the spatially ordered launch
+// Workgroups per CU of the spatially ordered launch: about one per 4 wavefront
+// tiles at 8000 grains (the dispatcher then balances tiles of uneven cost);
+// sweep of the comparison segment: 32: 3.59, 64: 3.35, 128: 3.27, 256: 3.18,
+// 512: 3.16, 1024: 3.18 ms.
+constexpr int kColGridPerCUSpace = 512;
 // ---- spatial work list ----------------------------------------------------
 // The wavefront tiles of all grains that read the same neighbour rows are put
 // next to each other: a tile's key is the cell of its centre (x: the middle of
