@@ -218,14 +218,12 @@ __device__ inline void cmp_tile(const SimParams& sp, int g, int cb, int zc, cons
       }
     }
     const int64_t li = (int64_t)(qlo + q) * nn2 + col;
-    // the previous IDLocal and both outputs are touched once here: nontemporal,
-    // so they take less of the L2 that the neighbour gathers reuse
     uint32_t idv = bid_get<B16>(bid, q);
-    if (idv == 0xFFFFFFFFu) idv = li < oldsize ? __builtin_nontemporal_load(&idl_old[li]) : 0u;
-    __builtin_nontemporal_store(idv, &idl_new[li]);
+    if (idv == 0xFFFFFFFFu) idv = li < oldsize ? idl_old[li] : 0u;
+    idl_new[li] = idv;
     const T a_in = own[li];
     const T o = (a_in < T(0) ? -a_in : a_in) < lim ? (T)0.5 * (a_in - best[q]) : a_in;
-    __builtin_nontemporal_store(o, &dst[li]);
+    dst[li] = o;
     if (o >= T(0)) {
       kmin = min(kmin, k);
       kmax = max(kmax, k);
