@@ -87,9 +87,10 @@ __device__ inline bool deep_inside(int j, int i, int k, int gb, int m) {
 // B16: the winning ids of the run's layers are kept as 16-bit halves, two per
 // register (grain ids below 0xFFFF; 0xFFFF stands for "not covered").  With
 // the spatial kernel's request of eight wavefronts per SIMD (64 vector
-// registers) the FP32 body spills about ten registers and still runs faster
-// than at seven wavefronts with unpacked ids (3.33 against 3.65 ms at 8000
-// grains).
+// registers) the FP32 body spills about ten registers, outside the gather
+// loop, and still runs faster than at seven wavefronts with unpacked ids
+// (3.33 against 3.65 ms at 8000 grains, measured with a grid-stride launch of
+// 64 workgroups per CU).
 template <bool B16>
 __device__ inline uint32_t bid_get(const uint32_t* bid, int q) {
   if constexpr (!B16) return bid[q];
